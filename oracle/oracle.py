@@ -1,0 +1,351 @@
+"""ctypes view of the FP64 CPU oracle (liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg import this module, and only as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+c_int, c_uint, c_double, c_void_p = ctypes.c_int, ctypes.c_uint, ctypes.c_double, ctypes.c_void_p
+_P = ctypes.POINTER
+
+
+def _ptr(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        L.orx_network_new.restype = c_void_p
+        L.orx_network_new.argtypes = [c_int, c_void_p, c_int, c_double, c_int]
+        L.orx_network_free.argtypes = [c_void_p]
+        L.orx_network_set_l.argtypes = [c_void_p, c_uint]
+        L.orx_network_n_baselines.argtypes = [c_void_p]
+        L.orx_plan_info.argtypes = [c_void_p, c_int, c_void_p]
+        L.orx_plan_geometry.argtypes = [c_void_p, c_int, c_void_p]
+        L.orx_plan_get_dp.argtypes = [c_void_p, c_int, c_void_p]
+        L.orx_plan_get_power_1d.argtypes = [c_void_p, c_int, c_void_p]
+        L.orx_plan_get_D.argtypes = [c_void_p, c_int, c_void_p]
+        L.orx_network_integrate.argtypes = [c_void_p, c_void_p, c_uint, c_void_p]
+        L.orx_segment_pipeline.argtypes = [c_void_p, c_void_p, c_void_p, c_uint, c_double, c_void_p, c_void_p]
+        L.orx_tseries_to_fseries.argtypes = [c_void_p, c_void_p, c_int]
+        L.orx_ctseries_to_fseries.argtypes = [c_void_p, c_void_p, c_int]
+        L.orx_dft.argtypes = [c_void_p, c_int, c_int]
+        L.orx_gl_nodes.argtypes = [c_int, c_void_p, c_void_p]
+        L.orx_sphPlm_array.argtypes = [c_uint, c_int, c_double, c_void_p]
+        L.orx_wigner_3j.restype = c_double
+        L.orx_wigner_3j.argtypes = [c_int] * 6
+        L.orx_mesh_nodes.argtypes = [c_uint, c_void_p, c_void_p]
+        L.orx_Y.argtypes = [c_uint, c_int, c_double, c_double, c_void_p]
+        L.orx_eval.argtypes = [c_uint, c_int, c_void_p, c_double, c_double, c_void_p]
+        L.orx_to_mesh.argtypes = [c_uint, c_int, c_void_p, c_void_p]
+        L.orx_from_mesh.argtypes = [c_uint, c_int, c_void_p, c_void_p]
+        L.orx_from_realmesh.argtypes = [c_uint, c_int, c_void_p, c_void_p]
+        L.orx_impulse.argtypes = [c_uint, c_double, c_double, c_void_p]
+        L.orx_euler_rotation_matrix.argtypes = [c_double, c_double, c_double, c_void_p]
+        L.orx_rot_matrix.argtypes = [c_double, c_double, c_void_p]
+        L.orx_invrot_matrix.argtypes = [c_double, c_double, c_void_p]
+        L.orx_rotation_D.argtypes = [c_uint, c_void_p, c_void_p]
+        L.orx_rotate.argtypes = [c_uint, c_int, c_void_p, c_void_p, c_void_p]
+        L.orx_rotate_z.argtypes = [c_uint, c_void_p, c_double, c_void_p]
+        L.orx_product.argtypes = [c_uint, c_int, c_uint, c_int, c_void_p, c_uint, c_int, c_void_p, c_int, c_void_p]
+        L.orx_product_plan_length.argtypes = [c_uint, c_int, c_uint, c_int, c_uint, c_int]
+        L.orx_projection_matrix_delay.argtypes = [c_uint, c_uint, c_void_p, c_double, c_void_p]
+        L.orx_projection_element_general.argtypes = [c_uint, c_int, c_void_p, c_double, c_int, c_void_p]
+        L.orx_params.argtypes = [c_double, c_double, c_void_p]
+        L.orx_rms_error.restype = c_double
+        L.orx_rms_error.argtypes = [c_uint, c_int, c_void_p, c_uint, c_int, c_void_p]
+        L.orx_series_real.argtypes = [c_uint, c_void_p]
+        L.orx_series_conj.argtypes = [c_uint, c_int, c_void_p]
+        L.orx_scale_power_l.argtypes = [c_uint, c_void_p, c_double, c_double]
+        L.orx_array_set_l.argtypes = [c_int, c_uint, c_int, c_void_p, c_uint, c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def length(l_max, polar):
+    return l_max + 1 if polar else (l_max + 1) ** 2
+
+
+def lmoffset(l_max, l, m):
+    L1 = l_max + 1
+    x = m * (m - 1)
+    base = L1 * m - x // 2 if m >= 0 else L1 * (L1 + m) + x // 2
+    return base + l - abs(m)
+
+
+class Network:
+    """A network FD plan (correlator_network_plan_fd) built by the oracle."""
+
+    def __init__(self, positions, n, delta_t, target_l=-1):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        self.n_instruments = pos.shape[0]
+        self.n = n
+        self._h = lib().orx_network_new(self.n_instruments, _ptr(pos), n, delta_t, target_l)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orx_network_free(self._h)
+            self._h = None
+
+    @property
+    def n_baselines(self):
+        return lib().orx_network_n_baselines(self._h)
+
+    def set_l(self, l):
+        if lib().orx_network_set_l(self._h, l):
+            raise RuntimeError("set_l failed")
+
+    def info(self, k):
+        a = np.zeros(8, dtype=np.int32)
+        lib().orx_plan_info(self._h, k, _ptr(a))
+        keys = ["l_max", "n", "stride", "transient", "polar", "rot_l_max", "index_a", "index_b"]
+        return dict(zip(keys, [int(v) for v in a]))
+
+    def geometry(self, k):
+        g = np.zeros(5)
+        lib().orx_plan_geometry(self._h, k, _ptr(g))
+        return g
+
+    def delay_product(self, k):
+        inf = self.info(k)
+        out = np.zeros((inf["n"], inf["stride"]), dtype=np.complex128)
+        lib().orx_plan_get_dp(self._h, k, _ptr(out))
+        return out
+
+    def power_1d(self, k):
+        inf = self.info(k)
+        out = np.zeros(inf["l_max"] + 1, dtype=np.complex128)
+        lib().orx_plan_get_power_1d(self._h, k, _ptr(out))
+        return out
+
+    def wigner_blocks(self, k):
+        inf = self.info(k)
+        L = inf["rot_l_max"]
+        tot = sum((2 * l + 1) ** 2 for l in range(1, L + 1))
+        out = np.zeros(tot, dtype=np.complex128)
+        lib().orx_plan_get_D(self._h, k, _ptr(out))
+        return out
+
+    def integrate(self, fseries, sky_l):
+        fs = np.ascontiguousarray(fseries, dtype=np.complex128)
+        sky = np.zeros((sky_l + 1) ** 2, dtype=np.complex128)
+        if lib().orx_network_integrate(self._h, _ptr(fs), sky_l, _ptr(sky)):
+            raise RuntimeError("integrate failed")
+        return sky
+
+    def segment(self, tseries, sky_l, window=None, gmst=0.0, mesh=True):
+        ts = np.ascontiguousarray(tseries, dtype=np.float64)
+        w = None if window is None else np.ascontiguousarray(window, dtype=np.float64)
+        sky = np.zeros((sky_l + 1) ** 2, dtype=np.complex128)
+        B2 = 2 * (sky_l + 1)
+        m = np.zeros((B2, B2), dtype=np.complex128) if mesh else None
+        rc = lib().orx_segment_pipeline(self._h, _ptr(ts), None if w is None else _ptr(w), sky_l, gmst,
+                                        _ptr(sky), None if m is None else _ptr(m))
+        if rc:
+            raise RuntimeError("segment pipeline failed")
+        return sky, m
+
+
+def tseries_to_fseries(ts):
+    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    out = np.zeros(ts.shape[0], dtype=np.complex128)
+    lib().orx_tseries_to_fseries(_ptr(ts), _ptr(out), ts.shape[0])
+    return out
+
+
+def ctseries_to_fseries(ts):
+    ts = np.ascontiguousarray(ts, dtype=np.complex128)
+    out = np.zeros(ts.shape[0], dtype=np.complex128)
+    lib().orx_ctseries_to_fseries(_ptr(ts), _ptr(out), ts.shape[0])
+    return out
+
+
+def dft(x, sign=-1):
+    d = np.array(x, dtype=np.complex128)
+    lib().orx_dft(_ptr(d), d.shape[0], sign)
+    return d
+
+
+def gl_nodes(n):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orx_gl_nodes(n, _ptr(x), _ptr(w))
+    return x, w
+
+
+def mesh_nodes(l):
+    n = 2 * (l + 1)
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orx_mesh_nodes(l, _ptr(x), _ptr(w))
+    return x, w
+
+
+def sphPlm_array(l_max, m, x):
+    out = np.zeros(l_max - m + 1)
+    lib().orx_sphPlm_array(l_max, m, x, _ptr(out))
+    return out
+
+
+def wigner_3j(*a):
+    return lib().orx_wigner_3j(*[int(v) for v in a])
+
+
+def Y(l, m, theta, phi):
+    out = np.zeros(1, dtype=np.complex128)
+    lib().orx_Y(l, m, theta, phi, _ptr(out))
+    return out[0]
+
+
+def eval_series(coeff, l, polar, theta, phi):
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    out = np.zeros(1, dtype=np.complex128)
+    lib().orx_eval(l, polar, _ptr(c), theta, phi, _ptr(out))
+    return out[0]
+
+
+def to_mesh(coeff, l, polar=0):
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    n = 2 * (l + 1)
+    out = np.zeros((n, n), dtype=np.complex128)
+    lib().orx_to_mesh(l, polar, _ptr(c), _ptr(out))
+    return out
+
+
+def from_mesh(mesh, l, polar=0):
+    m = np.ascontiguousarray(mesh, dtype=np.complex128)
+    out = np.zeros(length(l, polar), dtype=np.complex128)
+    lib().orx_from_mesh(l, polar, _ptr(m), _ptr(out))
+    return out
+
+
+def from_realmesh(mesh, l, polar=0):
+    m = np.ascontiguousarray(mesh, dtype=np.float64)
+    out = np.zeros(length(l, polar), dtype=np.complex128)
+    lib().orx_from_realmesh(l, polar, _ptr(m), _ptr(out))
+    return out
+
+
+def impulse(l, theta, phi):
+    out = np.zeros((l + 1) ** 2, dtype=np.complex128)
+    lib().orx_impulse(l, theta, phi, _ptr(out))
+    return out
+
+
+def euler_rotation_matrix(gamma, beta, alpha):
+    R = np.zeros(9)
+    lib().orx_euler_rotation_matrix(gamma, beta, alpha, _ptr(R))
+    return R.reshape(3, 3)
+
+
+def rot_matrix(theta, phi):
+    R = np.zeros(9)
+    lib().orx_rot_matrix(theta, phi, _ptr(R))
+    return R.reshape(3, 3)
+
+
+def invrot_matrix(theta, phi):
+    R = np.zeros(9)
+    lib().orx_invrot_matrix(theta, phi, _ptr(R))
+    return R.reshape(3, 3)
+
+
+def rotation_D(l_max, R):
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(9)
+    tot = sum((2 * l + 1) ** 2 for l in range(1, l_max + 1))
+    out = np.zeros(tot, dtype=np.complex128)
+    lib().orx_rotation_D(l_max, _ptr(R), _ptr(out))
+    blocks, off = {}, 0
+    for l in range(1, l_max + 1):
+        n = 2 * l + 1
+        # storage index (2l+1) * (m'+l) + (m+l): row m', column m
+        blocks[l] = out[off:off + n * n].reshape(n, n)
+        off += n * n
+    return blocks
+
+
+def rotate(coeff, l, polar, R):
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(9)
+    out = np.zeros((l + 1) ** 2, dtype=np.complex128)
+    if lib().orx_rotate(l, polar, _ptr(c), _ptr(R), _ptr(out)):
+        raise RuntimeError("rotate failed")
+    return out
+
+
+def rotate_z(coeff, l, phi):
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    out = np.zeros_like(c)
+    lib().orx_rotate_z(l, _ptr(c), phi, _ptr(out))
+    return out
+
+
+def product(a, al, ap, b, bl, bp, dl, dp, mode=-1):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    out = np.zeros(length(dl, dp), dtype=np.complex128)
+    if lib().orx_product(dl, dp, al, ap, _ptr(a), bl, bp, _ptr(b), mode, _ptr(out)):
+        raise RuntimeError("product failed")
+    return out
+
+
+def projection_matrix_delay(n, l_max, r, dt):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    polar = r[0] == 0.0 and r[1] == 0.0
+    out = np.zeros((n, length(l_max, polar)), dtype=np.complex128)
+    lib().orx_projection_matrix_delay(n, l_max, _ptr(r), dt, _ptr(out))
+    return out
+
+
+def projection_element_general(l_max, polar, r, dt, k):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(length(l_max, polar), dtype=np.complex128)
+    lib().orx_projection_element_general(l_max, polar, _ptr(r), dt, k, _ptr(out))
+    return out
+
+
+def params(dt, D):
+    out = np.zeros(3, dtype=np.int32)
+    lib().orx_params(dt, D, _ptr(out))
+    return dict(P=int(out[0]), a=int(out[1]), n_p=int(out[2]))
+
+
+def rms_error(m, lm, pm, e, le, pe):
+    m = np.ascontiguousarray(m, dtype=np.complex128)
+    e = np.ascontiguousarray(e, dtype=np.complex128)
+    return lib().orx_rms_error(lm, pm, _ptr(m), le, pe, _ptr(e))
+
+
+def series_real(coeff, l):
+    c = np.array(coeff, dtype=np.complex128)
+    lib().orx_series_real(l, _ptr(c))
+    return c
+
+
+def series_conj(coeff, l, polar=0):
+    c = np.array(coeff, dtype=np.complex128)
+    lib().orx_series_conj(l, polar, _ptr(c))
+    return c
+
+
+def scale_power_l(coeff, l, power, overall):
+    c = np.array(coeff, dtype=np.complex128)
+    lib().orx_scale_power_l(l, _ptr(c), power, overall)
+    return c
+
+
+def array_set_l(arr, l_old, polar, l_new):
+    a = np.ascontiguousarray(arr, dtype=np.complex128)
+    n = a.shape[0]
+    out = np.zeros((n, length(l_new, polar)), dtype=np.complex128)
+    lib().orx_array_set_l(n, l_old, polar, _ptr(a), l_new, _ptr(out))
+    return out

@@ -1,0 +1,103 @@
+"""Synthetic detector data for the cross-power path.
+
+Restates the conventions of the reference's injection helpers
+(src/lib/inject.c): white Gaussian noise (inject.c:48-56) and plane-wave
+injection by a circular FFT phase shift, x_i(t) = h(t + r_i . s)
+(inject_transform_point_source, inject.c:119-152; inject_into,
+inject.c:187-203, with theta = pi/2 - dec and phi = ra - gmst).  Random
+numbers come from numpy's PCG64 so a seed reproduces the same data on
+every machine.
+
+The workloads are the ones the throughput and parity runs use:
+
+  two_site      2 sites (46.45N, 119.41W), (30.56N, 90.77W), 4096 Hz,
+                16 x 4096-sample segments, Hann window, l_max 25.
+  three_site    adds (43.63N, 10.50E); 1024 x 1024-sample segments,
+                l_max 32, white noise plus one sine-Gaussian burst
+                (f0 ~ U(100, 400) Hz, Q = 9, network SNR 20, isotropic
+                direction) in every segment.
+"""
+import math
+
+import numpy as np
+
+C_SI = 299792458.0
+EARTH_RADIUS_SPHERE = 6371e3
+
+SITES = {
+    "A": (46.45, -119.41),
+    "B": (30.56, -90.77),
+    "C": (43.63, 10.50),
+}
+
+
+def site_position(lat_deg, lon_deg, radius=EARTH_RADIUS_SPHERE):
+    """Phase centre in light-seconds, as instrument_new_from_r_theta_phi
+    (instrument.c:164-172) builds it from (r, theta = pi/2 - lat, phi = lon)."""
+    r = radius / C_SI
+    th = math.pi / 2 - math.radians(lat_deg)
+    ph = math.radians(lon_deg)
+    return np.array([r * math.sin(th) * math.cos(ph), r * math.sin(th) * math.sin(ph), r * math.cos(th)])
+
+
+def positions(names):
+    return np.array([site_position(*SITES[n]) for n in names])
+
+
+def hann(n):
+    """Symmetric Hann window w[k] = 0.5 - 0.5 cos(2 pi k / (n - 1))."""
+    k = np.arange(n)
+    return 0.5 - 0.5 * np.cos(2 * np.pi * k / (n - 1))
+
+
+def delay_circular(x, delay_s, delta_t):
+    """x(t + delay), circularly, by an FFT phase shift (inject.c:119-152)."""
+    n = x.shape[-1]
+    ft = np.fft.rfft(x)
+    f = np.arange(ft.shape[-1])
+    ft = ft * np.exp(2j * np.pi * delay_s * f / (delta_t * n))
+    return np.fft.irfft(ft, n)
+
+
+def r_dot_s(r, theta, phi):
+    """instrument.c:77-80"""
+    return (r[0] * math.cos(phi) + r[1] * math.sin(phi)) * math.sin(theta) + r[2] * math.cos(theta)
+
+
+def sine_gaussian(n, delta_t, f0, q, t0):
+    t = np.arange(n) * delta_t - t0
+    tau = q / (math.sqrt(2.0) * math.pi * f0)
+    return np.sin(2 * math.pi * f0 * t) * np.exp(-(t / tau) ** 2)
+
+
+def isotropic_direction(rng):
+    """(theta, phi) uniform on the sphere."""
+    return math.acos(rng.uniform(-1.0, 1.0)), rng.uniform(0.0, 2 * math.pi)
+
+
+def two_site(n_segments=16, n=4096, seed=0):
+    """White N(0,1) float64 at 4096 Hz, cut into non-overlapping segments.
+    Returns tseries [n_det][n_segments][n]."""
+    rng = np.random.default_rng(seed)
+    stream = rng.standard_normal((2, n_segments * n))
+    return stream.reshape(2, n_segments, n)
+
+
+def three_site(n_segments=1024, n=1024, seed=0, snr=20.0, delta_t=1.0 / 4096):
+    """White N(0,1) float64 plus one sine-Gaussian plane wave per segment,
+    delayed per site by r_i . s.  Returns (tseries [3][S][n], injections),
+    injections = [(theta, phi, f0)] per segment."""
+    rng = np.random.default_rng(seed)
+    pos = positions("ABC")
+    data = rng.standard_normal((3, n_segments, n))
+    inj = []
+    for s in range(n_segments):
+        f0 = rng.uniform(100.0, 400.0)
+        theta, phi = isotropic_direction(rng)
+        h = sine_gaussian(n, delta_t, f0, 9.0, 0.5 * n * delta_t)
+        # network SNR^2 = sum over the 3 sites of sum_t h^2 (unit white noise)
+        h *= snr / math.sqrt(3.0 * float(np.dot(h, h)))
+        for d in range(3):
+            data[d, s] += delay_circular(h, r_dot_s(pos[d], theta, phi), delta_t)
+        inj.append((theta, phi, f0))
+    return data, inj
