@@ -1,0 +1,171 @@
+"""Throughput of the batched cross-power path on one MI355X.
+
+Workload (the metric's configuration): three sites (46.45N 119.41W),
+(30.56N 90.77W), (43.63N 10.50E) on a 6371 km sphere; 1024 segments of
+1024 samples (0.25 s at 4096 Hz) of white noise plus one sine-Gaussian
+burst per segment; sky l_max 32.  One step = one pass of the batched
+pipeline over the 1024 segments (FFT, cross spectra x delay-product
+projection, rotation and baseline average, mesh synthesis), inputs
+resident in HBM.  value = sky maps/s of the mixed-precision (float32)
+path; the float64 path's rate and the a_lm / map error of the mixed path
+against it are reported beside it, with the CPU oracle's rate on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(info, S, esz, in_bytes_per_sample=8):
+    """Algorithmic HBM bytes per launch of each stage."""
+    D, nb, N, L, CT, ns = info["detectors"], info["baselines"], info["N"], info["sky_l"], info["columns"], info["k_splits"]
+    Nh = N // 2 + 1
+    ncoef = (L + 1) ** 2
+    n = 2 * (L + 1)
+    spec = D * S * Nh * 2 * esz
+    fft = D * S * N * in_bytes_per_sample + spec
+    xproj = spec + (N + 2) * CT * esz + ns * S * CT * esz
+    alm = ns * S * CT * esz + nb * ncoef * 2 * esz + S * ncoef * 2 * esz
+    synth = S * ncoef * 2 * esz + S * n * n * 2 * esz
+    return [fft, xproj, alm, synth]
+
+
+def run_path(S, net, L, d_ts, nseg, precision, steps, warmup):
+    cdt = np.complex128 if precision == S.FP64 else np.complex64
+    esz = np.dtype(cdt).itemsize // 2
+    b = S.Batch(net, L, precision, max_segments=nseg)
+    d_alm = S.DeviceBuffer(nseg * (L + 1) ** 2 * 2 * esz)
+    d_map = S.DeviceBuffer(nseg * (2 * L + 2) ** 2 * 2 * esz)
+    st = S.lib().sphr_stream()
+    for _ in range(warmup):
+        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, stream=st)
+    S.lib().sphr_stream_synchronize(st)
+    e0, e1 = S.lib().sphr_event_new(), S.lib().sphr_event_new()
+    S.lib().sphr_event_record(e0, st)
+    for _ in range(steps):
+        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, stream=st)
+    S.lib().sphr_event_record(e1, st)
+    S.lib().sphr_event_synchronize(e1)
+    ms = S.lib().sphr_event_elapsed_ms(e0, e1)
+    # per-stage times on the same stream (separate pass, events around each stage)
+    b.profile(True)
+    for _ in range(steps):
+        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, stream=st)
+    stage_ms, runs = b.profile_read()
+    b.profile(False)
+    alm = d_alm.to_numpy(cdt, (nseg, (L + 1) ** 2))
+    mp = d_map.to_numpy(cdt, (nseg, 2 * L + 2, 2 * L + 2))
+    info = b.info()
+    S.lib().sphr_event_free(e0)
+    S.lib().sphr_event_free(e1)
+    return dict(ms_per_step=ms / steps, stage_ms=[t / runs for t in stage_ms], alm=alm, map=mp, info=info, esz=esz)
+
+
+def cpu_baseline(ts, L, budget_s):
+    from oracle import oracle as O
+    from sphradiometer_amd import inject
+    pos = inject.positions("ABC")
+    t0 = time.time()
+    net = O.Network(pos, ts.shape[2], 1.0 / 4096, L)
+    plan_s = time.time() - t0
+    done = 0
+    t0 = time.time()
+    while done < ts.shape[1] and time.time() - t0 < budget_s:
+        net.segment(ts[:, done], L)
+        done += 1
+    el = time.time() - t0
+    return dict(value=done / el, unit="sky maps/s", cores=1, kind="port",
+                sample="%d of the 1024 three-site segments (a_lm + 66x66 map each), FP64 C oracle, 1 thread; "
+                       "plan build %.2f s not included" % (done, plan_s))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--segments", type=int, default=1024)
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (segments are independent; "
+              "see DESIGN.md for how the path would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import sphradiometer_amd as S
+    from sphradiometer_amd import inject
+    S.lib()
+    if not S.gpu_available():
+        print("no GPU: " + S.last_error(), file=sys.stderr)
+        sys.exit(1)
+
+    L, N, nseg = 32, 1024, args.segments
+    t0 = time.time()
+    net = S.Network(inject.positions("ABC"), N, 1.0 / 4096)
+    net.set_l(L)
+    plan_s = time.time() - t0
+    ts, _ = inject.three_site(nseg, N, seed=0)
+    d_ts = S.DeviceBuffer.from_numpy(ts)
+    print("plan built in %.2f s; data %s on device" % (plan_s, ts.shape), file=sys.stderr, flush=True)
+
+    r64 = run_path(S, net, L, d_ts, nseg, S.FP64, args.steps, args.warmup)
+    print("fp64: %.3f ms/step, stages %s" % (r64["ms_per_step"], ["%.4f" % t for t in r64["stage_ms"]]), file=sys.stderr, flush=True)
+    rmx = run_path(S, net, L, d_ts, nseg, S.MIXED, args.steps, args.warmup)
+    print("mixed: %.3f ms/step, stages %s" % (rmx["ms_per_step"], ["%.4f" % t for t in rmx["stage_ms"]]), file=sys.stderr, flush=True)
+
+    alm_err = np.linalg.norm(rmx["alm"] - r64["alm"], axis=1) / np.linalg.norm(r64["alm"], axis=1)
+    map_err = float(np.linalg.norm(rmx["map"] - r64["map"]) / np.linalg.norm(r64["map"]))
+    value = nseg / (rmx["ms_per_step"] * 1e-3)
+    fp64_value = nseg / (r64["ms_per_step"] * 1e-3)
+
+    names = ["fft", "xproj", "alm", "synth"]
+    sb = stage_bytes(rmx["info"], nseg, rmx["esz"])
+    k = int(np.argmax(rmx["stage_ms"]))
+    achieved = sb[k] / (rmx["stage_ms"][k] * 1e-3) / 1e9
+    sb64 = stage_bytes(r64["info"], nseg, r64["esz"])
+    k64 = int(np.argmax(r64["stage_ms"]))
+
+    out = {
+        "metric": "sky maps/s, 3-det l_max=32: mixed-prec batched vs FP64 plain at <=1e-5 a_lm err",
+        "value": round(value, 1),
+        "unit": "sky maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(rmx["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: white N(0,1) float64 + one sine-Gaussian burst per segment (seed 0)",
+        "config": {"workload": "three sites, %d segments x %d samples at 4096 Hz, sky l_max %d, a_lm + 66x66 map per segment" % (nseg, N, L),
+                   "segments": nseg, "samples": N, "l_max": L, "detectors": 3, "baselines": 3},
+        "fp64": {"value": round(fp64_value, 1), "ms_per_step": round(r64["ms_per_step"], 4),
+                 "stage_ms": [round(t, 5) for t in r64["stage_ms"]],
+                 "dominant": names[k64], "dominant_gbs": round(sb64[k64] / (r64["stage_ms"][k64] * 1e-3) / 1e9, 1)},
+        "mixed_vs_fp64": {"speedup": round(value / fp64_value, 3), "alm_rel_err_max": float(alm_err.max()),
+                          "alm_rel_err_median": float(np.median(alm_err)), "map_rel_err": map_err},
+        "stage_ms": [round(t, 5) for t in rmx["stage_ms"]],
+        "roofline": {"bound": "hbm", "kernel": names[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "plan_build_s": round(plan_s, 2),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ts, L, args.cpu_budget)
+        out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

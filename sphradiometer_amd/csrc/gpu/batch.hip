@@ -1,0 +1,312 @@
+/*
+ * Batched device pipeline (sphradiometer_batch.h): FFT -> cross spectrum x
+ * projection -> sky coefficients -> mesh synthesis for many segments per
+ * launch, in float64 (SPHR_FP64) or float32 (SPHR_MIXED).
+ *
+ * Plan tables built once per batch object from the host network plan:
+ *   W     [N+2][CT] the delay product folded for real data: for baseline
+ *         columns 2l, 2l+1 and frequency f <= N/2,
+ *           rows 2f   : (dp[f] + dp[N-f]) / N        (dp[f] alone at 0, N/2)
+ *           rows 2f+1 : i (dp[f] - dp[N-f]) / N      (i dp[f] at 0, N/2)
+ *         so sum_{f<N} dp[f] X[f] / N = sum_k A[k] W[k] with
+ *         A = (Re X[f], Im X[f]) over the half spectrum (X[N-f] = conj X[f]
+ *         for real time series)
+ *   R     [nb][(L+1)^2] (1/nb) D^l_{0 m'} of each baseline's rotation, in
+ *         the sky coefficient layout (zero above the baseline's l_max)
+ *   synthesis tables of order L and the FFT twiddles
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sphradiometer/sphradiometer_batch.h>
+#include "../sphr_internal.h"
+#include "kernels.h"
+#include "runtime.h"
+#include "synth_tables.h"
+
+struct ev_set {
+	hipEvent_t e[5];
+};
+
+struct sphr_batch {
+	int precision, D, nb, N, Nh, L, max_segments, nsplit, k_per_split;
+	struct xproj_geom g;
+	struct synth_dev syn;
+	void *W, *R, *tw, *spec, *partial;
+	int *col_bl;
+	int2 *lm;
+	int profile;
+	struct ev_set *ev;
+	int ev_used, ev_cap;
+};
+
+static size_t block_offset_h(int l)
+{
+	return ((size_t) 4 * l * l * l - l - 3) / 3;
+}
+
+template <typename T>
+static int upload(void **dst, const T *src, size_t count)
+{
+	if(sphr_rt_check(hipMalloc(dst, count * sizeof(T) + 16), "hipMalloc"))
+		return -1;
+	return sphr_rt_check(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+template <typename T>
+static int build_tables(struct sphr_batch *bt, const struct sphr_net_desc *d)
+{
+	const int N = bt->N, Nh = bt->Nh, CT = bt->g.CT, K2 = bt->g.K2, L = bt->L, nb = bt->nb;
+	const size_t ncoef = (size_t) (L + 1) * (L + 1);
+	T *W = (T *) calloc((size_t) K2 * CT, sizeof(T));
+	T *R = (T *) calloc(2 * ncoef * nb, sizeof(T));
+	T *tw = (T *) malloc(2 * (size_t) N * sizeof(T));
+	double *twd = (double *) malloc(2 * (size_t) N * sizeof(double));
+	int rc = -1;
+	if(!W || !R || !tw || !twd)
+		goto done;
+	for(int b = 0; b < nb; b++) {
+		const int Lb = d->Lb[b], c0 = bt->g.col_start[b];
+		const double *dp = d->dp[b];		/* N x (Lb+1) complex */
+		for(int f = 0; f < Nh; f++) {
+			const int fn = (N - f) % N;
+			const int self = (f == 0 || 2 * f == N);
+			for(int l = 0; l <= Lb; l++) {
+				const double ar = dp[2 * ((size_t) f * (Lb + 1) + l)], ai = dp[2 * ((size_t) f * (Lb + 1) + l) + 1];
+				const double br = self ? 0.0 : dp[2 * ((size_t) fn * (Lb + 1) + l)];
+				const double bi = self ? 0.0 : dp[2 * ((size_t) fn * (Lb + 1) + l) + 1];
+				/* G = a + b, H = i (a - b) */
+				const double gr = ar + br, gi = ai + bi;
+				const double hr = -(ai - bi), hi = ar - br;
+				W[(size_t) (2 * f) * CT + c0 + 2 * l] = (T) (gr / N);
+				W[(size_t) (2 * f) * CT + c0 + 2 * l + 1] = (T) (gi / N);
+				W[(size_t) (2 * f + 1) * CT + c0 + 2 * l] = (T) (hr / N);
+				W[(size_t) (2 * f + 1) * CT + c0 + 2 * l + 1] = (T) (hi / N);
+			}
+		}
+		/* rotation rows (sh_series_rotation.c:566-577), averaged over
+		 * baselines (correlator.c:1004) */
+		T *Rb = R + 2 * ncoef * b;
+		Rb[0] = (T) (1.0 / nb);
+		for(int l = 1; l <= Lb; l++)
+			for(int mp = -l; mp <= l; mp++) {
+				const size_t src = block_offset_h(l) + (size_t) (2 * l + 1) * (mp + l) + l;
+				const int dst = lm_offset(L, l, mp);
+				Rb[2 * dst] = (T) (d->blocks[b][2 * src] / nb);
+				Rb[2 * dst + 1] = (T) (d->blocks[b][2 * src + 1] / nb);
+			}
+	}
+	sphr_twiddles(N, -1, twd);
+	for(int k = 0; k < 2 * N; k++)
+		tw[k] = (T) twd[k];
+	if(upload(&bt->W, W, (size_t) K2 * CT) || upload(&bt->R, R, 2 * ncoef * nb) || upload(&bt->tw, tw, 2 * (size_t) N))
+		goto done;
+	rc = 0;
+done:
+	free(W);
+	free(R);
+	free(tw);
+	free(twd);
+	return rc;
+}
+
+extern "C" void sphr_batch_free(struct sphr_batch *bt)
+{
+	if(!bt)
+		return;
+	void *ptrs[] = { bt->W, bt->R, bt->tw, bt->spec, bt->partial, bt->col_bl, bt->lm };
+	for(unsigned i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
+		if(ptrs[i])
+			hipFree(ptrs[i]);
+	synth_tables_free(&bt->syn);
+	for(int i = 0; i < bt->ev_cap; i++)
+		for(int k = 0; k < 5; k++)
+			hipEventDestroy(bt->ev[i].e[k]);
+	free(bt->ev);
+	free(bt);
+}
+
+extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, unsigned sky_l, int precision, int max_segments)
+{
+	if(sphr_gpu_ensure())
+		return NULL;
+	const int N = d->N;
+	if(N < 64 || (N & (N - 1))) {
+		sphr_set_error("batched path needs a power-of-two segment length >= 64 (got %d)", N);
+		return NULL;
+	}
+	if(d->nb < 1 || d->nb > SPHR_MAX_BASELINES || d->n_det > 2 * SPHR_MAX_BASELINES || max_segments < 1) {
+		sphr_set_error("unsupported network shape");
+		return NULL;
+	}
+	const size_t esz = precision == SPHR_MIXED ? sizeof(float) : sizeof(double);
+	if((size_t) N * 2 * esz > sphr_rt_lds_max()) {
+		sphr_set_error("segment length %d does not fit the FFT's LDS tile", N);
+		return NULL;
+	}
+	struct sphr_batch *bt = (struct sphr_batch *) calloc(1, sizeof(*bt));
+	if(!bt)
+		return NULL;
+	bt->precision = precision;
+	bt->D = d->n_det;
+	bt->nb = d->nb;
+	bt->N = N;
+	bt->Nh = N / 2 + 1;
+	bt->L = (int) sky_l;
+	bt->max_segments = max_segments;
+	bt->g.D = d->n_det;
+	bt->g.nb = d->nb;
+	bt->g.N = N;
+	bt->g.Nh = bt->Nh;
+	bt->g.K2 = 2 * bt->Nh;
+	int CT = 0;
+	for(int b = 0; b < d->nb; b++) {
+		if(!d->polar[b] || d->Lb[b] > (int) sky_l) {
+			sphr_set_error("baseline %d: the batched path needs polar powers with l_max <= the sky's", b);
+			free(bt);
+			return NULL;
+		}
+		bt->g.ia[b] = d->ia[b];
+		bt->g.ib[b] = d->ib[b];
+		bt->g.Lb[b] = d->Lb[b];
+		bt->g.col_start[b] = CT;
+		CT += 2 * (d->Lb[b] + 1);
+	}
+	bt->g.CT = CT;
+	/* split K so the projection launch has enough workgroups */
+	{
+		const int base = ((max_segments + 15) / 16) * ((CT + 255) / 256);
+		const int target = 2 * sphr_rt_cus();
+		int ns = (target + base - 1) / base;
+		const int maxs = (bt->g.K2 + 63) / 64;
+		ns = ns < 1 ? 1 : ns > maxs ? maxs : ns;
+		int kps = (bt->g.K2 + ns - 1) / ns;
+		kps = (kps + 31) / 32 * 32;
+		bt->k_per_split = kps;
+		bt->nsplit = (bt->g.K2 + kps - 1) / kps;
+	}
+	int *colbl = (int *) malloc(CT * sizeof(int));
+	const int ncoef = (bt->L + 1) * (bt->L + 1);
+	int2 *lm = (int2 *) malloc(ncoef * sizeof(int2));
+	if(!colbl || !lm)
+		goto fail;
+	for(int b = 0; b < d->nb; b++)
+		for(int c = 0; c < 2 * (d->Lb[b] + 1); c++)
+			colbl[bt->g.col_start[b] + c] = b;
+	for(int i = 0; i < ncoef; i++) {
+		int l, m;
+		lm_decode(bt->L, i, &l, &m);
+		lm[i].x = l;
+		lm[i].y = m;
+	}
+	if(upload((void **) &bt->col_bl, colbl, CT) || upload((void **) &bt->lm, lm, ncoef))
+		goto fail;
+	free(colbl);
+	free(lm);
+	colbl = NULL;
+	lm = NULL;
+	if((precision == SPHR_MIXED ? build_tables<float>(bt, d) : build_tables<double>(bt, d)))
+		goto fail;
+	if(synth_tables_build(bt->L, &bt->syn))
+		goto fail;
+	if(sphr_rt_check(hipMalloc(&bt->spec, (size_t) bt->D * max_segments * bt->Nh * 2 * esz), "hipMalloc spectra")
+	   || sphr_rt_check(hipMalloc(&bt->partial, (size_t) bt->nsplit * max_segments * CT * esz), "hipMalloc partials"))
+		goto fail;
+	return bt;
+fail:
+	free(colbl);
+	free(lm);
+	sphr_batch_free(bt);
+	return NULL;
+}
+
+template <typename T>
+static int run_t(struct sphr_batch *bt, const void *ts, int f32, int S, const double *window, const double *gmst, void *alm, void *map, hipStream_t st, hipEvent_t *ev)
+{
+	if(ev)
+		hipEventRecord(ev[0], st);
+	if(launch_rfft_pairs<T>(ts, f32, window, bt->D * S, bt->N, (const cplx<T> *) bt->tw, (cplx<T> *) bt->spec, st))
+		return -1;
+	if(ev)
+		hipEventRecord(ev[1], st);
+	if(launch_xproj<T>((const cplx<T> *) bt->spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) bt->partial, st))
+		return -1;
+	if(ev)
+		hipEventRecord(ev[2], st);
+	if(launch_alm<T, T>((const T *) bt->partial, bt->nsplit, S, &bt->g, bt->L, (const cplx<T> *) bt->R, bt->lm, gmst, (cplx<T> *) alm, st))
+		return -1;
+	if(ev)
+		hipEventRecord(ev[3], st);
+	if(map && launch_synth<T, T, T>((const cplx<T> *) alm, S, 0, &bt->syn.g, &bt->syn.tb, (cplx<T> *) map, st))
+		return -1;
+	if(ev)
+		hipEventRecord(ev[4], st);
+	return 0;
+}
+
+extern "C" int sphr_batch_run(struct sphr_batch *bt, const void *ts, int f32, int S, const double *window, const double *gmst, void *alm, void *map, void *stream)
+{
+	if(!bt || S < 1 || S > bt->max_segments || !ts || (!alm && map) || !alm) {
+		sphr_set_error("sphr_batch_run: bad arguments (segments %d of at most %d)", S, bt ? bt->max_segments : 0);
+		return -1;
+	}
+	hipStream_t st = stream ? (hipStream_t) stream : sphr_rt_stream();
+	hipEvent_t *ev = NULL;
+	if(bt->profile) {
+		if(bt->ev_used == bt->ev_cap) {
+			const int cap = bt->ev_cap ? 2 * bt->ev_cap : 64;
+			struct ev_set *n = (struct ev_set *) realloc(bt->ev, cap * sizeof(*n));
+			if(!n)
+				return -1;
+			bt->ev = n;
+			for(int i = bt->ev_cap; i < cap; i++)
+				for(int k = 0; k < 5; k++)
+					hipEventCreate(&bt->ev[i].e[k]);
+			bt->ev_cap = cap;
+		}
+		ev = bt->ev[bt->ev_used++].e;
+	}
+	const int r = bt->precision == SPHR_MIXED ? run_t<float>(bt, ts, f32, S, window, gmst, alm, map, st, ev) : run_t<double>(bt, ts, f32, S, window, gmst, alm, map, st, ev);
+	if(r)
+		sphr_set_error("sphr_batch_run: kernel launch failed (%s)", hipGetErrorString(hipGetLastError()));
+	return r;
+}
+
+extern "C" int sphr_batch_profile(struct sphr_batch *bt, int enable)
+{
+	bt->profile = enable;
+	bt->ev_used = 0;
+	return 0;
+}
+
+extern "C" int sphr_batch_profile_read(struct sphr_batch *bt, double ms[4], int *runs)
+{
+	for(int k = 0; k < 4; k++)
+		ms[k] = 0.0;
+	for(int i = 0; i < bt->ev_used; i++) {
+		if(sphr_rt_check(hipEventSynchronize(bt->ev[i].e[4]), "hipEventSynchronize"))
+			return -1;
+		for(int k = 0; k < 4; k++) {
+			float t = 0.f;
+			hipEventElapsedTime(&t, bt->ev[i].e[k], bt->ev[i].e[k + 1]);
+			ms[k] += t;
+		}
+	}
+	if(runs)
+		*runs = bt->ev_used;
+	bt->ev_used = 0;
+	return 0;
+}
+
+extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[8])
+{
+	info[0] = bt->D;
+	info[1] = bt->nb;
+	info[2] = bt->N;
+	info[3] = bt->L;
+	info[4] = bt->g.CT;
+	info[5] = bt->nsplit;
+	info[6] = bt->precision;
+	info[7] = bt->k_per_split;
+}

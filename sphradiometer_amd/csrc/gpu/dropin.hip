@@ -1,0 +1,490 @@
+/*
+ * GPU back ends of the synchronous drop-in entry points (sh_series_array_
+ * dotc, sh_series_rotate(_z), sh_series_to_mesh, the FD integrators and
+ * the FFT helpers).  Each call copies its operands to the device, runs the
+ * kernels on the library stream and copies the results back before
+ * returning, as the reference's functions return with results in host
+ * memory.  Calls are serialised: they share the scratch buffers.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../sphr_internal.h"
+#include "kernels.h"
+#include "runtime.h"
+#include "synth_tables.h"
+
+#define LOCK() pthread_mutex_lock(&sphr_dropin_lock)
+#define UNLOCK() pthread_mutex_unlock(&sphr_dropin_lock)
+
+enum { SL_A, SL_B, SL_C, SL_D, SL_E, SL_F, SL_G, SL_H };
+
+void sphr_twiddles(int n, int sign, double *out)
+{
+	const long double two_pi = 6.283185307179586476925286766559005768L;
+	for(int k = 0; k < n; k++) {
+		const long double a = two_pi * (long double) k / (long double) n;
+		out[2 * k] = (double) cosl(a);
+		out[2 * k + 1] = (double) (sign < 0 ? -sinl(a) : sinl(a));
+	}
+}
+
+int sphr_factor(int n, int *fac, int max)
+{
+	int k = 0;
+	while(n % 4 == 0 && k < max) {
+		fac[k++] = 4;
+		n /= 4;
+	}
+	for(int p = 2; n > 1 && k < max; p++)
+		while(n % p == 0 && k < max) {
+			fac[k++] = p;
+			n /= p;
+		}
+	return n == 1 ? k : -1;
+}
+
+/* ---- synthesis tables (host build, device copy) ---- */
+
+int synth_tables_build(int L, struct synth_dev *t)
+{
+	memset(t, 0, sizeof(*t));
+	const int B = L + 1, n = 2 * B;
+	t->g.L = L;
+	t->g.B = B;
+	t->g.n = n;
+	t->g.nfac = sphr_factor(n, t->g.fac, SPHR_MAX_FACTORS);
+	if(t->g.nfac < 0) {
+		sphr_set_error("cannot factor the phi transform length %d", n);
+		return -1;
+	}
+	const double *x;
+	sphr_mesh_rule(L, &x, NULL);
+	double *hx = (double *) malloc(B * sizeof(double));
+	double *hs = (double *) malloc((size_t) B * (L + 1) * sizeof(double));
+	double *ha = (double *) calloc((size_t) (L + 1) * (L + 1), sizeof(double));
+	double *ht = (double *) malloc(2 * (size_t) n * sizeof(double));
+	double *tmp = (double *) malloc((L + 1) * sizeof(double));
+	int rc = -1;
+	if(!hx || !hs || !ha || !ht || !tmp)
+		goto done;
+	for(int j = 0; j < B; j++) {
+		hx[j] = x[j];
+		for(int m = 0; m <= L; m++) {
+			sphr_plm(m, m, x[j], tmp);
+			hs[(size_t) j * (L + 1) + m] = tmp[0];
+		}
+	}
+	for(int m = 0; m <= L; m++)
+		for(int l = m + 1; l <= L; l++)
+			ha[(size_t) m * (L + 1) + l] = sqrt((4.0 * l * l - 1.0) / ((double) (l - m) * (l + m)));
+	sphr_twiddles(n, +1, ht);
+	if(sphr_rt_check(hipMalloc(&t->x, B * sizeof(double)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&t->smm, (size_t) B * (L + 1) * sizeof(double)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&t->alpha, (size_t) (L + 1) * (L + 1) * sizeof(double)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&t->twp, 2 * (size_t) n * sizeof(double)), "hipMalloc"))
+		goto done;
+	if(sphr_rt_check(hipMemcpy(t->x, hx, B * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
+	   || sphr_rt_check(hipMemcpy(t->smm, hs, (size_t) B * (L + 1) * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
+	   || sphr_rt_check(hipMemcpy(t->alpha, ha, (size_t) (L + 1) * (L + 1) * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
+	   || sphr_rt_check(hipMemcpy(t->twp, ht, 2 * (size_t) n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
+		goto done;
+	t->tb.x = t->x;
+	t->tb.smm = t->smm;
+	t->tb.alpha = t->alpha;
+	t->tb.twp = t->twp;
+	rc = 0;
+done:
+	free(hx);
+	free(hs);
+	free(ha);
+	free(ht);
+	free(tmp);
+	if(rc)
+		synth_tables_free(t);
+	return rc;
+}
+
+void synth_tables_free(struct synth_dev *t)
+{
+	if(t->x)
+		hipFree(t->x);
+	if(t->smm)
+		hipFree(t->smm);
+	if(t->alpha)
+		hipFree(t->alpha);
+	if(t->twp)
+		hipFree(t->twp);
+	memset(t, 0, sizeof(*t));
+}
+
+/* drop-in caches (accessed under the drop-in lock) */
+struct synth_cache_entry {
+	struct synth_dev t;
+	struct synth_cache_entry *next;
+};
+static struct synth_cache_entry *synth_cache;
+
+static struct synth_dev *synth_for(int L)
+{
+	for(struct synth_cache_entry *e = synth_cache; e; e = e->next)
+		if(e->t.g.L == L)
+			return &e->t;
+	struct synth_cache_entry *e = (struct synth_cache_entry *) calloc(1, sizeof(*e));
+	if(!e || synth_tables_build(L, &e->t)) {
+		free(e);
+		return NULL;
+	}
+	e->next = synth_cache;
+	synth_cache = e;
+	return &e->t;
+}
+
+struct tw_cache_entry {
+	int n;
+	cplx<double> *tw;
+	struct tw_cache_entry *next;
+};
+static struct tw_cache_entry *tw_cache;
+
+static const cplx<double> *twiddles_for(int n)
+{
+	for(struct tw_cache_entry *e = tw_cache; e; e = e->next)
+		if(e->n == n)
+			return e->tw;
+	double *h = (double *) malloc(2 * (size_t) n * sizeof(double));
+	struct tw_cache_entry *e = (struct tw_cache_entry *) calloc(1, sizeof(*e));
+	if(!h || !e) {
+		free(h);
+		free(e);
+		return NULL;
+	}
+	sphr_twiddles(n, -1, h);
+	if(sphr_rt_check(hipMalloc(&e->tw, 2 * (size_t) n * sizeof(double)), "hipMalloc") || sphr_rt_check(hipMemcpy(e->tw, h, 2 * (size_t) n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")) {
+		free(h);
+		free(e);
+		return NULL;
+	}
+	free(h);
+	e->n = n;
+	e->next = tw_cache;
+	tw_cache = e;
+	return e->tw;
+}
+
+static size_t block_offset_host(int l)
+{
+	return ((size_t) 4 * l * l * l - l - 3) / 3;
+}
+
+/* D^l_{0 m'} rows of packed blocks, in the coefficient layout of order L */
+static void polar_rows(int L, const double *blocks, double *rows)
+{
+	rows[0] = 1.0;
+	rows[1] = 0.0;
+	for(int l = 1; l <= L; l++)
+		for(int mp = -l; mp <= l; mp++) {
+			const size_t src = block_offset_host(l) + (size_t) (2 * l + 1) * (mp + l) + l;
+			const int dst = lm_offset(L, l, mp);
+			rows[2 * dst] = blocks[2 * src];
+			rows[2 * dst + 1] = blocks[2 * src + 1];
+		}
+}
+
+#define CK(x, what) do { if(sphr_rt_check((x), what)) goto fail; } while(0)
+
+extern "C" int sphr_gpu_dotc(const double *coeff, int n, int stride, const double *vec, int vec_is_real, double *result)
+{
+	if(sphr_gpu_ensure())
+		return -1;
+	if(n <= 0) {
+		memset(result, 0, 2 * (size_t) stride * sizeof(double));
+		return 0;
+	}
+	LOCK();
+	hipStream_t st = sphr_rt_stream();
+	const int nsplit = n >= 64 * 256 ? 256 : (n + 63) / 64;
+	cplx<double> *dc = (cplx<double> *) sphr_rt_scratch(SL_A, (size_t) n * stride * sizeof(cplx<double>));
+	void *dv = sphr_rt_scratch(SL_B, (size_t) n * sizeof(cplx<double>));
+	cplx<double> *dp = (cplx<double> *) sphr_rt_scratch(SL_C, (size_t) nsplit * stride * sizeof(cplx<double>));
+	cplx<double> *dr = (cplx<double> *) sphr_rt_scratch(SL_D, (size_t) stride * sizeof(cplx<double>));
+	if(!dc || !dv || !dp || !dr)
+		goto fail;
+	CK(hipMemcpyAsync(dc, coeff, (size_t) n * stride * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy coefficients");
+	CK(hipMemcpyAsync(dv, vec, (size_t) n * (vec_is_real ? 1 : 2) * sizeof(double), hipMemcpyHostToDevice, st), "copy vector");
+	if(launch_dotc(dc, n, stride, vec_is_real ? NULL : (const cplx<double> *) dv, vec_is_real ? (const double *) dv : NULL, 1.0, dp, nsplit, dr, st))
+		goto launch;
+	CK(hipMemcpyAsync(result, dr, (size_t) stride * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy result");
+	CK(hipStreamSynchronize(st), "synchronize");
+	UNLOCK();
+	return 0;
+launch:
+	sphr_set_error("dotc kernel launch failed");
+fail:
+	UNLOCK();
+	return -1;
+}
+
+extern "C" int sphr_gpu_to_mesh(unsigned l_max, int polar, const double *coeff, double *mesh)
+{
+	if(sphr_gpu_ensure())
+		return -1;
+	LOCK();
+	hipStream_t st = sphr_rt_stream();
+	const int L = (int) l_max, n = 2 * (L + 1);
+	const size_t nc = polar ? (size_t) L + 1 : (size_t) (L + 1) * (L + 1);
+	struct synth_dev *t = synth_for(L);
+	cplx<double> *da = (cplx<double> *) sphr_rt_scratch(SL_A, nc * sizeof(cplx<double>));
+	cplx<double> *dm = (cplx<double> *) sphr_rt_scratch(SL_B, (size_t) n * n * sizeof(cplx<double>));
+	if(!t || !da || !dm)
+		goto fail;
+	CK(hipMemcpyAsync(da, coeff, nc * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy coefficients");
+	if(launch_synth<double, double, double>(da, 1, polar, &t->g, &t->tb, dm, st)) {
+		sphr_set_error("synthesis launch failed");
+		goto fail;
+	}
+	CK(hipMemcpyAsync(mesh, dm, (size_t) n * n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy mesh");
+	CK(hipStreamSynchronize(st), "synchronize");
+	UNLOCK();
+	return 0;
+fail:
+	UNLOCK();
+	return -1;
+}
+
+extern "C" int sphr_gpu_rotate(unsigned l_max, int polar, const double *coeff, const double *blocks, double *result)
+{
+	if(sphr_gpu_ensure())
+		return -1;
+	const int L = (int) l_max;
+	const size_t nc = (size_t) (L + 1) * (L + 1);
+	double *rows = NULL;
+	LOCK();
+	hipStream_t st = sphr_rt_stream();
+	cplx<double> *din = (cplx<double> *) sphr_rt_scratch(SL_A, (polar ? L + 1 : nc) * sizeof(cplx<double>));
+	cplx<double> *dout = (cplx<double> *) sphr_rt_scratch(SL_B, nc * sizeof(cplx<double>));
+	const size_t nbl = polar ? nc : block_offset_host(L + 1);
+	cplx<double> *dbl = (cplx<double> *) sphr_rt_scratch(SL_C, (nbl ? nbl : 1) * sizeof(cplx<double>));
+	if(!din || !dout || !dbl)
+		goto fail;
+	CK(hipMemcpyAsync(din, coeff, (polar ? L + 1 : nc) * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy series");
+	if(polar) {
+		rows = (double *) malloc(2 * nc * sizeof(double));
+		if(!rows)
+			goto fail;
+		polar_rows(L, blocks, rows);
+		CK(hipMemcpyAsync(dbl, rows, nc * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy rotation rows");
+	} else if(nbl && L > 0) {
+		CK(hipMemcpyAsync(dbl, blocks, nbl * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy rotation blocks");
+	}
+	{
+		const cplx<double> *pw[1] = { din };
+		const cplx<double> *bk[1] = { dbl };
+		const int Lb[1] = { L }, pol[1] = { polar };
+		if(launch_rotsum(1, pw, Lb, pol, bk, L, 1.0, dout, st)) {
+			sphr_set_error("rotation launch failed");
+			goto fail;
+		}
+	}
+	CK(hipMemcpyAsync(result, dout, nc * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy result");
+	CK(hipStreamSynchronize(st), "synchronize");
+	UNLOCK();
+	free(rows);
+	return 0;
+fail:
+	UNLOCK();
+	free(rows);
+	return -1;
+}
+
+extern "C" int sphr_gpu_rotate_z(unsigned l_max, const double *coeff, double phi, double *result)
+{
+	if(sphr_gpu_ensure())
+		return -1;
+	const size_t nc = (size_t) (l_max + 1) * (l_max + 1);
+	LOCK();
+	hipStream_t st = sphr_rt_stream();
+	cplx<double> *din = (cplx<double> *) sphr_rt_scratch(SL_A, nc * sizeof(cplx<double>));
+	cplx<double> *dout = (cplx<double> *) sphr_rt_scratch(SL_B, nc * sizeof(cplx<double>));
+	if(!din || !dout)
+		goto fail;
+	CK(hipMemcpyAsync(din, coeff, nc * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy series");
+	if(launch_rotate_z((int) l_max, din, phi, dout, st)) {
+		sphr_set_error("rotate_z launch failed");
+		goto fail;
+	}
+	CK(hipMemcpyAsync(result, dout, nc * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy result");
+	CK(hipStreamSynchronize(st), "synchronize");
+	UNLOCK();
+	return 0;
+fail:
+	UNLOCK();
+	return -1;
+}
+
+extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int n, int nb, const struct sphr_fd_baseline *bl, unsigned sky_l, double scale, double *sky)
+{
+	if(sphr_gpu_ensure())
+		return -1;
+	if(nb > SPHR_MAX_BASELINES || n_inst > 2 * SPHR_MAX_BASELINES || n <= 0) {
+		sphr_set_error("network of %d baselines / %d instruments / %d samples is outside the supported range", nb, n_inst, n);
+		return -1;
+	}
+	const size_t nsky = (size_t) (sky_l + 1) * (sky_l + 1);
+	size_t dp_total = 0, pw_total = 0, rot_total = 0;
+	size_t dp_off[SPHR_MAX_BASELINES], pw_off[SPHR_MAX_BASELINES], rot_off[SPHR_MAX_BASELINES];
+	for(int b = 0; b < nb; b++) {
+		dp_off[b] = dp_total;
+		dp_total += (size_t) n * bl[b].stride;
+		pw_off[b] = pw_total;
+		pw_total += bl[b].stride;
+		rot_off[b] = rot_total;
+		rot_total += bl[b].polar ? (size_t) (bl[b].l_max + 1) * (bl[b].l_max + 1) : block_offset_host(bl[b].l_max + 1) + 1;
+	}
+	const int nsplit = n >= 64 * 256 ? 256 : (n + 63) / 64;
+	int maxstride = 1;
+	for(int b = 0; b < nb; b++)
+		maxstride = bl[b].stride > maxstride ? bl[b].stride : maxstride;
+	double *rows = NULL;
+	LOCK();
+	hipStream_t st = sphr_rt_stream();
+	cplx<double> *dfs = (cplx<double> *) sphr_rt_scratch(SL_A, (size_t) n_inst * n * sizeof(cplx<double>));
+	cplx<double> *dfsp = (cplx<double> *) sphr_rt_scratch(SL_B, (size_t) nb * n * sizeof(cplx<double>));
+	cplx<double> *ddp = (cplx<double> *) sphr_rt_scratch(SL_C, dp_total * sizeof(cplx<double>));
+	cplx<double> *dpw = (cplx<double> *) sphr_rt_scratch(SL_D, pw_total * sizeof(cplx<double>));
+	cplx<double> *drot = (cplx<double> *) sphr_rt_scratch(SL_E, rot_total * sizeof(cplx<double>));
+	cplx<double> *dpart = (cplx<double> *) sphr_rt_scratch(SL_F, (size_t) nsplit * maxstride * sizeof(cplx<double>));
+	cplx<double> *dsky = (cplx<double> *) sphr_rt_scratch(SL_G, nsky * sizeof(cplx<double>));
+	if(!dfs || !dfsp || !ddp || !dpw || !drot || !dpart || !dsky)
+		goto fail;
+	{
+		const cplx<double> *fsd[2 * SPHR_MAX_BASELINES];
+		int ia[SPHR_MAX_BASELINES], ib[SPHR_MAX_BASELINES];
+		for(int i = 0; i < n_inst; i++) {
+			CK(hipMemcpyAsync(dfs + (size_t) i * n, fseries[i], (size_t) n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy spectra");
+			fsd[i] = dfs + (size_t) i * n;
+		}
+		for(int b = 0; b < nb; b++) {
+			ia[b] = bl[b].index_a;
+			ib[b] = bl[b].index_b;
+		}
+		if(launch_xspec(fsd, n, nb, ia, ib, dfsp, st)) {
+			sphr_set_error("cross-spectrum launch failed");
+			goto fail;
+		}
+	}
+	for(int b = 0; b < nb; b++) {
+		CK(hipMemcpyAsync(ddp + dp_off[b], bl[b].dp, (size_t) n * bl[b].stride * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy delay product");
+		/* power_1d = (1/n) sum_f dp[f] X[f] (correlator.c:690-703) */
+		if(launch_dotc(ddp + dp_off[b], n, bl[b].stride, dfsp + (size_t) b * n, NULL, 1.0 / n, dpart, nsplit, dpw + pw_off[b], st)) {
+			sphr_set_error("projection launch failed");
+			goto fail;
+		}
+		const int Lb = (int) bl[b].l_max;
+		if(bl[b].polar) {
+			const size_t nr = (size_t) (Lb + 1) * (Lb + 1);
+			double *r = (double *) realloc(rows, 2 * nr * sizeof(double));
+			if(!r)
+				goto fail;
+			rows = r;
+			if(Lb > 0)
+				polar_rows(Lb, bl[b].blocks, rows);
+			else {
+				rows[0] = 1.0;
+				rows[1] = 0.0;
+			}
+			CK(hipMemcpyAsync(drot + rot_off[b], rows, nr * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy rotation rows");
+			/* the rows must reach the device before the buffer is
+			 * reused for the next baseline */
+			CK(hipStreamSynchronize(st), "synchronize");
+		} else if(Lb > 0) {
+			CK(hipMemcpyAsync(drot + rot_off[b], bl[b].blocks, block_offset_host(Lb + 1) * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy rotation blocks");
+		}
+	}
+	{
+		const cplx<double> *pw[SPHR_MAX_BASELINES], *bk[SPHR_MAX_BASELINES];
+		int Lb[SPHR_MAX_BASELINES], pol[SPHR_MAX_BASELINES];
+		for(int b = 0; b < nb; b++) {
+			pw[b] = dpw + pw_off[b];
+			bk[b] = drot + rot_off[b];
+			Lb[b] = (int) bl[b].l_max;
+			pol[b] = bl[b].polar;
+		}
+		if(launch_rotsum(nb, pw, Lb, pol, bk, (int) sky_l, scale, dsky, st)) {
+			sphr_set_error("rotation launch failed");
+			goto fail;
+		}
+	}
+	CK(hipMemcpyAsync(sky, dsky, nsky * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy sky");
+	for(int b = 0; b < nb; b++) {
+		CK(hipMemcpyAsync(bl[b].power_1d, dpw + pw_off[b], (size_t) bl[b].stride * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy power");
+		if(bl[b].fsp)
+			CK(hipMemcpyAsync(bl[b].fsp, dfsp + (size_t) b * n, (size_t) n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy cross spectrum");
+	}
+	CK(hipStreamSynchronize(st), "synchronize");
+	UNLOCK();
+	free(rows);
+	return 0;
+fail:
+	UNLOCK();
+	free(rows);
+	return -1;
+}
+
+/* ---- FFT helpers ---- */
+
+struct sphr_fft {
+	int n, real_input;
+	double *in, *out;
+};
+
+extern "C" struct sphr_fft *sphr_fft_new(int n, int real_input, double *in, double *out)
+{
+	if(sphr_gpu_ensure())
+		return NULL;
+	struct sphr_fft *f = (struct sphr_fft *) malloc(sizeof(*f));
+	if(!f)
+		return NULL;
+	f->n = n;
+	f->real_input = real_input;
+	f->in = in;
+	f->out = out;
+	return f;
+}
+
+extern "C" void sphr_fft_free(struct sphr_fft *f)
+{
+	free(f);
+}
+
+extern "C" int sphr_fft_exec(struct sphr_fft *f)
+{
+	if(!f || sphr_gpu_ensure())
+		return -1;
+	const int n = f->n;
+	const size_t in_bytes = (size_t) n * (f->real_input ? 1 : 2) * sizeof(double);
+	LOCK();
+	hipStream_t st = sphr_rt_stream();
+	const cplx<double> *tw = twiddles_for(n);
+	double *din = (double *) sphr_rt_scratch(SL_A, in_bytes);
+	cplx<double> *dout = (cplx<double> *) sphr_rt_scratch(SL_B, (size_t) n * sizeof(cplx<double>));
+	if(!tw || !din || !dout)
+		goto fail;
+	CK(hipMemcpyAsync(din, f->in, in_bytes, hipMemcpyHostToDevice, st), "copy series");
+	{
+		const int pow2 = !(n & (n - 1));
+		const int r = (pow2 && (size_t) n * sizeof(cplx<double>) <= sphr_rt_lds_max()) ? launch_fft_single(din, f->real_input, n, tw, dout, st) : launch_dft_direct(din, f->real_input, n, tw, dout, st);
+		if(r) {
+			sphr_set_error("FFT launch failed");
+			goto fail;
+		}
+	}
+	CK(hipMemcpyAsync(f->out, dout, (size_t) n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy spectrum");
+	CK(hipStreamSynchronize(st), "synchronize");
+	UNLOCK();
+	return 0;
+fail:
+	UNLOCK();
+	return -1;
+}

@@ -1,0 +1,60 @@
+/*
+ * Launch wrappers of the HIP kernels (kernels.hip).  All launches are
+ * asynchronous on the given stream; pointers are device pointers.
+ */
+#ifndef SPHR_GPU_KERNELS_H
+#define SPHR_GPU_KERNELS_H
+
+#include "common.h"
+
+#define SPHR_MAX_BASELINES 16
+#define SPHR_MAX_FACTORS 16
+
+/* per-plan constants of the cross-power projection (real time series) */
+struct xproj_geom {
+	int D, nb, N, Nh, CT, K2;
+	int ia[SPHR_MAX_BASELINES], ib[SPHR_MAX_BASELINES];
+	int col_start[SPHR_MAX_BASELINES], Lb[SPHR_MAX_BASELINES];
+};
+
+/* phi-synthesis constants */
+struct synth_geom {
+	int L, B, n;		/* n = 2B points along phi */
+	int nfac, fac[SPHR_MAX_FACTORS];
+};
+
+/* device tables of the mesh synthesis at order L */
+struct synth_tables {
+	const double *x;	/* B: cos(theta_j), j < B (descending) */
+	const double *smm;	/* B x (L+1): Pbar_mm(x_j) */
+	const double *alpha;	/* (L+1) x (L+1): alpha[m][l] of the l recurrence */
+	const double *twp;	/* 2n: exp(+2 pi i t / n) */
+};
+
+/* batched real FFT of length N (power of two), two real series per
+ * workgroup, half spectra out (N/2+1 bins per series).  in_f32 selects a
+ * float input.  window may be NULL. */
+template <typename T>
+int launch_rfft_pairs(const void *in, int in_f32, const double *window, int nseries, int N, const cplx<T> *tw, cplx<T> *spec, hipStream_t st);
+
+/* projection GEMM: partial[split][S][CT] */
+template <typename T>
+int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *col_bl, int k_per_split, int nsplit, T *partial, hipStream_t st);
+
+/* sum partials, rotate, average baselines, rotate_z: alm[S][(L+1)^2] */
+template <typename T, typename TO>
+int launch_alm(const T *partial, int nsplit, int S, const struct xproj_geom *g, int L, const cplx<T> *R, const int2 *lm, const double *gmst, cplx<TO> *alm, hipStream_t st);
+
+/* synthesis of S series of order L (polar flag) onto 2B x 2B meshes */
+template <typename TI, typename T, typename TO>
+int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom *g, const struct synth_tables *tb, cplx<TO> *map, hipStream_t st);
+
+/* drop-in helpers (FP64) */
+int launch_xspec(const cplx<double> *const *fs_dev, int n, int nb, const int *ia, const int *ib, cplx<double> *fsp, hipStream_t st);
+int launch_dotc(const cplx<double> *coeff, int n, int stride, const cplx<double> *vec, const double *vec_real, double scale, cplx<double> *partial, int nsplit, cplx<double> *out, hipStream_t st);
+int launch_rotsum(int nb, const cplx<double> *const *power, const int *Lb, const int *polar, const cplx<double> *const *blocks, int L, double scale, cplx<double> *sky, hipStream_t st);
+int launch_rotate_z(int L, const cplx<double> *in, double phi, cplx<double> *out, hipStream_t st);
+int launch_fft_single(const double *in, int real_input, int N, const cplx<double> *tw, cplx<double> *out, hipStream_t st);
+int launch_dft_direct(const double *in, int real_input, int N, const cplx<double> *tw, cplx<double> *out, hipStream_t st);
+
+#endif

@@ -1,0 +1,95 @@
+/*
+ * Interface between the host C library (object management and plan-time
+ * numerics) and the HIP side (kernels and device state).  Plain C types
+ * only; complex arrays are interleaved (re, im) doubles.
+ */
+#ifndef SPHR_INTERNAL_H
+#define SPHR_INTERNAL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- host numerics (numerics.c) ---- */
+
+/* n-point Gauss-Legendre rule on [-1, 1], nodes ascending */
+void sphr_gl_rule(int n, double *x, double *w);
+/* the rule of the (2(l_max+1))^2 pixel mesh: cos(theta) descending (theta
+ * ascending), cached per l_max; returned arrays are owned by the cache */
+void sphr_mesh_rule(unsigned l_max, const double **cos_theta, const double **weights);
+/* orthonormal associated Legendre functions (Condon-Shortley phase),
+ * out[l - m] for l = m .. l_max */
+void sphr_plm(unsigned l_max, int m, double x, double *out);
+/* all 3j (j1 j2 j; m1 m2 -m1-m2) for j = jmin .. j1+j2 into out[j - jmin];
+ * returns jmin, or -1 when every symbol vanishes */
+int sphr_3j_range(int j1, int j2, int m1, int m2, double *out);
+/* in-place unnormalised complex DFT, sign = -1 forward, +1 backward */
+void sphr_host_dft(double *data, int n, int sign);
+
+/* packed Wigner-D blocks of a rotation plan (l = 1 first, each
+ * (2l+1)^2 in storage order); NULL when l_max = 0 */
+struct sh_series_rotation_plan;
+const double *sphr_rotation_blocks(const struct sh_series_rotation_plan *);
+
+/* ---- GPU side (runtime.hip, dropin.hip) ---- */
+
+/* 0 on success; on failure -1 with a message for sphr_last_error() */
+int sphr_gpu_ensure(void);
+void sphr_set_error(const char *fmt, ...);
+
+/* result[c] = sum_i coeff[i * stride + c] * vec[i] over n rows */
+int sphr_gpu_dotc(const double *coeff, int n, int stride, const double *vec, int vec_is_real, double *result);
+
+/* synthesis of one series onto the 2(l+1) x 2(l+1) mesh (row-major
+ * [theta][phi], complex) */
+int sphr_gpu_to_mesh(unsigned l_max, int polar, const double *coeff, double *mesh);
+
+/* rotation.  For a polar input only D^l_{0 m'} is used.  blocks: packed
+ * (2l+1)^2 blocks for l = 1..l_max in the reference storage order. */
+int sphr_gpu_rotate(unsigned l_max, int polar, const double *coeff, const double *blocks, double *result);
+int sphr_gpu_rotate_z(unsigned l_max, const double *coeff, double phi, double *result);
+
+/* network frequency-domain integrate.
+ * n_inst detector spectra fseries[i] (length n each, full complex);
+ * per baseline k: index_a/b, delay product dp[k] (n x stride_k complex,
+ * l_max_k, polar_k), rotation blocks blocks[k] (packed as above; polar
+ * baselines may pass only the m = 0 rows, see sphr_gpu_network_fd);
+ * outputs: power_1d[k] (stride_k complex), fsp[k] (n complex, the
+ * fseries_product scratch of the reference plan, may be NULL) and
+ * sky (non-polar, sky_l) = (1/nb) sum_k rotate(power_1d[k]). */
+struct sphr_fd_baseline {
+	int index_a, index_b;
+	unsigned l_max;
+	int polar;
+	int stride;
+	const double *dp;          /* n * stride complex */
+	const double *blocks;      /* packed D blocks l = 1..l_max */
+	double *power_1d;          /* out: stride complex */
+	double *fsp;               /* out: n complex, optional */
+};
+int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int n, int nb, const struct sphr_fd_baseline *bl, unsigned sky_l, double scale, double *sky);
+
+/* a network FD plan flattened for the batched device pipeline */
+#define SPHR_DESC_MAX 16
+struct sphr_net_desc {
+	int n_det, nb, N;
+	int ia[SPHR_DESC_MAX], ib[SPHR_DESC_MAX], Lb[SPHR_DESC_MAX], polar[SPHR_DESC_MAX];
+	const double *dp[SPHR_DESC_MAX];       /* N x stride complex */
+	const double *blocks[SPHR_DESC_MAX];   /* packed D blocks */
+};
+struct sphr_batch;
+struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, unsigned sky_l, int precision, int max_segments);
+
+/* FFT helpers of correlator.h */
+struct sphr_fft;
+struct sphr_fft *sphr_fft_new(int n, int real_input, double *in, double *out);
+int sphr_fft_exec(struct sphr_fft *);
+void sphr_fft_free(struct sphr_fft *);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -76,22 +76,28 @@ def isotropic_direction(rng):
 
 
 def two_site(n_segments=16, n=4096, seed=0):
-    """White N(0,1) float64 at 4096 Hz, cut into non-overlapping segments.
-    Returns tseries [n_det][n_segments][n]."""
-    rng = np.random.default_rng(seed)
-    stream = rng.standard_normal((2, n_segments * n))
-    return stream.reshape(2, n_segments, n)
+    """White N(0,1) float64 at 4096 Hz, one stream per detector (generator
+    seeded with (seed, detector)), cut into non-overlapping segments, so the
+    first k segments do not depend on n_segments.  Returns tseries
+    [n_det][n_segments][n]."""
+    out = np.empty((2, n_segments, n))
+    for d in range(2):
+        out[d] = np.random.default_rng([seed, d]).standard_normal(n_segments * n).reshape(n_segments, n)
+    return out
 
 
 def three_site(n_segments=1024, n=1024, seed=0, snr=20.0, delta_t=1.0 / 4096):
     """White N(0,1) float64 plus one sine-Gaussian plane wave per segment,
     delayed per site by r_i . s.  Returns (tseries [3][S][n], injections),
-    injections = [(theta, phi, f0)] per segment."""
-    rng = np.random.default_rng(seed)
+    injections = [(theta, phi, f0)] per segment.  Segment s draws from a
+    generator seeded with (seed, s), so the first k segments do not depend
+    on n_segments."""
     pos = positions("ABC")
-    data = rng.standard_normal((3, n_segments, n))
+    data = np.empty((3, n_segments, n))
     inj = []
     for s in range(n_segments):
+        rng = np.random.default_rng([seed, s])
+        data[:, s] = rng.standard_normal((3, n))
         f0 = rng.uniform(100.0, 400.0)
         theta, phi = isotropic_direction(rng)
         h = sine_gaussian(n, delta_t, f0, 9.0, 0.5 * n * delta_t)

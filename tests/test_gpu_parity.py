@@ -1,0 +1,215 @@
+"""GPU parity: the HIP path of the C library against the FP64 oracle.
+
+Every call goes through the C ABI of libsphradiometer_amd.so.  Bars:
+float64 paths match the oracle to 1e-12 relative L2 (the drop-in
+tolerance); the SPHR_MIXED batched path matches the float64 path to
+1e-5 relative L2 on a_lm.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import sphradiometer_amd as S
+from sphradiometer_amd import inject
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "oracle_cases.npz")
+DT = 1.0 / 4096
+TOL64 = 1e-12
+TOL_MIXED = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    S.lib()
+    if not S.gpu_available():
+        pytest.fail("no GPU: " + S.last_error())
+    print("device:", S.last_error() or "ok", flush=True)
+
+
+@pytest.fixture(scope="module")
+def two_site():
+    net = S.Network(inject.positions("AB"), 4096, DT, 25)
+    return net
+
+
+@pytest.fixture(scope="module")
+def three_site():
+    return S.Network(inject.positions("ABC"), 1024, DT, 32)
+
+
+def test_fft_helpers_match_oracle(oracle):
+    rng = np.random.default_rng(1)
+    for n in (64, 1024, 4096, 584, 1000):
+        x = rng.standard_normal(n)
+        assert rel(S.tseries_to_fseries(x), oracle.tseries_to_fseries(x)) < TOL64
+        z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        assert rel(S.ctseries_to_fseries(z), oracle.ctseries_to_fseries(z)) < TOL64
+
+
+@pytest.mark.parametrize("L,polar", [(0, 0), (1, 0), (7, 1), (25, 0), (32, 0), (50, 0), (64, 0), (130, 1)])
+def test_to_mesh_matches_oracle(oracle, L, polar):
+    rng = np.random.default_rng(L + 100 * polar)
+    n = L + 1 if polar else (L + 1) ** 2
+    c = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    got = S.to_mesh(c, L, polar)
+    ref = oracle.to_mesh(c, L, polar)
+    assert got.shape == ref.shape
+    assert rel(got, ref) < TOL64
+
+
+def test_dotc_matches_numpy():
+    L = S.lib()
+    rng = np.random.default_rng(2)
+    for n, lmax, polar in [(1, 3, 1), (1000, 20, 1), (584, 12, 0)]:
+        arr = L.sh_series_array_new(n, lmax, polar)
+        A = S.array_view(arr)
+        A[:] = rng.standard_normal(A.shape) + 1j * rng.standard_normal(A.shape)
+        v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        res = L.sh_series_new(lmax, polar)
+        assert L.sh_series_array_dotc(res, arr, S._ptr(v))
+        ref = (A * v[:, None]).sum(axis=0)
+        assert rel(S.coeff_view(res), ref) < TOL64
+        L.sh_series_free(res)
+        L.sh_series_array_free(arr)
+
+
+def test_rotate_and_rotate_z_match_oracle(oracle):
+    L = S.lib()
+    rng = np.random.default_rng(3)
+    for lmax, polar in [(1, 0), (13, 0), (32, 1), (50, 0)]:
+        n = lmax + 1 if polar else (lmax + 1) ** 2
+        c = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        th, ph = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi)
+        s = S.series_from_numpy(c, lmax, polar)
+        R = L.sh_series_rot_matrix(th, ph)
+        plan = L.sh_series_rotation_plan_new(s, R)
+        out = L.sh_series_new(lmax, 0)
+        assert L.sh_series_rotate(out, s, plan)
+        ref = oracle.rotate(c, lmax, polar, np.ctypeslib.as_array(R, shape=(9,)))
+        assert rel(S.coeff_view(out), ref) < TOL64
+        if not polar:
+            assert L.sh_series_rotate_z(out, s, 0.37)
+            assert rel(S.coeff_view(out), oracle.rotate_z(c, lmax, 0.37)) < TOL64
+        for p in (out, s):
+            L.sh_series_free(p)
+        L.sh_series_rotation_plan_free(plan)
+
+
+def test_rotated_impulse_on_gpu():
+    # rotation_validate.c:396-437 through the GPU rotation
+    L = S.lib()
+    rng = np.random.default_rng(4)
+    for lmax in (1, 13):
+        for _ in range(5):
+            th, ph = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi)
+            dth, dph = rng.uniform(0, math.pi) - th, rng.uniform(0, 2 * math.pi) - ph
+            s = L.sh_series_impulse(lmax, th, ph)
+            c = L.sh_series_impulse(lmax, th + dth, ph + dph)
+            R = L.euler_rotation_matrix(-ph, dth, ph + dph)
+            plan = L.sh_series_rotation_plan_new(s, R)
+            out = L.sh_series_new(lmax, 0)
+            assert L.sh_series_rotate(out, s, plan)
+            assert L.diagnostics_rms_error(out, c) < 1e-10
+            for p in (out, s, c):
+                L.sh_series_free(p)
+            L.sh_series_rotation_plan_free(plan)
+
+
+def _segment_oracle(oracle, pos, n, L, ts, window=None, gmst=0.0):
+    net = oracle.Network(pos, n, DT, L)
+    return net.segment(ts, L, window=window, gmst=gmst)
+
+
+def test_network_integrate_two_site(oracle, two_site):
+    ts = inject.two_site(16, 4096, seed=0)[:, 0, :]
+    w = inject.hann(4096)
+    fs = np.array([oracle.tseries_to_fseries(ts[d] * w) for d in range(2)])
+    sky = two_site.integrate(fs, 25)
+    onet = oracle.Network(inject.positions("AB"), 4096, DT, 25)
+    ref = onet.integrate(fs, 25)
+    assert rel(sky, ref) < TOL64
+    # the plan's power_1d is left holding the baseline power (correlator.c:703)
+    assert rel(two_site.power_1d(0), onet.power_1d(0)) < TOL64
+    gold = np.load(GOLD)
+    assert rel(sky, gold["two_site_alm_0"]) < TOL64
+
+
+def test_network_integrate_three_site_and_golden(oracle, three_site):
+    ts, _ = inject.three_site(2, 1024, seed=0)
+    gold = np.load(GOLD)
+    for s in (0, 1):
+        fs = np.array([S.tseries_to_fseries(ts[d, s]) for d in range(3)])
+        sky = three_site.integrate(fs, 32)
+        assert rel(sky, gold["three_site_alm_%d" % s]) < TOL64
+    mesh = S.to_mesh(three_site.integrate(np.array([S.tseries_to_fseries(ts[d, 0]) for d in range(3)]), 32), 32)
+    assert rel(mesh, gold["three_site_map_0"]) < TOL64
+
+
+def _run_batch(net, L, ts, precision, window=None, gmst=None):
+    D, Sg, N = ts.shape
+    b = S.Batch(net, L, precision, max_segments=Sg)
+    cdt = np.complex128 if precision == S.FP64 else np.complex64
+    d_ts = S.DeviceBuffer.from_numpy(ts.astype(np.float64))
+    d_alm = S.DeviceBuffer(Sg * (L + 1) ** 2 * np.dtype(cdt).itemsize)
+    d_map = S.DeviceBuffer(Sg * (2 * L + 2) ** 2 * np.dtype(cdt).itemsize)
+    d_w = S.DeviceBuffer.from_numpy(window) if window is not None else None
+    d_g = S.DeviceBuffer.from_numpy(np.asarray(gmst, dtype=np.float64)) if gmst is not None else None
+    b.run(d_ts.ptr, Sg, d_alm.ptr, d_map.ptr, d_window=d_w.ptr if d_w else None, d_gmst=d_g.ptr if d_g else None)
+    S.synchronize()
+    alm = d_alm.to_numpy(cdt, (Sg, (L + 1) ** 2))
+    mp = d_map.to_numpy(cdt, (Sg, 2 * L + 2, 2 * L + 2))
+    return alm, mp, b.info()
+
+
+def test_batch_fp64_two_site_matches_oracle_and_golden(oracle, two_site):
+    ts = inject.two_site(16, 4096, seed=0)
+    w = inject.hann(4096)
+    alm, mp, info = _run_batch(two_site, 25, ts, S.FP64, window=w)
+    print("batch info", info, flush=True)
+    gold = np.load(GOLD)
+    assert rel(alm[0], gold["two_site_alm_0"]) < TOL64
+    assert rel(alm[1], gold["two_site_alm_1"]) < TOL64
+    assert rel(mp[0], gold["two_site_map_0"]) < TOL64
+    onet = oracle.Network(inject.positions("AB"), 4096, DT, 25)
+    for s in (5, 15):
+        sky, mesh = onet.segment(ts[:, s], 25, window=w)
+        assert rel(alm[s], sky) < TOL64
+        assert rel(mp[s], mesh) < TOL64
+
+
+@pytest.mark.parametrize("nseg", [1, 3, 37])
+def test_batch_fp64_three_site_ragged_and_gmst(oracle, three_site, nseg):
+    ts, _ = inject.three_site(nseg, 1024, seed=nseg)
+    gm = np.linspace(0.0, 2.0, nseg)
+    alm, mp, _ = _run_batch(three_site, 32, ts, S.FP64, gmst=gm)
+    onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
+    for s in sorted({0, nseg // 2, nseg - 1}):
+        sky, mesh = onet.segment(ts[:, s], 32, gmst=float(gm[s]))
+        assert rel(alm[s], sky) < TOL64
+        assert rel(mp[s], mesh) < TOL64
+
+
+def test_batch_full_size_three_site_properties(three_site):
+    # the 1024-segment workload: golden segments, linearity (x -> 2x gives
+    # 4x power), and a real sky for real data
+    ts, _ = inject.three_site(1024, 1024, seed=0)
+    alm, mp, _ = _run_batch(three_site, 32, ts, S.FP64)
+    gold = np.load(GOLD)
+    assert rel(alm[0], gold["three_site_alm_0"]) < TOL64
+    assert rel(alm[1], gold["three_site_alm_1"]) < TOL64
+    alm2, _, _ = _run_batch(three_site, 32, 2.0 * ts, S.FP64)
+    assert rel(alm2, 4.0 * alm) < 1e-14
+    assert np.max(np.abs(mp.imag)) < 1e-12 * np.max(np.abs(mp.real))
+    # mixed precision: a_lm within 1e-5 of float64 per segment
+    almm, mpm, _ = _run_batch(three_site, 32, ts, S.MIXED)
+    err = np.linalg.norm(almm - alm, axis=1) / np.linalg.norm(alm, axis=1)
+    print("mixed a_lm rel err: max %.3g median %.3g; map %.3g" % (err.max(), np.median(err), rel(mpm, mp)), flush=True)
+    assert err.max() < TOL_MIXED
