@@ -171,12 +171,12 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 		bt->g.ib[b] = d->ib[b];
 		bt->g.Lb[b] = d->Lb[b];
 		bt->g.col_start[b] = CT;
-		CT += 2 * (d->Lb[b] + 1);
+		CT += (2 * (d->Lb[b] + 1) + 15) / 16 * 16;	/* whole MFMA tiles */
 	}
 	bt->g.CT = CT;
 	/* split K so the projection launch has enough workgroups */
 	{
-		const int base = ((max_segments + 15) / 16) * ((CT + 255) / 256);
+		const int base = ((max_segments + 15) / 16) * ((CT / 16 + 15) / 16);
 		const int target = 2 * sphr_rt_cus();
 		int ns = (target + base - 1) / base;
 		const int maxs = (bt->g.K2 + 63) / 64;
@@ -186,21 +186,21 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 		bt->k_per_split = kps;
 		bt->nsplit = (bt->g.K2 + kps - 1) / kps;
 	}
-	int *colbl = (int *) malloc(CT * sizeof(int));
+	int *colbl = (int *) malloc((CT / 16) * sizeof(int));
 	const int ncoef = (bt->L + 1) * (bt->L + 1);
 	int2 *lm = (int2 *) malloc(ncoef * sizeof(int2));
 	if(!colbl || !lm)
 		goto fail;
 	for(int b = 0; b < d->nb; b++)
-		for(int c = 0; c < 2 * (d->Lb[b] + 1); c++)
-			colbl[bt->g.col_start[b] + c] = b;
+		for(int c = 0; c < 2 * (d->Lb[b] + 1); c += 16)
+			colbl[(bt->g.col_start[b] + c) / 16] = b;
 	for(int i = 0; i < ncoef; i++) {
 		int l, m;
 		lm_decode(bt->L, i, &l, &m);
 		lm[i].x = l;
 		lm[i].y = m;
 	}
-	if(upload((void **) &bt->col_bl, colbl, CT) || upload((void **) &bt->lm, lm, ncoef))
+	if(upload((void **) &bt->col_bl, colbl, CT / 16) || upload((void **) &bt->lm, lm, ncoef))
 		goto fail;
 	free(colbl);
 	free(lm);
@@ -234,11 +234,9 @@ static int run_t(struct sphr_batch *bt, const void *ts, int f32, int S, const do
 		return -1;
 	if(ev)
 		hipEventRecord(ev[2], st);
-	if(launch_alm<T, T>((const T *) bt->partial, bt->nsplit, S, &bt->g, bt->L, (const cplx<T> *) bt->R, bt->lm, gmst, (cplx<T> *) alm, st))
-		return -1;
 	if(ev)
 		hipEventRecord(ev[3], st);
-	if(map && launch_synth<T, T, T>((const cplx<T> *) alm, S, 0, &bt->syn.g, &bt->syn.tb, (cplx<T> *) map, st))
+	if(launch_sky<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st))
 		return -1;
 	if(ev)
 		hipEventRecord(ev[4], st);

@@ -63,7 +63,8 @@ int synth_tables_build(int L, struct synth_dev *t)
 	sphr_mesh_rule(L, &x, NULL);
 	double *hx = (double *) malloc(B * sizeof(double));
 	double *hs = (double *) malloc((size_t) B * (L + 1) * sizeof(double));
-	double *ha = (double *) calloc((size_t) (L + 1) * (L + 1), sizeof(double));
+	const size_t nal = (size_t) (L + 1) * (L + 2) / 2;
+	double *ha = (double *) calloc(nal, sizeof(double));
 	double *ht = (double *) malloc(2 * (size_t) n * sizeof(double));
 	double *tmp = (double *) malloc((L + 1) * sizeof(double));
 	int rc = -1;
@@ -76,18 +77,20 @@ int synth_tables_build(int L, struct synth_dev *t)
 			hs[(size_t) j * (L + 1) + m] = tmp[0];
 		}
 	}
+	/* alpha_lm = sqrt((4l^2 - 1) / (l^2 - m^2)) at the (l, m >= 0) slot of
+	 * the coefficient layout */
 	for(int m = 0; m <= L; m++)
 		for(int l = m + 1; l <= L; l++)
-			ha[(size_t) m * (L + 1) + l] = sqrt((4.0 * l * l - 1.0) / ((double) (l - m) * (l + m)));
+			ha[lm_offset(L, l, m)] = sqrt((4.0 * l * l - 1.0) / ((double) (l - m) * (l + m)));
 	sphr_twiddles(n, +1, ht);
 	if(sphr_rt_check(hipMalloc(&t->x, B * sizeof(double)), "hipMalloc")
 	   || sphr_rt_check(hipMalloc(&t->smm, (size_t) B * (L + 1) * sizeof(double)), "hipMalloc")
-	   || sphr_rt_check(hipMalloc(&t->alpha, (size_t) (L + 1) * (L + 1) * sizeof(double)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&t->alpha, nal * sizeof(double)), "hipMalloc")
 	   || sphr_rt_check(hipMalloc(&t->twp, 2 * (size_t) n * sizeof(double)), "hipMalloc"))
 		goto done;
 	if(sphr_rt_check(hipMemcpy(t->x, hx, B * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
 	   || sphr_rt_check(hipMemcpy(t->smm, hs, (size_t) B * (L + 1) * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
-	   || sphr_rt_check(hipMemcpy(t->alpha, ha, (size_t) (L + 1) * (L + 1) * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
+	   || sphr_rt_check(hipMemcpy(t->alpha, ha, nal * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
 	   || sphr_rt_check(hipMemcpy(t->twp, ht, 2 * (size_t) n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
 		goto done;
 	t->tb.x = t->x;

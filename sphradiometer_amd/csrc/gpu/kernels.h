@@ -10,7 +10,9 @@
 #define SPHR_MAX_BASELINES 16
 #define SPHR_MAX_FACTORS 16
 
-/* per-plan constants of the cross-power projection (real time series) */
+/* per-plan constants of the cross-power projection (real time series);
+ * CT counts the projection columns, each baseline padded to whole
+ * 16-column tiles starting at col_start[b] */
 struct xproj_geom {
 	int D, nb, N, Nh, CT, K2;
 	int ia[SPHR_MAX_BASELINES], ib[SPHR_MAX_BASELINES];
@@ -27,7 +29,8 @@ struct synth_geom {
 struct synth_tables {
 	const double *x;	/* B: cos(theta_j), j < B (descending) */
 	const double *smm;	/* B x (L+1): Pbar_mm(x_j) */
-	const double *alpha;	/* (L+1) x (L+1): alpha[m][l] of the l recurrence */
+	const double *alpha;	/* alpha_lm of the l recurrence at the (l, m >= 0)
+				 * slot of the coefficient layout */
 	const double *twp;	/* 2n: exp(+2 pi i t / n) */
 };
 
@@ -37,9 +40,10 @@ struct synth_tables {
 template <typename T>
 int launch_rfft_pairs(const void *in, int in_f32, const double *window, int nseries, int N, const cplx<T> *tw, cplx<T> *spec, hipStream_t st);
 
-/* projection GEMM: partial[split][S][CT] */
+/* projection GEMM: partial[split][S][CT]; tile_bl[t] is the baseline of
+ * column tile t; k_per_split a multiple of the K chunk (32) */
 template <typename T>
-int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *col_bl, int k_per_split, int nsplit, T *partial, hipStream_t st);
+int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st);
 
 /* sum partials, rotate, average baselines, rotate_z: alm[S][(L+1)^2] */
 template <typename T, typename TO>
@@ -48,6 +52,12 @@ int launch_alm(const T *partial, int nsplit, int S, const struct xproj_geom *g, 
 /* synthesis of S series of order L (polar flag) onto 2B x 2B meshes */
 template <typename TI, typename T, typename TO>
 int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom *g, const struct synth_tables *tb, cplx<TO> *map, hipStream_t st);
+
+/* fused: partials -> a_lm -> mesh (map may be NULL), one workgroup per
+ * segment; falls back to launch_alm + launch_synth when the coefficients
+ * exceed the LDS budget */
+template <typename T>
+int launch_sky(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st);
 
 /* drop-in helpers (FP64) */
 int launch_xspec(const cplx<double> *const *fs_dev, int n, int nb, const int *ia, const int *ib, cplx<double> *fsp, hipStream_t st);

@@ -5,14 +5,15 @@
  *                series, in-place radix-4 decimation in time in LDS from a
  *                bit-reversed load, then split into two half spectra
  *   xproj        cross spectrum X = F_b conj(F_a) formed in LDS and
- *                contracted with the packed delay-product matrix (a real
- *                GEMM over K = N + 2), split over K into partial sums
+ *                contracted with the packed delay-product matrix: a real
+ *                GEMM over K = N + 2 on MFMA 16x16x4 (f32 / f64), split
+ *                over K into partial sums
  *   alm          partial sums -> per-baseline polar power -> rotation into
  *                the Earth frame (the D^l_{0 m'} row), 1/nb average and
  *                rotate_z, written in the reference coefficient layout
  *   synth        Legendre stage per (ring pair, m) with the l recurrence
  *                run in registers (equatorial symmetry gives both rings),
- *                then a Stockham mixed-radix DFT along phi in LDS
+ *                then the phi DFT, two columns (k, k + n/2) per thread
  * plus the FP64 helpers of the synchronous drop-in entry points.
  */
 #include <hip/hip_runtime.h>
@@ -162,78 +163,123 @@ int launch_dft_direct(const double *in, int real_input, int N, const cplx<double
 /* cross spectrum x delay-product projection                           */
 /* ------------------------------------------------------------------ */
 
-#define XP_TS 16	/* segments per workgroup */
-#define XP_KC 16	/* frequencies per LDS chunk */
-#define XP_UC 4		/* 64-column groups per workgroup (256 columns) */
+/*
+ * Real GEMM partial[z][s][c] = sum_{k in split z} A_b(c)[s][k] W[k][c] on
+ * MFMA 16x16x4 (f32 or f64), A_b[s][2f] = Re X_b[s][f],
+ * A_b[s][2f+1] = Im X_b[s][f], X_b = F_ib conj(F_ia).  Each baseline's
+ * columns are padded to whole 16-column tiles so a tile reads one
+ * baseline's A.  Workgroup: 16 segments x up to 16 tiles (4 per wave),
+ * one K split; per K chunk the A strip (all baselines) and the W slab are
+ * staged in LDS.
+ *
+ * MFMA 16x16x4 operand layout (64 lanes, measured on gfx950): A[i][k] in
+ * lane i + 16 k, B[k][j] in lane j + 16 k; the 4 accumulator elements e of
+ * lane j + 16 q hold D[i][j] with i = 4 q + e for f32 and i = q + 4 e for
+ * f64.
+ */
+template <typename T> struct mfma16;
+template <> struct mfma16<float> {
+	typedef float v4 __attribute__((ext_vector_type(4)));
+	static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+	static __device__ inline int row(int q, int e) { return 4 * q + e; }
+	enum { KC = 32 };
+};
+template <> struct mfma16<double> {
+	typedef double v4 __attribute__((ext_vector_type(4)));
+	static __device__ inline v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+	static __device__ inline int row(int q, int e) { return q + 4 * e; }
+	enum { KC = 16 };
+};
+
+#define XP_MS 16	/* segments per workgroup */
+#define XP_TPW 4	/* column tiles per wave */
+#define XP_NW 4		/* waves per workgroup */
 
 template <typename T>
-__global__ void __launch_bounds__(NT) k_xproj(const cplx<T> *__restrict__ spec, int S, struct xproj_geom g, const T *__restrict__ W, const int *__restrict__ col_bl, int k_per_split, T *__restrict__ partial)
+__global__ void __launch_bounds__(NT) k_xproj(const cplx<T> *__restrict__ spec, int S, struct xproj_geom g, const T *__restrict__ W, const int *__restrict__ tile_bl, int k_per_split, T *__restrict__ partial)
 {
+	typedef typename mfma16<T>::v4 v4;
+	const int KC = mfma16<T>::KC;
+	const int CTW = XP_NW * XP_TPW * 16;	/* columns per workgroup */
 	extern __shared__ unsigned char smem_raw[];
-	cplx<T> *sF = reinterpret_cast<cplx<T> *>(smem_raw);		/* [D][TS][KC] */
-	T *sA = reinterpret_cast<T *>(sF + g.D * XP_TS * XP_KC);	/* [nb][TS][2 KC] */
-	const int s0 = blockIdx.x * XP_TS;
-	const int c0 = blockIdx.y * 64 * XP_UC;
+	const int KA = KC + 1;					/* padded row: no bank conflicts */
+	T *sA = reinterpret_cast<T *>(smem_raw);		/* [nb][MS][KA] */
+	T *sW = sA + g.nb * XP_MS * KA;				/* [KC][CTW] */
+	const int s0 = blockIdx.x * XP_MS;
+	const int t0 = blockIdx.y * XP_NW * XP_TPW;		/* first tile */
+	const int ntile = g.CT / 16;
 	const int kbeg = blockIdx.z * k_per_split;
 	const int kend = min(g.K2, kbeg + k_per_split);
-	const int fbeg = kbeg / 2, fend = kend / 2;
-	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-	int cidx[XP_UC], cb[XP_UC];
-	bool cv[XP_UC];
-	for(int u = 0; u < XP_UC; u++) {
-		cidx[u] = c0 + tx + 64 * u;
-		cv[u] = cidx[u] < g.CT;
-		cb[u] = cv[u] ? col_bl[cidx[u]] : 0;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int li = lane & 15, lk = lane >> 4;
+	int tb[XP_TPW];
+	for(int u = 0; u < XP_TPW; u++) {
+		const int t = t0 + wave * XP_TPW + u;
+		tb[u] = t < ntile ? tile_bl[t] : 0;
 	}
-	T acc[XP_UC][4];
-	for(int u = 0; u < XP_UC; u++)
-		for(int v = 0; v < 4; v++)
-			acc[u][v] = (T) 0;
-	for(int f0 = fbeg; f0 < fend; f0 += XP_KC) {
-		const int nf = min(XP_KC, fend - f0);
-		for(int i = threadIdx.x; i < g.D * XP_TS * XP_KC; i += NT) {
-			const int d = i / (XP_TS * XP_KC), r = i % (XP_TS * XP_KC);
-			const int s = r / XP_KC, ff = r % XP_KC;
-			sF[i] = (s0 + s < S && ff < nf) ? spec[((size_t) d * S + s0 + s) * g.Nh + f0 + ff] : mk((T) 0, (T) 0);
+	v4 acc[XP_TPW];
+	for(int u = 0; u < XP_TPW; u++)
+		acc[u] = (v4) 0;
+	const int c0 = t0 * 16;
+	for(int k0 = kbeg; k0 < kend; k0 += KC) {
+		const int f0 = k0 / 2;
+		/* A strip: X_b for MS segments x KC/2 frequencies */
+		for(int i = threadIdx.x; i < g.nb * XP_MS * (KC / 2); i += NT) {
+			const int b = i / (XP_MS * (KC / 2)), r = i - b * XP_MS * (KC / 2);
+			const int sl = r / (KC / 2), ff = r - sl * (KC / 2);
+			const int s = s0 + sl, f = f0 + ff;
+			T xr = 0, xi = 0;
+			if(s < S && 2 * f < kend) {
+				const cplx<T> Fa = spec[((size_t) g.ia[b] * S + s) * g.Nh + f];
+				const cplx<T> Fb = spec[((size_t) g.ib[b] * S + s) * g.Nh + f];
+				const cplx<T> X = Fb * conjg(Fa);
+				xr = X.re;
+				xi = X.im;
+			}
+			T *a = sA + (b * XP_MS + sl) * KA + 2 * ff;
+			a[0] = xr;
+			a[1] = xi;
+		}
+		/* W slab: KC rows x CTW columns */
+		for(int i = threadIdx.x; i < KC * CTW; i += NT) {
+			const int kk = i / CTW, c = i - kk * CTW;
+			const int k = k0 + kk, cg = c0 + c;
+			sW[i] = (k < kend && cg < g.CT) ? W[(size_t) k * g.CT + cg] : (T) 0;
 		}
 		__syncthreads();
-		for(int i = threadIdx.x; i < g.nb * XP_TS * XP_KC; i += NT) {
-			const int b = i / (XP_TS * XP_KC), r = i % (XP_TS * XP_KC);
-			const int s = r / XP_KC, ff = r % XP_KC;
-			const cplx<T> Fa = sF[(g.ia[b] * XP_TS + s) * XP_KC + ff];
-			const cplx<T> Fb = sF[(g.ib[b] * XP_TS + s) * XP_KC + ff];
-			const cplx<T> X = Fb * conjg(Fa);
-			sA[(b * XP_TS + s) * 2 * XP_KC + 2 * ff] = X.re;
-			sA[(b * XP_TS + s) * 2 * XP_KC + 2 * ff + 1] = X.im;
-		}
-		__syncthreads();
-		for(int kk = 0; kk < 2 * nf; kk++) {
-			const size_t row = (size_t) (2 * f0 + kk) * g.CT;
-			T w[XP_UC];
-			for(int u = 0; u < XP_UC; u++)
-				w[u] = cv[u] ? W[row + cidx[u]] : (T) 0;
-			for(int u = 0; u < XP_UC; u++) {
-				const T *a = sA + (cb[u] * XP_TS + ty * 4) * 2 * XP_KC + kk;
-				for(int v = 0; v < 4; v++)
-					acc[u][v] = fma(a[v * 2 * XP_KC], w[u], acc[u][v]);
+		for(int ks = 0; ks < KC; ks += 4) {
+			for(int u = 0; u < XP_TPW; u++) {
+				const int tl = wave * XP_TPW + u;
+				if(t0 + tl >= ntile)
+					break;
+				const T a = sA[(tb[u] * XP_MS + li) * KA + ks + lk];
+				const T b = sW[(ks + lk) * CTW + tl * 16 + li];
+				acc[u] = mfma16<T>::mma(a, b, acc[u]);
 			}
 		}
 		__syncthreads();
 	}
-	for(int u = 0; u < XP_UC; u++)
-		for(int v = 0; v < 4; v++) {
-			const int s = s0 + ty * 4 + v;
-			if(s < S && cv[u])
-				partial[((size_t) blockIdx.z * S + s) * g.CT + cidx[u]] = acc[u][v];
+	for(int u = 0; u < XP_TPW; u++) {
+		const int tl = wave * XP_TPW + u;
+		if(t0 + tl >= ntile)
+			break;
+		const int c = c0 + tl * 16 + li;
+		for(int e = 0; e < 4; e++) {
+			const int s = s0 + mfma16<T>::row(lk, e);
+			if(s < S)
+				partial[((size_t) blockIdx.z * S + s) * g.CT + c] = acc[u][e];
 		}
+	}
 }
 
 template <typename T>
-int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *col_bl, int k_per_split, int nsplit, T *partial, hipStream_t st)
+int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st)
 {
-	const size_t lds = (size_t) g->D * XP_TS * XP_KC * sizeof(cplx<T>) + (size_t) g->nb * XP_TS * 2 * XP_KC * sizeof(T);
-	dim3 grid((S + XP_TS - 1) / XP_TS, (g->CT + 64 * XP_UC - 1) / (64 * XP_UC), nsplit);
-	hipLaunchKernelGGL((k_xproj<T>), grid, dim3(NT), lds, st, spec, S, *g, W, col_bl, k_per_split, partial);
+	const int KC = mfma16<T>::KC;
+	const size_t lds = ((size_t) g->nb * XP_MS * (KC + 1) + (size_t) KC * XP_NW * XP_TPW * 16) * sizeof(T);
+	const int ntile = g->CT / 16;
+	dim3 grid((S + XP_MS - 1) / XP_MS, (ntile + XP_NW * XP_TPW - 1) / (XP_NW * XP_TPW), nsplit);
+	hipLaunchKernelGGL((k_xproj<T>), grid, dim3(NT), lds, st, spec, S, *g, W, tile_bl, k_per_split, partial);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -291,119 +337,261 @@ template int launch_alm<float, float>(const float *, int, int, const struct xpro
 /* synthesis onto the Gauss-Legendre x equispaced mesh                 */
 /* ------------------------------------------------------------------ */
 
-#define SY_RP 4		/* ring pairs per workgroup */
+/*
+ * Synthesis stages.  Legendre stage: one item per (ring pair, m) runs the
+ * l recurrence in registers and accumulates the even-(l+|m|) and
+ * odd-(l+|m|) sums E, O, giving F(x_j) = E + O and F(-x_j) = E - O.  phi
+ * stage: each thread produces columns k and k + n/2 of a ring,
+ *   map[k] = sum_m F_m w_k^m,  map[k + n/2] = sum_m (-1)^m F_m w_k^m,
+ * w_k = exp(2 pi i k / n), with the powers of w_k formed by recurrence in
+ * registers (m = 0 .. L up from 1, m = -L .. -1 up from the tabulated
+ * w_k^{-L}), so the twiddle table is read once per thread.
+ */
+template <typename T>
+struct synth_lds {
+	const cplx<T> *tw;	/* [n] twiddles exp(+2 pi i t / n) */
+	const cplx<T> *sa;	/* series coefficients in LDS, or NULL */
+	const T *al;		/* recurrence coefficients in LDS, or NULL */
+};
 
-template <typename TI, typename T, typename TO>
-__global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, int polar, struct synth_geom g, struct synth_tables tb, cplx<TO> *__restrict__ map)
+template <typename TI, typename T>
+__device__ void legendre_stage(const cplx<TI> *__restrict__ ag, int polar, int rp, int j0, const struct synth_geom &g, const struct synth_tables &tb, const synth_lds<T> &ls, cplx<T> *F)
 {
-	extern __shared__ unsigned char smem_raw[];
 	const int n = g.n, L = g.L, B = g.B;
-	cplx<T> *buf0 = reinterpret_cast<cplx<T> *>(smem_raw);		/* [2 RP][n] */
-	cplx<T> *buf1 = buf0 + 2 * SY_RP * n;
-	cplx<T> *twp = buf1 + 2 * SY_RP * n;				/* [n] */
-	const int s = blockIdx.y;
-	const int j0 = blockIdx.x * SY_RP;
-	const int ncoef = polar ? L + 1 : (L + 1) * (L + 1);
-	const cplx<TI> *a = alm + (size_t) s * ncoef;
-	for(int t = threadIdx.x; t < n; t += blockDim.x)
-		twp[t] = mk((T) tb.twp[2 * t], (T) tb.twp[2 * t + 1]);
-	for(int t = threadIdx.x; t < 2 * SY_RP * n; t += blockDim.x)
-		buf0[t] = mk((T) 0, (T) 0);
-	__syncthreads();
-	/* Legendre stage: item = (ring pair, m) */
 	const int nm = polar ? 1 : 2 * L + 1;
-	for(int it = threadIdx.x; it < SY_RP * nm; it += blockDim.x) {
-		const int rp = it / nm, mi = it % nm;
-		const int j = j0 + rp;
+	for(int it = threadIdx.x; it < rp * nm; it += NT) {
+		const int r = it / nm, mi = it - r * nm;
+		const int j = j0 + r;
 		if(j >= B)
 			continue;
 		const int m = polar ? 0 : mi - L;
 		const int am = m < 0 ? -m : m;
 		const T x = (T) tb.x[j];
-		const T *al = nullptr;
-		(void) al;
-		const double *alpha = tb.alpha + (size_t) am * (L + 1);
-		T pprev = 0, pcur = (T) tb.smm[(size_t) j * (L + 1) + am];
+		const int abase = polar ? 0 : lm_offset(L, am, m);	/* (l = |m|, m) */
+		const int albase = polar ? 0 : lm_offset(L, am, am);	/* alpha at (l = |m|, |m|) */
+		T pprev = 0, pcur = (T) tb.smm[(size_t) j * (L + 1) + am], aprev = 1;
 		cplx<T> E = mk((T) 0, (T) 0), O = mk((T) 0, (T) 0);
-		const int base = polar ? 0 : lm_offset(L, am, m);	/* offset of (l = |m|, m) */
-		T alpha_prev = 0;
 		for(int l = am; l <= L; l++) {
-			if(l == am + 1) {
-				pprev = pcur;
-				alpha_prev = (T) alpha[l];
-				pcur = alpha_prev * x * pprev;
-			} else if(l > am + 1) {
-				const T al_l = (T) alpha[l];
-				const T pn = al_l * (x * pcur - pprev / alpha_prev);
+			if(l > am) {
+				const T a_l = ls.al ? ls.al[albase + l - am] : (T) tb.alpha[albase + l - am];
+				const T pn = (l == am + 1) ? a_l * x * pcur : a_l * (x * pcur - pprev / aprev);
 				pprev = pcur;
 				pcur = pn;
-				alpha_prev = al_l;
+				aprev = a_l;
 			}
-			const cplx<TI> c = a[base + l - am];
-			const cplx<T> cc = mk((T) c.re, (T) c.im);
+			cplx<T> c;
+			if(ls.sa)
+				c = ls.sa[abase + l - am];
+			else {
+				const cplx<TI> cg = ag[abase + l - am];
+				c = mk((T) cg.re, (T) cg.im);
+			}
 			if((l + am) & 1)
-				cfma(O, cc, pcur);
+				cfma(O, c, pcur);
 			else
-				cfma(E, cc, pcur);
+				cfma(E, c, pcur);
 		}
 		const int col = m >= 0 ? m : n + m;
 		const T sg = (m < 0 && (am & 1)) ? (T) -1 : (T) 1;
-		buf0[(2 * rp) * n + col] = (E + O) * sg;
-		buf0[(2 * rp + 1) * n + col] = (E - O) * sg;
+		F[(2 * r) * n + col] = (E + O) * sg;
+		F[(2 * r + 1) * n + col] = (E - O) * sg;
 	}
-	__syncthreads();
-	/* backward DFT along phi, Stockham, one output per thread per stage */
-	cplx<T> *src = buf0, *dst = buf1;
-	int Ns = 1;
-	for(int fi = 0; fi < g.nfac; fi++) {
-		const int R = g.fac[fi];
-		const int NsR = Ns * R, nR = n / R, tstep = n / NsR;
-		for(int o = threadIdx.x; o < 2 * SY_RP * n; o += blockDim.x) {
-			const int ring = o / n, oo = o % n;
-			const int e = oo % NsR;
-			const int jj = (oo / NsR) * Ns + (oo % Ns);
-			const cplx<T> *sr = src + ring * n;
-			cplx<T> acc = mk((T) 0, (T) 0);
-			int t = 0;
-			const int tinc = (e * tstep) % n;
-			for(int r = 0; r < R; r++) {
-				cfma(acc, sr[jj + r * nR], twp[t]);
-				t += tinc;
-				if(t >= n)
-					t -= n;
-			}
-			dst[o] = acc;
-		}
-		__syncthreads();
-		cplx<T> *tmp = src;
-		src = dst;
-		dst = tmp;
-		Ns = NsR;
-	}
-	/* write the rows: slot 2rp -> ring j, slot 2rp+1 -> ring 2B-1-j */
-	for(int o = threadIdx.x; o < 2 * SY_RP * n; o += blockDim.x) {
-		const int slot = o / n, k = o % n;
-		const int rp = slot >> 1;
-		const int j = j0 + rp;
+}
+
+template <typename T, typename TO>
+__device__ void phi_stage(const cplx<T> *F, int polar, int rp, int j0, const struct synth_geom &g, const synth_lds<T> &ls, cplx<TO> *__restrict__ mapseg)
+{
+	const int n = g.n, B = g.B, half = g.n / 2;
+	const int mmax = polar ? 0 : g.L;
+	for(int o = threadIdx.x; o < 2 * rp * half; o += NT) {
+		const int slot = o / half, k = o - slot * half;
+		const int j = j0 + (slot >> 1);
 		if(j >= B)
 			continue;
+		const cplx<T> *Fr = F + slot * n;
+		const cplx<T> wk = ls.tw[k];
+		cplx<T> a0 = mk((T) 0, (T) 0), a1 = mk((T) 0, (T) 0);
+		cplx<T> w = mk((T) 1, (T) 0);
+		for(int m = 0; m <= mmax; m++) {
+			const cplx<T> e = Fr[m] * w;
+			a0 = a0 + e;
+			a1 = (m & 1) ? a1 - e : a1 + e;
+			w = w * wk;
+		}
+		if(mmax > 0) {
+			w = ls.tw[(int) ((n - (long) mmax * k % n) % n)];
+			for(int m = -mmax; m < 0; m++) {
+				const cplx<T> e = Fr[n + m] * w;
+				a0 = a0 + e;
+				a1 = (m & 1) ? a1 - e : a1 + e;
+				w = w * wk;
+			}
+		}
 		const int ring = (slot & 1) ? 2 * B - 1 - j : j;
-		const cplx<T> v = src[o];
-		map[((size_t) s * n + ring) * n + k] = mk((TO) v.re, (TO) v.im);
+		cplx<TO> *row = mapseg + (size_t) ring * n;
+		row[k] = mk((TO) a0.re, (TO) a0.im);
+		row[k + half] = mk((TO) a1.re, (TO) a1.im);
 	}
+}
+
+/* synthesis of a batch of series: one workgroup per (series, group of
+ * rp ring pairs); coefficients and recurrence table in LDS when they fit */
+template <typename TI, typename T, typename TO>
+__global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, int polar, int rp, int a_in_lds, int al_in_lds, struct synth_geom g, struct synth_tables tb, cplx<TO> *__restrict__ map)
+{
+	extern __shared__ unsigned char smem_raw[];
+	const int n = g.n, L = g.L;
+	const int ncoef = polar ? L + 1 : (L + 1) * (L + 1);
+	const int nal = polar ? L + 1 : (L + 1) * (L + 2) / 2;
+	cplx<T> *F = reinterpret_cast<cplx<T> *>(smem_raw);	/* [2 rp][n] */
+	cplx<T> *tw = F + 2 * rp * n;				/* [n] */
+	cplx<T> *sa = tw + n;					/* [ncoef] when staged */
+	T *al = reinterpret_cast<T *>(sa + (a_in_lds ? ncoef : 0));	/* [nal] when staged */
+	const int s = blockIdx.y, j0 = blockIdx.x * rp;
+	const cplx<TI> *ag = alm + (size_t) s * ncoef;
+	for(int t = threadIdx.x; t < n; t += NT)
+		tw[t] = mk((T) tb.twp[2 * t], (T) tb.twp[2 * t + 1]);
+	if(a_in_lds)
+		for(int t = threadIdx.x; t < ncoef; t += NT) {
+			const cplx<TI> c = ag[t];
+			sa[t] = mk((T) c.re, (T) c.im);
+		}
+	if(al_in_lds)
+		for(int t = threadIdx.x; t < nal; t += NT)
+			al[t] = (T) tb.alpha[t];
+	__syncthreads();
+	synth_lds<T> ls;
+	ls.tw = tw;
+	ls.sa = a_in_lds ? sa : nullptr;
+	ls.al = al_in_lds ? al : nullptr;
+	legendre_stage<TI, T>(ag, polar, rp, j0, g, tb, ls, F);
+	__syncthreads();
+	phi_stage<T, TO>(F, polar, rp, j0, g, ls, map + (size_t) s * n * n);
+}
+
+/*
+ * Fused sky stage of the batched pipeline, one workgroup per segment:
+ * sum the projection's K-split partials, rotate each baseline's polar
+ * power into the Earth frame with the D^l_{0 m'} rows (pre-scaled by
+ * 1/nb), apply rotate_z(gmst), write a_lm, then synthesise the mesh ring
+ * group by ring group from the coefficients held in LDS.
+ */
+template <typename T>
+__global__ void __launch_bounds__(NT) k_sky(const T *__restrict__ partial, int nsplit, int S, struct xproj_geom xg, const cplx<T> *__restrict__ R, const int2 *__restrict__ lm, const double *__restrict__ gmst, struct synth_geom g, struct synth_tables tb, int rp, cplx<T> *__restrict__ alm, cplx<T> *__restrict__ map)
+{
+	extern __shared__ unsigned char smem_raw[];
+	const int n = g.n, L = g.L, ncoef = (L + 1) * (L + 1), nal = (L + 1) * (L + 2) / 2;
+	cplx<T> *F = reinterpret_cast<cplx<T> *>(smem_raw);	/* [2 rp][n] */
+	cplx<T> *tw = F + 2 * rp * n;				/* [n] */
+	cplx<T> *sa = tw + n;					/* [ncoef] */
+	T *al = reinterpret_cast<T *>(sa + ncoef);		/* [nal] */
+	T *pw = al + nal;					/* [CT] */
+	const int s = blockIdx.x;
+	for(int c = threadIdx.x; c < xg.CT; c += NT) {
+		T v = 0;
+		for(int sp = 0; sp < nsplit; sp++)
+			v += partial[((size_t) sp * S + s) * xg.CT + c];
+		pw[c] = v;
+	}
+	for(int t = threadIdx.x; t < n; t += NT)
+		tw[t] = mk((T) tb.twp[2 * t], (T) tb.twp[2 * t + 1]);
+	for(int t = threadIdx.x; t < nal; t += NT)
+		al[t] = (T) tb.alpha[t];
+	__syncthreads();
+	double gs = 0.0;
+	if(gmst)
+		gs = gmst[s];
+	for(int idx = threadIdx.x; idx < ncoef; idx += NT) {
+		const int2 q = lm[idx];
+		const int l = q.x, m = q.y;
+		cplx<T> acc = mk((T) 0, (T) 0);
+		for(int b = 0; b < xg.nb; b++) {
+			if(l > xg.Lb[b])
+				continue;
+			const T *p = pw + xg.col_start[b] + 2 * l;
+			cfma(acc, R[(size_t) b * ncoef + idx], mk(p[0], p[1]));
+		}
+		if(gmst && m) {
+			double sn, cs;
+			sincos(-(double) m * gs, &sn, &cs);
+			acc = acc * mk((T) cs, (T) sn);
+		}
+		sa[idx] = acc;
+		alm[(size_t) s * ncoef + idx] = acc;
+	}
+	__syncthreads();
+	if(!map)
+		return;
+	synth_lds<T> ls;
+	ls.tw = tw;
+	ls.sa = sa;
+	ls.al = al;
+	for(int j0 = 0; j0 < g.B; j0 += rp) {
+		legendre_stage<T, T>(sa, 0, rp, j0, g, tb, ls, F);
+		__syncthreads();
+		phi_stage<T, T>(F, 0, rp, j0, g, ls, map + (size_t) s * n * n);
+		__syncthreads();
+	}
+}
+
+/* ring pairs per pass of k_sky that fit the LDS budget, or 0 */
+template <typename T>
+static int sky_rp(const struct xproj_geom *xg, int L, size_t *lds)
+{
+	const size_t budget = 65536;
+	const int n = 2 * (L + 1), B = L + 1;
+	const size_t fixed = (size_t) n * sizeof(cplx<T>) + (size_t) (L + 1) * (L + 1) * sizeof(cplx<T>) + ((size_t) (L + 1) * (L + 2) / 2 + xg->CT) * sizeof(T);
+	if(fixed + 2 * n * sizeof(cplx<T>) > budget)
+		return 0;
+	int rp = (int) ((budget - fixed) / (2 * n * sizeof(cplx<T>)));
+	rp = rp > B ? B : rp;
+	/* equal passes */
+	const int passes = (B + rp - 1) / rp;
+	rp = (B + passes - 1) / passes;
+	*lds = fixed + (size_t) 2 * rp * n * sizeof(cplx<T>);
+	return rp;
+}
+
+template <typename T>
+int launch_sky(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st)
+{
+	size_t lds = 0;
+	const int rp = sky_rp<T>(xg, g->L, &lds);
+	if(!rp) {
+		/* coefficients too large for one workgroup's LDS: two kernels */
+		if(launch_alm<T, T>(partial, nsplit, S, xg, g->L, R, lm, gmst, alm, st))
+			return -1;
+		return map ? launch_synth<T, T, T>(alm, S, 0, g, tb, map, st) : 0;
+	}
+	hipLaunchKernelGGL((k_sky<T>), dim3(S), dim3(NT), lds, st, partial, nsplit, S, *xg, R, lm, gmst, *g, *tb, rp, alm, map);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <typename TI, typename T, typename TO>
 int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom *g, const struct synth_tables *tb, cplx<TO> *map, hipStream_t st)
 {
-	const size_t lds = (size_t) (2 * 2 * SY_RP * g->n + g->n) * sizeof(cplx<T>);
-	dim3 grid((g->B + SY_RP - 1) / SY_RP, S);
-	hipLaunchKernelGGL((k_synth<TI, T, TO>), grid, dim3(NT), lds, st, alm, polar, *g, *tb, map);
+	const size_t budget = 65536;
+	const int L = g->L, n = g->n;
+	const size_t ncoef = polar ? L + 1 : (size_t) (L + 1) * (L + 1);
+	const size_t nal = polar ? L + 1 : (size_t) (L + 1) * (L + 2) / 2;
+	int rp = (int) (24576 / (2 * n * sizeof(cplx<T>)));
+	rp = rp < 1 ? 1 : rp > g->B ? g->B : rp;
+	size_t lds = (size_t) (2 * rp + 1) * n * sizeof(cplx<T>);
+	const int a_in_lds = lds + ncoef * sizeof(cplx<T>) <= budget - 4096;
+	if(a_in_lds)
+		lds += ncoef * sizeof(cplx<T>);
+	const int al_in_lds = lds + nal * sizeof(T) <= budget;
+	if(al_in_lds)
+		lds += nal * sizeof(T);
+	dim3 grid((g->B + rp - 1) / rp, S);
+	hipLaunchKernelGGL((k_synth<TI, T, TO>), grid, dim3(NT), lds, st, alm, polar, rp, a_in_lds, al_in_lds, *g, *tb, map);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template int launch_synth<double, double, double>(const cplx<double> *, int, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, hipStream_t);
 template int launch_synth<float, float, float>(const cplx<float> *, int, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, hipStream_t);
+template int launch_sky<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
+template int launch_sky<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);
 
 /* ------------------------------------------------------------------ */
 /* drop-in helpers (FP64)                                              */
