@@ -168,6 +168,15 @@ double complex sh_series_rotation_plan_wigner_D(const struct sh_series_rotation_
 struct sh_series *sh_series_rotate(struct sh_series *, const struct sh_series *, const struct sh_series_rotation_plan *);
 struct sh_series *sh_series_rotate_z(struct sh_series *, const struct sh_series *, double);
 
+/* derivatives used by the extremizer (sh_series_deriv.c:241-543) */
+struct sh_series *sh_series_d_by_dphi(struct sh_series *);
+struct sh_series *sh_series_sintheta_d_by_dtheta(const struct sh_series *);
+const struct sh_series *sh_series_sintheta_grad(const struct sh_series *, struct sh_series **, struct sh_series **);
+
+/* minimum of Re f: value returned, co-ordinates stored (GPU search);
+ * sh_series_extremize.c:273-360 */
+double sh_series_real_minimum(const struct sh_series *, double *, double *);
+
 struct sh_series_array *sh_series_array_new(int, unsigned int, int);
 void sh_series_array_free(struct sh_series_array *);
 struct sh_series_array *sh_series_array_set_len(struct sh_series_array *, int);

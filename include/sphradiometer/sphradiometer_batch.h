@@ -61,6 +61,47 @@ int sphr_batch_run(struct sphr_batch *, const void *tseries, int tseries_is_f32,
 int sphr_batch_profile(struct sphr_batch *, int enable);
 int sphr_batch_profile_read(struct sphr_batch *, double ms_per_stage[4], int *runs);
 
+/* Peak of each segment's sky: the maximum (find_max != 0) or minimum of
+ * Re f for the n_segments series in alm, as sh_series_real_minimum
+ * (sh_series_extremize.c:273-360) finds it, one wavefront per segment.
+ * alm and map are the device outputs of sphr_batch_run (same precision);
+ * with map the seed is the map's extreme pixel, with map NULL it is the
+ * extreme pixel of the l = 6 low-pass mesh, as in the reference.  The
+ * refinement evaluates the series in float64 whatever its precision.
+ * theta, phi, value: device float64 [n_segments], theta in [0, pi], phi
+ * in [0, 2 pi).  Asynchronous on stream; 0 when queued. */
+int sphr_batch_peaks(struct sphr_batch *, const void *alm, const void *map, int n_segments, int find_max, double *theta, double *phi, double *value, void *stream);
+
+/*
+ * Batched localization skies (no counterpart in the reference): for many
+ * events, what RapidLocalization_.sphcoeff does per event
+ * (RapidLocalization.py:246-298): preprocess_SNRTimeSeries, the c2c FFT
+ * per detector, generate_alm_sky for the beta = +1 and beta = -1 plans
+ * (inspiral_sky_map.c:869-939) and sh_series_scale_power_l(power,
+ * overall).  plansp / plansn have been through
+ * correlator_network_plan_mult_by_projection (beta = +1 / -1) and set_l;
+ * logprior has order l_max (the sky order); N of the plans must be
+ * precalculated_TimeSeries_length(length, delta_t).
+ *
+ * Layouts (row-major, device):
+ *   series [n_det][n_events][length]   complex (float32 pairs, or float64
+ *                                       pairs with series_f64)
+ *   gmst   [n_events] float64, may be NULL (no rotate_z)
+ *   alm    [2][n_events][(L+1)^2]      complex, beta = +1 first
+ *   map    [2][n_events][2(L+1)][2(L+1)] complex, may be NULL
+ *   theta, phi, value [2][n_events]    float64: the maximum of each sky
+ */
+struct sphr_loc;
+struct sphr_loc *sphr_loc_new(const struct correlator_network_plan_fd *plansp, const struct correlator_network_plan_fd *plansn, const struct sh_series *logprior, unsigned l_max, double power, double overall, int length, double delta_t, int precision, int max_events);
+void sphr_loc_free(struct sphr_loc *);
+int sphr_loc_run(struct sphr_loc *, const void *series, int series_f64, int n_events, const double *gmst, void *alm, void *map, void *stream);
+/* peaks of the 2 n_events skies of the last run's alm (and map, which
+ * seeds the search when given) */
+int sphr_loc_peaks(struct sphr_loc *, const void *alm, const void *map, int n_events, double *theta, double *phi, double *value, void *stream);
+/* [0] detectors, [1] baselines, [2] N, [3] length, [4] l_max,
+ * [5] precision, [6] max_events */
+void sphr_loc_info(const struct sphr_loc *, int info[8]);
+
 /* Shape information: [0] detectors, [1] baselines, [2] N, [3] sky l_max,
  * [4] projection columns, [5] K splits, [6] precision */
 void sphr_batch_info(const struct sphr_batch *, int info[8]);

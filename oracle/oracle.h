@@ -150,6 +150,21 @@ double complex or_wigner_D(const struct or_rotation_plan *, unsigned l, int m, i
 struct or_series *or_rotate(struct or_series *result, const struct or_series *s, const struct or_rotation_plan *);
 struct or_series *or_rotate_z(struct or_series *result, const struct or_series *s, double phi);
 
+/* extremizer and derivative: sh_series_extremize.c, sh_series_deriv.c */
+double or_find_near_minimum(const struct or_series *, unsigned l_lowpass, double *theta, double *phi);
+double or_real_minimum(const struct or_series *, double *theta, double *phi);
+struct or_series *or_sintheta_d_by_dtheta(const struct or_series *);
+
+/* localization: inspiral_sky_map.c, RapidLocalization.py */
+void or_det_response(const double resp[9], double ra, double dec, double psi, double gmst, double *fp, double *fx);
+double complex or_cbc_projection(double theta, double phi, int i, int j, const double *resp, int n, double beta, double psi, const double *psds);
+int or_network_mult_by_projection(struct or_network *, const double *resp, double beta, double psi, const double *psd);
+struct or_series *or_log_uniformsky_prior(unsigned l_max);
+double or_tukey(int n, int i, double beta);
+long or_precalc_length(long length, double delta_t);
+void or_preprocess_snr(const double complex *series, int n_series, int len, double delta_t, double complex *out);
+struct or_series *or_generate_alm_sky(struct or_series *sky, struct or_network *, double complex **fseries, double gmst, const struct or_series *logprior);
+
 /* products: sh_series_product.c */
 struct or_product_plan *or_product_plan_new(const struct or_series *dest, const struct or_series *a, const struct or_series *b);
 struct or_product_plan *or_product_plan_new_forced(const struct or_series *dest, const struct or_series *a, const struct or_series *b, int microcode);

@@ -69,6 +69,18 @@ def lib():
         L.orx_series_conj.argtypes = [c_uint, c_int, c_void_p]
         L.orx_scale_power_l.argtypes = [c_uint, c_void_p, c_double, c_double]
         L.orx_array_set_l.argtypes = [c_int, c_uint, c_int, c_void_p, c_uint, c_void_p]
+        L.orx_real_minimum.argtypes = [c_uint, c_int, c_void_p, c_void_p]
+        L.orx_find_near_minimum.argtypes = [c_uint, c_int, c_void_p, c_uint, c_void_p]
+        L.orx_sintheta_d_by_dtheta.argtypes = [c_uint, c_int, c_void_p, c_void_p]
+        L.orx_det_response.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_void_p]
+        L.orx_cbc_projection.argtypes = [c_double, c_double, c_int, c_int, c_void_p, c_int, c_double, c_double, c_void_p, c_void_p]
+        L.orx_network_mult_by_projection.argtypes = [c_void_p, c_void_p, c_double, c_double, c_void_p]
+        L.orx_log_uniformsky_prior.argtypes = [c_uint, c_void_p]
+        L.orx_tukey.argtypes = [c_int, c_double, c_void_p]
+        L.orx_precalc_length.restype = ctypes.c_long
+        L.orx_precalc_length.argtypes = [ctypes.c_long, c_double]
+        L.orx_preprocess_snr.argtypes = [c_void_p, c_int, c_int, c_double, c_void_p]
+        L.orx_localize.argtypes = [c_void_p, c_void_p, c_int, c_double, c_double, c_void_p, c_uint, c_double, c_double, c_void_p]
         _LIB = L
     return _LIB
 
@@ -143,6 +155,24 @@ class Network:
         if lib().orx_network_integrate(self._h, _ptr(fs), sky_l, _ptr(sky)):
             raise RuntimeError("integrate failed")
         return sky
+
+    def mult_by_projection(self, responses, beta, psi, psd):
+        """correlator_network_plan_mult_by_projection (inspiral_sky_map.c:
+        423-549); responses [n_inst][3][3], psd [n][n_inst]."""
+        r = np.ascontiguousarray(responses, dtype=np.float64)
+        p = np.ascontiguousarray(psd, dtype=np.float64)
+        if lib().orx_network_mult_by_projection(self._h, _ptr(r), beta, psi, _ptr(p)):
+            raise RuntimeError("mult_by_projection failed")
+
+    def localize(self, series, delta_t, gmst, logprior, sky_l, power=0.6, overall=2.7):
+        """generate_alm_sky + scale_power_l for one event: raw complex
+        series [n_inst][len] (preprocessed here), logprior at sky_l."""
+        x = np.ascontiguousarray(series, dtype=np.complex128)
+        lp = np.ascontiguousarray(logprior, dtype=np.complex128)
+        out = np.zeros((sky_l + 1) ** 2, dtype=np.complex128)
+        if lib().orx_localize(self._h, _ptr(x), x.shape[1], delta_t, gmst, _ptr(lp), sky_l, power, overall, _ptr(out)):
+            raise RuntimeError("localize failed")
+        return out
 
     def segment(self, tseries, sky_l, window=None, gmst=0.0, mesh=True):
         ts = np.ascontiguousarray(tseries, dtype=np.float64)
@@ -348,4 +378,70 @@ def array_set_l(arr, l_old, polar, l_new):
     n = a.shape[0]
     out = np.zeros((n, length(l_new, polar)), dtype=np.complex128)
     lib().orx_array_set_l(n, l_old, polar, _ptr(a), l_new, _ptr(out))
+    return out
+
+
+def real_minimum(coeff, l, polar=0):
+    """(theta, phi, value) of the minimum of Re f: reference seed search,
+    Nelder-Mead refinement (sh_series_extremize.c:273-360)."""
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    out = np.zeros(3)
+    lib().orx_real_minimum(l, polar, _ptr(c), _ptr(out))
+    return tuple(out)
+
+
+def find_near_minimum(coeff, l, polar=0, l_lowpass=6):
+    """(theta, phi, value) of the minimum pixel of the low-passed series
+    (sh_series_extremize.c:168-241)."""
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    out = np.zeros(3)
+    lib().orx_find_near_minimum(l, polar, _ptr(c), l_lowpass, _ptr(out))
+    return tuple(out)
+
+
+def sintheta_d_by_dtheta(coeff, l, polar=0):
+    """sin(theta) d/dtheta through the 3j form of sh_series_deriv.c:288-543;
+    the result has l_max + 1."""
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    out = np.zeros(length(l + 1, polar), dtype=np.complex128)
+    lib().orx_sintheta_d_by_dtheta(l, polar, _ptr(c), _ptr(out))
+    return out
+
+
+def det_response(resp, ra, dec, psi, gmst=0.0):
+    r = np.ascontiguousarray(resp, dtype=np.float64)
+    out = np.zeros(2)
+    lib().orx_det_response(_ptr(r), ra, dec, psi, gmst, _ptr(out))
+    return out[0], out[1]
+
+
+def cbc_projection(theta, phi, i, j, responses, beta, psi, psds):
+    r = np.ascontiguousarray(responses, dtype=np.float64)
+    p = np.ascontiguousarray(psds, dtype=np.float64)
+    out = np.zeros(1, dtype=np.complex128)
+    lib().orx_cbc_projection(theta, phi, i, j, _ptr(r), r.shape[0], beta, psi, _ptr(p), _ptr(out))
+    return complex(out[0])
+
+
+def log_uniformsky_prior(l):
+    out = np.zeros((l + 1) ** 2, dtype=np.complex128)
+    lib().orx_log_uniformsky_prior(l, _ptr(out))
+    return out
+
+
+def tukey(n, beta):
+    out = np.zeros(n)
+    lib().orx_tukey(n, beta, _ptr(out))
+    return out
+
+
+def precalc_length(length, delta_t):
+    return int(lib().orx_precalc_length(length, delta_t))
+
+
+def preprocess_snr(series, delta_t):
+    x = np.ascontiguousarray(series, dtype=np.complex128)
+    n, ln = x.shape
+    out = np.zeros((n, precalc_length(ln, delta_t)), dtype=np.complex128)
+    lib().orx_preprocess_snr(_ptr(x), n, ln, delta_t, _ptr(out))
     return out

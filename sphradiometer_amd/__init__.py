@@ -88,6 +88,11 @@ class NetworkPlanFD(ctypes.Structure):
     _fields_ = [("baselines", P(NetworkBaselines)), ("plans", P(P(PlanFD)))]
 
 
+class Detector(ctypes.Structure):
+    """struct sphr_detector: LALDetector's leading fields."""
+    _fields_ = [("location", c_double * 3), ("response", (ctypes.c_float * 3) * 3)]
+
+
 # name: (restype, argtypes)
 _PROTOS = {
     "sh_series_new": (P(ShSeries), [c_uint, c_int]),
@@ -127,6 +132,10 @@ _PROTOS = {
     "sh_series_rotation_plan_set_l": (P(RotationPlan), [P(RotationPlan), c_uint]),
     "sh_series_rotate": (P(ShSeries), [P(ShSeries), P(ShSeries), P(RotationPlan)]),
     "sh_series_rotate_z": (P(ShSeries), [P(ShSeries), P(ShSeries), c_double]),
+    "sh_series_d_by_dphi": (P(ShSeries), [P(ShSeries)]),
+    "sh_series_sintheta_d_by_dtheta": (P(ShSeries), [P(ShSeries)]),
+    "sh_series_sintheta_grad": (P(ShSeries), [P(ShSeries), P(P(ShSeries)), P(P(ShSeries))]),
+    "sh_series_real_minimum": (c_double, [P(ShSeries), P(c_double), P(c_double)]),
     "sh_series_array_new": (P(ShSeriesArray), [c_int, c_uint, c_int]),
     "sh_series_array_free": (None, [P(ShSeriesArray)]),
     "sh_series_array_set_l": (P(ShSeriesArray), [P(ShSeriesArray), c_uint]),
@@ -192,6 +201,17 @@ _PROTOS = {
     "sphr_batch_profile": (c_int, [c_void_p, c_int]),
     "sphr_batch_profile_read": (c_int, [c_void_p, P(c_double), P(c_int)]),
     "sphr_batch_info": (None, [c_void_p, P(c_int)]),
+    "sphr_batch_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "precalculated_TimeSeries_length": (ctypes.c_long, [ctypes.c_long, c_double]),
+    "sh_series_log_uniformsky_prior": (P(ShSeries), [c_int]),
+    "correlator_network_plan_mult_by_projection": (c_int, [P(NetworkPlanFD), c_double, c_double, P(P(c_double))]),
+    "preprocess_SNR_arrays": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p]),
+    "generate_alm_skys_arrays": (c_int, [P(P(ShSeries)), P(P(ShSeries)), P(NetworkPlanFD), P(NetworkPlanFD), c_void_p, c_int, c_double, P(ShSeries)]),
+    "sphr_loc_new": (c_void_p, [P(NetworkPlanFD), P(NetworkPlanFD), P(ShSeries), c_uint, c_double, c_double, c_int, c_double, c_int, c_int]),
+    "sphr_loc_free": (None, [c_void_p]),
+    "sphr_loc_run": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sphr_loc_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sphr_loc_info": (None, [c_void_p, P(c_int)]),
 }
 
 
@@ -274,11 +294,21 @@ class Network:
     """Instruments + network baselines + FD network plan, built through the
     C API (correlator_network_plan_fd_new, then _set_l)."""
 
-    def __init__(self, positions, n, delta_t, l_max=None):
+    def __init__(self, positions, n, delta_t, l_max=None, responses=None):
         L = lib()
         self._inst = L.instrument_array_new(0)
-        for x, y, z in np.asarray(positions, dtype=np.float64):
-            L.instrument_array_append(self._inst, L.instrument_new(float(x), float(y), float(z), None))
+        # detector descriptors (struct sphr_detector) stay alive with the network
+        self._det = []
+        for k, (x, y, z) in enumerate(np.asarray(positions, dtype=np.float64)):
+            data = None
+            if responses is not None:
+                d = Detector()
+                for i in range(3):
+                    for j in range(3):
+                        d.response[i][j] = float(responses[k][i][j])
+                self._det.append(d)
+                data = ctypes.cast(ctypes.pointer(d), c_void_p)
+            L.instrument_array_append(self._inst, L.instrument_new(float(x), float(y), float(z), data))
         self._bl = L.correlator_network_baselines_new(self._inst)
         self._plan = L.correlator_network_plan_fd_new(self._bl, n, delta_t)
         if not self._plan:
@@ -308,6 +338,13 @@ class Network:
 
     def power_1d(self, k):
         return coeff_view(self.baseline_plan(k).contents.power_1d).copy()
+
+    def mult_by_projection(self, beta, psi, psd):
+        """correlator_network_plan_mult_by_projection; psd [n][n_inst]."""
+        p = np.ascontiguousarray(psd, dtype=np.float64)
+        rows = (P(c_double) * p.shape[0])(*[p[i].ctypes.data_as(P(c_double)) for i in range(p.shape[0])])
+        if lib().correlator_network_plan_mult_by_projection(self._plan, beta, psi, rows):
+            raise RuntimeError("mult_by_projection failed: " + last_error())
 
     def integrate(self, fseries, sky_l):
         """correlator_network_integrate_power_fd on full spectra [n_inst][n]."""
@@ -439,6 +476,13 @@ class Batch:
         if r:
             raise RuntimeError("sphr_batch_run failed: " + last_error())
 
+    def peaks(self, d_alm, n_segments, d_theta, d_phi, d_value, d_map=None, find_max=True, stream=None):
+        """Queue the per-segment peak search (sphr_batch_peaks); theta, phi
+        and value are device float64 buffers of n_segments."""
+        r = lib().sphr_batch_peaks(self.h, d_alm, d_map, n_segments, int(find_max), d_theta, d_phi, d_value, stream)
+        if r:
+            raise RuntimeError("sphr_batch_peaks failed: " + last_error())
+
     def profile(self, enable):
         lib().sphr_batch_profile(self.h, int(enable))
 
@@ -453,6 +497,85 @@ class Batch:
         try:
             if getattr(self, "h", None):
                 lib().sphr_batch_free(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+
+def log_uniformsky_prior(l_max):
+    s = lib().sh_series_log_uniformsky_prior(l_max)
+    if not s:
+        raise RuntimeError("sh_series_log_uniformsky_prior failed")
+    out = coeff_view(s).copy()
+    lib().sh_series_free(s)
+    return out
+
+
+def precalculated_length(length, delta_t):
+    return int(lib().precalculated_TimeSeries_length(length, delta_t))
+
+
+def preprocess_snr(series, delta_t):
+    """preprocess_SNR_arrays on [n][length] complex series."""
+    x = np.ascontiguousarray(series, dtype=np.complex128)
+    n, ln = x.shape
+    out = np.zeros((n, precalculated_length(ln, delta_t)), dtype=np.complex128)
+    ip = (c_void_p * n)(*[x[i].ctypes.data for i in range(n)])
+    op = (c_void_p * n)(*[out[i].ctypes.data for i in range(n)])
+    if lib().preprocess_SNR_arrays(ip, n, ln, delta_t, op):
+        raise RuntimeError("preprocess_SNR_arrays failed")
+    return out
+
+
+def generate_alm_skys(net_p, net_n, series, gmst, logprior, l_max):
+    """generate_alm_skys_arrays on preprocessed [n][N] series; returns
+    (sky_p, sky_n) coefficient arrays."""
+    L = lib()
+    x = np.ascontiguousarray(series, dtype=np.complex128)
+    n, ln = x.shape
+    ip = (c_void_p * n)(*[x[i].ctypes.data for i in range(n)])
+    lp = series_from_numpy(logprior, l_max, 0)
+    sp, sn = P(ShSeries)(), P(ShSeries)()
+    r = L.generate_alm_skys_arrays(ctypes.byref(sp), ctypes.byref(sn), net_p.plan, net_n.plan, ip, ln, gmst, lp)
+    L.sh_series_free(lp)
+    if r:
+        raise RuntimeError("generate_alm_skys_arrays failed: " + last_error())
+    out = coeff_view(sp).copy(), coeff_view(sn).copy()
+    L.sh_series_free(sp)
+    L.sh_series_free(sn)
+    return out
+
+
+class Localizer:
+    """sphradiometer_batch.h sphr_loc_*: batched localization skies."""
+
+    def __init__(self, net_p, net_n, logprior, l_max, length, delta_t, precision=FP64, max_events=1024, power=0.6, overall=2.7):
+        lp = series_from_numpy(logprior, l_max, 0)
+        self.h = lib().sphr_loc_new(net_p.plan, net_n.plan, lp, l_max, power, overall, length, delta_t, precision, max_events)
+        lib().sh_series_free(lp)
+        if not self.h:
+            raise RuntimeError("sphr_loc_new failed: " + last_error())
+        self.l_max = l_max
+        self.precision = precision
+        self._keep = (net_p, net_n)
+
+    def info(self):
+        a = (c_int * 8)()
+        lib().sphr_loc_info(self.h, a)
+        return dict(zip(["detectors", "baselines", "N", "length", "l_max", "precision", "max_events"], list(a)[:7]))
+
+    def run(self, d_series, n_events, d_alm, d_map=None, d_gmst=None, series_f64=False, stream=None):
+        if lib().sphr_loc_run(self.h, d_series, int(series_f64), n_events, d_gmst, d_alm, d_map, stream):
+            raise RuntimeError("sphr_loc_run failed: " + last_error())
+
+    def peaks(self, d_alm, n_events, d_theta, d_phi, d_value, d_map=None, stream=None):
+        if lib().sphr_loc_peaks(self.h, d_alm, d_map, n_events, d_theta, d_phi, d_value, stream):
+            raise RuntimeError("sphr_loc_peaks failed: " + last_error())
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().sphr_loc_free(self.h)
         except Exception:
             pass
         self.h = None

@@ -308,3 +308,24 @@ extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[8])
 	info[6] = bt->precision;
 	info[7] = bt->k_per_split;
 }
+
+extern "C" int sphr_batch_peaks(struct sphr_batch *bt, const void *alm, const void *map, int S, int find_max, double *theta, double *phi, double *value, void *stream)
+{
+	if(!bt || S < 1 || S > bt->max_segments || !alm || !theta || !phi || !value) {
+		sphr_set_error("sphr_batch_peaks: bad arguments (segments %d of at most %d)", S, bt ? bt->max_segments : 0);
+		return -1;
+	}
+	hipStream_t st = stream ? (hipStream_t) stream : sphr_rt_stream();
+	const double *sx = map ? NULL : sphr_gpu_seed_nodes();
+	if(!map && !sx)
+		return -1;
+	const int ns = 2 * (SPHR_L_SEED + 1);
+	int r;
+	if(bt->precision == SPHR_MIXED)
+		r = launch_extremize<float>((const cplx<float> *) alm, S, bt->L, find_max, SPHR_L_SEED, (const cplx<float> *) map, bt->syn.x, bt->syn.cm, bt->syn.alpha, sx, ns, 100, theta, phi, value, st);
+	else
+		r = launch_extremize<double>((const cplx<double> *) alm, S, bt->L, find_max, SPHR_L_SEED, (const cplx<double> *) map, bt->syn.x, bt->syn.cm, bt->syn.alpha, sx, ns, 100, theta, phi, value, st);
+	if(r)
+		sphr_set_error("sphr_batch_peaks: kernel launch failed (%s)", hipGetErrorString(hipGetLastError()));
+	return r;
+}

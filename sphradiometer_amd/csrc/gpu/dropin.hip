@@ -93,6 +93,14 @@ int synth_tables_build(int L, struct synth_dev *t)
 	   || sphr_rt_check(hipMemcpy(t->alpha, ha, nal * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
 	   || sphr_rt_check(hipMemcpy(t->twp, ht, 2 * (size_t) n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
 		goto done;
+	/* Pbar_mm at the equator is its sectoral constant */
+	for(int m = 0; m <= L; m++) {
+		double one;
+		sphr_plm(m, m, 0.0, &one);
+		tmp[m] = one;
+	}
+	if(sphr_rt_check(hipMalloc(&t->cm, (L + 1) * sizeof(double)), "hipMalloc") || sphr_rt_check(hipMemcpy(t->cm, tmp, (L + 1) * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
+		goto done;
 	t->tb.x = t->x;
 	t->tb.smm = t->smm;
 	t->tb.alpha = t->alpha;
@@ -119,6 +127,8 @@ void synth_tables_free(struct synth_dev *t)
 		hipFree(t->alpha);
 	if(t->twp)
 		hipFree(t->twp);
+	if(t->cm)
+		hipFree(t->cm);
 	memset(t, 0, sizeof(*t));
 }
 
@@ -486,6 +496,59 @@ extern "C" int sphr_fft_exec(struct sphr_fft *f)
 	CK(hipMemcpyAsync(f->out, dout, (size_t) n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy spectrum");
 	CK(hipStreamSynchronize(st), "synchronize");
 	UNLOCK();
+	return 0;
+fail:
+	UNLOCK();
+	return -1;
+}
+
+/* ---- extremum (sh_series_real_minimum) ---- */
+
+static double *seed_x_dev;
+
+static const double *seed_nodes(void)
+{
+	if(!seed_x_dev) {
+		const double *x;
+		sphr_mesh_rule(SPHR_L_SEED, &x, NULL);
+		const int n = 2 * (SPHR_L_SEED + 1);
+		if(sphr_rt_check(hipMalloc(&seed_x_dev, n * sizeof(double)), "hipMalloc") || sphr_rt_check(hipMemcpy(seed_x_dev, x, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
+			return NULL;
+	}
+	return seed_x_dev;
+}
+
+extern "C" const double *sphr_gpu_seed_nodes(void)
+{
+	return seed_nodes();
+}
+
+extern "C" int sphr_gpu_extremize(unsigned l_max, const double *coeff, int find_max, double *theta, double *phi, double *value)
+{
+	if(sphr_gpu_ensure())
+		return -1;
+	const int L = (int) l_max;
+	const size_t nc = (size_t) (L + 1) * (L + 1);
+	LOCK();
+	hipStream_t st = sphr_rt_stream();
+	struct synth_dev *t = synth_for(L);
+	const double *sx = seed_nodes();
+	cplx<double> *da = (cplx<double> *) sphr_rt_scratch(SL_A, nc * sizeof(cplx<double>));
+	double *dout = (double *) sphr_rt_scratch(SL_B, 3 * sizeof(double));
+	double hout[3];
+	if(!t || !sx || !da || !dout)
+		goto fail;
+	CK(hipMemcpyAsync(da, coeff, nc * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy series");
+	if(launch_extremize<double>(da, 1, L, find_max, SPHR_L_SEED, NULL, NULL, t->cm, t->alpha, sx, 2 * (SPHR_L_SEED + 1), 100, dout, dout + 1, dout + 2, st)) {
+		sphr_set_error("extremize launch failed");
+		goto fail;
+	}
+	CK(hipMemcpyAsync(hout, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, st), "copy result");
+	CK(hipStreamSynchronize(st), "synchronize");
+	UNLOCK();
+	*theta = hout[0];
+	*phi = hout[1];
+	*value = hout[2];
 	return 0;
 fail:
 	UNLOCK();

@@ -1,0 +1,126 @@
+/*
+ * Real GEMM on MFMA for gfx950: C[M][N] = A[M][K] B[K][N], all row-major
+ * with leading dimensions, float64 (v_mfma_f64_16x16x4f64) or float32
+ * (v_mfma_f32_16x16x4f32) with accumulation in the same type.  A may be
+ * stored in float32 and widened on the way into LDS.
+ *
+ * A complex product with interleaved (re, im) storage is the same kernel:
+ * read the complex A[M][K] as real [M][2K] and expand the complex B[K][N]
+ * to real [2K][2N] with rows 2k = (Br, Bi) pairs and 2k+1 = (-Bi, Br)
+ * pairs; the real [M][2N] result is the complex C.  Eight real flops per
+ * complex multiply-add, as in the complex product itself.
+ *
+ * Tiling: workgroup 256 threads = 4 waves in 2 x 2, block tile 128 x 128,
+ * wave tile 64 x 64 = 4 x 4 MFMA tiles of 16 x 16; K chunks of KC staged
+ * in LDS.  Workgroup ids are
+ * remapped so the 8 XCDs each work on contiguous column panels of B (each
+ * XCD has its own L2).
+ *
+ * MFMA 16x16x4 operand layout (64 lanes): A[i][k] in lane i + 16 k, B[k][j]
+ * in lane j + 16 k; accumulator element e of lane j + 16 q holds D[i][j]
+ * with i = 4 q + e (f32) or i = q + 4 e (f64).
+ */
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+
+namespace {
+
+template <typename T> struct mf;
+template <> struct mf<float> {
+	typedef float v4 __attribute__((ext_vector_type(4)));
+	static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+	static __device__ inline int row(int q, int e) { return 4 * q + e; }
+	enum { KC = 32 };
+};
+template <> struct mf<double> {
+	typedef double v4 __attribute__((ext_vector_type(4)));
+	static __device__ inline v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+	static __device__ inline int row(int q, int e) { return q + 4 * e; }
+	enum { KC = 16 };
+};
+
+#define GM_BM 128
+#define GM_BN 128
+#define GM_NT 256
+
+template <typename T, typename TA>
+__global__ void __launch_bounds__(GM_NT) k_gemm(int M, int N, int K, const TA *__restrict__ A, long lda, const T *__restrict__ B, long ldb, T *__restrict__ C, long ldc, int tiles_m, int tiles_n)
+{
+	typedef typename mf<T>::v4 v4;
+	const int KC = mf<T>::KC;
+	/* rows padded to 16 elements past a multiple of 32 (64-bit) or 64
+	 * (32-bit) words: the four 16-lane groups of an operand read land on
+	 * disjoint banks */
+	__shared__ T sA[KC][GM_BM + 16];	/* transposed: [k][row] */
+	__shared__ T sB[KC][GM_BN + 16];
+	/* XCD-aware remap: hardware places workgroup id on XCD id % 8; give
+	 * each XCD a contiguous range of tiles (column panel major) */
+	const int nwg = tiles_m * tiles_n;
+	const int id = blockIdx.x;
+	int t = id;
+	if(nwg % 8 == 0)
+		t = (id % 8) * (nwg / 8) + id / 8;
+	const int tm = t % tiles_m, tn = t / tiles_m;
+	const int m0 = tm * GM_BM, n0 = tn * GM_BN;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+	const int li = lane & 15, lk = lane >> 4;
+	v4 acc[4][4];
+	for(int a = 0; a < 4; a++)
+		for(int b = 0; b < 4; b++)
+			acc[a][b] = (v4) 0;
+	for(int k0 = 0; k0 < K; k0 += KC) {
+		/* A tile: rows m0.., cols k0..k0+KC, coalesced along k */
+		for(int i = threadIdx.x; i < GM_BM * KC; i += GM_NT) {
+			const int r = i / KC, c = i - r * KC;
+			const int gm = m0 + r, gk = k0 + c;
+			sA[c][r] = (gm < M && gk < K) ? (T) A[(size_t) gm * lda + gk] : (T) 0;
+		}
+		/* B tile: rows k0.., cols n0..n0+BN, coalesced along n */
+		for(int i = threadIdx.x; i < KC * GM_BN; i += GM_NT) {
+			const int r = i / GM_BN, c = i - r * GM_BN;
+			const int gk = k0 + r, gn = n0 + c;
+			sB[r][c] = (gk < K && gn < N) ? B[(size_t) gk * ldb + gn] : (T) 0;
+		}
+		__syncthreads();
+		for(int ks = 0; ks < KC; ks += 4) {
+			T af[4], bf[4];
+			for(int a = 0; a < 4; a++)
+				af[a] = sA[ks + lk][wm + 16 * a + li];
+			for(int b = 0; b < 4; b++)
+				bf[b] = sB[ks + lk][wn + 16 * b + li];
+			for(int a = 0; a < 4; a++)
+				for(int b = 0; b < 4; b++)
+					acc[a][b] = mf<T>::mma(af[a], bf[b], acc[a][b]);
+		}
+		__syncthreads();
+	}
+	for(int a = 0; a < 4; a++)
+		for(int b = 0; b < 4; b++) {
+			const int gn = n0 + wn + 16 * b + li;
+			if(gn >= N)
+				continue;
+			for(int e = 0; e < 4; e++) {
+				const int gm = m0 + wm + 16 * a + mf<T>::row(lk, e);
+				if(gm < M)
+					C[(size_t) gm * ldc + gn] = acc[a][b][e];
+			}
+		}
+}
+
+}	/* namespace */
+
+template <typename T, typename TA>
+int launch_gemm(int M, int N, int K, const TA *A, long lda, const T *B, long ldb, T *C, long ldc, hipStream_t st)
+{
+	if(M <= 0 || N <= 0 || K <= 0)
+		return 0;
+	const int tm = (M + GM_BM - 1) / GM_BM, tn = (N + GM_BN - 1) / GM_BN;
+	hipLaunchKernelGGL((k_gemm<T, TA>), dim3(tm * tn), dim3(GM_NT), 0, st, M, N, K, A, lda, B, ldb, C, ldc, tm, tn);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template int launch_gemm<double, double>(int, int, int, const double *, long, const double *, long, double *, long, hipStream_t);
+template int launch_gemm<double, float>(int, int, int, const float *, long, const double *, long, double *, long, hipStream_t);
+template int launch_gemm<float, float>(int, int, int, const float *, long, const float *, long, float *, long, hipStream_t);
+template int launch_gemm<float, double>(int, int, int, const double *, long, const float *, long, float *, long, hipStream_t);

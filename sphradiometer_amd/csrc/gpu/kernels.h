@@ -59,6 +59,17 @@ int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom 
 template <typename T>
 int launch_sky(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st);
 
+/* extremum of sign * Re f for S series (one wavefront each): seed from
+ * seed_map (S meshes of order L, rings from map_x) when given, else from
+ * the l_seed low-pass on the n_seed-ring mesh seed_x; then Newton */
+template <typename TI>
+int launch_extremize(const cplx<TI> *alm, int S, int L, int find_max, int l_seed, const cplx<TI> *seed_map, const double *map_x, const double *cm, const double *alpha, const double *seed_x, int n_seed, int max_iter, double *theta, double *phi, double *value, hipStream_t st);
+
+/* C[M][N] = A[M][K] B[K][N] on MFMA, row-major with leading dimensions;
+ * T is the arithmetic type, TA the storage type of A (widened on load) */
+template <typename T, typename TA>
+int launch_gemm(int M, int N, int K, const TA *A, long lda, const T *B, long ldb, T *C, long ldc, hipStream_t st);
+
 /* drop-in helpers (FP64) */
 int launch_xspec(const cplx<double> *const *fs_dev, int n, int nb, const int *ia, const int *ib, cplx<double> *fsp, hipStream_t st);
 int launch_dotc(const cplx<double> *coeff, int n, int stride, const cplx<double> *vec, const double *vec_real, double scale, cplx<double> *partial, int nsplit, cplx<double> *out, hipStream_t st);

@@ -1,0 +1,257 @@
+/*
+ * Batched localization skies on the GPU (sphradiometer_batch.h,
+ * sphr_loc_*): for E events of D complex SNR series,
+ *
+ *   1. spectra  F[d][e] = series[d][e] . DFT    (MFMA GEMM, the SNR
+ *      preprocessing folded into the [len][N] DFT matrix)
+ *   2. cross spectra X[e][b N + f] = F_ib[e][f] conj(F_ia[e][f])
+ *   3. skies S[e][beta][lm] = X[e] . W           (MFMA GEMM, K = nb N,
+ *      both beta = +1 / -1 plans side by side in the columns)
+ *   4. alm[beta][e][lm] = (S + s_l prior) e^{-i m gmst_e}
+ *   5. optional meshes (synthesis) and peaks (extremizer).
+ *
+ * Complex operands of the GEMMs are real-expanded (see gemm.hip): the
+ * right-hand matrices are stored expanded, the left-hand ones are the
+ * interleaved complex arrays read as real.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sphradiometer/sphradiometer_batch.h>
+#include "../sphr_internal.h"
+#include "kernels.h"
+#include "runtime.h"
+#include "synth_tables.h"
+
+struct sphr_loc {
+	int precision, D, nb, N, len, L, max_events;
+	int ia[16], ib[16];
+	void *Wx;	/* T [2 nb N][4 ncoef] */
+	void *Dx;	/* T [2 len][2 N] */
+	void *prior;	/* cplx<T> [ncoef] */
+	void *F;	/* T [D max_events][2 N] */
+	void *X;	/* T [max_events][2 nb N] */
+	void *S;	/* T [max_events][4 ncoef] */
+	struct synth_dev syn;
+};
+
+struct loc_pairs {
+	int ia[16], ib[16];
+};
+
+template <typename T>
+__global__ void k_loc_xspec(const cplx<T> *__restrict__ F, int E, int N, int nb, struct loc_pairs p, cplx<T> *__restrict__ X)
+{
+	const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	const long tot = (long) E * nb * N;
+	if(i >= tot)
+		return;
+	const int e = (int) (i / ((long) nb * N));
+	const int r = (int) (i - (long) e * nb * N);
+	const int b = r / N, f = r - b * N;
+	const cplx<T> Fa = F[((size_t) p.ia[b] * E + e) * N + f];
+	const cplx<T> Fb = F[((size_t) p.ib[b] * E + e) * N + f];
+	X[i] = Fb * conjg(Fa);
+}
+
+template <typename T>
+__global__ void k_loc_epilogue(const T *__restrict__ S, int E, int L, const cplx<T> *__restrict__ prior, const double *__restrict__ gmst, cplx<T> *__restrict__ alm)
+{
+	const int ncoef = (L + 1) * (L + 1);
+	const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= 2L * E * ncoef)
+		return;
+	const int beta = (int) (i / ((long) E * ncoef));
+	const long r = i - (long) beta * E * ncoef;
+	const int e = (int) (r / ncoef), c = (int) (r - (long) e * ncoef);
+	const T *s = S + (size_t) e * 4 * ncoef + (size_t) beta * 2 * ncoef + 2 * c;
+	cplx<T> v = mk(s[0], s[1]) + prior[c];
+	if(gmst) {
+		int l, m;
+		lm_decode(L, c, &l, &m);
+		double sn, cs;
+		sincos(-(double) m * gmst[e], &sn, &cs);
+		v = v * mk((T) cs, (T) sn);
+	}
+	alm[i] = v;
+}
+
+template <typename T>
+static int upload_as(void **dst, const double *src, size_t n)
+{
+	T *h = (T *) malloc(n * sizeof(T));
+	if(!h)
+		return -1;
+	for(size_t i = 0; i < n; i++)
+		h[i] = (T) src[i];
+	int rc = sphr_rt_check(hipMalloc(dst, n * sizeof(T)), "hipMalloc") || sphr_rt_check(hipMemcpy(*dst, h, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+	free(h);
+	return rc ? -1 : 0;
+}
+
+/* complex [K][N] -> real [2K][2N] expansion on the host, as T */
+template <typename T>
+static int upload_expanded(void **dst, const double *src, size_t K, size_t N)
+{
+	T *h = (T *) malloc(4 * K * N * sizeof(T));
+	if(!h)
+		return -1;
+	for(size_t k = 0; k < K; k++)
+		for(size_t n = 0; n < N; n++) {
+			const double re = src[2 * (k * N + n)], im = src[2 * (k * N + n) + 1];
+			T *r0 = h + (2 * k) * (2 * N) + 2 * n, *r1 = r0 + 2 * N;
+			r0[0] = (T) re;
+			r0[1] = (T) im;
+			r1[0] = (T) -im;
+			r1[1] = (T) re;
+		}
+	int rc = sphr_rt_check(hipMalloc(dst, 4 * K * N * sizeof(T)), "hipMalloc") || sphr_rt_check(hipMemcpy(*dst, h, 4 * K * N * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+	free(h);
+	return rc ? -1 : 0;
+}
+
+extern "C" void sphr_loc_free(struct sphr_loc *lc)
+{
+	if(!lc)
+		return;
+	void *p[] = { lc->Wx, lc->Dx, lc->prior, lc->F, lc->X, lc->S };
+	for(unsigned i = 0; i < sizeof(p) / sizeof(p[0]); i++)
+		if(p[i])
+			hipFree(p[i]);
+	synth_tables_free(&lc->syn);
+	free(lc);
+}
+
+template <typename T>
+static int create_t(struct sphr_loc *lc, const struct sphr_loc_desc *d)
+{
+	const size_t ncoef = (size_t) (d->L + 1) * (d->L + 1);
+	/* W: rows (b, f), columns (beta, lm): reorder [beta][b][f][lm] */
+	const size_t K = (size_t) d->nb * d->N;
+	double *Wr = (double *) malloc(2 * K * 2 * ncoef * sizeof(double));
+	if(!Wr)
+		return -1;
+	for(size_t k = 0; k < K; k++)
+		for(int beta = 0; beta < 2; beta++)
+			memcpy(Wr + 2 * (k * 2 * ncoef + beta * ncoef), d->W + 2 * ((beta * K + k) * ncoef), 2 * ncoef * sizeof(double));
+	int rc = upload_expanded<T>(&lc->Wx, Wr, K, 2 * ncoef);
+	free(Wr);
+	if(rc || upload_expanded<T>(&lc->Dx, d->dft, d->len, d->N) || upload_as<T>(&lc->prior, d->prior, 2 * ncoef))
+		return -1;
+	const size_t E = d->max_events;
+	if(sphr_rt_check(hipMalloc(&lc->F, (size_t) d->D * E * 2 * d->N * sizeof(T)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&lc->X, E * 2 * K * sizeof(T)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&lc->S, E * 4 * ncoef * sizeof(T)), "hipMalloc"))
+		return -1;
+	return 0;
+}
+
+extern "C" struct sphr_loc *sphr_loc_create(const struct sphr_loc_desc *d)
+{
+	if(sphr_gpu_ensure())
+		return NULL;
+	if(d->nb > 16 || d->D > 16 || (d->precision != SPHR_FP64 && d->precision != SPHR_MIXED)) {
+		sphr_set_error("sphr_loc_new: unsupported shape or precision");
+		return NULL;
+	}
+	struct sphr_loc *lc = (struct sphr_loc *) calloc(1, sizeof(*lc));
+	if(!lc)
+		return NULL;
+	lc->precision = d->precision;
+	lc->D = d->D;
+	lc->nb = d->nb;
+	lc->N = d->N;
+	lc->len = d->len;
+	lc->L = d->L;
+	lc->max_events = d->max_events;
+	memcpy(lc->ia, d->ia, sizeof(lc->ia));
+	memcpy(lc->ib, d->ib, sizeof(lc->ib));
+	const int rc = d->precision == SPHR_MIXED ? create_t<float>(lc, d) : create_t<double>(lc, d);
+	if(rc || synth_tables_build(d->L, &lc->syn)) {
+		sphr_loc_free(lc);
+		return NULL;
+	}
+	return lc;
+}
+
+template <typename T>
+static int run_t(struct sphr_loc *lc, const void *series, int series_f64, int E, const double *gmst, void *alm, void *map, hipStream_t st)
+{
+	const int N = lc->N, nb = lc->nb, L = lc->L;
+	const int ncoef = (L + 1) * (L + 1);
+	/* 1: spectra, [D E][2 len] x [2 len][2 N] */
+	int r;
+	if(series_f64)
+		r = launch_gemm<T, double>(lc->D * E, 2 * N, 2 * lc->len, (const double *) series, 2L * lc->len, (const T *) lc->Dx, 2L * N, (T *) lc->F, 2L * N, st);
+	else
+		r = launch_gemm<T, float>(lc->D * E, 2 * N, 2 * lc->len, (const float *) series, 2L * lc->len, (const T *) lc->Dx, 2L * N, (T *) lc->F, 2L * N, st);
+	if(r)
+		return -1;
+	/* 2: cross spectra */
+	struct loc_pairs p;
+	memcpy(p.ia, lc->ia, sizeof(p.ia));
+	memcpy(p.ib, lc->ib, sizeof(p.ib));
+	const long nx = (long) E * nb * N;
+	hipLaunchKernelGGL((k_loc_xspec<T>), dim3((unsigned) ((nx + 255) / 256)), dim3(256), 0, st, (const cplx<T> *) lc->F, E, N, nb, p, (cplx<T> *) lc->X);
+	if(hipGetLastError() != hipSuccess)
+		return -1;
+	/* 3: skies, [E][2 nb N] x [2 nb N][4 ncoef] */
+	if(launch_gemm<T, T>(E, 4 * ncoef, 2 * nb * N, (const T *) lc->X, 2L * nb * N, (const T *) lc->Wx, 4L * ncoef, (T *) lc->S, 4L * ncoef, st))
+		return -1;
+	/* 4: prior, gmst */
+	const long ne = 2L * E * ncoef;
+	hipLaunchKernelGGL((k_loc_epilogue<T>), dim3((unsigned) ((ne + 255) / 256)), dim3(256), 0, st, (const T *) lc->S, E, L, (const cplx<T> *) lc->prior, gmst, (cplx<T> *) alm);
+	if(hipGetLastError() != hipSuccess)
+		return -1;
+	/* 5: meshes of both skies */
+	if(map && launch_synth<T, T, T>((const cplx<T> *) alm, 2 * E, 0, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st))
+		return -1;
+	return 0;
+}
+
+extern "C" int sphr_loc_run(struct sphr_loc *lc, const void *series, int series_f64, int n_events, const double *gmst, void *alm, void *map, void *stream)
+{
+	if(!lc || !series || !alm || n_events < 1 || n_events > lc->max_events) {
+		sphr_set_error("sphr_loc_run: bad arguments (events %d of at most %d)", n_events, lc ? lc->max_events : 0);
+		return -1;
+	}
+	hipStream_t st = stream ? (hipStream_t) stream : sphr_rt_stream();
+	const int r = lc->precision == SPHR_MIXED ? run_t<float>(lc, series, series_f64, n_events, gmst, alm, map, st) : run_t<double>(lc, series, series_f64, n_events, gmst, alm, map, st);
+	if(r)
+		sphr_set_error("sphr_loc_run: kernel launch failed (%s)", hipGetErrorString(hipGetLastError()));
+	return r;
+}
+
+extern "C" int sphr_loc_peaks(struct sphr_loc *lc, const void *alm, const void *map, int n_events, double *theta, double *phi, double *value, void *stream)
+{
+	if(!lc || !alm || n_events < 1 || n_events > lc->max_events || !theta || !phi || !value) {
+		sphr_set_error("sphr_loc_peaks: bad arguments");
+		return -1;
+	}
+	hipStream_t st = stream ? (hipStream_t) stream : sphr_rt_stream();
+	const double *sx = map ? NULL : sphr_gpu_seed_nodes();
+	if(!map && !sx)
+		return -1;
+	const int ns = 2 * (SPHR_L_SEED + 1), S = 2 * n_events;
+	int r;
+	if(lc->precision == SPHR_MIXED)
+		r = launch_extremize<float>((const cplx<float> *) alm, S, lc->L, 1, SPHR_L_SEED, (const cplx<float> *) map, lc->syn.x, lc->syn.cm, lc->syn.alpha, sx, ns, 100, theta, phi, value, st);
+	else
+		r = launch_extremize<double>((const cplx<double> *) alm, S, lc->L, 1, SPHR_L_SEED, (const cplx<double> *) map, lc->syn.x, lc->syn.cm, lc->syn.alpha, sx, ns, 100, theta, phi, value, st);
+	if(r)
+		sphr_set_error("sphr_loc_peaks: kernel launch failed (%s)", hipGetErrorString(hipGetLastError()));
+	return r;
+}
+
+extern "C" void sphr_loc_info(const struct sphr_loc *lc, int info[8])
+{
+	info[0] = lc->D;
+	info[1] = lc->nb;
+	info[2] = lc->N;
+	info[3] = lc->len;
+	info[4] = lc->L;
+	info[5] = lc->precision;
+	info[6] = lc->max_events;
+	info[7] = 0;
+}

@@ -28,6 +28,9 @@ int sphr_3j_range(int j1, int j2, int m1, int m2, double *out);
 /* in-place unnormalised complex DFT, sign = -1 forward, +1 backward */
 void sphr_host_dft(double *data, int n, int sign);
 
+/* LAL's Tukey window of n samples, taper fraction beta (localize.c) */
+void sphr_tukey_window(int n, double beta, double *w);
+
 /* packed Wigner-D blocks of a rotation plan (l = 1 first, each
  * (2l+1)^2 in storage order); NULL when l_max = 0 */
 struct sh_series_rotation_plan;
@@ -41,6 +44,21 @@ void sphr_set_error(const char *fmt, ...);
 
 /* result[c] = sum_i coeff[i * stride + c] * vec[i] over n rows */
 int sphr_gpu_dotc(const double *coeff, int n, int stride, const double *vec, int vec_is_real, double *result);
+
+/* batched localization tables (host-built, localize.c) */
+struct sphr_loc;
+struct sphr_loc_desc {
+	int precision, D, nb, N, len, L, max_events;
+	int ia[16], ib[16];
+	const double *W;	/* complex [2][nb][N][(L+1)^2]: sky per unit X */
+	const double *prior;	/* complex [(L+1)^2], scale_power_l applied */
+	const double *dft;	/* complex [len][N]: preprocessing + c2c DFT */
+};
+struct sphr_loc *sphr_loc_create(const struct sphr_loc_desc *);
+
+/* complex float64 C[M][N] = A[M][K] B[K][N] (row-major host arrays) on
+ * the MFMA GEMM; plan-time use */
+int sphr_gpu_zgemm(int M, int N, int K, const double *A, const double *B, double *C);
 
 /* synthesis of one series onto the 2(l+1) x 2(l+1) mesh (row-major
  * [theta][phi], complex) */
@@ -81,6 +99,12 @@ struct sphr_net_desc {
 };
 struct sphr_batch;
 struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, unsigned sky_l, int precision, int max_segments);
+
+/* extremum of Re f (find_max = 0: minimum) of a non-polar series of
+ * order l_max, seeded like sh_series_real_minimum */
+int sphr_gpu_extremize(unsigned l_max, const double *coeff, int find_max, double *theta, double *phi, double *value);
+/* device copy of the seed mesh's ring nodes */
+const double *sphr_gpu_seed_nodes(void);
 
 /* FFT helpers of correlator.h */
 struct sphr_fft;

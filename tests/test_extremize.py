@@ -1,0 +1,276 @@
+"""The extremizer and the gradient operators it uses
+(sh_series_extremize.c:168-360, sh_series_deriv.c:241-543).
+
+Pins of the oracle: sin(theta) d/dtheta against a numerical derivative of
+the series and against the closed form for Y_10; the seed search against
+an impulse centred on a seed-mesh pixel; the minimum of a negated impulse
+sits at the impulse's centre with value -(L+1)^2/(4 pi) (addition theorem);
+the Nelder-Mead minimum is below every point of a dense grid.
+
+The reference refines with GSL's BFGS2 and stops at |grad| < 1e-3; the
+library and the oracle both converge to the stationary point itself, so the
+BFGS iterates are parity unpinned and the tests compare stationary points.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import sphradiometer_amd as S
+
+
+def random_series(rng, l, polar=0, real=True, decay=0.0):
+    n = (l + 1) if polar else (l + 1) ** 2
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    if decay:
+        from oracle import oracle as O
+        for ll in range(l + 1):
+            for m in ([0] if polar else range(-ll, ll + 1)):
+                c[O.lmoffset(l, ll, m) if not polar else ll] *= math.exp(-decay * ll)
+    if real and not polar:
+        from oracle import oracle as O
+        c = O.series_real(c, l)
+    if polar and real:
+        c = c.real.astype(np.complex128)
+    return c
+
+
+# ---------------------------------------------------------------- oracle pins
+
+
+def test_oracle_sintheta_dtheta_Y10_closed_form(oracle):
+    # sin(theta) d/dtheta [sqrt(3/4pi) cos(theta)] = -sqrt(3/4pi) sin^2(theta)
+    c = np.zeros(2, dtype=np.complex128)
+    c[1] = 1.0
+    r = oracle.sintheta_d_by_dtheta(c, 1, polar=1)
+    for th in np.linspace(0.05, 3.0, 11):
+        got = oracle.eval_series(r, 2, 1, th, 0.3)
+        assert abs(got - (-math.sqrt(3 / (4 * math.pi)) * math.sin(th) ** 2)) < 1e-14
+
+
+@pytest.mark.parametrize("polar", [0, 1])
+def test_oracle_sintheta_dtheta_numerical(oracle, polar):
+    rng = np.random.default_rng(11 + polar)
+    L = 12
+    c = random_series(rng, L, polar=polar, real=False)
+    r = oracle.sintheta_d_by_dtheta(c, L, polar=polar)
+    h = 1e-5
+    for _ in range(12):
+        th, ph = rng.uniform(0.2, 2.9), rng.uniform(0, 2 * math.pi)
+        num = math.sin(th) * (oracle.eval_series(c, L, polar, th + h, ph) - oracle.eval_series(c, L, polar, th - h, ph)) / (2 * h)
+        got = oracle.eval_series(r, L + 1, polar, th, ph)
+        assert abs(got - num) < 1e-6 * (1 + abs(num))
+
+
+def test_oracle_seed_on_mesh_pixel(oracle):
+    # an impulse centred on a pixel of the l = 6 seed mesh: the seed search
+    # of its negation returns exactly that pixel
+    x, _ = oracle.mesh_nodes(6)
+    j, i = 4, 9
+    th0, ph0 = math.acos(x[j]), 2 * math.pi * i / 14
+    L = 12
+    c = -oracle.impulse(L, th0, ph0)
+    th, ph, v = oracle.find_near_minimum(c, L)
+    assert th == th0 and ph == ph0
+    assert v < 0
+
+
+def test_oracle_minimum_of_impulse(oracle):
+    rng = np.random.default_rng(3)
+    for L in (8, 16, 32):
+        th0, ph0 = rng.uniform(0.3, 2.8), rng.uniform(0, 2 * math.pi)
+        c = -oracle.impulse(L, th0, ph0)
+        th, ph, v = oracle.real_minimum(c, L)
+        assert abs(th - th0) < 1e-6 and abs((ph - ph0 + math.pi) % (2 * math.pi) - math.pi) < 1e-6
+        assert abs(v + (L + 1) ** 2 / (4 * math.pi)) < 1e-11 * (L + 1) ** 2
+
+
+def test_oracle_minimum_below_dense_grid(oracle):
+    rng = np.random.default_rng(5)
+    L = 5
+    c = random_series(rng, L, real=True)
+    th, ph, v = oracle.real_minimum(c, L)
+    # dense grid brute force through the series' own mesh at a high order
+    # (zero padding changes nothing but the sampling)
+    big = np.zeros((65) ** 2, dtype=np.complex128)
+    for l in range(L + 1):
+        for m in range(-l, l + 1):
+            big[oracle.lmoffset(64, l, m)] = c[oracle.lmoffset(L, l, m)]
+    mesh = oracle.to_mesh(big, 64).real
+    assert v <= mesh.min() + 1e-13
+    # and the minimum is a stationary point: no lower value nearby
+    for d in (1e-4, -1e-4):
+        assert oracle.eval_series(c, L, 0, th + d, ph).real >= v
+        assert oracle.eval_series(c, L, 0, th, ph + d).real >= v
+
+
+# ------------------------------------------------ library host-side operators
+
+
+@pytest.mark.parametrize("polar", [0, 1])
+def test_lib_sintheta_dtheta_matches_oracle(oracle, polar):
+    rng = np.random.default_rng(21 + polar)
+    L = 24
+    c = random_series(rng, L, polar=polar, real=False)
+    s = S.series_from_numpy(c, L, polar)
+    r = S.lib().sh_series_sintheta_d_by_dtheta(s)
+    got = S.coeff_view(r).copy()
+    want = oracle.sintheta_d_by_dtheta(c, L, polar=polar)
+    assert r.contents.l_max == L + 1 and r.contents.polar == polar
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+    S.lib().sh_series_free(r)
+    S.lib().sh_series_free(s)
+
+
+def test_lib_d_by_dphi_numerical(oracle):
+    rng = np.random.default_rng(31)
+    L = 10
+    c = random_series(rng, L, real=False)
+    s = S.series_from_numpy(c, L, 0)
+    S.lib().sh_series_d_by_dphi(s)
+    d = S.coeff_view(s).copy()
+    h = 1e-5
+    for _ in range(10):
+        th, ph = rng.uniform(0.2, 2.9), rng.uniform(0, 2 * math.pi)
+        num = (oracle.eval_series(c, L, 0, th, ph + h) - oracle.eval_series(c, L, 0, th, ph - h)) / (2 * h)
+        assert abs(oracle.eval_series(d, L, 0, th, ph) - num) < 1e-6 * (1 + abs(num))
+    S.lib().sh_series_free(s)
+
+
+def test_lib_sintheta_grad(oracle):
+    rng = np.random.default_rng(41)
+    L = 9
+    c = random_series(rng, L, real=True)
+    s = S.series_from_numpy(c, L, 0)
+    u = ctypes.POINTER(S.ShSeries)()
+    v = ctypes.POINTER(S.ShSeries)()
+    ret = S.lib().sh_series_sintheta_grad(s, ctypes.byref(u), ctypes.byref(v))
+    assert ctypes.addressof(ret.contents) == ctypes.addressof(s.contents)
+    uu = S.coeff_view(u).copy()
+    vv = S.coeff_view(v).copy()
+    want_u = oracle.sintheta_d_by_dtheta(c, L)
+    assert np.linalg.norm(uu - want_u) <= 1e-13 * np.linalg.norm(want_u)
+    for l in range(L + 1):
+        for m in range(-l, l + 1):
+            k = oracle.lmoffset(L, l, m)
+            assert abs(vv[k] - 1j * m * c[k]) <= 1e-15 * abs(c[k]) * (abs(m) + 1)
+    for p in (u, v, s):
+        S.lib().sh_series_free(p)
+
+
+# ------------------------------------------------------------------- GPU
+
+
+def lib_minimum(c, L, polar=0):
+    s = S.series_from_numpy(c, L, polar)
+    th = ctypes.c_double()
+    ph = ctypes.c_double()
+    v = S.lib().sh_series_real_minimum(s, ctypes.byref(th), ctypes.byref(ph))
+    S.lib().sh_series_free(s)
+    return th.value, ph.value, v
+
+
+def ang(a, b):
+    return abs((a - b + math.pi) % (2 * math.pi) - math.pi)
+
+
+@pytest.mark.gpu
+def test_gpu_minimum_of_impulse(oracle):
+    rng = np.random.default_rng(7)
+    for L in (6, 16, 32, 64):
+        th0, ph0 = rng.uniform(0.3, 2.8), rng.uniform(0, 2 * math.pi)
+        c = -oracle.impulse(L, th0, ph0)
+        print(f"impulse L={L} at ({th0:.6f}, {ph0:.6f})", flush=True)
+        th, ph, v = lib_minimum(c, L)
+        assert abs(th - th0) < 1e-7 and ang(ph, ph0) < 1e-7, (th, ph, th0, ph0)
+        assert abs(v + (L + 1) ** 2 / (4 * math.pi)) < 1e-11 * (L + 1) ** 2
+        assert 0 <= th <= math.pi and 0 <= ph < 2 * math.pi
+
+
+@pytest.mark.gpu
+def test_gpu_minimum_near_pole(oracle):
+    # the minimum at the north pole exactly and 1e-3 rad from it
+    for th0 in (0.0, 1e-3, math.pi):
+        c = -oracle.impulse(20, th0, 1.0)
+        th, ph, v = lib_minimum(c, 20)
+        assert abs(v + 21 ** 2 / (4 * math.pi)) < 1e-10 * 21 ** 2
+        # distance on the sphere
+        cosd = math.cos(th) * math.cos(th0) + math.sin(th) * math.sin(th0) * math.cos(ph - 1.0)
+        assert math.acos(min(1.0, cosd)) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,polar", [(4, 0), (12, 0), (32, 0), (20, 1)])
+def test_gpu_minimum_matches_oracle(oracle, L, polar):
+    rng = np.random.default_rng(100 + L)
+    for trial in range(4):
+        c = random_series(rng, L, polar=polar, real=True, decay=0.05)
+        print(f"random L={L} polar={polar} trial={trial}", flush=True)
+        th, ph, v = lib_minimum(c, L, polar)
+        to, po, vo = oracle.real_minimum(c if not polar else _polar_to_full(oracle, c, L), L)
+        scale = np.abs(c).max()
+        # same seed, same stationary point
+        assert abs(v - vo) <= 1e-10 * scale * (L + 1), (v, vo)
+        cosd = math.cos(th) * math.cos(to) + math.sin(th) * math.sin(to) * math.cos(ph - po)
+        if not polar:
+            assert math.acos(min(1.0, cosd)) < 1e-5
+        else:
+            assert abs(th - to) < 1e-5
+
+
+def _polar_to_full(oracle, c, L):
+    full = np.zeros((L + 1) ** 2, dtype=np.complex128)
+    for l in range(L + 1):
+        full[oracle.lmoffset(L, l, 0)] = c[l]
+    return full
+
+
+# ------------------------------------------------ batched peaks (GPU)
+
+
+@pytest.mark.gpu
+def test_gpu_batch_peaks_match_oracle(oracle):
+    from sphradiometer_amd import inject
+
+    net = S.Network(inject.positions("ABC"), 1024, 1.0 / 4096, 32)
+    ts, _ = inject.three_site(24, 1024, seed=3)
+    Sg, L = ts.shape[1], 32
+    out = {}
+    for prec, cdt in ((S.FP64, np.complex128), (S.MIXED, np.complex64)):
+        b = S.Batch(net, L, prec, max_segments=Sg)
+        d_ts = S.DeviceBuffer.from_numpy(ts)
+        d_alm = S.DeviceBuffer(Sg * (L + 1) ** 2 * np.dtype(cdt).itemsize)
+        d_map = S.DeviceBuffer(Sg * (2 * L + 2) ** 2 * np.dtype(cdt).itemsize)
+        b.run(d_ts.ptr, Sg, d_alm.ptr, d_map.ptr)
+        res = {}
+        for seed in ("lowpass", "map"):
+            d_t, d_p, d_v = (S.DeviceBuffer(Sg * 8) for _ in range(3))
+            print(f"batch peaks precision={prec} seed={seed}", flush=True)
+            b.peaks(d_alm.ptr, Sg, d_t.ptr, d_p.ptr, d_v.ptr, d_map=d_map.ptr if seed == "map" else None)
+            S.synchronize()
+            res[seed] = tuple(d.to_numpy(np.float64, (Sg,)) for d in (d_t, d_p, d_v))
+        alm = d_alm.to_numpy(cdt, (Sg, (L + 1) ** 2)).astype(np.complex128)
+        mp = d_map.to_numpy(cdt, (Sg, 2 * L + 2, 2 * L + 2)).real.astype(np.float64)
+        out[prec] = (res, alm, mp)
+    res, alm, mp = out[S.FP64]
+    for s in range(Sg):
+        scale = np.abs(alm[s]).max()
+        # the reference seed: the same stationary point as the oracle
+        to, po, vo = oracle.real_minimum(-alm[s], L)
+        t, p, v = (res["lowpass"][k][s] for k in range(3))
+        assert abs(v + vo) <= 1e-10 * scale * (L + 1), (s, v, vo)
+        cosd = math.cos(t) * math.cos(to) + math.sin(t) * math.sin(to) * math.cos(p - po)
+        assert math.acos(min(1.0, cosd)) < 1e-5, s
+        # the map seed: never below the map's own maximum
+        assert res["map"][2][s] >= mp[s].max() - 1e-12 * scale
+    # mixed precision peaks against float64 peaks, both map seeded
+    resm = out[S.MIXED][0]
+    seps = []
+    for s in range(Sg):
+        t, p = res["map"][0][s], res["map"][1][s]
+        tm, pm = resm["map"][0][s], resm["map"][1][s]
+        cosd = math.cos(t) * math.cos(tm) + math.sin(t) * math.sin(tm) * math.cos(p - pm)
+        seps.append(math.acos(min(1.0, cosd)))
+    print("mixed vs fp64 peak separation: max %.3g rad median %.3g" % (max(seps), np.median(seps)), flush=True)
+    assert np.median(seps) < 1e-4
