@@ -107,3 +107,63 @@ def three_site(n_segments=1024, n=1024, seed=0, snr=20.0, delta_t=1.0 / 4096):
             data[d, s] += delay_circular(h, r_dot_s(pos[d], theta, phi), delta_t)
         inj.append((theta, phi, f0))
     return data, inj
+
+
+# ---------------------------------------------------------------- localization
+
+# x-arm azimuths (degrees east of north) of the L-shaped detectors placed at
+# the three sites; the y arm is 90 degrees further east.  A fixed choice of
+# this module, used only to give the projection operator a response tensor.
+ARM_AZIMUTH = {"A": 36.0, "B": 108.0, "C": 19.0}
+
+
+def detector_response(lat_deg, lon_deg, azimuth_deg):
+    """Response tensor D = (u u^T - v v^T) / 2 of an L-shaped detector with
+    arms u (azimuth) and v (azimuth + 90 deg) in the local horizontal plane,
+    Earth-fixed Cartesian axes."""
+    la, lo, az = math.radians(lat_deg), math.radians(lon_deg), math.radians(azimuth_deg)
+    east = np.array([-math.sin(lo), math.cos(lo), 0.0])
+    north = np.array([-math.sin(la) * math.cos(lo), -math.sin(la) * math.sin(lo), math.cos(la)])
+    u = math.cos(az) * north + math.sin(az) * east
+    v = math.cos(az + math.pi / 2) * north + math.sin(az + math.pi / 2) * east
+    return 0.5 * (np.outer(u, u) - np.outer(v, v))
+
+
+def responses(names):
+    return np.array([detector_response(*SITES[n], ARM_AZIMUTH[n]) for n in names])
+
+
+def snr_events(n_events, length=410, delta_t=1.0 / 4096, seed=0, snr_range=(8.0, 30.0), names="ABC"):
+    """Matched-filter SNR series around a trigger, one event at a time:
+    unit-variance complex Gaussian noise plus a chirp-like complex peak
+    A_d exp(-(t - t_d)^2 / (2 s^2)) exp(2 pi i (f_c (t - t_d) + k (t - t_d)^2 / 2))
+    with s = 2 ms, f_c = 150 Hz, k = 2000 Hz/s; t_d = -r_d . n for an
+    isotropic direction n (Earth frame), A_d = rho a_d / |a| with
+    a_d = (F+_d + i Fx_d) e^{i phi0}, F+_d and Fx_d uniform in [-1, 1] (random
+    antenna factors), rho uniform in snr_range, phi0 uniform.  Samples at
+    t_k = (k - length // 2) delta_t.  Event e draws from a generator seeded
+    with (seed, e).  Returns (series [n_det][n_events][length] complex64,
+    meta dict of per-event theta, phi, snr, gmst)."""
+    pos = positions(names)
+    D = len(names)
+    out = np.empty((D, n_events, length), dtype=np.complex64)
+    meta = {"theta": np.empty(n_events), "phi": np.empty(n_events), "snr": np.empty(n_events), "gmst": np.empty(n_events)}
+    t = (np.arange(length) - length // 2) * delta_t
+    s, fc, k = 2e-3, 150.0, 2000.0
+    for e in range(n_events):
+        rng = np.random.default_rng([seed, e])
+        noise = (rng.standard_normal((D, length)) + 1j * rng.standard_normal((D, length))) / math.sqrt(2.0)
+        theta, phi = isotropic_direction(rng)
+        rho = rng.uniform(*snr_range)
+        fpx = rng.uniform(-1.0, 1.0, (D, 2))
+        phi0 = rng.uniform(0.0, 2 * math.pi)
+        a = (fpx[:, 0] + 1j * fpx[:, 1]) * np.exp(1j * phi0)
+        A = rho * a / np.linalg.norm(a)
+        for d in range(D):
+            td = -r_dot_s(pos[d], theta, phi)
+            u = t - td
+            noise[d] += A[d] * np.exp(-u * u / (2 * s * s)) * np.exp(2j * math.pi * (fc * u + 0.5 * k * u * u))
+        out[:, e] = noise
+        meta["theta"][e], meta["phi"][e], meta["snr"][e] = theta, phi, rho
+        meta["gmst"][e] = rng.uniform(0.0, 2 * math.pi)
+    return out, meta
