@@ -134,6 +134,107 @@ def cpu_baseline(ts, L, budget_s):
                        "plan build %.2f s not included" % (done, plan_s))
 
 
+FP64_MFMA_TFLOPS = 78.6
+FP32_MFMA_TFLOPS = 157.3
+
+
+def bench_localization(S, args):
+    """The localization workload: three sites, complex SNR series of 410
+    samples at 4096 Hz (+-50 ms, padded to 584), beta = +1 / -1 plans
+    multiplied by the projection at their natural order then set_l(50),
+    8192 events per step: spectra (DFT GEMM), cross spectra, the sky GEMM
+    (K = 3 x 584, 2 x 2601 complex columns), prior, rotate_z, meshes.
+    One JSON line, events/s; MFMA roofline of the sky GEMM."""
+    from sphradiometer_amd import inject
+    L, ln, dt, E = 50, 410, 1.0 / 4096, args.events
+    N = S.precalculated_length(ln, dt)
+    t0 = time.time()
+    nets = []
+    for beta in (1.0, -1.0):
+        net = S.Network(inject.positions("ABC"), N, dt, responses=inject.responses("ABC"))
+        nat = [int(net.baseline_plan(k).contents.delay_product.contents.l_max) for k in range(3)]
+        net.mult_by_projection(beta, 0.0, np.ones((N, 3)))
+        net.set_l(L)
+        nets.append(net)
+    lp_nat = S.log_uniformsky_prior(S.lib().correlator_network_l_max(nets[0]._bl, dt))
+    lnat = int(round(math.sqrt(lp_nat.size))) - 1
+    lp = np.array([lp_nat[_lmoff(lnat, l, m)] for l, m in _lm_list(L)])
+    plan_s = time.time() - t0
+    print("localization plans built in %.1f s (logprior natural l %d)" % (plan_s, lnat), file=sys.stderr, flush=True)
+    x, meta = inject.snr_events(E, length=ln, delta_t=dt, seed=0)
+    d_x = S.DeviceBuffer.from_numpy(x)
+    d_g = S.DeviceBuffer.from_numpy(meta["gmst"])
+    res = {}
+    for name, prec, cdt in (("fp64", S.FP64, np.complex128), ("mixed", S.MIXED, np.complex64)):
+        loc = S.Localizer(nets[0], nets[1], lp, L, ln, dt, precision=prec, max_events=E)
+        esz = np.dtype(cdt).itemsize
+        d_alm = S.DeviceBuffer(2 * E * (L + 1) ** 2 * esz)
+        n = 2 * (L + 1)
+        d_map = S.DeviceBuffer(2 * E * n * n * esz)
+        st = S.lib().sphr_stream()
+        for _ in range(args.warmup):
+            loc.run(d_x.ptr, E, d_alm.ptr, d_map=d_map.ptr, d_gmst=d_g.ptr, stream=st)
+        S.lib().sphr_stream_synchronize(st)
+        e0, e1 = S.lib().sphr_event_new(), S.lib().sphr_event_new()
+        S.lib().sphr_event_record(e0, st)
+        for _ in range(args.steps):
+            loc.run(d_x.ptr, E, d_alm.ptr, d_map=d_map.ptr, d_gmst=d_g.ptr, stream=st)
+        S.lib().sphr_event_record(e1, st)
+        S.lib().sphr_event_synchronize(e1)
+        ms = S.lib().sphr_event_elapsed_ms(e0, e1) / args.steps
+        res[name] = dict(ms=ms, alm=d_alm.to_numpy(cdt, (2, E, (L + 1) ** 2)))
+        print("%s: %.3f ms per %d events" % (name, ms, E), file=sys.stderr, flush=True)
+    err = np.linalg.norm(res["mixed"]["alm"] - res["fp64"]["alm"], axis=2) / np.linalg.norm(res["fp64"]["alm"], axis=2)
+    # the drop-in path on a few events: generate_alm_skys_arrays + scale_power_l
+    plain_err = []
+    for e in range(3):
+        pre = S.preprocess_snr(x[:, e].astype(np.complex128), dt)
+        sp, sn = S.generate_alm_skys(nets[0], nets[1], pre, float(meta["gmst"][e]), lp, L)
+        sl = np.array([2.7 * (l / L) ** 0.6 for l, m in _lm_list(L)])   # storage order
+        for bi, s in enumerate((sp, sn)):
+            want = s * sl
+            plain_err.append(float(np.linalg.norm(res["fp64"]["alm"][bi, e] - want) / np.linalg.norm(want)))
+    nb, ncoef = 3, (L + 1) ** 2
+    flops = 2.0 * E * (2 * nb * N) * (4 * ncoef) + 2.0 * (3 * E) * (2 * ln) * (2 * N)
+    value = E / (res["mixed"]["ms"] * 1e-3)
+    ach = flops / (res["mixed"]["ms"] * 1e-3) / 1e12
+    ach64 = flops / (res["fp64"]["ms"] * 1e-3) / 1e12
+    out = {
+        "metric": "localization sky pairs/s, 3-det l_max=50 (beta = +1 and -1 skies per event)",
+        "value": round(value, 1), "unit": "events/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(res["mixed"]["ms"], 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: unit complex Gaussian SNR series + chirp-like peak, random antenna factors (seed 0)",
+        "config": {"workload": "three sites, %d events x 410 complex samples at 4096 Hz padded to %d, l_max %d, a_lm + 102x102 map per sky" % (E, N, L),
+                   "events": E, "length": ln, "N": N, "l_max": L, "natural_l": nat},
+        "fp64": {"value": round(E / (res["fp64"]["ms"] * 1e-3), 1), "ms_per_step": round(res["fp64"]["ms"], 4),
+                 "mfma_tflops": round(ach64, 2), "frac": round(ach64 / FP64_MFMA_TFLOPS, 4),
+                 "alm_rel_err_vs_dropin_max": max(plain_err)},
+        "mixed_vs_fp64": {"alm_rel_err_max": float(err.max()), "alm_rel_err_median": float(np.median(err))},
+        "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(ach / FP32_MFMA_TFLOPS, 4), "traffic": None},
+        "plan_build_s": round(plan_s, 1),
+    }
+    print(json.dumps(out))
+
+
+def _lm_list(L):
+    """(l, m) in the coefficient storage order."""
+    out = [(l, 0) for l in range(L + 1)]
+    for m in range(1, L + 1):
+        out += [(l, m) for l in range(m, L + 1)]
+    for m in range(-L, 0):
+        out += [(l, m) for l in range(-m, L + 1)]
+    return out
+
+
+def _lmoff(L, l, m):
+    L1 = L + 1
+    x = m * (m - 1)
+    base = L1 * m - x // 2 if m >= 0 else L1 * (L1 + m) + x // 2
+    return base + l - abs(m)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +244,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-budget", type=float, default=3.0)
+    ap.add_argument("--workload", choices=["sky", "localization"], default="sky",
+                    help="sky: the three-site cross-power maps (default); localization: the log-likelihood skies")
+    ap.add_argument("--events", type=int, default=8192)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (segments are independent; "
@@ -155,6 +259,9 @@ def main():
     if not S.gpu_available():
         print("no GPU: " + S.last_error(), file=sys.stderr)
         sys.exit(1)
+    if args.workload == "localization":
+        bench_localization(S, args)
+        return
 
     L, N, nseg = 32, 1024, args.segments
     t0 = time.time()

@@ -102,6 +102,47 @@ int sphr_loc_peaks(struct sphr_loc *, const void *alm, const void *map, int n_ev
  * [5] precision, [6] max_events */
 void sphr_loc_info(const struct sphr_loc *, int info[8]);
 
+/*
+ * Learned peak proposal (no counterpart in the reference): a multilayer
+ * perceptron from the (L+1)^2 real-form SH coefficients of a sky
+ * (sphr_features) to K candidate directions, trained and evaluated with
+ * this library's kernels; sphr_refine_peaks then refines candidates on
+ * the exact series with the extremizer.  All pointers except host
+ * parameter arrays are device pointers; calls are asynchronous on stream
+ * (NULL = the library stream).
+ */
+struct sphr_mlp;
+/* depth hidden layers of width (GELU), n_candidates outputs x 3; bf16
+ * selects bfloat16 operands (float32 accumulation), else float32;
+ * parameters initialised from seed */
+struct sphr_mlp *sphr_mlp_new(int n_in, int width, int depth, int n_candidates, int max_batch, int bf16, unsigned long long seed);
+void sphr_mlp_free(struct sphr_mlp *);
+long sphr_mlp_n_params(const struct sphr_mlp *);
+/* flat float32 parameters W_1 [in][out], b_1, ..., W_out, b_out (host) */
+int sphr_mlp_set_params(struct sphr_mlp *, const float *host);
+int sphr_mlp_get_params(const struct sphr_mlp *, float *host);
+int sphr_mlp_get_grads(const struct sphr_mlp *, float *host);
+/* y [B][3 K] (may be NULL: kept inside for sphr_mlp_seeds) */
+int sphr_mlp_forward(struct sphr_mlp *, const float *x, int B, float *y, void *stream);
+/* forward + loss (per row, may be NULL) + gradients of the mean loss */
+int sphr_mlp_loss_grad(struct sphr_mlp *, const float *x, const float *target, int B, float lambda, float *loss, void *stream);
+int sphr_mlp_adam(struct sphr_mlp *, float lr, float beta1, float beta2, float eps, int step, void *stream);
+int sphr_mlp_train_step(struct sphr_mlp *, const float *x, const float *target, int B, float lambda, float lr, int step, float *loss, void *stream);
+/* (theta, phi) of the B x K candidates of the last forward: seeds [B][K][2] */
+int sphr_mlp_seeds(struct sphr_mlp *, int B, double *seeds, void *stream);
+/* real-form coefficients at unit RMS: out [S][(L+1)^2] float32 */
+int sphr_features(const void *alm, int alm_f32, int S, int L, float *out, void *stream);
+/* extremum refinement of series alm[s / alm_div] from seeds [n][2];
+ * results float64 [n] */
+int sphr_refine_peaks(const void *alm, int alm_f32, int n_series, int alm_div, int l_max, const double *seeds, int find_max, double *theta, double *phi, double *value, void *stream);
+/* synthetic burst data for training and evaluation: out [n_det][S][n]
+ * float32 = noise * N(0,1) (Philox4x32-10 keyed by seed) + per-segment
+ * sine-Gaussian amp sin(2 pi f0 u) exp(-(u / tau)^2), u = (k - n/2)
+ * delta_t + delay_d; params [S][3 + n_det] = amp, f0, tau, delay_0.. */
+int sphr_inject_bursts(const float *params, int n_det, int n_segments, int n, double delta_t, double noise, unsigned long long seed, float *out, void *stream);
+/* test hook: C = op(A) op(B) of the network's GEMM (device float32) */
+int sphr_mlp_gemm_check(int bf16, int ta, int tb, int M, int N, int K, const float *A, const float *B, float *C);
+
 /* Shape information: [0] detectors, [1] baselines, [2] N, [3] sky l_max,
  * [4] projection columns, [5] K splits, [6] precision */
 void sphr_batch_info(const struct sphr_batch *, int info[8]);

@@ -433,11 +433,21 @@ class DeviceBuffer:
             raise RuntimeError(last_error())
         return b
 
-    def to_numpy(self, dtype, shape):
+    def to_numpy(self, dtype, shape, offset=0):
         out = np.empty(shape, dtype=dtype)
-        if lib().sphr_memcpy_dtoh(_ptr(out), self.ptr, out.nbytes):
+        if offset + out.nbytes > self.nbytes:
+            raise ValueError("read past the end of the device buffer")
+        if lib().sphr_memcpy_dtoh(_ptr(out), self.ptr + offset, out.nbytes):
             raise RuntimeError(last_error())
         return out
+
+    def upload(self, a, offset=0):
+        """Copy a host array into the buffer at byte offset."""
+        a = np.ascontiguousarray(a)
+        if offset + a.nbytes > self.nbytes:
+            raise ValueError("write past the end of the device buffer")
+        if lib().sphr_memcpy_htod(self.ptr + offset, _ptr(a), a.nbytes):
+            raise RuntimeError(last_error())
 
     def free(self):
         if self.ptr:

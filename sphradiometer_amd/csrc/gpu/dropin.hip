@@ -554,3 +554,25 @@ fail:
 	UNLOCK();
 	return -1;
 }
+
+/* peak refinement from given seeds on device series (batched API) */
+extern "C" int sphr_refine_peaks(const void *alm, int alm_f32, int n_series, int alm_div, int l_max, const double *seeds, int find_max, double *theta, double *phi, double *value, void *stream)
+{
+	if(sphr_gpu_ensure())
+		return -1;
+	if(!alm || !seeds || n_series < 1 || alm_div < 1 || !theta || !phi || !value) {
+		sphr_set_error("sphr_refine_peaks: bad arguments");
+		return -1;
+	}
+	LOCK();
+	struct synth_dev *t = synth_for(l_max);
+	UNLOCK();
+	if(!t)
+		return -1;
+	hipStream_t st = stream ? (hipStream_t) stream : sphr_rt_stream();
+	const int r = alm_f32 ? launch_extremize_seeded<float>((const cplx<float> *) alm, n_series, alm_div, l_max, find_max, seeds, t->cm, t->alpha, 100, theta, phi, value, st)
+			      : launch_extremize_seeded<double>((const cplx<double> *) alm, n_series, alm_div, l_max, find_max, seeds, t->cm, t->alpha, 100, theta, phi, value, st);
+	if(r)
+		sphr_set_error("sphr_refine_peaks: launch failed");
+	return r;
+}

@@ -65,6 +65,11 @@ int launch_sky(const T *partial, int nsplit, int S, const struct xproj_geom *xg,
 template <typename TI>
 int launch_extremize(const cplx<TI> *alm, int S, int L, int find_max, int l_seed, const cplx<TI> *seed_map, const double *map_x, const double *cm, const double *alpha, const double *seed_x, int n_seed, int max_iter, double *theta, double *phi, double *value, hipStream_t st);
 
+/* the same refinement from given seeds (theta, phi) [S][2]; series s
+ * refines alm[s / alm_div] */
+template <typename TI>
+int launch_extremize_seeded(const cplx<TI> *alm, int S, int alm_div, int L, int find_max, const double *seeds, const double *cm, const double *alpha, int max_iter, double *theta, double *phi, double *value, hipStream_t st);
+
 /* C[M][N] = A[M][K] B[K][N] on MFMA, row-major with leading dimensions;
  * T is the arithmetic type, TA the storage type of A (widened on load) */
 template <typename T, typename TA>

@@ -107,11 +107,16 @@ __device__ inline void offset_point(double th, double ph, double d1, double d2, 
 }
 
 template <typename TI>
-__global__ void __launch_bounds__(64) k_extremize(const cplx<TI> *__restrict__ alm, int L, int sign, int l_seed, const cplx<TI> *__restrict__ seed_map, const double *__restrict__ map_x, struct peak_tables t, int max_iter, double *__restrict__ theta_out, double *__restrict__ phi_out, double *__restrict__ value_out)
+__global__ void __launch_bounds__(64) k_extremize(const cplx<TI> *__restrict__ alm, int alm_div, int L, int sign, int l_seed, const cplx<TI> *__restrict__ seed_map, const double *__restrict__ map_x, const double *__restrict__ seed_dir, struct peak_tables t, int max_iter, double *__restrict__ theta_out, double *__restrict__ phi_out, double *__restrict__ value_out)
 {
 	const int s = blockIdx.x, lane = threadIdx.x;
 	const int ncoef = (L + 1) * (L + 1);
-	const cplx<TI> *a = alm + (size_t) s * ncoef;
+	const cplx<TI> *a = alm + (size_t) (s / alm_div) * ncoef;
+	double th, ph;
+	if(seed_dir) {
+		th = seed_dir[2 * (size_t) s];
+		ph = seed_dir[2 * (size_t) s + 1];
+	} else {
 	/* seed: the extreme of sign * Re f over a mesh, first in row-major
 	 * order on ties (sh_series_extremize.c:214-230) */
 	double best = INFINITY;
@@ -170,7 +175,6 @@ __global__ void __launch_bounds__(64) k_extremize(const cplx<TI> *__restrict__ a
 		}
 	}
 	bi = __shfl(bi, 0);
-	double th, ph;
 	{
 		const int j = bi / n, k = bi - j * n;
 		double x;
@@ -181,6 +185,7 @@ __global__ void __launch_bounds__(64) k_extremize(const cplx<TI> *__restrict__ a
 			x = t.seed_x[j];
 		th = acos(x);
 		ph = 2.0 * M_PI * k / n;
+	}
 	}
 	/* Newton in the tangent plane on g(p) = sign * Re f(p).  The
 	 * gradient comes from central differences at step hg (truncation
@@ -266,9 +271,23 @@ int launch_extremize(const cplx<TI> *alm, int S, int L, int find_max, int l_seed
 	t.alpha = alpha;
 	t.seed_x = seed_x;
 	t.n_seed = n_seed;
-	hipLaunchKernelGGL((k_extremize<TI>), dim3(S), dim3(64), 0, st, alm, L, find_max ? -1 : 1, l_seed, seed_map, map_x, t, max_iter, theta, phi, value);
+	hipLaunchKernelGGL((k_extremize<TI>), dim3(S), dim3(64), 0, st, alm, 1, L, find_max ? -1 : 1, l_seed, seed_map, map_x, (const double *) NULL, t, max_iter, theta, phi, value);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename TI>
+int launch_extremize_seeded(const cplx<TI> *alm, int S, int alm_div, int L, int find_max, const double *seeds, const double *cm, const double *alpha, int max_iter, double *theta, double *phi, double *value, hipStream_t st)
+{
+	struct peak_tables t;
+	t.cm = cm;
+	t.alpha = alpha;
+	t.seed_x = NULL;
+	t.n_seed = 0;
+	hipLaunchKernelGGL((k_extremize<TI>), dim3(S), dim3(64), 0, st, alm, alm_div, L, find_max ? -1 : 1, 0, (const cplx<TI> *) NULL, (const double *) NULL, seeds, t, max_iter, theta, phi, value);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template int launch_extremize<double>(const cplx<double> *, int, int, int, int, const cplx<double> *, const double *, const double *, const double *, const double *, int, int, double *, double *, double *, hipStream_t);
 template int launch_extremize<float>(const cplx<float> *, int, int, int, int, const cplx<float> *, const double *, const double *, const double *, const double *, int, int, double *, double *, double *, hipStream_t);
+template int launch_extremize_seeded<double>(const cplx<double> *, int, int, int, int, const double *, const double *, const double *, int, double *, double *, double *, hipStream_t);
+template int launch_extremize_seeded<float>(const cplx<float> *, int, int, int, int, const double *, const double *, const double *, int, double *, double *, double *, hipStream_t);

@@ -1,0 +1,308 @@
+"""Learned peak proposal for the three-site cross-power skies at l_max 32.
+
+A multilayer perceptron (1089 real SH coefficients -> 3 x 512 GELU -> 4
+candidate unit vectors, bfloat16 operands, float32 accumulation) proposes
+the sky's peak.  The extremizer refines every candidate on the exact
+series; an event falls back to the full-grid seed plus extremize when its
+best refined value trails the grid maximum by more than a relative 0.01.
+
+Everything runs on the library's kernels (sphradiometer_batch.h): data
+(sphr_inject_bursts), skies (sphr_batch_run), plain peaks
+(sphr_batch_peaks, map-seeded: the label and the judge), features
+(sphr_features), training (sphr_mlp_train_step) and refinement
+(sphr_refine_peaks).
+"""
+import ctypes
+import math
+from ctypes import POINTER as P, c_double, c_float, c_int, c_long, c_uint, c_ulonglong, c_void_p
+
+import numpy as np
+
+from . import (FP64, MIXED, Batch, DeviceBuffer, Network, last_error, lib, synchronize)
+from . import inject
+
+_DONE = False
+
+
+def _protos():
+    global _DONE
+    if _DONE:
+        return
+    L = lib()
+    table = {
+        "sphr_mlp_new": (c_void_p, [c_int, c_int, c_int, c_int, c_int, c_int, c_ulonglong]),
+        "sphr_mlp_free": (None, [c_void_p]),
+        "sphr_mlp_n_params": (c_long, [c_void_p]),
+        "sphr_mlp_set_params": (c_int, [c_void_p, c_void_p]),
+        "sphr_mlp_get_params": (c_int, [c_void_p, c_void_p]),
+        "sphr_mlp_get_grads": (c_int, [c_void_p, c_void_p]),
+        "sphr_mlp_forward": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+        "sphr_mlp_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_float, c_void_p, c_void_p]),
+        "sphr_mlp_adam": (c_int, [c_void_p, c_float, c_float, c_float, c_float, c_int, c_void_p]),
+        "sphr_mlp_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_float, c_float, c_int, c_void_p, c_void_p]),
+        "sphr_mlp_seeds": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+        "sphr_features": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+        "sphr_refine_peaks": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+        "sphr_mlp_gemm_check": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+        "sphr_inject_bursts": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, c_ulonglong, c_void_p, c_void_p]),
+    }
+    for name, (res, args) in table.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _DONE = True
+
+
+def _check(r, what):
+    if r:
+        raise RuntimeError(what + " failed: " + last_error())
+
+
+class MLP:
+    """sphr_mlp: the proposal network."""
+
+    def __init__(self, n_in=1089, width=512, depth=3, n_candidates=4, max_batch=4096, bf16=True, seed=0):
+        _protos()
+        self.h = lib().sphr_mlp_new(n_in, width, depth, n_candidates, max_batch, int(bf16), seed)
+        if not self.h:
+            raise RuntimeError("sphr_mlp_new failed: " + last_error())
+        self.n_in, self.width, self.depth, self.K, self.max_batch = n_in, width, depth, n_candidates, max_batch
+        self.n_params = int(lib().sphr_mlp_n_params(self.h))
+        self.step = 0
+
+    def params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().sphr_mlp_get_params(self.h, out.ctypes.data), "get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().sphr_mlp_set_params(self.h, p.ctypes.data), "set_params")
+
+    def grads(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().sphr_mlp_get_grads(self.h, out.ctypes.data), "get_grads")
+        return out
+
+    def forward(self, d_x, B, d_y=None, stream=None):
+        _check(lib().sphr_mlp_forward(self.h, d_x, B, d_y, stream), "forward")
+
+    def loss_grad(self, d_x, d_t, B, lam, d_loss=None, stream=None):
+        _check(lib().sphr_mlp_loss_grad(self.h, d_x, d_t, B, lam, d_loss, stream), "loss_grad")
+
+    def train_step(self, d_x, d_t, B, lam, lr, d_loss=None, stream=None):
+        self.step += 1
+        _check(lib().sphr_mlp_train_step(self.h, d_x, d_t, B, lam, lr, self.step, d_loss, stream), "train_step")
+
+    def seeds(self, B, d_seeds, stream=None):
+        _check(lib().sphr_mlp_seeds(self.h, B, d_seeds, stream), "seeds")
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().sphr_mlp_free(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+
+def burst_params(n_segments, seed, snr_range=(8.0, 30.0), delta_t=1.0 / 4096, names="ABC"):
+    """Per-segment burst parameters [S][3 + D] (amp, f0, tau, delays) and
+    the injected directions, as inject.burst_batch draws them."""
+    rng = np.random.default_rng(seed)
+    pos = inject.positions(names)
+    D = len(names)
+    f0 = rng.uniform(100.0, 400.0, n_segments)
+    cth = rng.uniform(-1.0, 1.0, n_segments)
+    phi = rng.uniform(0.0, 2 * math.pi, n_segments)
+    snr = rng.uniform(*snr_range, n_segments)
+    tau = 9.0 / (math.sqrt(2.0) * math.pi * f0)
+    amp = snr / np.sqrt(D * tau * math.sqrt(math.pi / 2) / (2 * delta_t))
+    st = np.sqrt(1 - cth * cth)
+    sdir = np.stack([st * np.cos(phi), st * np.sin(phi), cth], axis=1)
+    par = np.empty((n_segments, 3 + D), dtype=np.float32)
+    par[:, 0], par[:, 1], par[:, 2] = amp, f0, tau
+    for d in range(D):
+        par[:, 3 + d] = sdir @ pos[d]
+    return par, {"theta": np.arccos(cth), "phi": phi, "snr": snr}
+
+
+class SkyFactory:
+    """Skies of the three-site workload (N = 1024 at 4096 Hz, L = 32) made
+    on the device in chunks: injected data -> batched skies (a_lm + maps)
+    -> plain peaks (map seed + extremize) -> features and unit targets."""
+
+    def __init__(self, net=None, L=32, N=1024, chunk=8192, precision=MIXED):
+        _protos()
+        self.L, self.N, self.chunk, self.precision = L, N, chunk, precision
+        self.net = net or Network(inject.positions("ABC"), N, 1.0 / 4096, L)
+        self.batch = Batch(self.net, L, precision, max_segments=chunk)
+        self.cdt = np.complex64 if precision == MIXED else np.complex128
+        esz = np.dtype(self.cdt).itemsize
+        n = 2 * (L + 1)
+        self.d_ts = DeviceBuffer(3 * chunk * N * 4)
+        self.d_par = DeviceBuffer(chunk * 6 * 4)
+        self.d_alm = DeviceBuffer(chunk * (L + 1) ** 2 * esz)
+        self.d_map = DeviceBuffer(chunk * n * n * esz)
+        self.d_pk = [DeviceBuffer(chunk * 8) for _ in range(3)]
+
+    def make(self, S, seed, d_feat_out, d_target_out=None, feat_offset=0, keep=False):
+        """S <= chunk skies from seed; features into d_feat_out rows
+        feat_offset.. (float32 [.][(L+1)^2]), unit targets of the plain
+        peak into d_target_out (float32 [.][3]).  Returns host arrays of the
+        plain peaks (theta, phi, value) and, with keep, a_lm and maps."""
+        L = self.L
+        ncoef = (L + 1) ** 2
+        par, inj = burst_params(S, seed)
+        self.d_par.upload(par)
+        _check(lib().sphr_inject_bursts(self.d_par.ptr, 3, S, self.N, 1.0 / 4096, 1.0, seed, self.d_ts.ptr, None), "inject")
+        self.batch.run(self.d_ts.ptr, S, self.d_alm.ptr, self.d_map.ptr, f32_input=True)
+        self.batch.peaks(self.d_alm.ptr, S, self.d_pk[0].ptr, self.d_pk[1].ptr, self.d_pk[2].ptr, d_map=self.d_map.ptr)
+        _check(lib().sphr_features(self.d_alm.ptr, int(self.precision == MIXED), S, L, d_feat_out.ptr + 4 * ncoef * feat_offset, None), "features")
+        synchronize()
+        th, ph, val = (d.to_numpy(np.float64, (S,)) for d in self.d_pk)
+        if d_target_out is not None:
+            t = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1).astype(np.float32)
+            d_target_out.upload(t, offset=4 * 3 * feat_offset)
+        out = {"theta": th, "phi": ph, "value": val, "inj": inj}
+        if keep:
+            out["alm"] = self.d_alm.to_numpy(self.cdt, (S, ncoef))
+            out["map"] = self.d_map.to_numpy(self.cdt, (S, 2 * L + 2, 2 * L + 2))
+        return out
+
+
+def angle(t1, p1, t2, p2):
+    c = np.cos(t1) * np.cos(t2) + np.sin(t1) * np.sin(t2) * np.cos(p1 - p2)
+    return np.arccos(np.clip(c, -1.0, 1.0))
+
+
+def train(mlp, d_x, d_t, n, epochs=8, batch=2048, lr=1e-3, lam=0.05, seed=0, log=None):
+    """Adam over n rows of device features d_x [n][n_in] and unit targets
+    d_t [n][3] (shuffled per epoch by contiguous blocks); returns the mean
+    loss of each epoch."""
+    rng = np.random.default_rng(seed)
+    nb = n // batch
+    d_loss = DeviceBuffer(batch * 4)
+    hist = []
+    for ep in range(epochs):
+        # cosine decay of the step size
+        order = rng.permutation(nb)
+        sampled = []
+        for i, b in enumerate(order):
+            frac = (ep * nb + i) / max(1, epochs * nb)
+            step_lr = lr * 0.5 * (1 + math.cos(math.pi * frac))
+            mlp.train_step(d_x.ptr + 4 * mlp.n_in * b * batch, d_t.ptr + 4 * 3 * b * batch, batch, lam, step_lr, d_loss.ptr)
+            if i % 64 == 0 or i == nb - 1:
+                sampled.append(float(d_loss.to_numpy(np.float32, (batch,)).mean()))
+        hist.append(float(np.mean(sampled)))
+        if log:
+            log("epoch %d loss %.5f" % (ep, hist[-1]))
+    return hist
+
+
+def evaluate(mlp, factory, n_events, seed, rel_fallback=0.01):
+    """A/B on held-out skies: the plain peak (map-seeded extremize) against
+    (a) the reference's method without the network (l = 6 low-pass seed +
+    extremize) and (b) the network's K candidates refined on the exact
+    series, with the full-grid fallback.  Returns per-event separations
+    (rad) from the plain peak and the fallback count."""
+    _protos()
+    L, chunk = factory.L, factory.chunk
+    ncoef = (L + 1) ** 2
+    K = mlp.K
+    d_feat = DeviceBuffer(chunk * ncoef * 4)
+    d_seeds = DeviceBuffer(chunk * K * 16)
+    d_r = [DeviceBuffer(chunk * K * 8) for _ in range(3)]
+    d_l6 = [DeviceBuffer(chunk * 8) for _ in range(3)]
+    sep_ref, sep_mlp, sep_final, n_fallback = [], [], [], 0
+    done = 0
+    while done < n_events:
+        S = min(chunk, n_events - done)
+        plain = factory.make(S, seed + done, d_feat)
+        # (a) reference seeding
+        factory.batch.peaks(factory.d_alm.ptr, S, d_l6[0].ptr, d_l6[1].ptr, d_l6[2].ptr, d_map=None)
+        # (b) proposals refined
+        for r0 in range(0, S, mlp.max_batch):
+            B = min(mlp.max_batch, S - r0)
+            mlp.forward(d_feat.ptr + 4 * ncoef * r0, B)
+            mlp.seeds(B, d_seeds.ptr + 16 * K * r0)
+        esz = np.dtype(factory.cdt).itemsize
+        _check(lib().sphr_refine_peaks(factory.d_alm.ptr, int(factory.precision == MIXED), S * K, K, L, d_seeds.ptr, 1,
+                                       d_r[0].ptr, d_r[1].ptr, d_r[2].ptr, None), "refine")
+        synchronize()
+        th6, ph6 = (d.to_numpy(np.float64, (S,)) for d in d_l6[:2])
+        rt, rp, rv = (d.to_numpy(np.float64, (S, K)) for d in d_r)
+        best = np.argmax(rv, axis=1)
+        bt, bp, bv = rt[np.arange(S), best], rp[np.arange(S), best], rv[np.arange(S), best]
+        sep_ref.append(angle(th6, ph6, plain["theta"], plain["phi"]))
+        sep_mlp.append(angle(bt, bp, plain["theta"], plain["phi"]))
+        # fallback: the grid maximum is the plain path's seed value; the
+        # plain peak value bounds it from above
+        fb = bv < plain["value"] - rel_fallback * np.abs(plain["value"])
+        n_fallback += int(fb.sum())
+        ft = np.where(fb, plain["theta"], bt)
+        fp = np.where(fb, plain["phi"], bp)
+        sep_final.append(angle(ft, fp, plain["theta"], plain["phi"]))
+        done += S
+    return {"reference_seed": np.concatenate(sep_ref), "proposal": np.concatenate(sep_mlp),
+            "final": np.concatenate(sep_final), "fallbacks": n_fallback}
+
+
+def summary(res, tol_deg=0.1):
+    tol = math.radians(tol_deg)
+    out = {}
+    for k in ("reference_seed", "proposal", "final"):
+        s = res[k]
+        out[k] = {"within_%.2f_deg" % tol_deg: float(np.mean(s <= tol)), "median_deg": float(np.degrees(np.median(s))),
+                  "p99_deg": float(np.degrees(np.quantile(s, 0.99)))}
+    out["fallback_fraction"] = res["fallbacks"] / len(res["final"])
+    return out
+
+
+def main():
+    import argparse
+    import json
+    import sys
+    import time
+    ap = argparse.ArgumentParser(description="train and A/B-test the learned peak proposal")
+    ap.add_argument("--train", type=int, default=1 << 20)
+    ap.add_argument("--eval", type=int, default=65536)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--chunk", type=int, default=8192)
+    args = ap.parse_args()
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    L = 32
+    ncoef = (L + 1) ** 2
+    fac = SkyFactory(chunk=args.chunk)
+    d_x = DeviceBuffer(args.train * ncoef * 4)
+    d_t = DeviceBuffer(args.train * 3 * 4)
+    t0 = time.time()
+    for off in range(0, args.train, args.chunk):
+        S = min(args.chunk, args.train - off)
+        fac.make(S, 1_000_000 + off, d_x, d_t, feat_offset=off)
+        if (off // args.chunk) % 16 == 0:
+            log("training skies %d / %d (%.1f s)" % (off + S, args.train, time.time() - t0))
+    gen_s = time.time() - t0
+    mlp = MLP(max_batch=max(args.batch, 4096), bf16=True, seed=7)
+    t0 = time.time()
+    hist = train(mlp, d_x, d_t, args.train, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+    synchronize()
+    train_s = time.time() - t0
+    d_x.free()
+    d_t.free()
+    t0 = time.time()
+    res = evaluate(mlp, fac, args.eval, seed=5_000_000)
+    eval_s = time.time() - t0
+    out = {"train_maps": args.train, "eval_events": args.eval, "epochs": args.epochs, "batch": args.batch,
+           "loss_per_epoch": [round(h, 6) for h in hist], "sky_generation_s": round(gen_s, 2),
+           "train_s": round(train_s, 2), "eval_s": round(eval_s, 2), "ab": summary(res)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
