@@ -143,6 +143,36 @@ int sphr_inject_bursts(const float *params, int n_det, int n_segments, int n, do
 /* test hook: C = op(A) op(B) of the network's GEMM (device float32) */
 int sphr_mlp_gemm_check(int bf16, int ta, int tb, int M, int N, int K, const float *A, const float *B, float *C);
 
+/*
+ * Batches of independent jobs (no counterpart in the reference): job j
+ * has its own network of n_det[j] sites (positions in light-seconds,
+ * [sum n_det][3]), segment length N[j] and sky order l_max[j], with one
+ * real segment each; the result per job is what
+ * correlator_network_plan_fd_new + _set_l(l_max) + tseries_to_fseries +
+ * correlator_network_integrate_power_fd + to_mesh give for it.  Plans are
+ * generated on the GPU by sphr_jobs_new (the lag-domain form of the
+ * frequency-domain plan, (2 n_p - 1) x (l + 1) values per baseline); each
+ * baseline's tap count n_p must not exceed its job's N.
+ *
+ * Layouts (device, packed in job order; offsets from sphr_jobs_offsets):
+ *   in   float32, job j at in[j]: [n_det][N]
+ *   alm  complex (float64 or float32 pairs), job j at alm[j]: (L+1)^2
+ *   map  complex, job j at map[j]: [2(L+1)][2(L+1)], may be NULL
+ * sphr_jobs_run replays a captured HIP graph while the buffers and the
+ * stream stay the same.
+ */
+struct sphr_jobs;
+struct sphr_jobs *sphr_jobs_new(int n_jobs, const int *n_det, const double *positions, const int *N, const int *l_max, double delta_t, int precision);
+void sphr_jobs_free(struct sphr_jobs *);
+int sphr_jobs_run(struct sphr_jobs *, const float *in, void *alm, void *map, void *stream);
+/* autotuner: times the lag-correlation tiles on these buffers, keeps the
+ * fastest; returns the chosen candidate index, its time per run */
+int sphr_jobs_tune(struct sphr_jobs *, const float *in, void *alm, double *best_ms);
+/* [0] input floats, [1] a_lm elements, [2] map elements, [3] baselines,
+ * [4] l_max classes, [5..6] lag tile, [7] plan bytes */
+void sphr_jobs_info(const struct sphr_jobs *, long info[8]);
+void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map);
+
 /* Shape information: [0] detectors, [1] baselines, [2] N, [3] sky l_max,
  * [4] projection columns, [5] K splits, [6] precision */
 void sphr_batch_info(const struct sphr_batch *, int info[8]);

@@ -202,6 +202,12 @@ _PROTOS = {
     "sphr_batch_profile_read": (c_int, [c_void_p, P(c_double), P(c_int)]),
     "sphr_batch_info": (None, [c_void_p, P(c_int)]),
     "sphr_batch_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sphr_jobs_new": (c_void_p, [c_int, P(c_int), P(c_double), P(c_int), P(c_int), c_double, c_int]),
+    "sphr_jobs_free": (None, [c_void_p]),
+    "sphr_jobs_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sphr_jobs_tune": (c_int, [c_void_p, c_void_p, c_void_p, P(c_double)]),
+    "sphr_jobs_info": (None, [c_void_p, P(ctypes.c_long)]),
+    "sphr_jobs_offsets": (None, [c_void_p, P(ctypes.c_long), P(ctypes.c_long), P(ctypes.c_long)]),
     "precalculated_TimeSeries_length": (ctypes.c_long, [ctypes.c_long, c_double]),
     "sh_series_log_uniformsky_prior": (P(ShSeries), [c_int]),
     "correlator_network_plan_mult_by_projection": (c_int, [P(NetworkPlanFD), c_double, c_double, P(P(c_double))]),
@@ -586,6 +592,53 @@ class Localizer:
         try:
             if getattr(self, "h", None):
                 lib().sphr_loc_free(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+
+class Jobs:
+    """sphradiometer_batch.h sphr_jobs_*: a batch of independent jobs with
+    their own networks, lengths and orders (plans made on the GPU)."""
+
+    def __init__(self, jobs, delta_t, precision=FP64):
+        n = len(jobs)
+        nd = (c_int * n)(*[len(j["positions"]) for j in jobs])
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(j["positions"], dtype=np.float64) for j in jobs]))
+        Ns = (c_int * n)(*[int(j["N"]) for j in jobs])
+        Ls = (c_int * n)(*[int(j["l_max"]) for j in jobs])
+        self.h = lib().sphr_jobs_new(n, nd, pos.ctypes.data_as(P(c_double)), Ns, Ls, delta_t, precision)
+        if not self.h:
+            raise RuntimeError("sphr_jobs_new failed: " + last_error())
+        self.n = n
+        self.precision = precision
+        self.l_max = [int(j["l_max"]) for j in jobs]
+        a = (ctypes.c_long * n)()
+        b = (ctypes.c_long * n)()
+        c = (ctypes.c_long * n)()
+        lib().sphr_jobs_offsets(self.h, a, b, c)
+        self.off_in, self.off_alm, self.off_map = list(a), list(b), list(c)
+
+    def info(self):
+        a = (ctypes.c_long * 8)()
+        lib().sphr_jobs_info(self.h, a)
+        return dict(zip(["in_floats", "alm_elems", "map_elems", "baselines", "l_classes", "lag_samples", "lag_threads", "plan_bytes"], list(a)))
+
+    def run(self, d_in, d_alm, d_map=None, stream=None):
+        if lib().sphr_jobs_run(self.h, d_in, d_alm, d_map, stream):
+            raise RuntimeError("sphr_jobs_run failed: " + last_error())
+
+    def tune(self, d_in, d_alm):
+        ms = c_double()
+        k = lib().sphr_jobs_tune(self.h, d_in, d_alm, ctypes.byref(ms))
+        if k < 0:
+            raise RuntimeError("sphr_jobs_tune failed: " + last_error())
+        return k, ms.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().sphr_jobs_free(self.h)
         except Exception:
             pass
         self.h = None

@@ -1,0 +1,761 @@
+/*
+ * Batches of independent sky-map jobs, each with its own detector network,
+ * segment length N and sky order L (the uneven workload): plan generation
+ * on the GPU and a lag-domain data path, launched as one captured HIP
+ * graph.
+ *
+ * The frequency-domain plan of a baseline (correlator.c:424-519, then
+ * set_l to Lt = min(P, L)),
+ *   dp[(N-f) mod N]_d = (2 pi / N) sum_q w_q conj(A_f(x_q)) B_f(x_q) Pbar_d0(x_q),
+ *   A_f(x) = sum_i e^{-2 pi i f i / N} T_i(x),  T_i(x) = sum_{l<=a} C_il Pbar_l0(x),
+ * with C the tap projections (projection.c:99-131), is used by
+ * power_1d = (1/N) sum_f dp[f] X[f], X = F_b conj(F_a).  Summing over f
+ * first turns this into lags of the circular cross-correlation
+ *   c[u] = sum_f X[f] e^{2 pi i f u / N} = N sum_s x_a[s] x_b[(s+u) mod N],
+ *   power_1d[d] = sum_{|u| < n_p} c[u mod N] M[u][d],
+ *   M[u][d] = (2 pi / N^2) sum_q w_q Pbar_d0(x_q) sum_i T_i(x_q) T'_{i+u}(x_q),
+ * where T' are the taps of the other phase centre, T'_i(x) = T_i(-x).  The
+ * identity is exact (a reordering of finite sums); the plan per baseline
+ * is (2 n_p - 1) x (Lt + 1) real numbers instead of N x (Lt + 1) complex
+ * ones, and no FFT is needed.  q runs over an n_q-point Gauss-Legendre
+ * rule with 2 n_q - 1 >= 2a + Lt, exact for the polynomial integrand, as
+ * the reference's microcode and pixel-domain products are.
+ *
+ * Plan generation (float64, per baseline): Legendre tables on the tap and
+ * product nodes; C (sinc samples x tables); T (C x tables); the lag
+ * correlation K of T with its mirror; M (K x weighted tables); the polar
+ * rotation rows D^l_{0m}(baseline) / nb.  Data path: lag correlations
+ * (LDS-tiled), M and rotation per job into a_lm, synthesis per l_max
+ * class with gathered offsets.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sphradiometer/sphradiometer_batch.h>
+#include "../sphr_internal.h"
+#include "kernels.h"
+#include "runtime.h"
+#include "synth_tables.h"
+
+struct jb_bl {
+	int job, ia, ib;
+	int a, np, nt, nq, Lt, lags, L;
+	double r_dt, theta, phi, inv_nb, scale;	/* scale = 1 / N^2 */
+	long off_nt, off_nq;		/* node arrays (x then w) in the chunk's node buffer */
+	long off_Pt, off_Pq, off_C, off_T, off_K;	/* chunk workspace (doubles) */
+	long off_M, off_R, off_c, off_pw;	/* persistent: M (T), R (cplx T), c (T), power slot */
+};
+
+struct jb_job {
+	int D, N, L, nb, first_bl;
+	long off_in, off_alm, off_map;
+};
+
+struct jb_class {
+	int L, count;
+	long *d_aoff, *d_moff;
+	struct synth_dev syn;
+};
+
+struct sphr_jobs {
+	int precision, n_jobs, n_bl, n_classes;
+	double delta_t;
+	struct jb_job *jobs;
+	struct jb_bl *bl;
+	struct jb_job *d_jobs;
+	struct jb_bl *d_bl;
+	void *M, *R, *c;
+	long n_in, n_alm, n_map, n_pw_max;
+	struct jb_class *cls;
+	int lag_sc, lag_nt;		/* tuned lag-kernel tile */
+	hipGraph_t graph;
+	hipGraphExec_t exec;
+	const void *g_in;
+	void *g_alm, *g_map;
+	hipStream_t g_st;
+};
+
+/* ---------------------------------------------------------------- plan */
+
+__device__ inline double sinc_pi(double x)
+{
+	const double px = M_PI * x;
+	return px == 0.0 ? 1.0 : sin(px) / px;
+}
+
+/* normalised Legendre Pbar_l0(x), l = 0..lmax, into row[0..lmax] */
+__device__ inline void pbar_row(double x, int lmax, double *row)
+{
+	double p0 = 0.28209479177387814347, p1 = 0.0, am1 = 1.0;
+	row[0] = p0;
+	for(int l = 1; l <= lmax; l++) {
+		const double al = sqrt((4.0 * l * l - 1.0) / ((double) l * l));
+		const double pn = l == 1 ? al * x * p0 : al * (x * p0 - p1 / am1);
+		p1 = p0;
+		p0 = pn;
+		am1 = al;
+		row[l] = pn;
+	}
+}
+
+__global__ void k_pg_tables(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, double *__restrict__ ws)
+{
+	const struct jb_bl &B = bl[b0 + blockIdx.x];
+	const int lq = B.a > B.Lt ? B.a : B.Lt;
+	for(int i = threadIdx.x; i < B.nt + B.nq; i += blockDim.x) {
+		if(i < B.nt)
+			pbar_row(nodes[B.off_nt + i], B.a, ws + B.off_Pt + (size_t) i * (B.a + 1));
+		else {
+			const int q = i - B.nt;
+			pbar_row(nodes[B.off_nq + q], lq, ws + B.off_Pq + (size_t) q * (lq + 1));
+		}
+	}
+}
+
+#define PG_TC 8		/* taps per workgroup */
+
+/* C[i][l] = sum_j (f_ij nt)(w_j 2 pi / nt) Pbar_l0(x_j), f_ij the delay
+ * element of tap k = i - (np-1)/2 at node j (projection.c:99-107) */
+__global__ void k_pg_taps(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, double *__restrict__ ws)
+{
+	extern __shared__ double pg_sm[];
+	const struct jb_bl &B = bl[b0 + blockIdx.y];
+	const int i0 = blockIdx.x * PG_TC;
+	if(i0 >= B.np)
+		return;
+	double *F = pg_sm;	/* [PG_TC][nt] */
+	const double *x = nodes + B.off_nt, *w = x + B.nt;
+	for(int e = threadIdx.x; e < PG_TC * B.nt; e += blockDim.x) {
+		const int t = e / B.nt, j = e - t * B.nt;
+		const int i = i0 + t;
+		double v = 0.0;
+		if(i < B.np) {
+			const int k = -(B.np - 1) / 2 + i;
+			const double f = sinc_pi(0 - k - B.r_dt * cos(acos(x[j])));
+			v = (f * (double) B.nt) * (w[j] * (2. * M_PI / B.nt));
+		}
+		F[e] = v;
+	}
+	__syncthreads();
+	const double *Pt = ws + B.off_Pt;
+	for(int e = threadIdx.x; e < PG_TC * (B.a + 1); e += blockDim.x) {
+		const int t = e / (B.a + 1), l = e - t * (B.a + 1);
+		if(i0 + t >= B.np)
+			continue;
+		double acc = 0.0;
+		for(int j = 0; j < B.nt; j++)
+			acc += F[t * B.nt + j] * Pt[(size_t) j * (B.a + 1) + l];
+		ws[B.off_C + (size_t) (i0 + t) * (B.a + 1) + l] = acc;
+	}
+}
+
+/* T[i][q] = sum_{l <= a} C[i][l] Pbar_l0(x_q) */
+__global__ void k_pg_synth(const struct jb_bl *__restrict__ bl, int b0, double *__restrict__ ws)
+{
+	extern __shared__ double pg_sm[];
+	const struct jb_bl &B = bl[b0 + blockIdx.y];
+	const int i0 = blockIdx.x * PG_TC;
+	if(i0 >= B.np)
+		return;
+	const int lq = B.a > B.Lt ? B.a : B.Lt;
+	double *Cs = pg_sm;	/* [PG_TC][a+1] */
+	for(int e = threadIdx.x; e < PG_TC * (B.a + 1); e += blockDim.x) {
+		const int t = e / (B.a + 1);
+		Cs[e] = i0 + t < B.np ? ws[B.off_C + (size_t) i0 * (B.a + 1) + e] : 0.0;
+	}
+	__syncthreads();
+	const double *Pq = ws + B.off_Pq;
+	for(int e = threadIdx.x; e < PG_TC * B.nq; e += blockDim.x) {
+		const int t = e / B.nq, q = e - t * B.nq;
+		if(i0 + t >= B.np)
+			continue;
+		double acc = 0.0;
+		for(int l = 0; l <= B.a; l++)
+			acc += Cs[t * (B.a + 1) + l] * Pq[(size_t) q * (lq + 1) + l];
+		ws[B.off_T + (size_t) (i0 + t) * B.nq + q] = acc;
+	}
+}
+
+/* K[u][q] = sum_i T[i][q] T[i+u][nq-1-q], u = uu - (np-1) */
+__global__ void k_pg_lag(const struct jb_bl *__restrict__ bl, int b0, double *__restrict__ ws)
+{
+	const struct jb_bl &B = bl[b0 + blockIdx.y];
+	const long e = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if(e >= (long) B.lags * B.nq)
+		return;
+	const int uu = (int) (e / B.nq), q = (int) (e - (long) uu * B.nq);
+	const int u = uu - (B.np - 1);
+	const double *T = ws + B.off_T;
+	const int lo = u < 0 ? -u : 0, hi = u > 0 ? B.np - u : B.np;
+	double acc = 0.0;
+	for(int i = lo; i < hi; i++)
+		acc += T[(size_t) i * B.nq + q] * T[(size_t) (i + u) * B.nq + (B.nq - 1 - q)];
+	ws[B.off_K + e] = acc;
+}
+
+/* M[u][d] = scale sum_q (2 pi w_q) K[u][q] Pbar_d0(x_q), d <= Lt */
+template <typename T>
+__global__ void k_pg_M(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, const double *__restrict__ ws, T *__restrict__ M)
+{
+	const struct jb_bl &B = bl[b0 + blockIdx.y];
+	const long e = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if(e >= (long) B.lags * (B.Lt + 1))
+		return;
+	const int uu = (int) (e / (B.Lt + 1)), d = (int) (e - (long) uu * (B.Lt + 1));
+	const int lq = B.a > B.Lt ? B.a : B.Lt;
+	const double *K = ws + B.off_K + (size_t) uu * B.nq, *Pq = ws + B.off_Pq, *w = nodes + B.off_nq + B.nq;
+	double acc = 0.0;
+	for(int q = 0; q < B.nq; q++)
+		acc += (2. * M_PI * w[q]) * K[q] * Pq[(size_t) q * (lq + 1) + d];
+	M[B.off_M + e] = (T) (acc * B.scale);
+}
+
+/* R[(l, m)] = D^l_{0m} / nb = sqrt(4 pi / (2l+1)) conj(Y_lm(theta, phi)) / nb
+ * for the sky order L of the job (zero above Lt is handled by the user) */
+template <typename T>
+__global__ void k_pg_rot(const struct jb_bl *__restrict__ bl, int b0, cplx<T> *__restrict__ R)
+{
+	const struct jb_bl &B = bl[b0 + blockIdx.x];
+	const int L = B.L;
+	const double x = cos(B.theta), s = sin(B.theta);
+	for(int mi = threadIdx.x; mi < 2 * L + 1; mi += blockDim.x) {
+		const int m = mi - L, am = m < 0 ? -m : m;
+		/* Pbar_mm by the sectoral recurrence */
+		double pmm = 0.28209479177387814347;
+		for(int k = 1; k <= am; k++)
+			pmm *= -sqrt((2.0 * k + 1.0) / (2.0 * k)) * s;
+		double sn, cs;
+		sincos(-(double) m * B.phi, &sn, &cs);
+		const double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;
+		double p0 = pmm, p1 = 0.0, am1 = 1.0;
+		for(int l = am; l <= L; l++) {
+			if(l > am) {
+				const double al = sqrt((4.0 * l * l - 1.0) / ((double) l * l - (double) am * am));
+				const double pn = l == am + 1 ? al * x * p0 : al * (x * p0 - p1 / am1);
+				p1 = p0;
+				p0 = pn;
+				am1 = al;
+			}
+			const double f = sqrt(4 * M_PI / (2 * l + 1)) * sg * p0 * B.inv_nb;
+			R[B.off_R + lm_offset(L, l, m)] = mk((T) (f * cs), (T) (f * sn));
+		}
+	}
+}
+
+/* ---------------------------------------------------------- data path */
+
+/* c[u] = N sum_s x_a[s] x_b[(s+u) mod N] for the baseline's lags; LDS
+ * tiles of sc samples, nt lags per workgroup */
+template <typename T>
+__global__ void k_lagcorr(const struct jb_job *__restrict__ jobs, const struct jb_bl *__restrict__ bl, const float *__restrict__ in, int sc, T *__restrict__ c)
+{
+	extern __shared__ unsigned char lag_sm_raw[];
+	T *sa = reinterpret_cast<T *>(lag_sm_raw);	/* [sc] */
+	T *sb = sa + sc;				/* [sc + nt - 1] */
+	const struct jb_bl &B = bl[blockIdx.y];
+	const struct jb_job &J = jobs[B.job];
+	const int nt = blockDim.x;
+	const int uu0 = blockIdx.x * nt;
+	if(uu0 >= B.lags)
+		return;
+	const int N = J.N;
+	const int u0 = uu0 - (B.np - 1);
+	const float *xa = in + J.off_in + (size_t) B.ia * N, *xb = in + J.off_in + (size_t) B.ib * N;
+	T acc = 0;
+	for(int s0 = 0; s0 < N; s0 += sc) {
+		const int len = N - s0 < sc ? N - s0 : sc;
+		for(int t = threadIdx.x; t < sc; t += nt)
+			sa[t] = t < len ? (T) xa[s0 + t] : (T) 0;
+		for(int t = threadIdx.x; t < sc + nt - 1; t += nt) {
+			/* x_b at s0 + t + u0, circular */
+			long idx = ((long) s0 + t + u0) % N;
+			if(idx < 0)
+				idx += N;
+			sb[t] = (T) xb[idx];
+		}
+		__syncthreads();
+		for(int t = 0; t < len; t++)
+			acc += sa[t] * sb[t + threadIdx.x];
+		__syncthreads();
+	}
+	const int uu = uu0 + threadIdx.x;
+	if(uu < B.lags)
+		c[B.off_c + uu] = acc * (T) N;
+}
+
+/* per job: power_1d_b = c_b M_b, a_lm = sum_b R_b power_1d_b[l] */
+template <typename T>
+__global__ void k_job_alm(const struct jb_job *__restrict__ jobs, const struct jb_bl *__restrict__ bl, const T *__restrict__ c, const T *__restrict__ M, const cplx<T> *__restrict__ R, cplx<T> *__restrict__ alm)
+{
+	extern __shared__ unsigned char alm_sm_raw[];
+	T *pw = reinterpret_cast<T *>(alm_sm_raw);
+	const struct jb_job &J = jobs[blockIdx.x];
+	int tot = 0;
+	for(int b = 0; b < J.nb; b++)
+		tot += bl[J.first_bl + b].Lt + 1;
+	for(int e = threadIdx.x; e < tot; e += blockDim.x) {
+		int b = 0, d = e;
+		while(d > bl[J.first_bl + b].Lt) {
+			d -= bl[J.first_bl + b].Lt + 1;
+			b++;
+		}
+		const struct jb_bl &B = bl[J.first_bl + b];
+		const T *cb = c + B.off_c, *Mb = M + B.off_M + d;
+		T acc = 0;
+		for(int uu = 0; uu < B.lags; uu++)
+			acc += cb[uu] * Mb[(size_t) uu * (B.Lt + 1)];
+		pw[e] = acc;
+	}
+	__syncthreads();
+	const int L = J.L, ncoef = (L + 1) * (L + 1);
+	for(int idx = threadIdx.x; idx < ncoef; idx += blockDim.x) {
+		int l, m;
+		lm_decode(L, idx, &l, &m);
+		cplx<T> acc = mk((T) 0, (T) 0);
+		int off = 0;
+		for(int b = 0; b < J.nb; b++) {
+			const struct jb_bl &B = bl[J.first_bl + b];
+			if(l <= B.Lt)
+				cfma(acc, R[B.off_R + idx], pw[off + l]);
+			off += B.Lt + 1;
+		}
+		alm[J.off_alm + idx] = acc;
+	}
+}
+
+/* ---------------------------------------------------------------- host */
+
+static unsigned pl_power_l(double D, double dt)
+{
+	return (unsigned) floor(M_PI / (2 * asin(dt / (2 * D))) + 1);
+}
+
+extern "C" void sphr_jobs_free(struct sphr_jobs *J)
+{
+	if(!J)
+		return;
+	if(J->exec)
+		hipGraphExecDestroy(J->exec);
+	if(J->graph)
+		hipGraphDestroy(J->graph);
+	void *p[] = { J->d_jobs, J->d_bl, J->M, J->R, J->c };
+	for(unsigned i = 0; i < sizeof(p) / sizeof(p[0]); i++)
+		if(p[i])
+			hipFree(p[i]);
+	for(int k = 0; k < J->n_classes; k++) {
+		if(J->cls[k].d_aoff)
+			hipFree(J->cls[k].d_aoff);
+		if(J->cls[k].d_moff)
+			hipFree(J->cls[k].d_moff);
+		synth_tables_free(&J->cls[k].syn);
+	}
+	free(J->cls);
+	free(J->jobs);
+	free(J->bl);
+	free(J);
+}
+
+template <typename T>
+static int generate_plans(struct sphr_jobs *J, hipStream_t st)
+{
+	/* plan-generation workspace in chunks of baselines */
+	const size_t budget = (size_t) 1 << 30;
+	double *ws = NULL, *nodes = NULL, *hnodes = NULL;
+	size_t ws_cap = 0, nd_cap = 0;
+	int rc = -1;
+	int b0 = 0;
+	while(b0 < J->n_bl) {
+		/* choose the chunk and lay out its workspace */
+		size_t wsz = 0, nsz = 0;
+		int b1 = b0;
+		while(b1 < J->n_bl) {
+			struct jb_bl &B = J->bl[b1];
+			const int lq = B.a > B.Lt ? B.a : B.Lt;
+			const size_t need = (size_t) B.nt * (B.a + 1) + (size_t) B.nq * (lq + 1) + (size_t) B.np * (B.a + 1) + (size_t) B.np * B.nq + (size_t) B.lags * B.nq;
+			if(b1 > b0 && (wsz + need) * sizeof(double) > budget)
+				break;
+			B.off_Pt = wsz;
+			B.off_Pq = B.off_Pt + (long) B.nt * (B.a + 1);
+			B.off_C = B.off_Pq + (long) B.nq * (lq + 1);
+			B.off_T = B.off_C + (long) B.np * (B.a + 1);
+			B.off_K = B.off_T + (long) B.np * B.nq;
+			wsz += need;
+			B.off_nt = nsz;
+			nsz += 2 * (size_t) B.nt;
+			B.off_nq = nsz;
+			nsz += 2 * (size_t) B.nq;
+			b1++;
+		}
+		if(wsz > ws_cap) {
+			if(ws)
+				hipFree(ws);
+			ws = NULL;
+			if(sphr_rt_check(hipMalloc(&ws, wsz * sizeof(double)), "hipMalloc"))
+				goto done;
+			ws_cap = wsz;
+		}
+		if(nsz > nd_cap) {
+			if(nodes)
+				hipFree(nodes);
+			nodes = NULL;
+			free(hnodes);
+			hnodes = (double *) malloc(nsz * sizeof(double));
+			if(!hnodes || sphr_rt_check(hipMalloc(&nodes, nsz * sizeof(double)), "hipMalloc"))
+				goto done;
+			nd_cap = nsz;
+		}
+		/* Gauss-Legendre rules: the tap mesh of order a lists cos theta
+		 * descending (sh_series_trans.c:149-196); the product rule is
+		 * any exact rule */
+		for(int b = b0; b < b1; b++) {
+			const struct jb_bl &B = J->bl[b];
+			const double *x, *w;
+			sphr_mesh_rule((unsigned) B.a, &x, &w);
+			memcpy(hnodes + B.off_nt, x, B.nt * sizeof(double));
+			memcpy(hnodes + B.off_nt + B.nt, w, B.nt * sizeof(double));
+			sphr_gl_rule(B.nq, hnodes + B.off_nq, hnodes + B.off_nq + B.nq);
+		}
+		if(sphr_rt_check(hipMemcpyAsync(nodes, hnodes, nsz * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpy")
+		   || sphr_rt_check(hipMemcpyAsync(J->d_bl + b0, J->bl + b0, (size_t) (b1 - b0) * sizeof(struct jb_bl), hipMemcpyHostToDevice, st), "hipMemcpy"))
+			goto done;
+		{
+			const int nb = b1 - b0;
+			int np_max = 0, a_max = 0, nt_max = 0, lags_nq_max = 0, lags_lt_max = 0;
+			for(int b = b0; b < b1; b++) {
+				const struct jb_bl &B = J->bl[b];
+				np_max = B.np > np_max ? B.np : np_max;
+				a_max = B.a > a_max ? B.a : a_max;
+				nt_max = B.nt > nt_max ? B.nt : nt_max;
+				lags_nq_max = B.lags * B.nq > lags_nq_max ? B.lags * B.nq : lags_nq_max;
+				lags_lt_max = B.lags * (B.Lt + 1) > lags_lt_max ? B.lags * (B.Lt + 1) : lags_lt_max;
+			}
+			const int ntap = (np_max + PG_TC - 1) / PG_TC;
+			hipLaunchKernelGGL(k_pg_tables, dim3(nb), dim3(256), 0, st, J->d_bl, b0, nodes, ws);
+			hipLaunchKernelGGL(k_pg_taps, dim3(ntap, nb), dim3(256), PG_TC * nt_max * sizeof(double), st, J->d_bl, b0, nodes, ws);
+			hipLaunchKernelGGL(k_pg_synth, dim3(ntap, nb), dim3(256), PG_TC * (a_max + 1) * sizeof(double), st, J->d_bl, b0, ws);
+			hipLaunchKernelGGL(k_pg_lag, dim3((lags_nq_max + 255) / 256, nb), dim3(256), 0, st, J->d_bl, b0, ws);
+			hipLaunchKernelGGL((k_pg_M<T>), dim3((lags_lt_max + 255) / 256, nb), dim3(256), 0, st, J->d_bl, b0, nodes, ws, (T *) J->M);
+			hipLaunchKernelGGL((k_pg_rot<T>), dim3(nb), dim3(128), 0, st, J->d_bl, b0, (cplx<T> *) J->R);
+			if(sphr_rt_check(hipGetLastError(), "plan kernels") || sphr_rt_check(hipStreamSynchronize(st), "plan"))
+				goto done;
+		}
+		b0 = b1;
+	}
+	rc = 0;
+done:
+	if(ws)
+		hipFree(ws);
+	if(nodes)
+		hipFree(nodes);
+	free(hnodes);
+	return rc;
+}
+
+extern "C" struct sphr_jobs *sphr_jobs_new(int n_jobs, const int *n_det, const double *positions, const int *N, const int *l_max, double delta_t, int precision)
+{
+	if(sphr_gpu_ensure())
+		return NULL;
+	if(n_jobs < 1 || !n_det || !positions || !N || !l_max || delta_t <= 0. || (precision != SPHR_FP64 && precision != SPHR_MIXED)) {
+		sphr_set_error("sphr_jobs_new: bad arguments");
+		return NULL;
+	}
+	struct sphr_jobs *J = (struct sphr_jobs *) calloc(1, sizeof(*J));
+	if(!J)
+		return NULL;
+	J->precision = precision;
+	J->n_jobs = n_jobs;
+	J->delta_t = delta_t;
+	J->lag_sc = 1024;
+	J->lag_nt = 256;
+	J->jobs = (struct jb_job *) calloc(n_jobs, sizeof(struct jb_job));
+	int nbl = 0;
+	for(int j = 0; j < n_jobs; j++) {
+		if(n_det[j] < 2 || n_det[j] > 16 || N[j] < 2 || l_max[j] < 0) {
+			sphr_set_error("sphr_jobs_new: job %d: %d detectors, N %d, l_max %d", j, n_det[j], N[j], l_max[j]);
+			sphr_jobs_free(J);
+			return NULL;
+		}
+		nbl += n_det[j] * (n_det[j] - 1) / 2;
+	}
+	J->n_bl = nbl;
+	J->bl = (struct jb_bl *) calloc(nbl, sizeof(struct jb_bl));
+	if(!J->jobs || !J->bl) {
+		sphr_jobs_free(J);
+		return NULL;
+	}
+	/* layout: inputs, outputs, baselines and their plan / lag slots */
+	long in = 0, aoff = 0, moff = 0, moff_M = 0, off_R = 0, off_c = 0;
+	int b = 0, pdet = 0;
+	const size_t esz = precision == SPHR_MIXED ? 4 : 8;
+	for(int j = 0; j < n_jobs; j++) {
+		struct jb_job &Jb = J->jobs[j];
+		Jb.D = n_det[j];
+		Jb.N = N[j];
+		Jb.L = l_max[j];
+		Jb.nb = Jb.D * (Jb.D - 1) / 2;
+		Jb.first_bl = b;
+		Jb.off_in = in;
+		Jb.off_alm = aoff;
+		Jb.off_map = moff;
+		in += (long) Jb.D * Jb.N;
+		aoff += (long) (Jb.L + 1) * (Jb.L + 1);
+		moff += 4L * (Jb.L + 1) * (Jb.L + 1);
+		/* baselines in the reference order: i = 1..D-1, j < i, d = r_i - r_j
+		 * (correlator.c:728-757) */
+		for(int i = 1; i < Jb.D; i++)
+			for(int k = 0; k < i; k++, b++) {
+				struct jb_bl &B = J->bl[b];
+				const double *ri = positions + 3 * (pdet + i), *rk = positions + 3 * (pdet + k);
+				const double d[3] = { ri[0] - rk[0], ri[1] - rk[1], ri[2] - rk[2] };
+				const double D = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+				if(D <= 0.) {
+					sphr_set_error("sphr_jobs_new: job %d has coincident sites", j);
+					sphr_jobs_free(J);
+					return NULL;
+				}
+				B.job = j;
+				B.ia = i;
+				B.ib = k;
+				B.a = (int) (M_PI * (D / 2) / delta_t + 1);
+				B.np = 2 * (int) (60 + (D / 2) / delta_t + 1) + 1;
+				if(B.np > Jb.N) {
+					sphr_set_error("sphr_jobs_new: job %d: %d taps exceed N = %d", j, B.np, Jb.N);
+					sphr_jobs_free(J);
+					return NULL;
+				}
+				B.nt = 2 * (B.a + 1);
+				unsigned P = pl_power_l(D, delta_t);
+				if(P > 2u * B.a)
+					P = 2 * B.a;
+				B.Lt = (int) P < Jb.L ? (int) P : Jb.L;
+				B.L = Jb.L;
+				B.nq = (2 * B.a + B.Lt + 2) / 2 + 1;
+				B.lags = 2 * B.np - 1;
+				B.r_dt = (D / 2) / delta_t;
+				/* vector_direction (instrument.c:90-100) */
+				B.theta = atan2(sqrt(d[0] * d[0] + d[1] * d[1]), d[2]);
+				B.phi = atan2(d[1], d[0]);
+				if(B.phi < 0)
+					B.phi += 2 * M_PI;
+				B.inv_nb = 1.0 / Jb.nb;
+				B.scale = 1.0 / ((double) Jb.N * Jb.N);
+				B.off_M = moff_M;
+				moff_M += (long) B.lags * (B.Lt + 1);
+				B.off_R = off_R;
+				off_R += (long) (Jb.L + 1) * (Jb.L + 1);
+				B.off_c = off_c;
+				off_c += B.lags;
+			}
+		pdet += Jb.D;
+	}
+	J->n_in = in;
+	J->n_alm = aoff;
+	J->n_map = moff;
+	hipStream_t st = sphr_rt_stream();
+	if(sphr_rt_check(hipMalloc(&J->d_jobs, n_jobs * sizeof(struct jb_job)), "hipMalloc") || sphr_rt_check(hipMalloc(&J->d_bl, nbl * sizeof(struct jb_bl)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&J->M, moff_M * esz), "hipMalloc") || sphr_rt_check(hipMalloc(&J->R, off_R * 2 * esz), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&J->c, off_c * esz), "hipMalloc")
+	   || sphr_rt_check(hipMemcpy(J->d_jobs, J->jobs, n_jobs * sizeof(struct jb_job), hipMemcpyHostToDevice), "hipMemcpy")) {
+		sphr_jobs_free(J);
+		return NULL;
+	}
+	if((precision == SPHR_MIXED ? generate_plans<float>(J, st) : generate_plans<double>(J, st))) {
+		sphr_jobs_free(J);
+		return NULL;
+	}
+	/* the device copy of the baseline table now holds the final layout */
+	if(sphr_rt_check(hipMemcpy(J->d_bl, J->bl, nbl * sizeof(struct jb_bl), hipMemcpyHostToDevice), "hipMemcpy")) {
+		sphr_jobs_free(J);
+		return NULL;
+	}
+	/* l_max classes for the synthesis */
+	J->cls = (struct jb_class *) calloc(n_jobs, sizeof(struct jb_class));
+	if(!J->cls) {
+		sphr_jobs_free(J);
+		return NULL;
+	}
+	for(int j = 0; j < n_jobs; j++) {
+		const int L = J->jobs[j].L;
+		int k = 0;
+		while(k < J->n_classes && J->cls[k].L != L)
+			k++;
+		if(k == J->n_classes) {
+			J->cls[k].L = L;
+			J->n_classes++;
+		}
+		J->cls[k].count++;
+	}
+	for(int k = 0; k < J->n_classes; k++) {
+		struct jb_class &C = J->cls[k];
+		long *ha = (long *) malloc(C.count * sizeof(long)), *hm = (long *) malloc(C.count * sizeof(long));
+		int n = 0;
+		for(int j = 0; j < n_jobs && ha && hm; j++)
+			if(J->jobs[j].L == C.L) {
+				ha[n] = J->jobs[j].off_alm;
+				hm[n] = J->jobs[j].off_map;
+				n++;
+			}
+		const int bad = !ha || !hm || sphr_rt_check(hipMalloc(&C.d_aoff, C.count * sizeof(long)), "hipMalloc") || sphr_rt_check(hipMalloc(&C.d_moff, C.count * sizeof(long)), "hipMalloc")
+				|| sphr_rt_check(hipMemcpy(C.d_aoff, ha, C.count * sizeof(long), hipMemcpyHostToDevice), "hipMemcpy")
+				|| sphr_rt_check(hipMemcpy(C.d_moff, hm, C.count * sizeof(long), hipMemcpyHostToDevice), "hipMemcpy")
+				|| synth_tables_build(C.L, &C.syn);
+		free(ha);
+		free(hm);
+		if(bad) {
+			sphr_jobs_free(J);
+			return NULL;
+		}
+	}
+	return J;
+}
+
+template <typename T>
+static int launch_run(struct sphr_jobs *J, const float *in, void *alm, void *map, int sc, int nt, hipStream_t st)
+{
+	int lags_max = 0;
+	for(int b = 0; b < J->n_bl; b++)
+		lags_max = J->bl[b].lags > lags_max ? J->bl[b].lags : lags_max;
+	hipLaunchKernelGGL((k_lagcorr<T>), dim3((lags_max + nt - 1) / nt, J->n_bl), dim3(nt), (size_t) (2 * sc + nt) * sizeof(T), st, J->d_jobs, J->d_bl, in, sc, (T *) J->c);
+	size_t pw = 0;
+	for(int j = 0; j < J->n_jobs; j++) {
+		size_t t = 0;
+		for(int b = 0; b < J->jobs[j].nb; b++)
+			t += J->bl[J->jobs[j].first_bl + b].Lt + 1;
+		pw = t > pw ? t : pw;
+	}
+	hipLaunchKernelGGL((k_job_alm<T>), dim3(J->n_jobs), dim3(256), pw * sizeof(T), st, J->d_jobs, J->d_bl, (const T *) J->c, (const T *) J->M, (const cplx<T> *) J->R, (cplx<T> *) alm);
+	if(hipGetLastError() != hipSuccess)
+		return -1;
+	if(map)
+		for(int k = 0; k < J->n_classes; k++) {
+			const struct jb_class &C = J->cls[k];
+			if(launch_synth<T, T, T>((const cplx<T> *) alm, C.count, 0, &C.syn.g, &C.syn.tb, (cplx<T> *) map, st, C.d_aoff, C.d_moff))
+				return -1;
+		}
+	return 0;
+}
+
+static int run_direct(struct sphr_jobs *J, const float *in, void *alm, void *map, hipStream_t st)
+{
+	return J->precision == SPHR_MIXED ? launch_run<float>(J, in, alm, map, J->lag_sc, J->lag_nt, st) : launch_run<double>(J, in, alm, map, J->lag_sc, J->lag_nt, st);
+}
+
+extern "C" int sphr_jobs_run(struct sphr_jobs *J, const float *in, void *alm, void *map, void *stream)
+{
+	if(!J || !in || !alm) {
+		sphr_set_error("sphr_jobs_run: bad arguments");
+		return -1;
+	}
+	hipStream_t st = stream ? (hipStream_t) stream : sphr_rt_stream();
+	/* replay the captured graph when the buffers and stream are the same */
+	if(!J->exec || J->g_in != in || J->g_alm != alm || J->g_map != map || J->g_st != st) {
+		if(J->exec) {
+			hipGraphExecDestroy(J->exec);
+			J->exec = NULL;
+		}
+		if(J->graph) {
+			hipGraphDestroy(J->graph);
+			J->graph = NULL;
+		}
+		if(sphr_rt_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture"))
+			return -1;
+		const int r = run_direct(J, in, alm, map, st);
+		hipGraph_t g = NULL;
+		const hipError_t e = hipStreamEndCapture(st, &g);
+		if(r || sphr_rt_check(e, "end capture")) {
+			if(g)
+				hipGraphDestroy(g);
+			return -1;
+		}
+		if(sphr_rt_check(hipGraphInstantiate(&J->exec, g, NULL, NULL, 0), "instantiate")) {
+			hipGraphDestroy(g);
+			return -1;
+		}
+		J->graph = g;
+		J->g_in = in;
+		J->g_alm = alm;
+		J->g_map = map;
+		J->g_st = st;
+	}
+	return sphr_rt_check(hipGraphLaunch(J->exec, st), "graph launch");
+}
+
+/* autotuner: time the lag-correlation tiles on the caller's data and keep
+ * the fastest (the graph is re-captured on the next run) */
+extern "C" int sphr_jobs_tune(struct sphr_jobs *J, const float *in, void *alm, double *best_ms)
+{
+	static const int cand[][2] = { { 512, 128 }, { 1024, 128 }, { 2048, 128 }, { 512, 256 }, { 1024, 256 }, { 2048, 256 } };
+	hipStream_t st = sphr_rt_stream();
+	hipEvent_t e0, e1;
+	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
+		return -1;
+	float best = 1e30f;
+	int bi = 0;
+	for(unsigned k = 0; k < sizeof(cand) / sizeof(cand[0]); k++) {
+		J->lag_sc = cand[k][0];
+		J->lag_nt = cand[k][1];
+		if(run_direct(J, in, alm, NULL, st))
+			return -1;
+		hipEventRecord(e0, st);
+		for(int r = 0; r < 3; r++)
+			run_direct(J, in, alm, NULL, st);
+		hipEventRecord(e1, st);
+		if(sphr_rt_check(hipEventSynchronize(e1), "tune"))
+			return -1;
+		float ms = 0.f;
+		hipEventElapsedTime(&ms, e0, e1);
+		if(ms < best) {
+			best = ms;
+			bi = (int) k;
+		}
+	}
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	J->lag_sc = cand[bi][0];
+	J->lag_nt = cand[bi][1];
+	if(J->exec) {
+		hipGraphExecDestroy(J->exec);
+		J->exec = NULL;
+		hipGraphDestroy(J->graph);
+		J->graph = NULL;
+	}
+	if(best_ms)
+		*best_ms = best / 3.0;
+	return bi;
+}
+
+/* [0] input floats, [1] a_lm complex elements, [2] map complex elements,
+ * [3] baselines, [4] l_max classes, [5] lag tile (samples), [6] lag tile
+ * (threads), [7] plan bytes */
+extern "C" void sphr_jobs_info(const struct sphr_jobs *J, long info[8])
+{
+	long plan = 0;
+	for(int b = 0; b < J->n_bl; b++)
+		plan += (long) J->bl[b].lags * (J->bl[b].Lt + 1);
+	const long esz = J->precision == SPHR_MIXED ? 4 : 8;
+	long rot = 0;
+	for(int b = 0; b < J->n_bl; b++)
+		rot += (long) (J->bl[b].L + 1) * (J->bl[b].L + 1) * 2;
+	info[0] = J->n_in;
+	info[1] = J->n_alm;
+	info[2] = J->n_map;
+	info[3] = J->n_bl;
+	info[4] = J->n_classes;
+	info[5] = J->lag_sc;
+	info[6] = J->lag_nt;
+	info[7] = (plan + rot) * esz;
+}
+
+/* per-job offsets: in (floats), alm and map (complex elements) */
+extern "C" void sphr_jobs_offsets(const struct sphr_jobs *J, long *in, long *alm, long *map)
+{
+	for(int j = 0; j < J->n_jobs; j++) {
+		if(in)
+			in[j] = J->jobs[j].off_in;
+		if(alm)
+			alm[j] = J->jobs[j].off_alm;
+		if(map)
+			map[j] = J->jobs[j].off_map;
+	}
+}

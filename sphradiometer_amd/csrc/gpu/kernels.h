@@ -49,9 +49,11 @@ int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T
 template <typename T, typename TO>
 int launch_alm(const T *partial, int nsplit, int S, const struct xproj_geom *g, int L, const cplx<T> *R, const int2 *lm, const double *gmst, cplx<TO> *alm, hipStream_t st);
 
-/* synthesis of S series of order L (polar flag) onto 2B x 2B meshes */
+/* synthesis of S series of order L (polar flag) onto 2B x 2B meshes;
+ * series s read from alm + aoff[s] and written to map + moff[s] (element
+ * offsets, device arrays) when given, else packed */
 template <typename TI, typename T, typename TO>
-int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom *g, const struct synth_tables *tb, cplx<TO> *map, hipStream_t st);
+int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom *g, const struct synth_tables *tb, cplx<TO> *map, hipStream_t st, const long *aoff = NULL, const long *moff = NULL);
 
 /* fused: partials -> a_lm -> mesh (map may be NULL), one workgroup per
  * segment; falls back to launch_alm + launch_synth when the coefficients

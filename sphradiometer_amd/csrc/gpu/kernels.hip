@@ -437,7 +437,7 @@ __device__ void phi_stage(const cplx<T> *F, int polar, int rp, int j0, const str
 /* synthesis of a batch of series: one workgroup per (series, group of
  * rp ring pairs); coefficients and recurrence table in LDS when they fit */
 template <typename TI, typename T, typename TO>
-__global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, int polar, int rp, int a_in_lds, int al_in_lds, struct synth_geom g, struct synth_tables tb, cplx<TO> *__restrict__ map)
+__global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, int polar, int rp, int a_in_lds, int al_in_lds, struct synth_geom g, struct synth_tables tb, cplx<TO> *__restrict__ map, const long *__restrict__ aoff, const long *__restrict__ moff)
 {
 	extern __shared__ unsigned char smem_raw[];
 	const int n = g.n, L = g.L;
@@ -448,7 +448,9 @@ __global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, 
 	cplx<T> *sa = tw + n;					/* [ncoef] when staged */
 	T *al = reinterpret_cast<T *>(sa + (a_in_lds ? ncoef : 0));	/* [nal] when staged */
 	const int s = blockIdx.y, j0 = blockIdx.x * rp;
-	const cplx<TI> *ag = alm + (size_t) s * ncoef;
+	/* series s at aoff[s] / moff[s] when given (ragged batches), else
+	 * packed at uniform strides */
+	const cplx<TI> *ag = alm + (aoff ? (size_t) aoff[s] : (size_t) s * ncoef);
 	for(int t = threadIdx.x; t < n; t += NT)
 		tw[t] = mk((T) tb.twp[2 * t], (T) tb.twp[2 * t + 1]);
 	if(a_in_lds)
@@ -466,7 +468,7 @@ __global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, 
 	ls.al = al_in_lds ? al : nullptr;
 	legendre_stage<TI, T>(ag, polar, rp, j0, g, tb, ls, F);
 	__syncthreads();
-	phi_stage<T, TO>(F, polar, rp, j0, g, ls, map + (size_t) s * n * n);
+	phi_stage<T, TO>(F, polar, rp, j0, g, ls, map + (moff ? (size_t) moff[s] : (size_t) s * n * n));
 }
 
 /*
@@ -568,7 +570,7 @@ int launch_sky(const T *partial, int nsplit, int S, const struct xproj_geom *xg,
 }
 
 template <typename TI, typename T, typename TO>
-int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom *g, const struct synth_tables *tb, cplx<TO> *map, hipStream_t st)
+int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom *g, const struct synth_tables *tb, cplx<TO> *map, hipStream_t st, const long *aoff, const long *moff)
 {
 	const size_t budget = 65536;
 	const int L = g->L, n = g->n;
@@ -584,12 +586,12 @@ int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom 
 	if(al_in_lds)
 		lds += nal * sizeof(T);
 	dim3 grid((g->B + rp - 1) / rp, S);
-	hipLaunchKernelGGL((k_synth<TI, T, TO>), grid, dim3(NT), lds, st, alm, polar, rp, a_in_lds, al_in_lds, *g, *tb, map);
+	hipLaunchKernelGGL((k_synth<TI, T, TO>), grid, dim3(NT), lds, st, alm, polar, rp, a_in_lds, al_in_lds, *g, *tb, map, aoff, moff);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template int launch_synth<double, double, double>(const cplx<double> *, int, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, hipStream_t);
-template int launch_synth<float, float, float>(const cplx<float> *, int, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, hipStream_t);
+template int launch_synth<double, double, double>(const cplx<double> *, int, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, hipStream_t, const long *, const long *);
+template int launch_synth<float, float, float>(const cplx<float> *, int, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, hipStream_t, const long *, const long *);
 template int launch_sky<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
 template int launch_sky<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);
 

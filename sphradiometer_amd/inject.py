@@ -198,3 +198,40 @@ def burst_batch(n_segments, n=1024, seed=0, snr_range=(8.0, 30.0), delta_t=1.0 /
         h = amp[:, None] * np.sin(2 * math.pi * f0[:, None] * u) * np.exp(-(u / tau[:, None]) ** 2)
         out[d] += h.astype(np.float32)
     return out, {"theta": theta, "phi": phi, "snr": snr, "f0": f0}
+
+
+def uneven_jobs(n_jobs, seed=0, delta_t=1.0 / 4096, l_range=(16, 64), det_range=(2, 5),
+                durations=(0.125, 0.25, 0.5, 1.0, 2.0), snr=20.0):
+    """The uneven batch: per job l_max ~ U{l_range}, detector count ~
+    U{det_range} with sites uniform on the 6371 km sphere, segment length
+    ~ U(durations) at 1 / delta_t, float32 white N(0,1) data and, in every
+    other job (odd index), one sine-Gaussian plane wave (f0 ~ U(100, 400)
+    Hz, Q = 9, network SNR snr, isotropic direction) delayed per site by
+    r . s in closed form.  Job j draws from a generator seeded (seed, j).
+    Returns (jobs, data): jobs a list of dicts (positions [D][3]
+    light-seconds, N, l_max, burst), data the float32 samples packed job by
+    job as [D][N]."""
+    jobs, chunks = [], []
+    r = EARTH_RADIUS_SPHERE / C_SI
+    for j in range(n_jobs):
+        rng = np.random.default_rng([seed, j])
+        D = int(rng.integers(det_range[0], det_range[1] + 1))
+        L = int(rng.integers(l_range[0], l_range[1] + 1))
+        N = int(round(durations[int(rng.integers(0, len(durations)))] / delta_t))
+        v = rng.standard_normal((D, 3))
+        pos = r * v / np.linalg.norm(v, axis=1, keepdims=True)
+        x = rng.standard_normal((D, N)).astype(np.float32)
+        burst = bool(j & 1)
+        if burst:
+            f0 = rng.uniform(100.0, 400.0)
+            th, ph = isotropic_direction(rng)
+            sdir = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+            tau = 9.0 / (math.sqrt(2.0) * math.pi * f0)
+            t = np.arange(N) * delta_t - 0.5 * N * delta_t
+            amp = snr / math.sqrt(D * tau * math.sqrt(math.pi / 2) / (2 * delta_t))
+            for d in range(D):
+                u = t + float(pos[d] @ sdir)
+                x[d] += (amp * np.sin(2 * math.pi * f0 * u) * np.exp(-(u / tau) ** 2)).astype(np.float32)
+        jobs.append({"positions": pos, "N": N, "l_max": L, "burst": burst})
+        chunks.append(x.ravel())
+    return jobs, np.concatenate(chunks)

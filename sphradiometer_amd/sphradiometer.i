@@ -24,6 +24,7 @@
 #include <sphradiometer/projection.h>
 #include <sphradiometer/correlator.h>
 #include <sphradiometer/diagnostics.h>
+#include <sphradiometer/inspiral_sky_map.h>
 #include <sphradiometer/sphradiometer_gpu.h>
 #include <sphradiometer/sphradiometer_batch.h>
 %}
@@ -41,6 +42,7 @@
 %include <sphradiometer/projection.h>
 %include <sphradiometer/correlator.h>
 %include <sphradiometer/diagnostics.h>
+%include <sphradiometer/inspiral_sky_map.h>
 %include <sphradiometer/sphradiometer_gpu.h>
 %include <sphradiometer/sphradiometer_batch.h>
 
