@@ -27,18 +27,20 @@ HBM_PEAK_GBS = 8000.0
 
 
 def stage_bytes(info, S, esz, in_bytes_per_sample=8):
-    """Algorithmic HBM bytes per launch of each stage."""
+    """Algorithmic HBM bytes per launch of each stage: FFT, projection,
+    a_lm + Legendre ring sums, phi DFT (GEMM)."""
     D, nb, N, L, CT, ns = info["detectors"], info["baselines"], info["N"], info["sky_l"], info["columns"], info["k_splits"]
     Nh = N // 2 + 1
     ncoef = (L + 1) ** 2
     n = 2 * (L + 1)
+    nm = 2 * L + 1
     spec = D * S * Nh * 2 * esz
     fft = D * S * N * in_bytes_per_sample + spec
     xproj = spec + (N + 2) * CT * esz + ns * S * CT * esz
-    # sky: K-split partials in, rotation rows, a_lm and the mesh out (one
-    # fused kernel; the third profiled stage is empty)
-    sky = ns * S * CT * esz + nb * ncoef * 2 * esz + S * ncoef * 2 * esz + S * n * n * 2 * esz
-    return [fft, xproj, 0, sky]
+    ring = S * n * nm * 2 * esz
+    lf = ns * S * CT * esz + nb * ncoef * 2 * esz + S * ncoef * 2 * esz + ring
+    phi = ring + 2 * nm * 2 * n * esz + S * n * n * 2 * esz
+    return [fft, xproj, lf, phi]
 
 
 def run_path(S, net, L, d_ts, nseg, precision, steps, warmup):
@@ -352,7 +354,7 @@ def main():
     value = nseg / (rmx["ms_per_step"] * 1e-3)
     fp64_value = nseg / (r64["ms_per_step"] * 1e-3)
 
-    names = ["fft", "xproj", "-", "sky"]
+    names = ["fft", "xproj", "alm_legendre", "phi_gemm"]
     sb = stage_bytes(rmx["info"], nseg, rmx["esz"])
     k = int(np.argmax(rmx["stage_ms"]))
     achieved = sb[k] / (rmx["stage_ms"][k] * 1e-3) / 1e9

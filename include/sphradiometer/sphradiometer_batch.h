@@ -55,9 +55,9 @@ void sphr_batch_free(struct sphr_batch *);
 int sphr_batch_run(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, void *stream);
 
 /* Per-stage timing: while enabled, every run records HIP events on its
- * stream around its stages (FFT, projection, [unused], sky: coefficients
- * and synthesis); _read synchronises and returns the summed milliseconds
- * per stage and the number of runs since enabling. */
+ * stream around its stages (FFT, projection, a_lm + Legendre ring sums,
+ * phi DFT); _read synchronises and returns the summed milliseconds per
+ * stage and the number of runs since enabling. */
 int sphr_batch_profile(struct sphr_batch *, int enable);
 int sphr_batch_profile_read(struct sphr_batch *, double ms_per_stage[4], int *runs);
 

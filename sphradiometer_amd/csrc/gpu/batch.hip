@@ -33,7 +33,7 @@ struct sphr_batch {
 	int precision, D, nb, N, Nh, L, max_segments, nsplit, k_per_split;
 	struct xproj_geom g;
 	struct synth_dev syn;
-	void *W, *R, *tw, *spec, *partial;
+	void *W, *R, *tw, *spec, *partial, *F;
 	int *col_bl;
 	int2 *lm;
 	int profile;
@@ -115,7 +115,7 @@ extern "C" void sphr_batch_free(struct sphr_batch *bt)
 {
 	if(!bt)
 		return;
-	void *ptrs[] = { bt->W, bt->R, bt->tw, bt->spec, bt->partial, bt->col_bl, bt->lm };
+	void *ptrs[] = { bt->W, bt->R, bt->tw, bt->spec, bt->partial, bt->col_bl, bt->lm, bt->F };
 	for(unsigned i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
 		if(ptrs[i])
 			hipFree(ptrs[i]);
@@ -211,7 +211,8 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 	if(synth_tables_build(bt->L, &bt->syn))
 		goto fail;
 	if(sphr_rt_check(hipMalloc(&bt->spec, (size_t) bt->D * max_segments * bt->Nh * 2 * esz), "hipMalloc spectra")
-	   || sphr_rt_check(hipMalloc(&bt->partial, (size_t) bt->nsplit * max_segments * CT * esz), "hipMalloc partials"))
+	   || sphr_rt_check(hipMalloc(&bt->partial, (size_t) bt->nsplit * max_segments * CT * esz), "hipMalloc partials")
+	   || sphr_rt_check(hipMalloc(&bt->F, (size_t) max_segments * 2 * (bt->L + 1) * (2 * bt->L + 1) * 2 * esz), "hipMalloc ring sums"))
 		goto fail;
 	return bt;
 fail:
@@ -234,10 +235,19 @@ static int run_t(struct sphr_batch *bt, const void *ts, int f32, int S, const do
 		return -1;
 	if(ev)
 		hipEventRecord(ev[2], st);
-	if(ev)
-		hipEventRecord(ev[3], st);
-	if(launch_sky<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st))
-		return -1;
+	/* a_lm and ring sums, then the phi DFT as a GEMM; the one-kernel form
+	 * when the coefficients do not fit the LDS budget */
+	if(launch_sky_lf<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) bt->F : (cplx<T> *) NULL, st) == 0) {
+		if(ev)
+			hipEventRecord(ev[3], st);
+		if(map && launch_phi<T>((const cplx<T> *) bt->F, S, &bt->syn.g, &bt->syn.tb, (cplx<T> *) map, st))
+			return -1;
+	} else {
+		if(ev)
+			hipEventRecord(ev[3], st);
+		if(launch_sky<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st))
+			return -1;
+	}
 	if(ev)
 		hipEventRecord(ev[4], st);
 	return 0;

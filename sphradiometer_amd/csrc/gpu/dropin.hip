@@ -105,6 +105,37 @@ int synth_tables_build(int L, struct synth_dev *t)
 	t->tb.smm = t->smm;
 	t->tb.alpha = t->alpha;
 	t->tb.twp = t->twp;
+	{
+		/* the expanded phi-DFT matrix [2 (2L+1)][2 n] */
+		const int nm = 2 * L + 1;
+		const size_t ne = (size_t) 2 * nm * 2 * n;
+		double *e64 = (double *) malloc(ne * sizeof(double));
+		float *e32 = (float *) malloc(ne * sizeof(float));
+		int bad = !e64 || !e32;
+		for(int i = 0; i < nm && !bad; i++) {
+			const int m = i <= L ? i : i - nm;
+			for(int k = 0; k < n; k++) {
+				const long r = (((long) k * m) % n + n) % n;
+				const double er = ht[2 * r], ei = ht[2 * r + 1];
+				double *r0 = e64 + (size_t) (2 * i) * 2 * n + 2 * k, *r1 = r0 + 2 * n;
+				r0[0] = er;
+				r0[1] = ei;
+				r1[0] = -ei;
+				r1[1] = er;
+			}
+		}
+		for(size_t q = 0; q < ne && !bad; q++)
+			e32[q] = (float) e64[q];
+		bad = bad || sphr_rt_check(hipMalloc(&t->E64, ne * sizeof(double)), "hipMalloc") || sphr_rt_check(hipMalloc(&t->E32, ne * sizeof(float)), "hipMalloc")
+		      || sphr_rt_check(hipMemcpy(t->E64, e64, ne * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
+		      || sphr_rt_check(hipMemcpy(t->E32, e32, ne * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+		free(e64);
+		free(e32);
+		if(bad)
+			goto done;
+		t->tb.E64 = t->E64;
+		t->tb.E32 = t->E32;
+	}
 	rc = 0;
 done:
 	free(hx);
@@ -129,6 +160,10 @@ void synth_tables_free(struct synth_dev *t)
 		hipFree(t->twp);
 	if(t->cm)
 		hipFree(t->cm);
+	if(t->E64)
+		hipFree(t->E64);
+	if(t->E32)
+		hipFree(t->E32);
 	memset(t, 0, sizeof(*t));
 }
 

@@ -32,6 +32,8 @@ struct synth_tables {
 	const double *alpha;	/* alpha_lm of the l recurrence at the (l, m >= 0)
 				 * slot of the coefficient layout */
 	const double *twp;	/* 2n: exp(+2 pi i t / n) */
+	const double *E64;	/* [2 (2L+1)][2 n]: the phi DFT, real-expanded */
+	const float *E32;
 };
 
 /* batched real FFT of length N (power of two), two real series per
@@ -60,6 +62,19 @@ int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom 
  * exceed the LDS budget */
 template <typename T>
 int launch_sky(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st);
+
+/* the same stage as two kernels: a_lm and the Legendre stage into F
+ * ([S][2(L+1)][2L+1] complex scratch; F NULL: a_lm only; -1 when the
+ * coefficients exceed the LDS budget), then the phi DFT of F as an MFMA
+ * GEMM into the meshes */
+template <typename T>
+int launch_sky_lf(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *F, hipStream_t st);
+template <typename T>
+int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *map, hipStream_t st);
+/* GEMM-form synthesis of S packed series (order L, non-polar) through the
+ * ring-sum scratch F; -1 when a series exceeds the LDS budget */
+template <typename T>
+int launch_synth_gemm(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, cplx<T> *map, hipStream_t st);
 
 /* extremum of sign * Re f for S series (one wavefront each): seed from
  * seed_map (S meshes of order L, rings from map_x) when given, else from

@@ -536,6 +536,179 @@ __global__ void __launch_bounds__(NT) k_sky(const T *__restrict__ partial, int n
 	}
 }
 
+/*
+ * Sky stage in two kernels (batched path).  k_sky_lf: one workgroup per
+ * segment sums the K-split partials, forms a_lm (rotation rows, baseline
+ * average, rotate_z) in LDS, writes a_lm, and runs the Legendre stage for
+ * all ring pairs into F [S][n rings][2L+1] in HBM (ring rows in map order,
+ * columns m = 0..L, -L..-1).  The phi DFT is then one GEMM on MFMA,
+ * map [S n][n] = F [S n][2L+1] E [2L+1][n] (complex, real-expanded E).
+ */
+template <typename T>
+__global__ void __launch_bounds__(NT) k_sky_lf(const T *__restrict__ partial, int nsplit, int S, struct xproj_geom xg, const cplx<T> *__restrict__ R, const int2 *__restrict__ lm, const double *__restrict__ gmst, struct synth_geom g, struct synth_tables tb, cplx<T> *__restrict__ alm, cplx<T> *__restrict__ Fg)
+{
+	extern __shared__ unsigned char smem_raw[];
+	const int L = g.L, B = g.B, n = g.n, nm = 2 * L + 1;
+	const int ncoef = (L + 1) * (L + 1), nal = (L + 1) * (L + 2) / 2;
+	cplx<T> *sa = reinterpret_cast<cplx<T> *>(smem_raw);	/* [ncoef] */
+	T *al = reinterpret_cast<T *>(sa + ncoef);		/* [nal] */
+	T *pw = al + nal;					/* [CT] */
+	const int s = blockIdx.x;
+	for(int c = threadIdx.x; c < xg.CT; c += NT) {
+		T v = 0;
+		for(int sp = 0; sp < nsplit; sp++)
+			v += partial[((size_t) sp * S + s) * xg.CT + c];
+		pw[c] = v;
+	}
+	for(int t = threadIdx.x; t < nal; t += NT)
+		al[t] = (T) tb.alpha[t];
+	__syncthreads();
+	double gs = 0.0;
+	if(gmst)
+		gs = gmst[s];
+	for(int idx = threadIdx.x; idx < ncoef; idx += NT) {
+		const int2 q = lm[idx];
+		const int l = q.x, m = q.y;
+		cplx<T> acc = mk((T) 0, (T) 0);
+		for(int b = 0; b < xg.nb; b++) {
+			if(l > xg.Lb[b])
+				continue;
+			const T *p = pw + xg.col_start[b] + 2 * l;
+			cfma(acc, R[(size_t) b * ncoef + idx], mk(p[0], p[1]));
+		}
+		if(gmst && m) {
+			double sn, cs;
+			sincos(-(double) m * gs, &sn, &cs);
+			acc = acc * mk((T) cs, (T) sn);
+		}
+		sa[idx] = acc;
+		alm[(size_t) s * ncoef + idx] = acc;
+	}
+	if(!Fg)
+		return;
+	__syncthreads();
+	cplx<T> *F = Fg + (size_t) s * n * nm;
+	for(int it = threadIdx.x; it < B * nm; it += NT) {
+		const int j = it / nm, mi = it - j * nm;
+		const int m = mi <= L ? mi : mi - nm;
+		const int am = m < 0 ? -m : m;
+		const T x = (T) tb.x[j];
+		const int abase = lm_offset(L, am, m);
+		const int albase = lm_offset(L, am, am);
+		T pprev = 0, pcur = (T) tb.smm[(size_t) j * (L + 1) + am], aprev = 1;
+		cplx<T> E = mk((T) 0, (T) 0), O = mk((T) 0, (T) 0);
+		for(int l = am; l <= L; l++) {
+			if(l > am) {
+				const T a_l = al[albase + l - am];
+				const T pn = (l == am + 1) ? a_l * x * pcur : a_l * (x * pcur - pprev / aprev);
+				pprev = pcur;
+				pcur = pn;
+				aprev = a_l;
+			}
+			const cplx<T> c = sa[abase + l - am];
+			if((l + am) & 1)
+				cfma(O, c, pcur);
+			else
+				cfma(E, c, pcur);
+		}
+		const T sg = (m < 0 && (am & 1)) ? (T) -1 : (T) 1;
+		F[(size_t) j * nm + mi] = (E + O) * sg;
+		F[(size_t) (2 * B - 1 - j) * nm + mi] = (E - O) * sg;
+	}
+}
+
+/* Legendre ring sums of given series (one workgroup per series), the
+ * first half of the GEMM-form synthesis */
+template <typename T>
+__global__ void __launch_bounds__(NT) k_alm_lf(const cplx<T> *__restrict__ alm, struct synth_geom g, struct synth_tables tb, cplx<T> *__restrict__ Fg)
+{
+	extern __shared__ unsigned char smem_raw[];
+	const int L = g.L, B = g.B, n = g.n, nm = 2 * L + 1;
+	const int ncoef = (L + 1) * (L + 1), nal = (L + 1) * (L + 2) / 2;
+	cplx<T> *sa = reinterpret_cast<cplx<T> *>(smem_raw);
+	T *al = reinterpret_cast<T *>(sa + ncoef);
+	const int s = blockIdx.x;
+	for(int t = threadIdx.x; t < ncoef; t += NT)
+		sa[t] = alm[(size_t) s * ncoef + t];
+	for(int t = threadIdx.x; t < nal; t += NT)
+		al[t] = (T) tb.alpha[t];
+	__syncthreads();
+	cplx<T> *F = Fg + (size_t) s * n * nm;
+	for(int it = threadIdx.x; it < B * nm; it += NT) {
+		const int j = it / nm, mi = it - j * nm;
+		const int m = mi <= L ? mi : mi - nm;
+		const int am = m < 0 ? -m : m;
+		const T x = (T) tb.x[j];
+		const int abase = lm_offset(L, am, m);
+		const int albase = lm_offset(L, am, am);
+		T pprev = 0, pcur = (T) tb.smm[(size_t) j * (L + 1) + am], aprev = 1;
+		cplx<T> E = mk((T) 0, (T) 0), O = mk((T) 0, (T) 0);
+		for(int l = am; l <= L; l++) {
+			if(l > am) {
+				const T a_l = al[albase + l - am];
+				const T pn = (l == am + 1) ? a_l * x * pcur : a_l * (x * pcur - pprev / aprev);
+				pprev = pcur;
+				pcur = pn;
+				aprev = a_l;
+			}
+			const cplx<T> c = sa[abase + l - am];
+			if((l + am) & 1)
+				cfma(O, c, pcur);
+			else
+				cfma(E, c, pcur);
+		}
+		const T sg = (m < 0 && (am & 1)) ? (T) -1 : (T) 1;
+		F[(size_t) j * nm + mi] = (E + O) * sg;
+		F[(size_t) (2 * B - 1 - j) * nm + mi] = (E - O) * sg;
+	}
+}
+
+template <typename T> struct phi_table;
+template <> struct phi_table<double> {
+	static const double *get(const struct synth_tables *tb) { return tb->E64; }
+};
+template <> struct phi_table<float> {
+	static const float *get(const struct synth_tables *tb) { return tb->E32; }
+};
+
+template <typename T>
+int launch_sky_lf(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *F, hipStream_t st)
+{
+	const int L = g->L;
+	const size_t lds = (size_t) (L + 1) * (L + 1) * sizeof(cplx<T>) + ((size_t) (L + 1) * (L + 2) / 2 + xg->CT) * sizeof(T);
+	if(lds > 65536)
+		return -1;
+	hipLaunchKernelGGL((k_sky_lf<T>), dim3(S), dim3(NT), lds, st, partial, nsplit, S, *xg, R, lm, gmst, *g, *tb, alm, F);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T>
+int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *map, hipStream_t st)
+{
+	const int n = g->n, nm = 2 * g->L + 1;
+	return launch_gemm<T, T>(S * n, 2 * n, 2 * nm, (const T *) F, 2L * nm, phi_table<T>::get(tb), 2L * n, (T *) map, 2L * n, st);
+}
+
+template int launch_sky_lf<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
+template int launch_sky_lf<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);
+template <typename T>
+int launch_synth_gemm(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, cplx<T> *map, hipStream_t st)
+{
+	const int L = g->L;
+	const size_t lds = (size_t) (L + 1) * (L + 1) * sizeof(cplx<T>) + (size_t) (L + 1) * (L + 2) / 2 * sizeof(T);
+	if(lds > 65536)
+		return -1;
+	hipLaunchKernelGGL((k_alm_lf<T>), dim3(S), dim3(NT), lds, st, alm, *g, *tb, F);
+	if(hipGetLastError() != hipSuccess)
+		return -1;
+	return launch_phi<T>(F, S, g, tb, map, st);
+}
+
+template int launch_synth_gemm<double>(const cplx<double> *, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
+template int launch_synth_gemm<float>(const cplx<float> *, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);
+template int launch_phi<double>(const cplx<double> *, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, hipStream_t);
+template int launch_phi<float>(const cplx<float> *, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, hipStream_t);
+
 /* ring pairs per pass of k_sky that fit the LDS budget, or 0 */
 template <typename T>
 static int sky_rp(const struct xproj_geom *xg, int L, size_t *lds)

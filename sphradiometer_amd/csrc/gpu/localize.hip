@@ -33,6 +33,7 @@ struct sphr_loc {
 	void *F;	/* T [D max_events][2 N] */
 	void *X;	/* T [max_events][2 nb N] */
 	void *S;	/* T [max_events][4 ncoef] */
+	void *Fr;	/* cplx<T> [2 max_events][2(L+1)][2L+1] ring sums */
 	struct synth_dev syn;
 };
 
@@ -115,7 +116,7 @@ extern "C" void sphr_loc_free(struct sphr_loc *lc)
 {
 	if(!lc)
 		return;
-	void *p[] = { lc->Wx, lc->Dx, lc->prior, lc->F, lc->X, lc->S };
+	void *p[] = { lc->Wx, lc->Dx, lc->prior, lc->F, lc->X, lc->S, lc->Fr };
 	for(unsigned i = 0; i < sizeof(p) / sizeof(p[0]); i++)
 		if(p[i])
 			hipFree(p[i]);
@@ -142,7 +143,8 @@ static int create_t(struct sphr_loc *lc, const struct sphr_loc_desc *d)
 	const size_t E = d->max_events;
 	if(sphr_rt_check(hipMalloc(&lc->F, (size_t) d->D * E * 2 * d->N * sizeof(T)), "hipMalloc")
 	   || sphr_rt_check(hipMalloc(&lc->X, E * 2 * K * sizeof(T)), "hipMalloc")
-	   || sphr_rt_check(hipMalloc(&lc->S, E * 4 * ncoef * sizeof(T)), "hipMalloc"))
+	   || sphr_rt_check(hipMalloc(&lc->S, E * 4 * ncoef * sizeof(T)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&lc->Fr, 2 * E * 2 * (d->L + 1) * (2 * d->L + 1) * 2 * sizeof(T)), "hipMalloc"))
 		return -1;
 	return 0;
 }
@@ -204,8 +206,10 @@ static int run_t(struct sphr_loc *lc, const void *series, int series_f64, int E,
 	hipLaunchKernelGGL((k_loc_epilogue<T>), dim3((unsigned) ((ne + 255) / 256)), dim3(256), 0, st, (const T *) lc->S, E, L, (const cplx<T> *) lc->prior, gmst, (cplx<T> *) alm);
 	if(hipGetLastError() != hipSuccess)
 		return -1;
-	/* 5: meshes of both skies */
-	if(map && launch_synth<T, T, T>((const cplx<T> *) alm, 2 * E, 0, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st))
+	/* 5: meshes of both skies: Legendre ring sums, then the phi DFT as a
+	 * GEMM (the one-kernel synthesis when the order is too large for it) */
+	if(map && launch_synth_gemm<T>((const cplx<T> *) alm, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) lc->Fr, (cplx<T> *) map, st)
+	   && launch_synth<T, T, T>((const cplx<T> *) alm, 2 * E, 0, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st))
 		return -1;
 	return 0;
 }

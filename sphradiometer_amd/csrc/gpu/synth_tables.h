@@ -13,6 +13,12 @@ struct synth_dev {
 	struct synth_tables tb;
 	double *x, *smm, *alpha, *twp;
 	double *cm;	/* L+1: Pbar_mm(cos theta) = cm[m] sin^m theta */
+	/* phi DFT as a real-expanded matrix for the GEMM form of the
+	 * synthesis: rows 2 i, 2 i + 1 for column i of F (m = i for i <= L,
+	 * m = i - (2L+1) above), columns (re, im) of point k:
+	 * E[i][k] = exp(2 pi i k m / n), expanded as (Er, Ei) / (-Ei, Er) */
+	double *E64;
+	float *E32;
 };
 
 /* the l_seed low-pass mesh of the extremizer's seed search */
