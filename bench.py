@@ -50,6 +50,7 @@ def run_path(S, net, L, d_ts, nseg, precision, steps, warmup):
     d_alm = S.DeviceBuffer(nseg * (L + 1) ** 2 * 2 * esz)
     d_map = S.DeviceBuffer(nseg * (2 * L + 2) ** 2 * 2 * esz)
     st = S.lib().sphr_stream()
+    tuned, tuned_ms = b.tune(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr)
     for _ in range(warmup):
         b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, stream=st)
     S.lib().sphr_stream_synchronize(st)
@@ -71,7 +72,7 @@ def run_path(S, net, L, d_ts, nseg, precision, steps, warmup):
     info = b.info()
     S.lib().sphr_event_free(e0)
     S.lib().sphr_event_free(e1)
-    return dict(ms_per_step=ms / steps, stage_ms=[t / runs for t in stage_ms], alm=alm, map=mp, info=info, esz=esz)
+    return dict(ms_per_step=ms / steps, stage_ms=[t / runs for t in stage_ms], alm=alm, map=mp, info=info, esz=esz, tuned=tuned)
 
 
 def plain_fp64(S, net, ts, L, budget_s, max_segments):
@@ -386,6 +387,8 @@ def main():
         "mixed_vs_fp64": {"speedup": round(value / fp64_value, 3), "alm_rel_err_max": float(alm_err.max()),
                           "alm_rel_err_median": float(np.median(alm_err)), "map_rel_err": map_err},
         "stage_ms": [round(t, 5) for t in rmx["stage_ms"]],
+        "stages": names,
+        "tuned": {"mixed": rmx["tuned"], "fp64": r64["tuned"], "info": rmx["info"]},
         "roofline": {"bound": "hbm", "kernel": names[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "plan_build_s": round(plan_s, 2),

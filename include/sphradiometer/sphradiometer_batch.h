@@ -173,8 +173,13 @@ int sphr_jobs_tune(struct sphr_jobs *, const float *in, void *alm, double *best_
 void sphr_jobs_info(const struct sphr_jobs *, long info[8]);
 void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map);
 
+/* Autotuner: runs the batch on these buffers with each candidate tile
+ * width and K split of the projection, keeps the fastest, returns its
+ * index (time per run in best_ms) or -1 */
+int sphr_batch_tune(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, double *best_ms);
+
 /* Shape information: [0] detectors, [1] baselines, [2] N, [3] sky l_max,
- * [4] projection columns, [5] K splits, [6] precision */
+ * [4] projection columns, [5] K splits, [6] precision, [7] K per split */
 void sphr_batch_info(const struct sphr_batch *, int info[8]);
 
 #ifdef __cplusplus

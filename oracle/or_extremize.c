@@ -74,15 +74,12 @@ static double f_at(const struct nm_ctx *c, const double d[2])
 	return creal(or_eval(c->s, th, ph));
 }
 
-/* sh_series_extremize.c:273-360 with the refinement described above */
-double or_real_minimum(const struct or_series *s, double *theta, double *phi)
+/* Nelder-Mead in the tangent plane from (th, ph), initial simplex size
+ * `size` rad, restarted around the best vertex with shrinking simplices */
+double or_refine_minimum(const struct or_series *s, double th, double ph, double size, double *theta, double *phi)
 {
-	double th, ph;
-	or_find_near_minimum(s, 6, &th, &ph);
 	struct nm_ctx c = { s, th, ph };
-	/* simplex in tangent co-ordinates, restarted around the best vertex
-	 * until the size stops shrinking */
-	double p[3][2] = { { 0, 0 }, { 0.05, 0 }, { 0, 0.05 } }, v[3];
+	double p[3][2] = { { 0, 0 }, { size, 0 }, { 0, size } }, v[3];
 	for(int restart = 0; restart < 6; restart++) {
 		for(int k = 0; k < 3; k++)
 			v[k] = f_at(&c, p[k]);
@@ -151,6 +148,14 @@ double or_real_minimum(const struct or_series *s, double *theta, double *phi)
 	return creal(or_eval(s, c.th0, c.ph0));
 }
 
+/* sh_series_extremize.c:273-360 with the refinement described above */
+double or_real_minimum(const struct or_series *s, double *theta, double *phi)
+{
+	double th, ph;
+	or_find_near_minimum(s, 6, &th, &ph);
+	return or_refine_minimum(s, th, ph, 0.05, theta, phi);
+}
+
 /* sh_series_deriv.c:288-~520: sin(theta) d/dtheta through the reference's
  * Wigner 3j form; result at l_max + 1 */
 struct or_series *or_sintheta_d_by_dtheta(const struct or_series *s)
@@ -205,4 +210,13 @@ void orx_sintheta_d_by_dtheta(unsigned l, int polar, const double complex *coeff
 	struct or_series *r = or_sintheta_d_by_dtheta(&s);
 	memcpy(out, r->coeff, or_length(r->l_max, r->polar) * sizeof(*out));
 	or_series_free(r);
+}
+
+void orx_refine_minimum(unsigned l, int polar, const double complex *coeff, double theta, double phi, double size, double *out)
+{
+	struct or_series s = { l, polar, (double complex *) coeff };
+	double th = NAN, ph = NAN;
+	out[2] = or_refine_minimum(&s, theta, phi, size, &th, &ph);
+	out[0] = th;
+	out[1] = ph;
 }

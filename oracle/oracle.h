@@ -153,6 +153,7 @@ struct or_series *or_rotate_z(struct or_series *result, const struct or_series *
 /* extremizer and derivative: sh_series_extremize.c, sh_series_deriv.c */
 double or_find_near_minimum(const struct or_series *, unsigned l_lowpass, double *theta, double *phi);
 double or_real_minimum(const struct or_series *, double *theta, double *phi);
+double or_refine_minimum(const struct or_series *, double th, double ph, double size, double *theta, double *phi);
 struct or_series *or_sintheta_d_by_dtheta(const struct or_series *);
 
 /* localization: inspiral_sky_map.c, RapidLocalization.py */

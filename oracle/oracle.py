@@ -71,6 +71,7 @@ def lib():
         L.orx_array_set_l.argtypes = [c_int, c_uint, c_int, c_void_p, c_uint, c_void_p]
         L.orx_real_minimum.argtypes = [c_uint, c_int, c_void_p, c_void_p]
         L.orx_find_near_minimum.argtypes = [c_uint, c_int, c_void_p, c_uint, c_void_p]
+        L.orx_refine_minimum.argtypes = [c_uint, c_int, c_void_p, c_double, c_double, c_double, c_void_p]
         L.orx_sintheta_d_by_dtheta.argtypes = [c_uint, c_int, c_void_p, c_void_p]
         L.orx_det_response.argtypes = [c_void_p, c_double, c_double, c_double, c_double, c_void_p]
         L.orx_cbc_projection.argtypes = [c_double, c_double, c_int, c_int, c_void_p, c_int, c_double, c_double, c_void_p, c_void_p]
@@ -387,6 +388,15 @@ def real_minimum(coeff, l, polar=0):
     c = np.ascontiguousarray(coeff, dtype=np.complex128)
     out = np.zeros(3)
     lib().orx_real_minimum(l, polar, _ptr(c), _ptr(out))
+    return tuple(out)
+
+
+def refine_minimum(coeff, l, theta, phi, size=1e-3, polar=0):
+    """(theta, phi, value): Nelder-Mead refinement of the minimum of Re f
+    from a given point (simplex of `size` rad)."""
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    out = np.zeros(3)
+    lib().orx_refine_minimum(l, polar, _ptr(c), theta, phi, size, _ptr(out))
     return tuple(out)
 
 

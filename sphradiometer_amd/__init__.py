@@ -202,6 +202,7 @@ _PROTOS = {
     "sphr_batch_profile_read": (c_int, [c_void_p, P(c_double), P(c_int)]),
     "sphr_batch_info": (None, [c_void_p, P(c_int)]),
     "sphr_batch_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sphr_batch_tune": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P(c_double)]),
     "sphr_jobs_new": (c_void_p, [c_int, P(c_int), P(c_double), P(c_int), P(c_int), c_double, c_int]),
     "sphr_jobs_free": (None, [c_void_p]),
     "sphr_jobs_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -498,6 +499,15 @@ class Batch:
         r = lib().sphr_batch_peaks(self.h, d_alm, d_map, n_segments, int(find_max), d_theta, d_phi, d_value, stream)
         if r:
             raise RuntimeError("sphr_batch_peaks failed: " + last_error())
+
+    def tune(self, d_tseries, n_segments, d_alm, d_map=None, d_window=None, d_gmst=None, f32_input=False):
+        """Autotune the projection tiles on these buffers (sphr_batch_tune);
+        returns (candidate index, ms per run)."""
+        ms = c_double()
+        k = lib().sphr_batch_tune(self.h, d_tseries, int(f32_input), n_segments, d_window, d_gmst, d_alm, d_map, ctypes.byref(ms))
+        if k < 0:
+            raise RuntimeError("sphr_batch_tune failed: " + last_error())
+        return k, ms.value
 
     def profile(self, enable):
         lib().sphr_batch_profile(self.h, int(enable))

@@ -31,6 +31,8 @@ struct ev_set {
 
 struct sphr_batch {
 	int precision, D, nb, N, Nh, L, max_segments, nsplit, k_per_split;
+	int tpw;			/* projection tiles per wave (autotuned) */
+	int ksplit_opt[3], nsplit_opt[3];	/* K-split candidates for 1, 2, 4 x CUs workgroups */
 	struct xproj_geom g;
 	struct synth_dev syn;
 	void *W, *R, *tw, *spec, *partial, *F;
@@ -174,17 +176,24 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 		CT += (2 * (d->Lb[b] + 1) + 15) / 16 * 16;	/* whole MFMA tiles */
 	}
 	bt->g.CT = CT;
-	/* split K so the projection launch has enough workgroups */
+	/* split K so the projection launch has enough workgroups; three
+	 * candidates (about 1, 2 and 4 workgroups per CU), the middle one by
+	 * default, the autotuner may pick another */
 	{
 		const int base = ((max_segments + 15) / 16) * ((CT / 16 + 15) / 16);
-		const int target = 2 * sphr_rt_cus();
-		int ns = (target + base - 1) / base;
 		const int maxs = (bt->g.K2 + 63) / 64;
-		ns = ns < 1 ? 1 : ns > maxs ? maxs : ns;
-		int kps = (bt->g.K2 + ns - 1) / ns;
-		kps = (kps + 31) / 32 * 32;
-		bt->k_per_split = kps;
-		bt->nsplit = (bt->g.K2 + kps - 1) / kps;
+		for(int o = 0; o < 3; o++) {
+			const int target = (1 << o) * sphr_rt_cus();
+			int ns = (target + base - 1) / base;
+			ns = ns < 1 ? 1 : ns > maxs ? maxs : ns;
+			int kps = (bt->g.K2 + ns - 1) / ns;
+			kps = (kps + 31) / 32 * 32;
+			bt->ksplit_opt[o] = kps;
+			bt->nsplit_opt[o] = (bt->g.K2 + kps - 1) / kps;
+		}
+		bt->k_per_split = bt->ksplit_opt[1];
+		bt->nsplit = bt->nsplit_opt[1];
+		bt->tpw = 4;
 	}
 	int *colbl = (int *) malloc((CT / 16) * sizeof(int));
 	const int ncoef = (bt->L + 1) * (bt->L + 1);
@@ -211,7 +220,7 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 	if(synth_tables_build(bt->L, &bt->syn))
 		goto fail;
 	if(sphr_rt_check(hipMalloc(&bt->spec, (size_t) bt->D * max_segments * bt->Nh * 2 * esz), "hipMalloc spectra")
-	   || sphr_rt_check(hipMalloc(&bt->partial, (size_t) bt->nsplit * max_segments * CT * esz), "hipMalloc partials")
+	   || sphr_rt_check(hipMalloc(&bt->partial, (size_t) bt->nsplit_opt[2] * max_segments * CT * esz), "hipMalloc partials")
 	   || sphr_rt_check(hipMalloc(&bt->F, (size_t) max_segments * 2 * (bt->L + 1) * (2 * bt->L + 1) * 2 * esz), "hipMalloc ring sums"))
 		goto fail;
 	return bt;
@@ -231,7 +240,7 @@ static int run_t(struct sphr_batch *bt, const void *ts, int f32, int S, const do
 		return -1;
 	if(ev)
 		hipEventRecord(ev[1], st);
-	if(launch_xproj<T>((const cplx<T> *) bt->spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) bt->partial, st))
+	if(launch_xproj<T>((const cplx<T> *) bt->spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) bt->partial, st, bt->tpw))
 		return -1;
 	if(ev)
 		hipEventRecord(ev[2], st);
@@ -338,4 +347,52 @@ extern "C" int sphr_batch_peaks(struct sphr_batch *bt, const void *alm, const vo
 	if(r)
 		sphr_set_error("sphr_batch_peaks: kernel launch failed (%s)", hipGetErrorString(hipGetLastError()));
 	return r;
+}
+
+/* autotuner: time the projection's tile width (1, 2, 4 tiles per wave)
+ * and K split (about 1, 2, 4 workgroups per CU) on the caller's data and
+ * keep the fastest whole run; returns the chosen candidate */
+extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, int S, const double *window, const double *gmst, void *alm, void *map, double *best_ms)
+{
+	if(!bt || S < 1 || S > bt->max_segments || !ts || !alm)
+		return -1;
+	hipStream_t st = sphr_rt_stream();
+	hipEvent_t e0, e1;
+	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
+		return -1;
+	static const int tpws[3] = { 1, 2, 4 };
+	float best = 1e30f;
+	int bo = 1, bw = 2;
+	const int prof = bt->profile;
+	bt->profile = 0;
+	for(int o = 0; o < 3; o++)
+		for(int w = 0; w < 3; w++) {
+			bt->k_per_split = bt->ksplit_opt[o];
+			bt->nsplit = bt->nsplit_opt[o];
+			bt->tpw = tpws[w];
+			if(sphr_batch_run(bt, ts, f32, S, window, gmst, alm, map, st))
+				return -1;
+			hipEventRecord(e0, st);
+			for(int r = 0; r < 3; r++)
+				sphr_batch_run(bt, ts, f32, S, window, gmst, alm, map, st);
+			hipEventRecord(e1, st);
+			if(sphr_rt_check(hipEventSynchronize(e1), "tune"))
+				return -1;
+			float ms = 0.f;
+			hipEventElapsedTime(&ms, e0, e1);
+			if(ms < best) {
+				best = ms;
+				bo = o;
+				bw = w;
+			}
+		}
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	bt->profile = prof;
+	bt->k_per_split = bt->ksplit_opt[bo];
+	bt->nsplit = bt->nsplit_opt[bo];
+	bt->tpw = tpws[bw];
+	if(best_ms)
+		*best_ms = best / 3.0;
+	return 3 * bo + bw;
 }

@@ -192,10 +192,9 @@ template <> struct mfma16<double> {
 };
 
 #define XP_MS 16	/* segments per workgroup */
-#define XP_TPW 4	/* column tiles per wave */
 #define XP_NW 4		/* waves per workgroup */
 
-template <typename T>
+template <typename T, int XP_TPW>
 __global__ void __launch_bounds__(NT) k_xproj(const cplx<T> *__restrict__ spec, int S, struct xproj_geom g, const T *__restrict__ W, const int *__restrict__ tile_bl, int k_per_split, T *__restrict__ partial)
 {
 	typedef typename mfma16<T>::v4 v4;
@@ -272,19 +271,32 @@ __global__ void __launch_bounds__(NT) k_xproj(const cplx<T> *__restrict__ spec, 
 	}
 }
 
-template <typename T>
-int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st)
+template <typename T, int XP_TPW>
+static void launch_xproj_tpw(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st)
 {
 	const int KC = mfma16<T>::KC;
 	const size_t lds = ((size_t) g->nb * XP_MS * (KC + 1) + (size_t) KC * XP_NW * XP_TPW * 16) * sizeof(T);
 	const int ntile = g->CT / 16;
 	dim3 grid((S + XP_MS - 1) / XP_MS, (ntile + XP_NW * XP_TPW - 1) / (XP_NW * XP_TPW), nsplit);
-	hipLaunchKernelGGL((k_xproj<T>), grid, dim3(NT), lds, st, spec, S, *g, W, tile_bl, k_per_split, partial);
+	hipLaunchKernelGGL((k_xproj<T, XP_TPW>), grid, dim3(NT), lds, st, spec, S, *g, W, tile_bl, k_per_split, partial);
+}
+
+/* tpw: 16-column tiles per wave (1, 2 or 4), the tile width the batch's
+ * autotuner picks */
+template <typename T>
+int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st, int tpw)
+{
+	if(tpw == 1)
+		launch_xproj_tpw<T, 1>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+	else if(tpw == 2)
+		launch_xproj_tpw<T, 2>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+	else
+		launch_xproj_tpw<T, 4>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template int launch_xproj<double>(const cplx<double> *, int, const struct xproj_geom *, const double *, const int *, int, int, double *, hipStream_t);
-template int launch_xproj<float>(const cplx<float> *, int, const struct xproj_geom *, const float *, const int *, int, int, float *, hipStream_t);
+template int launch_xproj<double>(const cplx<double> *, int, const struct xproj_geom *, const double *, const int *, int, int, double *, hipStream_t, int);
+template int launch_xproj<float>(const cplx<float> *, int, const struct xproj_geom *, const float *, const int *, int, int, float *, hipStream_t, int);
 
 /* ------------------------------------------------------------------ */
 /* per-baseline power -> sky coefficients                              */

@@ -53,6 +53,12 @@ def _protos():
     _DONE = True
 
 
+def _addr(p):
+    """A device address as a Python int (numpy integers are not accepted by
+    ctypes), None for a null pointer."""
+    return None if p is None else int(p)
+
+
 def _check(r, what):
     if r:
         raise RuntimeError(what + " failed: " + last_error())
@@ -85,17 +91,17 @@ class MLP:
         return out
 
     def forward(self, d_x, B, d_y=None, stream=None):
-        _check(lib().sphr_mlp_forward(self.h, d_x, B, d_y, stream), "forward")
+        _check(lib().sphr_mlp_forward(self.h, _addr(d_x), int(B), _addr(d_y), stream), "forward")
 
     def loss_grad(self, d_x, d_t, B, lam, d_loss=None, stream=None):
-        _check(lib().sphr_mlp_loss_grad(self.h, d_x, d_t, B, lam, d_loss, stream), "loss_grad")
+        _check(lib().sphr_mlp_loss_grad(self.h, _addr(d_x), _addr(d_t), int(B), lam, _addr(d_loss), stream), "loss_grad")
 
     def train_step(self, d_x, d_t, B, lam, lr, d_loss=None, stream=None):
         self.step += 1
-        _check(lib().sphr_mlp_train_step(self.h, d_x, d_t, B, lam, lr, self.step, d_loss, stream), "train_step")
+        _check(lib().sphr_mlp_train_step(self.h, _addr(d_x), _addr(d_t), int(B), lam, lr, self.step, _addr(d_loss), stream), "train_step")
 
     def seeds(self, B, d_seeds, stream=None):
-        _check(lib().sphr_mlp_seeds(self.h, B, d_seeds, stream), "seeds")
+        _check(lib().sphr_mlp_seeds(self.h, int(B), _addr(d_seeds), stream), "seeds")
 
     def __del__(self):
         try:
@@ -188,7 +194,7 @@ def train(mlp, d_x, d_t, n, epochs=8, batch=2048, lr=1e-3, lam=0.05, seed=0, log
         # cosine decay of the step size
         order = rng.permutation(nb)
         sampled = []
-        for i, b in enumerate(order):
+        for i, b in enumerate(order.tolist()):
             frac = (ep * nb + i) / max(1, epochs * nb)
             step_lr = lr * 0.5 * (1 + math.cos(math.pi * frac))
             mlp.train_step(d_x.ptr + 4 * mlp.n_in * b * batch, d_t.ptr + 4 * 3 * b * batch, batch, lam, step_lr, d_loss.ptr)
