@@ -28,7 +28,9 @@ HBM_PEAK_GBS = 8000.0
 
 def stage_bytes(info, S, esz, in_bytes_per_sample=8):
     """Algorithmic HBM bytes per launch of each stage: FFT, projection,
-    a_lm + Legendre ring sums, phi DFT (GEMM)."""
+    a_lm + Legendre ring sums, phi DFT (GEMM); with the fused sky kernel
+    the third stage is partials + rotation rows in, a_lm + maps out, and
+    the fourth is empty."""
     D, nb, N, L, CT, ns = info["detectors"], info["baselines"], info["N"], info["sky_l"], info["columns"], info["k_splits"]
     Nh = N // 2 + 1
     ncoef = (L + 1) ** 2
@@ -40,6 +42,8 @@ def stage_bytes(info, S, esz, in_bytes_per_sample=8):
     ring = S * n * nm * 2 * esz
     lf = ns * S * CT * esz + nb * ncoef * 2 * esz + S * ncoef * 2 * esz + ring
     phi = ring + 2 * nm * 2 * n * esz + S * n * n * 2 * esz
+    if info.get("fused"):
+        return [fft, xproj, ns * S * CT * esz + nb * ncoef * 2 * esz + S * ncoef * 2 * esz + S * n * n * 2 * esz, 0]
     return [fft, xproj, lf, phi]
 
 
@@ -355,7 +359,7 @@ def main():
     value = nseg / (rmx["ms_per_step"] * 1e-3)
     fp64_value = nseg / (r64["ms_per_step"] * 1e-3)
 
-    names = ["fft", "xproj", "alm_legendre", "phi_gemm"]
+    names = ["fft", "xproj", "sky_fused", "none"] if rmx["info"].get("fused") else ["fft", "xproj", "alm_legendre", "phi_gemm"]
     sb = stage_bytes(rmx["info"], nseg, rmx["esz"])
     k = int(np.argmax(rmx["stage_ms"]))
     achieved = sb[k] / (rmx["stage_ms"][k] * 1e-3) / 1e9

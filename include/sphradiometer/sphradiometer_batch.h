@@ -179,8 +179,16 @@ void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map)
 int sphr_batch_tune(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, double *best_ms);
 
 /* Shape information: [0] detectors, [1] baselines, [2] N, [3] sky l_max,
- * [4] projection columns, [5] K splits, [6] precision, [7] K per split */
-void sphr_batch_info(const struct sphr_batch *, int info[8]);
+ * [4] projection columns, [5] K splits, [6] precision, [7] K per split,
+ * [8] 1 when the sky stage runs as the one fused kernel (orders it is
+ * compiled for), 0 for the two-kernel form */
+void sphr_batch_info(const struct sphr_batch *, int info[9]);
+
+/* Select the sky stage's form: 1 (default) the fused kernel where it is
+ * compiled for the batch's order, 0 the two-kernel form (a_lm and ring
+ * sums, then the phi DFT as a GEMM).  Both give the same results to
+ * rounding. */
+void sphr_batch_set_fused(struct sphr_batch *, int on);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,7 @@ struct ev_set {
 struct sphr_batch {
 	int precision, D, nb, N, Nh, L, max_segments, nsplit, k_per_split;
 	int tpw;			/* projection tiles per wave (autotuned) */
+	int fused;			/* use the one-kernel sky stage where compiled (default 1) */
 	int ksplit_opt[3], nsplit_opt[3];	/* K-split candidates for 1, 2, 4 x CUs workgroups */
 	struct xproj_geom g;
 	struct synth_dev syn;
@@ -194,6 +195,7 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 		bt->k_per_split = bt->ksplit_opt[1];
 		bt->nsplit = bt->nsplit_opt[1];
 		bt->tpw = 4;
+		bt->fused = 1;
 	}
 	int *colbl = (int *) malloc((CT / 16) * sizeof(int));
 	const int ncoef = (bt->L + 1) * (bt->L + 1);
@@ -244,9 +246,13 @@ static int run_t(struct sphr_batch *bt, const void *ts, int f32, int S, const do
 		return -1;
 	if(ev)
 		hipEventRecord(ev[2], st);
-	/* a_lm and ring sums, then the phi DFT as a GEMM; the one-kernel form
-	 * when the coefficients do not fit the LDS budget */
-	if(launch_sky_lf<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) bt->F : (cplx<T> *) NULL, st) == 0) {
+	/* the fused sky kernel at the orders it is compiled for; else a_lm
+	 * and ring sums, then the phi DFT as a GEMM; the one-kernel form when
+	 * the coefficients do not fit the LDS budget */
+	if(bt->fused && launch_sky_fused<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st) == 0) {
+		if(ev)
+			hipEventRecord(ev[3], st);
+	} else if(launch_sky_lf<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) bt->F : (cplx<T> *) NULL, st) == 0) {
 		if(ev)
 			hipEventRecord(ev[3], st);
 		if(map && launch_phi<T>((const cplx<T> *) bt->F, S, &bt->syn.g, &bt->syn.tb, (cplx<T> *) map, st))
@@ -316,8 +322,15 @@ extern "C" int sphr_batch_profile_read(struct sphr_batch *bt, double ms[4], int 
 	return 0;
 }
 
-extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[8])
+extern "C" void sphr_batch_set_fused(struct sphr_batch *bt, int on)
 {
+	if(bt)
+		bt->fused = on ? 1 : 0;
+}
+
+extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[9])
+{
+	info[8] = bt->fused && sky_fused_order(bt->L);
 	info[0] = bt->D;
 	info[1] = bt->nb;
 	info[2] = bt->N;

@@ -70,6 +70,12 @@ int launch_sky(const T *partial, int nsplit, int S, const struct xproj_geom *xg,
  * GEMM into the meshes */
 template <typename T>
 int launch_sky_lf(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *F, hipStream_t st);
+/* the whole sky stage in one kernel (partials -> a_lm -> Legendre ring
+ * sums in LDS -> phi DFT on MFMA -> mesh) for the orders it is compiled
+ * for; -1 at other orders */
+int sky_fused_order(int L);	/* 1 when launch_sky_fused is compiled for order L */
+template <typename T>
+int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st);
 template <typename T>
 int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *map, hipStream_t st);
 /* GEMM-form synthesis of S packed series (order L, non-polar) through the

@@ -153,9 +153,10 @@ def test_network_integrate_three_site_and_golden(oracle, three_site):
     assert rel(mesh, gold["three_site_map_0"]) < TOL64
 
 
-def _run_batch(net, L, ts, precision, window=None, gmst=None):
+def _run_batch(net, L, ts, precision, window=None, gmst=None, fused=True):
     D, Sg, N = ts.shape
     b = S.Batch(net, L, precision, max_segments=Sg)
+    b.set_fused(fused)
     cdt = np.complex128 if precision == S.FP64 else np.complex64
     d_ts = S.DeviceBuffer.from_numpy(ts.astype(np.float64))
     d_alm = S.DeviceBuffer(Sg * (L + 1) ** 2 * np.dtype(cdt).itemsize)
@@ -195,6 +196,24 @@ def test_batch_fp64_three_site_ragged_and_gmst(oracle, three_site, nseg):
         sky, mesh = onet.segment(ts[:, s], 32, gmst=float(gm[s]))
         assert rel(alm[s], sky) < TOL64
         assert rel(mp[s], mesh) < TOL64
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_sky_stage_forms_agree(oracle, three_site, precision):
+    # the fused sky kernel (default at l_max 25 and 32) and the two-kernel
+    # form (ring sums, then the phi DFT as a GEMM) against the oracle
+    prec, tol = (S.FP64, TOL64) if precision == "fp64" else (S.MIXED, 1e-5)
+    ts, _ = inject.three_site(5, 1024, seed=11)
+    gm = np.linspace(0.3, 1.7, 5)
+    a1, m1, i1 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True)
+    a2, m2, i2 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=False)
+    assert i1["fused"] == 1 and i2["fused"] == 0
+    onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
+    for s in range(5):
+        sky, mesh = onet.segment(ts[:, s], 32, gmst=float(gm[s]))
+        for alm, mp in ((a1, m1), (a2, m2)):
+            assert rel(alm[s], sky) < tol
+            assert rel(mp[s], mesh) < (tol if prec == S.FP64 else 5e-6)
 
 
 def test_batch_full_size_three_site_properties(three_site):
