@@ -136,6 +136,14 @@ int synth_tables_build(int L, struct synth_dev *t)
 		t->tb.E64 = t->E64;
 		t->tb.E32 = t->E32;
 	}
+	if(sky_fused_order(L)) {
+		if(sky_fused_tables(L, &t->tb, &t->fz))
+			goto done;
+		t->tb.bf64 = t->fz.bf64;
+		t->tb.bf32 = t->fz.bf32;
+		t->tb.sk64 = t->fz.sk64;
+		t->tb.sk32 = t->fz.sk32;
+	}
 	rc = 0;
 done:
 	free(hx);
@@ -164,6 +172,14 @@ void synth_tables_free(struct synth_dev *t)
 		hipFree(t->E64);
 	if(t->E32)
 		hipFree(t->E32);
+	if(t->fz.bf64)
+		hipFree(t->fz.bf64);
+	if(t->fz.bf32)
+		hipFree(t->fz.bf32);
+	if(t->fz.sk64)
+		hipFree(t->fz.sk64);
+	if(t->fz.sk32)
+		hipFree(t->fz.sk32);
 	memset(t, 0, sizeof(*t));
 }
 

@@ -34,6 +34,19 @@ struct synth_tables {
 	const double *twp;	/* 2n: exp(+2 pi i t / n) */
 	const double *E64;	/* [2 (2L+1)][2 n]: the phi DFT, real-expanded */
 	const float *E32;
+	/* the fused sky kernel's tables at this order, NULL where it is not
+	 * compiled: B fragments of its phi DFT ([k-step][column tile][lane])
+	 * and the blob of small tables it stages into LDS */
+	const double *bf64;
+	const float *bf32;
+	const unsigned char *sk64, *sk32;
+};
+
+/* device allocations behind the fused kernel's tables */
+struct sky_fused_dev {
+	double *bf64;
+	float *bf32;
+	unsigned char *sk64, *sk32;
 };
 
 /* batched real FFT of length N (power of two), two real series per
@@ -74,6 +87,10 @@ int launch_sky_lf(const T *partial, int nsplit, int S, const struct xproj_geom *
  * sums in LDS -> phi DFT on MFMA -> mesh) for the orders it is compiled
  * for; -1 at other orders */
 int sky_fused_order(int L);	/* 1 when launch_sky_fused is compiled for order L */
+/* device tables of the fused kernel at order L, built from the synthesis
+ * tables tb (x, smm, alpha, twp set); all NULL at orders it is not
+ * compiled for; 0, or -1 with the error set */
+int sky_fused_tables(int L, const struct synth_tables *tb, struct sky_fused_dev *d);
 template <typename T>
 int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st);
 template <typename T>

@@ -16,11 +16,12 @@
  * 16-wide tile (one point for L = 32) and the rings beyond the last full
  * row tile (two for L = 32) are summed on the VALU.
  *
- * B operands (the DFT matrix) come from the n-entry twiddle table
- * exp(2 pi i t / n), t = m k mod n.  In float32 each wave keeps its B
- * fragments in registers for the life of the workgroup (the workgroups
- * are persistent over segments); in float64 they are formed in the loop,
- * hidden behind the longer f64 MFMA.
+ * B operands (the DFT matrix, entries exp(2 pi i t / n), t = m k mod n)
+ * are laid out once per order in the lane order of the MFMA fragments
+ * ([k-step][column tile][lane], coalesced).  In float32 each wave loads
+ * its B fragments into registers for the life of the workgroup (the
+ * workgroups are persistent over segments); in float64 they are read from
+ * the table (L2-resident) in the loop, behind the longer f64 MFMA.
  *
  * Legendre stage.  F[r][m] = sum_l a_lm Pbar_l|m|(x_r) for ring pairs
  * (x_j, -x_j) at once through the parity (-1)^(l+|m|).  Item (j, a)
@@ -28,7 +29,12 @@
  * steps (balanced lanes), each step serving m = +|m| and -|m|.  The
  * recurrence Pbar_l = alpha_l x Pbar_{l-1} - (alpha_l / alpha_{l-1})
  * Pbar_{l-2} uses a division-free coefficient table built once per
- * workgroup.
+ * workgroup, and runs two steps (one even, one odd l - |m|) per
+ * iteration so the parity needs no branch.
+ *
+ * Balance: the Legendre items that do not fill the last round land on
+ * the first wave, the VALU leftovers of the phi DFT that do not fill one
+ * round on the last wave.
  *
  * Layout (reference sh_series / mesh conventions, sh_series.c:33-80,
  * sh_series_to_mesh in sh_series_trans.c): a_lm in the m = 0, 1..L,
@@ -36,7 +42,9 @@
  * points phi_k = 2 pi k / n.
  */
 #include <hip/hip_runtime.h>
+#include <string.h>
 #include "kernels.h"
+#include "runtime.h"
 
 namespace {
 
@@ -47,12 +55,16 @@ template <> struct sf_mf<float> {
 	typedef float v4 __attribute__((ext_vector_type(4)));
 	static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 	static __device__ inline int row(int q, int e) { return 4 * q + e; }
+	static __device__ inline const float *btab(const struct synth_tables &tb) { return tb.bf32; }
+	static __device__ inline const unsigned char *blob(const struct synth_tables &tb) { return tb.sk32; }
 	enum { BREG = 1 };
 };
 template <> struct sf_mf<double> {
 	typedef double v4 __attribute__((ext_vector_type(4)));
 	static __device__ inline v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 	static __device__ inline int row(int q, int e) { return q + 4 * e; }
+	static __device__ inline const double *btab(const struct synth_tables &tb) { return tb.bf64; }
+	static __device__ inline const unsigned char *blob(const struct synth_tables &tb) { return tb.sk64; }
 	enum { BREG = 0 };
 };
 
@@ -85,197 +97,418 @@ template <int L> struct sf_geom {
 
 /* element (kr, c) of the real-expanded DFT matrix of one parity:
  * kr = 2 q + (0: re, 1: im of mode q), c = 2 k + (0: re, 1: im of point k) */
-template <typename T, int L>
-__device__ inline T sf_bval(const cplx<T> *tw, int m0, int nq, int kr, int c)
+template <int L>
+__device__ inline double sf_bval(const double *twp, int m0, int nq, int kr, int c)
 {
 	typedef sf_geom<L> G;
 	const int q = kr >> 1;
 	if(q >= nq)
-		return (T) 0;
+		return 0.0;
 	const int m = m0 + 2 * q, k = c >> 1;
 	int t = (m * k) % G::n;
 	if(t < 0)
 		t += G::n;
-	const cplx<T> w = tw[t];
+	const double wr = twp[2 * t], wi = twp[2 * t + 1];
 	const int ci = kr & 1, d = c & 1;
-	return ci == d ? w.re : (ci == 0 ? w.im : -w.im);
+	return ci == d ? wr : (ci == 0 ? wi : -wi);
 }
 
+/* B-fragment table: entry ((kidx NCT + ct) 64 + lane), kidx < KSE the even
+ * orders' k-steps, then the odd orders' */
 template <typename T, int L>
-__global__ void __launch_bounds__(SF_NT) k_sky_fused(const T *__restrict__ partial, int nsplit, int S, struct xproj_geom xg, const cplx<T> *__restrict__ R, const int2 *__restrict__ lm, const double *__restrict__ gmst, struct synth_tables tb, cplx<T> *__restrict__ alm, cplx<T> *__restrict__ map)
+__global__ void k_sky_bfrag(const double *__restrict__ twp, T *__restrict__ out)
+{
+	typedef sf_geom<L> G;
+	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+	if(idx >= (G::KSE + G::KSO) * G::NCT * 64)
+		return;
+	const int lane = idx & 63, ct = (idx >> 6) % G::NCT, kidx = (idx >> 6) / G::NCT;
+	const int li = lane & 15, lk = lane >> 4;
+	const int odd = kidx >= G::KSE, ks = odd ? kidx - G::KSE : kidx;
+	out[idx] = (T) sf_bval<L>(twp, odd ? G::MO0 : G::ME0, odd ? G::NO : G::NE, 4 * ks + lk, 16 * ct + li);
+}
+
+/* rings per Legendre item, recurrence steps per loaded batch (even) */
+#define SF_RG 2
+#define SF_US 4
+
+/* baselines handled with all rotation-row loads issued first; networks
+ * with more take the general loop */
+#define SF_NBF 4
+
+/* a_lm of one segment: sum over baselines of rotation row x the baseline's
+ * projected (l) column, then rotate_z; RP is an LDS or a global array of
+ * the rotation rows [nb][ncoef] */
+template <typename T, int L, typename RP>
+__device__ inline void sf_alm(RP Rr, const T *pw, const short *slm, const cplx<T> *rot, bool rz, const struct xproj_geom &xg, const int *csb, const int *lbb, int tid, cplx<T> *sa, cplx<T> *alm)
+{
+	typedef sf_geom<L> G;
+	const int nb = xg.nb;
+#pragma unroll 2
+	for(int idx = tid; idx < G::ncoef; idx += SF_NT) {
+		const int q = slm[idx];
+		const int l = q >> 8, m = (signed char) (q & 255);
+		cplx<T> acc = mk((T) 0, (T) 0);
+		if(nb <= SF_NBF) {
+			cplx<T> r[SF_NBF];
+#pragma unroll
+			for(int b = 0; b < SF_NBF; b++)
+				if(b < nb)
+					r[b] = Rr[b * G::ncoef + idx];
+#pragma unroll
+			for(int b = 0; b < SF_NBF; b++)
+				if(b < nb && l <= lbb[b]) {
+					const T *p = pw + csb[b] + 2 * l;
+					cfma(acc, r[b], mk(p[0], p[1]));
+				}
+		} else {
+			for(int b = 0; b < nb; b++)
+				if(l <= xg.Lb[b]) {
+					const T *p = pw + xg.col_start[b] + 2 * l;
+					cfma(acc, Rr[b * G::ncoef + idx], mk(p[0], p[1]));
+				}
+		}
+		if(rz && m)
+			acc = acc * (m > 0 ? rot[m] : conjg(rot[-m]));
+		sa[idx] = acc;
+		alm[idx] = acc;
+	}
+}
+
+/* the kernel's read-only tables as one blob, staged into LDS in one pass:
+ * recurrence coefficient pairs (alpha_l, alpha_l / alpha_{l-1}) with two
+ * zero pads, twiddles exp(2 pi i t / n), cos theta of the northern rings,
+ * Pbar_mm(x_j), and (l << 8) | (m & 255) per coefficient; each part
+ * 16-byte aligned */
+template <typename T, int L> struct sf_blob {
+	typedef sf_geom<L> G;
+	enum {
+		CF = 0,
+		TW = CF + (((G::nal + 2) * 2 * (int) sizeof(T) + 15) / 16) * 16,
+		XS = TW + ((G::n * 2 * (int) sizeof(T) + 15) / 16) * 16,
+		SM = XS + ((G::B * (int) sizeof(T) + 15) / 16) * 16,
+		LM = SM + ((G::B * (L + 1) * (int) sizeof(T) + 15) / 16) * 16,
+		BYTES = LM + ((G::ncoef * 2 + 15) / 16) * 16,
+	};
+};
+
+template <typename T, int L>
+__global__ void k_sky_blob(const double *__restrict__ alpha, const double *__restrict__ twp, const double *__restrict__ x, const double *__restrict__ sm, unsigned char *__restrict__ out)
+{
+	typedef sf_geom<L> G;
+	typedef sf_blob<T, L> Bl;
+	cplx<T> *cf = reinterpret_cast<cplx<T> *>(out + Bl::CF);
+	cplx<T> *tw = reinterpret_cast<cplx<T> *>(out + Bl::TW);
+	T *xs = reinterpret_cast<T *>(out + Bl::XS);
+	T *smm = reinterpret_cast<T *>(out + Bl::SM);
+	short *slm = reinterpret_cast<short *>(out + Bl::LM);
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if(t < G::nal) {
+		int l, m;
+		lm_decode(L, t, &l, &m);
+		const double a = l > m ? alpha[t] : 0.0;
+		cf[t] = mk((T) a, (T) (l > m + 1 ? a / alpha[t - 1] : 0.0));
+	} else if(t < G::nal + 2)
+		cf[t] = mk((T) 0, (T) 0);
+	if(t < G::n)
+		tw[t] = mk((T) twp[2 * t], (T) twp[2 * t + 1]);
+	if(t < G::B)
+		xs[t] = (T) x[t];
+	if(t < G::B * (L + 1))
+		smm[t] = (T) sm[t];
+	if(t < G::ncoef) {
+		int l, m;
+		lm_decode(L, t, &l, &m);
+		slm[t] = (short) ((l << 8) | (m & 255));
+	}
+}
+
+template <typename T, int L, int NSUB>
+__global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict__ partial, int nsplit, int S, struct xproj_geom xg, const cplx<T> *__restrict__ R, int rlds, const int2 *__restrict__ lm, const double *__restrict__ gmst, struct synth_tables tb, cplx<T> *__restrict__ alm, cplx<T> *__restrict__ map)
 {
 	typedef sf_geom<L> G;
 	typedef typename sf_mf<T>::v4 v4;
-	__shared__ T Fe[G::n * G::FS];		/* even orders [ring][2 q + re/im] */
-	__shared__ T Fo[G::n * G::FS];		/* odd orders */
-	__shared__ cplx<T> sa[G::ncoef];	/* a_lm of the segment */
-	__shared__ T ca[G::nal], cb[G::nal];	/* alpha_l, alpha_l / alpha_{l-1} */
-	__shared__ cplx<T> tw[G::n];		/* exp(2 pi i t / n) */
-	__shared__ cplx<T> rot[L + 1];		/* exp(-i m gmst) */
-	extern __shared__ unsigned char sf_dyn[];
-	T *pw = reinterpret_cast<T *>(sf_dyn);	/* [CT] partial sums */
+	/* per sub-group (one segment each) */
+	__shared__ T Fe_[NSUB][G::n * G::FS];	/* even orders [ring][2 q + re/im] */
+	__shared__ T Fo_[NSUB][G::n * G::FS];	/* odd orders */
+	__shared__ cplx<T> sa_[NSUB][G::ncoef];	/* a_lm of the segment */
+	__shared__ cplx<T> rot_[NSUB][L + 1];	/* exp(-i m gmst) */
+	/* shared by the sub-groups: the table blob */
+	typedef sf_blob<T, L> Bl;
+	__shared__ __align__(16) unsigned char tab[Bl::BYTES];
+	const cplx<T> *cf = reinterpret_cast<const cplx<T> *>(tab + Bl::CF);
+	const cplx<T> *tw = reinterpret_cast<const cplx<T> *>(tab + Bl::TW);
+	const T *xs = reinterpret_cast<const T *>(tab + Bl::XS);
+	const T *smm = reinterpret_cast<const T *>(tab + Bl::SM);
+	const short *slm = reinterpret_cast<const short *>(tab + Bl::LM);
+	extern __shared__ __align__(16) unsigned char sf_dyn[];
+	/* dynamic: the rotation rows [nb][ncoef] when rlds, then [NSUB][CT]
+	 * partial sums */
+	cplx<T> *sR = reinterpret_cast<cplx<T> *>(sf_dyn);
 
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int gtid = threadIdx.x, sub = gtid / SF_NT, tid = gtid % SF_NT;
+	const int lane = tid & 63, wave = tid >> 6;
 	const int li = lane & 15, lk = lane >> 4;
+	T *Fe = Fe_[sub], *Fo = Fo_[sub];
+	cplx<T> *sa = sa_[sub], *rot = rot_[sub];
+	T *pw = reinterpret_cast<T *>(sf_dyn + (rlds ? (size_t) xg.nb * G::ncoef * sizeof(cplx<T>) : 0)) + (size_t) sub * xg.CT;
 
-	/* once per workgroup: tables and the zero padding of the F tiles */
-	for(int t = tid; t < G::n * G::FS; t += SF_NT) {
-		Fe[t] = (T) 0;
-		Fo[t] = (T) 0;
+	/* once per workgroup: the table blob and the rotation rows to LDS
+	 * (all loads of a thread issued before its stores), the zero padding
+	 * of the F tiles */
+	{
+		constexpr int NT = SF_NT * NSUB, N16 = Bl::BYTES / 16, RB = (N16 + NT - 1) / NT;
+		const uint4 *src = reinterpret_cast<const uint4 *>(sf_mf<T>::blob(tb));
+		uint4 v[RB];
+#pragma unroll
+		for(int r = 0; r < RB; r++)
+			if(gtid + r * NT < N16)
+				v[r] = src[gtid + r * NT];
+		if(rlds) {
+			const int nr = xg.nb * G::ncoef;
+			for(int t = gtid; t < nr; t += 8 * NT) {
+				cplx<T> w[8];
+#pragma unroll
+				for(int u = 0; u < 8; u++)
+					if(t + u * NT < nr)
+						w[u] = R[t + u * NT];
+#pragma unroll
+				for(int u = 0; u < 8; u++)
+					if(t + u * NT < nr)
+						sR[t + u * NT] = w[u];
+			}
+		}
+#pragma unroll
+		for(int r = 0; r < RB; r++)
+			if(gtid + r * NT < N16)
+				reinterpret_cast<uint4 *>(tab)[gtid + r * NT] = v[r];
+		for(int t = gtid; t < NSUB * G::n * G::FS; t += NT) {
+			Fe_[0][t] = (T) 0;
+			Fo_[0][t] = (T) 0;
+		}
 	}
-	for(int t = tid; t < G::nal; t += SF_NT) {
-		int l, m;
-		lm_decode(L, t, &l, &m);
-		const double a = l > m ? tb.alpha[t] : 0.0;
-		ca[t] = (T) a;
-		cb[t] = (T) (l > m + 1 ? a / tb.alpha[t - 1] : 0.0);
+	/* per-baseline columns and orders of the first baselines, uniform */
+	int csb[SF_NBF], lbb[SF_NBF];
+#pragma unroll
+	for(int b = 0; b < SF_NBF; b++) {
+		csb[b] = xg.col_start[b];
+		lbb[b] = xg.Lb[b];
 	}
-	for(int t = tid; t < G::n; t += SF_NT)
-		tw[t] = mk((T) tb.twp[2 * t], (T) tb.twp[2 * t + 1]);
 	__syncthreads();
 
-	/* B fragments of this lane: column tile ct, k-step ks */
-	T bfe[sf_mf<T>::BREG ? G::KSE : 1][G::NCT], bfo[sf_mf<T>::BREG ? G::KSO : 1][G::NCT];
+	/* B fragments of this lane, [k-step][column tile] (float32: kept in
+	 * registers) */
+	const T *__restrict__ btab = sf_mf<T>::btab(tb);
+	T bf[sf_mf<T>::BREG ? G::KSE + G::KSO : 1][G::NCT];
 	if(sf_mf<T>::BREG) {
 #pragma unroll
-		for(int ks = 0; ks < (sf_mf<T>::BREG ? G::KSE : 1); ks++)
+		for(int k = 0; k < (sf_mf<T>::BREG ? G::KSE + G::KSO : 1); k++)
 #pragma unroll
 			for(int ct = 0; ct < G::NCT; ct++)
-				bfe[ks][ct] = sf_bval<T, L>(tw, G::ME0, G::NE, 4 * ks + lk, 16 * ct + li);
-#pragma unroll
-		for(int ks = 0; ks < (sf_mf<T>::BREG ? G::KSO : 1); ks++)
-#pragma unroll
-			for(int ct = 0; ct < G::NCT; ct++)
-				bfo[ks][ct] = sf_bval<T, L>(tw, G::MO0, G::NO, 4 * ks + lk, 16 * ct + li);
+				bf[k][ct] = btab[(k * G::NCT + ct) * 64 + lane];
 	}
 
-	for(int s = blockIdx.x; s < S; s += gridDim.x) {
+	for(int base = blockIdx.x * NSUB; base < S; base += gridDim.x * NSUB) {
+		const int s = base + sub;
+		const bool act = s < S;
 		/* ---- partial sums over the K splits, rotate_z phases ---- */
-		for(int c = tid; c < xg.CT; c += SF_NT) {
-			T v = 0;
-			for(int sp = 0; sp < nsplit; sp++)
-				v += partial[((size_t) sp * S + s) * xg.CT + c];
-			pw[c] = v;
-		}
-		if(tid <= L) {
-			double sn = 0.0, cs = 1.0;
-			if(gmst && tid)
-				sincos(-(double) tid * gmst[s], &sn, &cs);
-			rot[tid] = mk((T) cs, (T) sn);
+		if(act) {
+			for(int c = tid; c < xg.CT; c += SF_NT) {
+				/* four independent loads in flight */
+				const T *p = partial + (size_t) s * xg.CT + c;
+				const size_t st = (size_t) S * xg.CT;
+				T v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+				int sp = 0;
+				for(; sp + 4 <= nsplit; sp += 4) {
+					v0 += p[sp * st];
+					v1 += p[(sp + 1) * st];
+					v2 += p[(sp + 2) * st];
+					v3 += p[(sp + 3) * st];
+				}
+				for(; sp < nsplit; sp++)
+					v0 += p[sp * st];
+				pw[c] = (v0 + v1) + (v2 + v3);
+			}
+			if(tid <= L) {
+				double sn = 0.0, cs = 1.0;
+				if(gmst && tid)
+					sincos(-(double) tid * gmst[s], &sn, &cs);
+				rot[tid] = mk((T) cs, (T) sn);
+			}
 		}
 		__syncthreads();
 
 		/* ---- a_lm: rotation rows over the baselines, rotate_z ---- */
-		for(int idx = tid; idx < G::ncoef; idx += SF_NT) {
-			const int2 q = lm[idx];
-			const int l = q.x, m = q.y;
-			cplx<T> acc = mk((T) 0, (T) 0);
-			for(int b = 0; b < xg.nb; b++) {
-				if(l > xg.Lb[b])
-					continue;
-				const T *p = pw + xg.col_start[b] + 2 * l;
-				cfma(acc, R[(size_t) b * G::ncoef + idx], mk(p[0], p[1]));
-			}
-			if(gmst && m)
-				acc = acc * (m > 0 ? rot[m] : conjg(rot[-m]));
-			sa[idx] = acc;
-			alm[(size_t) s * G::ncoef + idx] = acc;
+		if(act) {
+			if(rlds)
+				sf_alm<T, L>(sR, pw, slm, rot, gmst != NULL, xg, csb, lbb, tid, sa, alm + (size_t) s * G::ncoef);
+			else
+				sf_alm<T, L>(R, pw, slm, rot, gmst != NULL, xg, csb, lbb, tid, sa, alm + (size_t) s * G::ncoef);
 		}
 		__syncthreads();
 		if(!map)
 			continue;
 
-		/* ---- Legendre ring sums into the F tiles ---- */
-		for(int it = tid; it < G::B * G::NA; it += SF_NT) {
-			const int j = it % G::B, a = it / G::B;
-			const T x = (T) tb.x[j];
+		/* ---- Legendre ring sums into the F tiles: item (a, ring group)
+		 * runs the recurrences of SF_RG rings at once, so each
+		 * coefficient and a_lm read from LDS serves SF_RG rings ---- */
+		if(act) {
+			constexpr int NG = (G::B + SF_RG - 1) / SF_RG;
+			for(int it = tid; it < NG * G::NA; it += SF_NT) {
+				const int tg = it % NG, a = it / NG;
+				T x[SF_RG];
 #pragma unroll
-			for(int h = 0; h < 2; h++) {
-				const int am = h ? L - a : a;
-				if(h && am == a)
-					break;
-				const int abase = lm_offset(L, am, am);	/* = coefficient table slot */
-				const int anbase = am ? lm_offset(L, am, -am) : 0;
-				T pprev = 0, pcur = (T) tb.smm[(size_t) j * (L + 1) + am];
-				cplx<T> Ep = mk((T) 0, (T) 0), Op = Ep, En = Ep, On = Ep;
-				for(int l = am; l <= L; l++) {
-					if(l > am) {
-						const T pn = ca[abase + l - am] * x * pcur - cb[abase + l - am] * pprev;
-						pprev = pcur;
-						pcur = pn;
-					}
-					const cplx<T> cp = sa[abase + l - am];
-					const cplx<T> cn = am ? sa[anbase + l - am] : cp;
-					if((l - am) & 1) {
-						cfma(Op, cp, pcur);
-						cfma(On, cn, pcur);
-					} else {
-						cfma(Ep, cp, pcur);
-						cfma(En, cn, pcur);
-					}
+				for(int r = 0; r < SF_RG; r++) {
+					const int j = SF_RG * tg + r;
+					x[r] = j < G::B ? xs[j] : (T) 0;
 				}
-				/* m = +am, then m = -am (Pbar_l,-m = (-1)^m Pbar_lm) */
 #pragma unroll
-				for(int sg = 0; sg < 2; sg++) {
-					if(sg && !am)
+				for(int h = 0; h < 2; h++) {
+					const int am = h ? L - a : a;
+					if(h && am == a)
 						break;
-					const int m = sg ? -am : am;
-					const T f = (sg && (am & 1)) ? (T) -1 : (T) 1;
-					const cplx<T> N = (sg ? En + On : Ep + Op) * f, Sth = (sg ? En - On : Ep - Op) * f;
-					T *Fp = (m & 1) ? Fo : Fe;
-					const int col = 2 * ((m - ((m & 1) ? G::MO0 : G::ME0)) >> 1);
-					Fp[j * G::FS + col] = N.re;
-					Fp[j * G::FS + col + 1] = N.im;
-					Fp[(G::n - 1 - j) * G::FS + col] = Sth.re;
-					Fp[(G::n - 1 - j) * G::FS + col + 1] = Sth.im;
+					const int cbase = lm_offset(L, am, am);	/* coefficient table slot too */
+					const int nbase = am ? lm_offset(L, am, -am) : cbase;
+					const cplx<T> *sp = sa + cbase, *sn = sa + nbase, *cc = cf + cbase;
+					const int cnt = L - am;	/* steps after l = am */
+					T p2[SF_RG], p1[SF_RG];
+					cplx<T> Ep[SF_RG], En[SF_RG], Op[SF_RG], On[SF_RG];
+					{
+						const cplx<T> s0 = sp[0], n0 = sn[0];
+						const cplx<T> c1 = cc[1], s1 = sp[1], n1 = sn[1];
+#pragma unroll
+						for(int r = 0; r < SF_RG; r++) {
+							const int j = SF_RG * tg + r;
+							p2[r] = j < G::B ? smm[j * (L + 1) + am] : (T) 0;
+							Ep[r] = s0 * p2[r];
+							En[r] = n0 * p2[r];
+							/* l = am + 1 (c1.im is 0; unused past the end) */
+							p1[r] = cnt >= 1 ? c1.re * x[r] * p2[r] : (T) 0;
+							Op[r] = s1 * p1[r];
+							On[r] = n1 * p1[r];
+						}
+					}
+					/* SF_US steps (E, O, ...) per iteration with all
+					 * operands loaded first, then two, then one */
+					int i = 2;
+					for(; i + SF_US - 1 <= cnt; i += SF_US) {
+						cplx<T> c[SF_US], a4[SF_US], b4[SF_US];
+#pragma unroll
+						for(int u = 0; u < SF_US; u++) {
+							c[u] = cc[i + u];
+							a4[u] = sp[i + u];
+							b4[u] = sn[i + u];
+						}
+#pragma unroll
+						for(int r = 0; r < SF_RG; r++) {
+#pragma unroll
+							for(int u = 0; u < SF_US; u += 2) {
+								const T pe = c[u].re * x[r] * p1[r] - c[u].im * p2[r];
+								const T po = c[u + 1].re * x[r] * pe - c[u + 1].im * p1[r];
+								cfma(Ep[r], a4[u], pe);
+								cfma(En[r], b4[u], pe);
+								cfma(Op[r], a4[u + 1], po);
+								cfma(On[r], b4[u + 1], po);
+								p2[r] = pe;
+								p1[r] = po;
+							}
+						}
+					}
+					if(i + 1 <= cnt) {
+						const cplx<T> c0 = cc[i], c1 = cc[i + 1], a0 = sp[i], b0 = sn[i], a1 = sp[i + 1], b1 = sn[i + 1];
+#pragma unroll
+						for(int r = 0; r < SF_RG; r++) {
+							const T pe = c0.re * x[r] * p1[r] - c0.im * p2[r];
+							const T po = c1.re * x[r] * pe - c1.im * p1[r];
+							cfma(Ep[r], a0, pe);
+							cfma(En[r], b0, pe);
+							cfma(Op[r], a1, po);
+							cfma(On[r], b1, po);
+							p2[r] = pe;
+							p1[r] = po;
+						}
+						i += 2;
+					}
+					if(i == cnt) {
+						const cplx<T> cl = cc[i], a0 = sp[i], b0 = sn[i];
+#pragma unroll
+						for(int r = 0; r < SF_RG; r++) {
+							const T pe = cl.re * x[r] * p1[r] - cl.im * p2[r];
+							cfma(Ep[r], a0, pe);
+							cfma(En[r], b0, pe);
+						}
+					}
+					/* m = +am, then m = -am (Pbar_l,-m = (-1)^m Pbar_lm) */
+#pragma unroll
+					for(int sg = 0; sg < 2; sg++) {
+						if(sg && !am)
+							break;
+						const int m = sg ? -am : am;
+						const T f = (sg && (am & 1)) ? (T) -1 : (T) 1;
+						T *Fp = (m & 1) ? Fo : Fe;
+						const int col = 2 * ((m - ((m & 1) ? G::MO0 : G::ME0)) >> 1);
+#pragma unroll
+						for(int r = 0; r < SF_RG; r++) {
+							const int j = SF_RG * tg + r;
+							if(j >= G::B)
+								continue;
+							const cplx<T> N = (sg ? En[r] + On[r] : Ep[r] + Op[r]) * f, Sth = (sg ? En[r] - On[r] : Ep[r] - Op[r]) * f;
+							Fp[j * G::FS + col] = N.re;
+							Fp[j * G::FS + col + 1] = N.im;
+							Fp[(G::n - 1 - j) * G::FS + col] = Sth.re;
+							Fp[(G::n - 1 - j) * G::FS + col + 1] = Sth.im;
+						}
+					}
 				}
 			}
 		}
 		__syncthreads();
 
-		/* ---- phi DFT of the full tiles on MFMA ---- */
-		T *mr = reinterpret_cast<T *>(map + (size_t) s * G::n * G::n);	/* [n][2n] real */
-		for(int rt = wave; rt < G::NRT; rt += SF_NT / 64) {
-			v4 ae[G::NCT], ao[G::NCT];
-#pragma unroll
-			for(int ct = 0; ct < G::NCT; ct++) {
-				ae[ct] = (v4) 0;
-				ao[ct] = (v4) 0;
-			}
-			const T *fe = Fe + (16 * rt + li) * G::FS + lk, *fo = Fo + (16 * rt + li) * G::FS + lk;
-#pragma unroll
-			for(int ks = 0; ks < G::KSE; ks++) {
-				const T av = fe[4 * ks];
+		if(act) {
+			/* ---- phi DFT of the full tiles on MFMA ---- */
+			T *mr = reinterpret_cast<T *>(map + (size_t) s * G::n * G::n);	/* [n][2n] real */
+			for(int rt = wave; rt < G::NRT; rt += SF_NT / 64) {
+				v4 ae[G::NCT], ao[G::NCT];
 #pragma unroll
 				for(int ct = 0; ct < G::NCT; ct++) {
-					const T bv = sf_mf<T>::BREG ? bfe[sf_mf<T>::BREG ? ks : 0][ct] : sf_bval<T, L>(tw, G::ME0, G::NE, 4 * ks + lk, 16 * ct + li);
-					ae[ct] = sf_mf<T>::mma(av, bv, ae[ct]);
+					ae[ct] = (v4) 0;
+					ao[ct] = (v4) 0;
 				}
+				const T *fe = Fe + (16 * rt + li) * G::FS + lk, *fo = Fo + (16 * rt + li) * G::FS + lk;
+#pragma unroll
+				for(int ks = 0; ks < G::KSE; ks++) {
+					const T av = fe[4 * ks];
+#pragma unroll
+					for(int ct = 0; ct < G::NCT; ct++) {
+						const T bv = sf_mf<T>::BREG ? bf[sf_mf<T>::BREG ? ks : 0][ct] : btab[(ks * G::NCT + ct) * 64 + lane];
+						ae[ct] = sf_mf<T>::mma(av, bv, ae[ct]);
+					}
+				}
+#pragma unroll
+				for(int ks = 0; ks < G::KSO; ks++) {
+					const T av = fo[4 * ks];
+#pragma unroll
+					for(int ct = 0; ct < G::NCT; ct++) {
+						const T bv = sf_mf<T>::BREG ? bf[sf_mf<T>::BREG ? G::KSE + ks : 0][ct] : btab[((G::KSE + ks) * G::NCT + ct) * 64 + lane];
+						ao[ct] = sf_mf<T>::mma(av, bv, ao[ct]);
+					}
+				}
+#pragma unroll
+				for(int ct = 0; ct < G::NCT; ct++)
+#pragma unroll
+					for(int e = 0; e < 4; e++) {
+						const int r = 16 * rt + sf_mf<T>::row(lk, e), c = 16 * ct + li;
+						mr[(size_t) r * 2 * G::n + c] = ae[ct][e] + ao[ct][e];
+						mr[(size_t) r * 2 * G::n + G::n + c] = ae[ct][e] - ao[ct][e];
+					}
 			}
-#pragma unroll
-			for(int ks = 0; ks < G::KSO; ks++) {
-				const T av = fo[4 * ks];
-#pragma unroll
-				for(int ct = 0; ct < G::NCT; ct++) {
-					const T bv = sf_mf<T>::BREG ? bfo[sf_mf<T>::BREG ? ks : 0][ct] : sf_bval<T, L>(tw, G::MO0, G::NO, 4 * ks + lk, 16 * ct + li);
-					ao[ct] = sf_mf<T>::mma(av, bv, ao[ct]);
-				}
-			}
-#pragma unroll
-			for(int ct = 0; ct < G::NCT; ct++)
-#pragma unroll
-				for(int e = 0; e < 4; e++) {
-					const int r = 16 * rt + sf_mf<T>::row(lk, e), c = 16 * ct + li;
-					mr[(size_t) r * 2 * G::n + c] = ae[ct][e] + ao[ct][e];
-					mr[(size_t) r * 2 * G::n + G::n + c] = ae[ct][e] - ao[ct][e];
-				}
-		}
 
-		/* ---- leftover rings (all points) and points (full tiles'
-		 * rings) on the VALU: one lane per (ring, point, parity) ---- */
-		{
+			/* ---- leftover rings (all points) and points (full tiles'
+			 * rings) on the VALU: one lane per (ring, point, parity),
+			 * units in reverse thread order so a partial last round
+			 * lands on the last wave ---- */
 			const int ntask = G::XR * G::B + 16 * G::NRT * G::XP;
-			for(int u = tid; u < 2 * ntask; u += SF_NT) {
+			for(int u = SF_NT - 1 - tid; u < 2 * ntask; u += SF_NT) {
 				const int task = u >> 1, par = u & 1;
 				int r, k;
 				if(task < G::XR * G::B) {
@@ -288,18 +521,42 @@ __global__ void __launch_bounds__(SF_NT) k_sky_fused(const T *__restrict__ parti
 				}
 				const T *Fr = (par ? Fo : Fe) + r * G::FS;
 				const int nq = par ? G::NO : G::NE, m0 = par ? G::MO0 : G::ME0;
-				cplx<T> acc = mk((T) 0, (T) 0);
+				cplx<T> acc = mk((T) 0, (T) 0), acc2 = acc;
 				int t = (m0 * k) % G::n;
 				if(t < 0)
 					t += G::n;
 				const int dt = (2 * k) % G::n;
-				for(int q = 0; q < nq; q++) {
+				/* four terms per iteration, loads first */
+				int q = 0;
+				for(; q + 4 <= nq; q += 4) {
+					int tq[4];
+#pragma unroll
+					for(int u4 = 0; u4 < 4; u4++) {
+						tq[u4] = t;
+						t += dt;
+						if(t >= G::n)
+							t -= G::n;
+					}
+					cplx<T> f[4], w[4];
+#pragma unroll
+					for(int u4 = 0; u4 < 4; u4++) {
+						f[u4] = mk(Fr[2 * (q + u4)], Fr[2 * (q + u4) + 1]);
+						w[u4] = tw[tq[u4]];
+					}
+					cfma(acc, f[0], w[0]);
+					cfma(acc2, f[1], w[1]);
+					cfma(acc, f[2], w[2]);
+					cfma(acc2, f[3], w[3]);
+				}
+				for(; q < nq; q++) {
 					cfma(acc, mk(Fr[2 * q], Fr[2 * q + 1]), tw[t]);
 					t += dt;
 					if(t >= G::n)
 						t -= G::n;
 				}
-				/* the pair (Ev, Od) sits in lanes u, u ^ 1 of one wave */
+				acc = acc + acc2;
+				/* the pair (Ev, Od) sits in lanes tid, tid ^ 1 of one
+				 * wave (u and u ^ 1: both in range or both out) */
 				const T ore = __shfl_xor(acc.re, 1), oim = __shfl_xor(acc.im, 1);
 				if(!par) {
 					cplx<T> *row = map + (size_t) s * G::n * G::n + (size_t) r * G::n;
@@ -319,11 +576,59 @@ int sky_fused_order(int L)
 	return L == 25 || L == 32;
 }
 
-/* the orders with a compiled fused kernel */
-template <typename T, int L>
-static int sky_fused_at(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int grid, hipStream_t st)
+template <int L>
+static int sky_tables_at(const struct synth_tables *tb, struct sky_fused_dev *d)
 {
-	hipLaunchKernelGGL((k_sky_fused<T, L>), dim3(grid), dim3(SF_NT), xg->CT * sizeof(T), st, partial, nsplit, S, *xg, R, lm, gmst, *tb, alm, map);
+	typedef sf_geom<L> G;
+	const int ne = (G::KSE + G::KSO) * G::NCT * 64;
+	const int nb64 = sf_blob<double, L>::BYTES, nb32 = sf_blob<float, L>::BYTES;
+	if(sphr_rt_check(hipMalloc(&d->bf64, ne * sizeof(double)), "hipMalloc") || sphr_rt_check(hipMalloc(&d->bf32, ne * sizeof(float)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&d->sk64, nb64), "hipMalloc") || sphr_rt_check(hipMalloc(&d->sk32, nb32), "hipMalloc"))
+		return -1;
+	hipLaunchKernelGGL((k_sky_bfrag<double, L>), dim3((ne + 255) / 256), dim3(256), 0, 0, tb->twp, d->bf64);
+	hipLaunchKernelGGL((k_sky_bfrag<float, L>), dim3((ne + 255) / 256), dim3(256), 0, 0, tb->twp, d->bf32);
+	const int nt = G::ncoef > G::B * (L + 1) ? G::ncoef : G::B * (L + 1);
+	hipLaunchKernelGGL((k_sky_blob<double, L>), dim3((nt + 255) / 256), dim3(256), 0, 0, tb->alpha, tb->twp, tb->x, tb->smm, d->sk64);
+	hipLaunchKernelGGL((k_sky_blob<float, L>), dim3((nt + 255) / 256), dim3(256), 0, 0, tb->alpha, tb->twp, tb->x, tb->smm, d->sk32);
+	return sphr_rt_check(hipGetLastError(), "sky tables") || sphr_rt_check(hipDeviceSynchronize(), "sky tables") ? -1 : 0;
+}
+
+int sky_fused_tables(int L, const struct synth_tables *tb, struct sky_fused_dev *d)
+{
+	memset(d, 0, sizeof(*d));
+	switch(L) {
+	case 25:
+		return sky_tables_at<25>(tb, d);
+	case 32:
+		return sky_tables_at<32>(tb, d);
+	default:
+		return 0;
+	}
+}
+
+/* the orders with a compiled fused kernel */
+/* sub-groups (segments in flight) per workgroup: float32 two, sharing the
+ * tables; float64 one (its F tiles take most of the LDS) */
+template <typename T> struct sf_sub {
+	enum { N = sizeof(T) == 4 ? 2 : 1 };
+};
+
+template <typename T, int L>
+static int sky_fused_at(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int ncu, hipStream_t st)
+{
+	typedef sf_geom<L> G;
+	const int NS = sf_sub<T>::N;
+	/* static LDS of the kernel (F tiles, a_lm, tables), with slack */
+	const size_t stat = NS * (2 * (size_t) G::n * G::FS * sizeof(T) + (size_t) (G::ncoef + L + 1) * sizeof(cplx<T>))
+			    + (size_t) (G::nal + 2 + G::n) * sizeof(cplx<T>) + G::ncoef * sizeof(short) + (size_t) G::B * (L + 2) * sizeof(T) + 1024;
+	const size_t rbytes = (size_t) xg->nb * G::ncoef * sizeof(cplx<T>), pbytes = (size_t) NS * xg->CT * sizeof(T);
+	/* the rotation rows go to LDS when they fit beside the rest (160 KiB
+	 * per CU, one workgroup per CU) */
+	const int rlds = stat + rbytes + pbytes <= 160 * 1024;
+	const size_t dyn = (rlds ? rbytes : 0) + pbytes;
+	const int nwg = (S + NS - 1) / NS;
+	const int grid = nwg < ncu ? nwg : ncu;
+	hipLaunchKernelGGL((k_sky_fused<T, L, sf_sub<T>::N>), dim3(grid), dim3(SF_NT * NS), dyn, st, partial, nsplit, S, *xg, R, rlds, lm, gmst, *tb, alm, map);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -339,14 +644,12 @@ int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geo
 	}
 	if(S <= 0)
 		return 0;
-	/* persistent workgroups: two per CU in float32, one in float64 (LDS) */
-	const int per_cu = sizeof(T) == 4 ? 2 : 1;
-	const int grid = S < per_cu * ncu ? S : per_cu * ncu;
+	/* persistent workgroups, one per CU */
 	switch(g->L) {
 	case 25:
-		return sky_fused_at<T, 25>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, grid, st);
+		return sky_fused_at<T, 25>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, st);
 	case 32:
-		return sky_fused_at<T, 32>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, grid, st);
+		return sky_fused_at<T, 32>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, st);
 	default:
 		return -1;
 	}

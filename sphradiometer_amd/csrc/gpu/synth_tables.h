@@ -19,6 +19,7 @@ struct synth_dev {
 	 * E[i][k] = exp(2 pi i k m / n), expanded as (Er, Ei) / (-Ei, Er) */
 	double *E64;
 	float *E32;
+	struct sky_fused_dev fz;	/* fused sky kernel tables (orders it is compiled for) */
 };
 
 /* the l_seed low-pass mesh of the extremizer's seed search */
