@@ -173,16 +173,17 @@ int sphr_jobs_tune(struct sphr_jobs *, const float *in, void *alm, double *best_
 void sphr_jobs_info(const struct sphr_jobs *, long info[8]);
 void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map);
 
-/* Autotuner: runs the batch on these buffers with each candidate tile
- * width and K split of the projection, keeps the fastest, returns its
- * index (time per run in best_ms) or -1 */
+/* Autotuner: runs the batch on these buffers with each candidate shape of
+ * the projection (tiles per wave, segments per workgroup, K split), keeps
+ * the fastest, returns its index (time per run in best_ms) or -1 */
 int sphr_batch_tune(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, double *best_ms);
 
 /* Shape information: [0] detectors, [1] baselines, [2] N, [3] sky l_max,
  * [4] projection columns, [5] K splits, [6] precision, [7] K per split,
  * [8] 1 when the sky stage runs as the one fused kernel (orders it is
- * compiled for), 0 for the two-kernel form */
-void sphr_batch_info(const struct sphr_batch *, int info[9]);
+ * compiled for), 0 for the two-kernel form, [9] projection tiles per wave,
+ * [10] projection segments per workgroup */
+void sphr_batch_info(const struct sphr_batch *, int info[11]);
 
 /* Select the sky stage's form: 1 (default) the fused kernel where it is
  * compiled for the batch's order, 0 the two-kernel form (a_lm and ring

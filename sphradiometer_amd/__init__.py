@@ -484,9 +484,10 @@ class Batch:
         self.network = network
 
     def info(self):
-        a = (c_int * 9)()
+        a = (c_int * 11)()
         lib().sphr_batch_info(self.h, a)
-        keys = ["detectors", "baselines", "N", "sky_l", "columns", "k_splits", "precision", "k_per_split", "fused"]
+        keys = ["detectors", "baselines", "N", "sky_l", "columns", "k_splits", "precision", "k_per_split", "fused",
+                "tiles_per_wave", "segments_per_workgroup"]
         return dict(zip(keys, list(a)))
 
     def set_fused(self, on):

@@ -32,8 +32,9 @@ struct ev_set {
 struct sphr_batch {
 	int precision, D, nb, N, Nh, L, max_segments, nsplit, k_per_split;
 	int tpw;			/* projection tiles per wave (autotuned) */
+	int xms;			/* projection segments per workgroup, 16 or 32 (autotuned) */
 	int fused;			/* use the one-kernel sky stage where compiled (default 1) */
-	int ksplit_opt[3], nsplit_opt[3];	/* K-split candidates for 1, 2, 4 x CUs workgroups */
+	int ksplit_opt[4], nsplit_opt[4];	/* K-split candidates: 1, 2, 4 x CUs workgroups, 64 per split */
 	struct xproj_geom g;
 	struct synth_dev syn;
 	void *W, *R, *tw, *spec, *partial, *F;
@@ -192,9 +193,13 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 			bt->ksplit_opt[o] = kps;
 			bt->nsplit_opt[o] = (bt->g.K2 + kps - 1) / kps;
 		}
+		/* one 64-wide chunk of K per workgroup */
+		bt->ksplit_opt[3] = 64;
+		bt->nsplit_opt[3] = (bt->g.K2 + 63) / 64;
 		bt->k_per_split = bt->ksplit_opt[1];
 		bt->nsplit = bt->nsplit_opt[1];
-		bt->tpw = 4;
+		bt->tpw = 0;
+		bt->xms = 64;
 		bt->fused = 1;
 	}
 	int *colbl = (int *) malloc((CT / 16) * sizeof(int));
@@ -222,7 +227,7 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 	if(synth_tables_build(bt->L, &bt->syn))
 		goto fail;
 	if(sphr_rt_check(hipMalloc(&bt->spec, (size_t) bt->D * max_segments * bt->Nh * 2 * esz), "hipMalloc spectra")
-	   || sphr_rt_check(hipMalloc(&bt->partial, (size_t) bt->nsplit_opt[2] * max_segments * CT * esz), "hipMalloc partials")
+	   || sphr_rt_check(hipMalloc(&bt->partial, (size_t) bt->nsplit_opt[3] * max_segments * CT * esz), "hipMalloc partials")
 	   || sphr_rt_check(hipMalloc(&bt->F, (size_t) max_segments * 2 * (bt->L + 1) * (2 * bt->L + 1) * 2 * esz), "hipMalloc ring sums"))
 		goto fail;
 	return bt;
@@ -242,7 +247,7 @@ static int run_t(struct sphr_batch *bt, const void *ts, int f32, int S, const do
 		return -1;
 	if(ev)
 		hipEventRecord(ev[1], st);
-	if(launch_xproj<T>((const cplx<T> *) bt->spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) bt->partial, st, bt->tpw))
+	if(launch_xproj<T>((const cplx<T> *) bt->spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) bt->partial, st, bt->tpw, bt->xms))
 		return -1;
 	if(ev)
 		hipEventRecord(ev[2], st);
@@ -328,9 +333,11 @@ extern "C" void sphr_batch_set_fused(struct sphr_batch *bt, int on)
 		bt->fused = on ? 1 : 0;
 }
 
-extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[9])
+extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[11])
 {
 	info[8] = bt->fused && sky_fused_order(bt->L);
+	info[9] = bt->tpw;
+	info[10] = bt->xms;
 	info[0] = bt->D;
 	info[1] = bt->nb;
 	info[2] = bt->N;
@@ -373,16 +380,19 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	hipEvent_t e0, e1;
 	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
 		return -1;
-	static const int tpws[3] = { 1, 2, 4 };
+	/* shapes: one baseline per workgroup (tpw 0), then tiles per wave x
+	 * segments per workgroup */
+	static const int tpws[7] = { 0, 1, 2, 4, 1, 2, 4 }, mss[7] = { 64, 16, 16, 16, 32, 32, 32 };
 	float best = 1e30f;
-	int bo = 1, bw = 2;
+	int bo = 1, bw = 0;
 	const int prof = bt->profile;
 	bt->profile = 0;
-	for(int o = 0; o < 3; o++)
-		for(int w = 0; w < 3; w++) {
+	for(int o = 0; o < 4; o++)
+		for(int w = 0; w < 7; w++) {
 			bt->k_per_split = bt->ksplit_opt[o];
 			bt->nsplit = bt->nsplit_opt[o];
 			bt->tpw = tpws[w];
+			bt->xms = mss[w];
 			if(sphr_batch_run(bt, ts, f32, S, window, gmst, alm, map, st))
 				return -1;
 			hipEventRecord(e0, st);
@@ -405,7 +415,8 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	bt->k_per_split = bt->ksplit_opt[bo];
 	bt->nsplit = bt->nsplit_opt[bo];
 	bt->tpw = tpws[bw];
+	bt->xms = mss[bw];
 	if(best_ms)
 		*best_ms = best / 3.0;
-	return 3 * bo + bw;
+	return 7 * bo + bw;
 }

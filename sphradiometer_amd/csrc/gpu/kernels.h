@@ -59,7 +59,7 @@ int launch_rfft_pairs(const void *in, int in_f32, const double *window, int nser
  * column tile t; k_per_split a multiple of the K chunk (32); tpw the
  * 16-column tiles per wave (1, 2 or 4) */
 template <typename T>
-int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st, int tpw);
+int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st, int tpw, int ms);
 
 /* sum partials, rotate, average baselines, rotate_z: alm[S][(L+1)^2] */
 template <typename T, typename TO>

@@ -95,11 +95,129 @@ __global__ void __launch_bounds__(NT) k_rfft_pairs(const TI *__restrict__ in, co
 	}
 }
 
+/*
+ * The same transform for N = 1024 NQ (a power of 4): radix-4 Stockham
+ * (autosort, natural order in and out) with 256 threads, NQ butterflies
+ * per thread per stage.  Stage 0 reads the series straight from global
+ * memory (all loads of a thread issued together); each later stage reads
+ * the previous one's output from an LDS ping-pong buffer, so one barrier
+ * per stage; the twiddles of all stages are loaded into registers first.
+ * Stage with sub-transform size p: butterfly q, k = q mod p, inputs
+ * u_r = a[q + r N/4] times exp(-2 pi i r k / 4p), radix-4 DFT, outputs to
+ * (q / p) 4p + k + r p.
+ */
+template <typename T, typename TI, int NQ>
+__global__ void __launch_bounds__(256) k_rfft4(const TI *__restrict__ in, const double *__restrict__ window, int nseries, const cplx<T> *__restrict__ tw, cplx<T> *__restrict__ spec)
+{
+	constexpr int N = 1024 * NQ, N4 = N / 4;
+	constexpr int NS = NQ == 1 ? 5 : NQ == 4 ? 6 : 0;	/* stages */
+	static_assert(NS > 0, "N must be 1024 or 4096");
+	__shared__ cplx<T> buf[2][N];
+	const int tid = threadIdx.x;
+	const int sx = 2 * blockIdx.x, sy = sx + 1;
+	const bool has_y = sy < nseries;
+	const TI *x = in + (size_t) sx * N;
+	const TI *y = in + (size_t) (has_y ? sy : sx) * N;
+	cplx<T> a[NQ][4];
+	/* stage 0 input: samples q + r N/4 */
+	{
+		TI xv[NQ][4], yv[NQ][4];
+		double wv[NQ][4];
+#pragma unroll
+		for(int u = 0; u < NQ; u++)
+#pragma unroll
+			for(int r = 0; r < 4; r++) {
+				const int i = tid + 256 * u + r * N4;
+				xv[u][r] = x[i];
+				yv[u][r] = y[i];
+				wv[u][r] = window ? window[i] : 1.0;
+			}
+#pragma unroll
+		for(int u = 0; u < NQ; u++)
+#pragma unroll
+			for(int r = 0; r < 4; r++) {
+				const T w = (T) wv[u][r];
+				a[u][r] = mk((T) xv[u][r] * w, has_y ? (T) yv[u][r] * w : (T) 0);
+			}
+	}
+	/* twiddles of every stage: w_r = tw[r k N / 4p], k = q mod p */
+	cplx<T> wt[NS][NQ][3];
+#pragma unroll
+	for(int st = 1; st < NS; st++) {
+		const int p = 1 << (2 * st);
+#pragma unroll
+		for(int u = 0; u < NQ; u++) {
+			const int k = (tid + 256 * u) & (p - 1);
+#pragma unroll
+			for(int r = 1; r < 4; r++)
+				wt[st][u][r - 1] = tw[r * k * (N / (4 * p))];
+		}
+	}
+#pragma unroll
+	for(int st = 0; st < NS; st++) {
+		const int p = 1 << (2 * st);
+		cplx<T> *o = buf[st & 1];
+		if(st > 0) {
+			const cplx<T> *iv = buf[(st - 1) & 1];
+#pragma unroll
+			for(int u = 0; u < NQ; u++)
+#pragma unroll
+				for(int r = 0; r < 4; r++)
+					a[u][r] = iv[tid + 256 * u + r * N4];
+		}
+#pragma unroll
+		for(int u = 0; u < NQ; u++) {
+			const int q = tid + 256 * u, k = q & (p - 1);
+			cplx<T> u0 = a[u][0], u1 = a[u][1], u2 = a[u][2], u3 = a[u][3];
+			if(st > 0) {
+				u1 = u1 * wt[st][u][0];
+				u2 = u2 * wt[st][u][1];
+				u3 = u3 * wt[st][u][2];
+			}
+			const cplx<T> s02 = u0 + u2, d02 = u0 - u2, s13 = u1 + u3, d13 = u1 - u3;
+			const cplx<T> jd13 = mk(d13.im, -d13.re);	/* -i (u1 - u3) */
+			const int base = (q - k) * 4 + k;
+			o[base] = s02 + s13;
+			o[base + p] = d02 + jd13;
+			o[base + 2 * p] = s02 - s13;
+			o[base + 3 * p] = d02 - jd13;
+		}
+		__syncthreads();
+	}
+	/* split the packed pair: X = (Z[f] + conj Z[N-f]) / 2,
+	 * Y = (Z[f] - conj Z[N-f]) / 2i */
+	const cplx<T> *z = buf[(NS - 1) & 1];
+	constexpr int Nh = N / 2 + 1;
+	cplx<T> *ox = spec + (size_t) sx * Nh;
+	cplx<T> *oy = spec + (size_t) sy * Nh;
+	for(int f = tid; f < Nh; f += 256) {
+		const cplx<T> zf = z[f & (N - 1)], zc = conjg(z[(N - f) & (N - 1)]);
+		const cplx<T> sum = zf + zc, dif = zf - zc;
+		ox[f] = mk(sum.re * (T) 0.5, sum.im * (T) 0.5);
+		if(has_y)
+			oy[f] = mk(dif.im * (T) 0.5, -dif.re * (T) 0.5);
+	}
+}
+
 template <typename T>
 int launch_rfft_pairs(const void *in, int in_f32, const double *window, int nseries, int N, const cplx<T> *tw, cplx<T> *spec, hipStream_t st)
 {
 	const size_t lds = (size_t) N * sizeof(cplx<T>);
 	dim3 grid((nseries + 1) / 2);
+	if(N == 1024 || (N == 4096 && 2 * N * sizeof(cplx<T>) <= 160 * 1024)) {
+		if(N == 1024) {
+			if(in_f32)
+				hipLaunchKernelGGL((k_rfft4<T, float, 1>), grid, dim3(256), 0, st, (const float *) in, window, nseries, tw, spec);
+			else
+				hipLaunchKernelGGL((k_rfft4<T, double, 1>), grid, dim3(256), 0, st, (const double *) in, window, nseries, tw, spec);
+		} else {
+			if(in_f32)
+				hipLaunchKernelGGL((k_rfft4<T, float, 4>), grid, dim3(256), 0, st, (const float *) in, window, nseries, tw, spec);
+			else
+				hipLaunchKernelGGL((k_rfft4<T, double, 4>), grid, dim3(256), 0, st, (const double *) in, window, nseries, tw, spec);
+		}
+		return hipGetLastError() == hipSuccess ? 0 : -1;
+	}
 	if(in_f32)
 		hipLaunchKernelGGL((k_rfft_pairs<T, float>), grid, dim3(NT), lds, st, (const float *) in, window, nseries, N, tw, spec);
 	else
@@ -191,15 +309,31 @@ template <> struct mfma16<double> {
 	enum { KC = 16 };
 };
 
-#define XP_MS 16	/* segments per workgroup */
 #define XP_NW 4		/* waves per workgroup */
 
-template <typename T, int XP_TPW>
+/* 16-byte vector of T for the W slab loads */
+template <typename T> struct vec16;
+template <> struct vec16<float> {
+	typedef float4 type;
+	enum { N = 4 };
+};
+template <> struct vec16<double> {
+	typedef double2 type;
+	enum { N = 2 };
+};
+
+template <typename T, int XP_MS, int XP_TPW, int NBM>
 __global__ void __launch_bounds__(NT) k_xproj(const cplx<T> *__restrict__ spec, int S, struct xproj_geom g, const T *__restrict__ W, const int *__restrict__ tile_bl, int k_per_split, T *__restrict__ partial)
 {
 	typedef typename mfma16<T>::v4 v4;
-	const int KC = mfma16<T>::KC;
+	typedef typename vec16<T>::type V;
+	const int KC = mfma16<T>::KC, VN = vec16<T>::N;
 	const int CTW = XP_NW * XP_TPW * 16;	/* columns per workgroup */
+	/* per chunk: W slab KC x CTW in 16-byte vectors, and the A strip
+	 * (all baselines) as complex products, staged through registers so
+	 * the next chunk's loads are in flight during this chunk's MFMAs */
+	const int WV = KC * CTW / VN / NT;	/* W vectors per thread */
+	const int AP = (NBM * XP_MS * (KC / 2) + NT - 1) / NT;	/* NBM >= nb */
 	extern __shared__ unsigned char smem_raw[];
 	const int KA = KC + 1;					/* padded row: no bank conflicts */
 	T *sA = reinterpret_cast<T *>(smem_raw);		/* [nb][MS][KA] */
@@ -211,49 +345,103 @@ __global__ void __launch_bounds__(NT) k_xproj(const cplx<T> *__restrict__ spec, 
 	const int kend = min(g.K2, kbeg + k_per_split);
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int li = lane & 15, lk = lane >> 4;
+	const int na = g.nb * XP_MS * (KC / 2);			/* A products per chunk */
 	int tb[XP_TPW];
 	for(int u = 0; u < XP_TPW; u++) {
 		const int t = t0 + wave * XP_TPW + u;
 		tb[u] = t < ntile ? tile_bl[t] : 0;
 	}
-	v4 acc[XP_TPW];
-	for(int u = 0; u < XP_TPW; u++)
-		acc[u] = (v4) 0;
+	constexpr int RT = XP_MS / 16;				/* row tiles */
+	v4 acc[RT][XP_TPW];
+	for(int rt = 0; rt < RT; rt++)
+		for(int u = 0; u < XP_TPW; u++)
+			acc[rt][u] = (v4) 0;
 	const int c0 = t0 * 16;
-	for(int k0 = kbeg; k0 < kend; k0 += KC) {
-		const int f0 = k0 / 2;
-		/* A strip: X_b for MS segments x KC/2 frequencies */
-		for(int i = threadIdx.x; i < g.nb * XP_MS * (KC / 2); i += NT) {
+	V wv[WV];
+	cplx<T> fa[AP], fb[AP];
+	/* this thread's A products: spectrum offsets of the two detectors
+	 * (frequency 0 of the chunk added per chunk), LDS slot, whether the
+	 * segment exists; fixed over the K loop, so the per-lane lookup of the
+	 * baseline's detectors happens once */
+	size_t oa[AP], ob[AP];
+	int sl_ff[AP];
+#pragma unroll
+	for(int p = 0; p < AP; p++) {
+		const int i = p * NT + threadIdx.x;
+		oa[p] = ob[p] = 0;
+		sl_ff[p] = -1;
+		if(i < na) {
 			const int b = i / (XP_MS * (KC / 2)), r = i - b * XP_MS * (KC / 2);
 			const int sl = r / (KC / 2), ff = r - sl * (KC / 2);
-			const int s = s0 + sl, f = f0 + ff;
-			T xr = 0, xi = 0;
-			if(s < S && 2 * f < kend) {
-				const cplx<T> Fa = spec[((size_t) g.ia[b] * S + s) * g.Nh + f];
-				const cplx<T> Fb = spec[((size_t) g.ib[b] * S + s) * g.Nh + f];
-				const cplx<T> X = Fb * conjg(Fa);
-				xr = X.re;
-				xi = X.im;
+			const int s = s0 + sl;
+			if(s < S) {
+				oa[p] = ((size_t) g.ia[b] * S + s) * g.Nh + ff;
+				ob[p] = ((size_t) g.ib[b] * S + s) * g.Nh + ff;
+				sl_ff[p] = ff;
 			}
-			T *a = sA + (b * XP_MS + sl) * KA + 2 * ff;
-			a[0] = xr;
-			a[1] = xi;
 		}
-		/* W slab: KC rows x CTW columns */
-		for(int i = threadIdx.x; i < KC * CTW; i += NT) {
+	}
+	/* issue the global loads of the chunk at k0 into registers */
+	auto load = [&](int k0) {
+		const int f0 = k0 / 2;
+#pragma unroll
+		for(int v = 0; v < WV; v++) {
+			const int i = (v * NT + threadIdx.x) * VN;
 			const int kk = i / CTW, c = i - kk * CTW;
 			const int k = k0 + kk, cg = c0 + c;
-			sW[i] = (k < kend && cg < g.CT) ? W[(size_t) k * g.CT + cg] : (T) 0;
+			if(k < kend && cg < g.CT)
+				wv[v] = *reinterpret_cast<const V *>(W + (size_t) k * g.CT + cg);
+			else
+				wv[v] = V{};
 		}
+#pragma unroll
+		for(int p = 0; p < AP; p++) {
+			fa[p] = mk((T) 0, (T) 0);
+			fb[p] = fa[p];
+			if(sl_ff[p] >= 0 && 2 * (f0 + sl_ff[p]) < kend) {
+				fa[p] = spec[oa[p] + f0];
+				fb[p] = spec[ob[p] + f0];
+			}
+		}
+	};
+	/* write the staged chunk to LDS */
+	auto store = [&]() {
+#pragma unroll
+		for(int v = 0; v < WV; v++)
+			*reinterpret_cast<V *>(sW + (v * NT + threadIdx.x) * VN) = wv[v];
+#pragma unroll
+		for(int p = 0; p < AP; p++) {
+			const int i = p * NT + threadIdx.x;
+			if(i < na) {
+				const int b = i / (XP_MS * (KC / 2)), r = i - b * XP_MS * (KC / 2);
+				const int sl = r / (KC / 2), ff = r - sl * (KC / 2);
+				const cplx<T> X = fb[p] * conjg(fa[p]);
+				T *a = sA + (b * XP_MS + sl) * KA + 2 * ff;
+				a[0] = X.re;
+				a[1] = X.im;
+			}
+		}
+	};
+	if(kbeg < kend)
+		load(kbeg);
+	for(int k0 = kbeg; k0 < kend; k0 += KC) {
+		store();
 		__syncthreads();
+		if(k0 + KC < kend)
+			load(k0 + KC);
+		/* all tiles unguarded: tiles past the last column multiply the
+		 * zero-filled W slab (their baseline index is 0, a valid row) */
+#pragma unroll
 		for(int ks = 0; ks < KC; ks += 4) {
+#pragma unroll
 			for(int u = 0; u < XP_TPW; u++) {
 				const int tl = wave * XP_TPW + u;
-				if(t0 + tl >= ntile)
-					break;
-				const T a = sA[(tb[u] * XP_MS + li) * KA + ks + lk];
 				const T b = sW[(ks + lk) * CTW + tl * 16 + li];
-				acc[u] = mfma16<T>::mma(a, b, acc[u]);
+#pragma unroll
+				for(int rt = 0; rt < RT; rt++) {
+					const T a = sA[(tb[u] * XP_MS + 16 * rt + li) * KA + ks + lk];
+					acc[rt][u] = mfma16<T>::mma(a, b, acc[rt][u]);
+				}
 			}
 		}
 		__syncthreads();
@@ -263,40 +451,202 @@ __global__ void __launch_bounds__(NT) k_xproj(const cplx<T> *__restrict__ spec, 
 		if(t0 + tl >= ntile)
 			break;
 		const int c = c0 + tl * 16 + li;
+#pragma unroll
+		for(int rt = 0; rt < RT; rt++)
+			for(int e = 0; e < 4; e++) {
+				const int s = s0 + 16 * rt + mfma16<T>::row(lk, e);
+				if(s < S)
+					partial[((size_t) blockIdx.z * S + s) * g.CT + c] = acc[rt][u][e];
+			}
+	}
+}
+
+/*
+ * The projection with one baseline per workgroup: 64 segments (one 16-row
+ * tile per wave) x the baseline's columns (up to NTB 16-column tiles) x a
+ * K split.  The baseline's detectors and columns are uniform per
+ * workgroup, so the A strip is that baseline's X alone and each A
+ * fragment feeds all the baseline's column tiles.  Chunks of KC reals are
+ * staged global -> registers -> LDS, the next chunk's loads in flight
+ * during this chunk's MFMAs.
+ */
+template <typename T> struct xpb;
+template <> struct xpb<float> {
+	enum { KC = 64, KA = 68, CW = 132 };	/* row strides = 4 mod 64 words */
+};
+template <> struct xpb<double> {
+	enum { KC = 32, KA = 36, CW = 132 };
+};
+#define XPB_MS 64
+
+template <typename T, int NTB>
+__global__ void __launch_bounds__(NT) k_xproj_bl(const cplx<T> *__restrict__ spec, int S, struct xproj_geom g, const T *__restrict__ W, int k_per_split, T *__restrict__ partial)
+{
+	typedef typename mfma16<T>::v4 v4;
+	typedef typename vec16<T>::type V;
+	const int KC = xpb<T>::KC, KA = xpb<T>::KA, CW = xpb<T>::CW, VN = vec16<T>::N;
+	const int AP = XPB_MS * (KC / 2) / NT;		/* A products per thread */
+	const int WV = (KC * NTB * 16 / VN + NT - 1) / NT;	/* W vectors per thread (at most) */
+	__shared__ __align__(16) T sA[XPB_MS * KA];
+	__shared__ __align__(16) T sW[KC * CW];
+	const int b = blockIdx.y;
+	const int s0 = blockIdx.x * XPB_MS;
+	const int kbeg = blockIdx.z * k_per_split;
+	const int kend = min(g.K2, kbeg + k_per_split);
+	const int cb0 = g.col_start[b];
+	const int ncol = (b + 1 < g.nb ? g.col_start[b + 1] : g.CT) - cb0;
+	const int ncb = ncol / 16;
+	const size_t da = (size_t) g.ia[b] * S, db = (size_t) g.ib[b] * S;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int li = lane & 15, lk = lane >> 4;
+	v4 acc[NTB];
+#pragma unroll
+	for(int ct = 0; ct < NTB; ct++)
+		acc[ct] = (v4) 0;
+	V wv[WV];
+	cplx<T> fa[AP], fb[AP];
+	auto load = [&](int k0) {
+		const int f0 = k0 / 2;
+#pragma unroll
+		for(int p = 0; p < AP; p++) {
+			const int q = p * NT + threadIdx.x;
+			const int sl = q / (KC / 2), ff = q - sl * (KC / 2);
+			const int s = s0 + sl, f = f0 + ff;
+			fa[p] = mk((T) 0, (T) 0);
+			fb[p] = fa[p];
+			if(s < S && 2 * f < kend) {
+				fa[p] = spec[(da + s) * g.Nh + f];
+				fb[p] = spec[(db + s) * g.Nh + f];
+			}
+		}
+#pragma unroll
+		for(int v = 0; v < WV; v++) {
+			const int i = (v * NT + threadIdx.x) * VN;
+			const int kk = i / (NTB * 16), c = i - kk * (NTB * 16);
+			const int k = k0 + kk;
+			if(kk < KC && k < kend && c < ncol)
+				wv[v] = *reinterpret_cast<const V *>(W + (size_t) k * g.CT + cb0 + c);
+			else
+				wv[v] = V{};
+		}
+	};
+	auto store = [&]() {
+#pragma unroll
+		for(int p = 0; p < AP; p++) {
+			const int q = p * NT + threadIdx.x;
+			const int sl = q / (KC / 2), ff = q - sl * (KC / 2);
+			const cplx<T> X = fb[p] * conjg(fa[p]);
+			sA[sl * KA + 2 * ff] = X.re;
+			sA[sl * KA + 2 * ff + 1] = X.im;
+		}
+#pragma unroll
+		for(int v = 0; v < WV; v++) {
+			const int i = (v * NT + threadIdx.x) * VN;
+			const int kk = i / (NTB * 16), c = i - kk * (NTB * 16);
+			if(kk < KC)
+				*reinterpret_cast<V *>(sW + kk * CW + c) = wv[v];
+		}
+	};
+	if(kbeg < kend)
+		load(kbeg);
+	for(int k0 = kbeg; k0 < kend; k0 += KC) {
+		store();
+		__syncthreads();
+		if(k0 + KC < kend)
+			load(k0 + KC);
+		/* all NTB tiles unguarded (tiles past this baseline's columns
+		 * multiply the zero-filled W slab), operands of the next k-step
+		 * read before this one's MFMAs */
+		const T *ar = sA + (16 * wave + li) * KA + lk;
+		const T *br = sW + lk * CW + li;
+		T an = ar[0], bn[NTB];
+#pragma unroll
+		for(int ct = 0; ct < NTB; ct++)
+			bn[ct] = br[ct * 16];
+#pragma unroll
+		for(int ks = 0; ks < KC; ks += 4) {
+			const T a = an;
+			T bb[NTB];
+#pragma unroll
+			for(int ct = 0; ct < NTB; ct++)
+				bb[ct] = bn[ct];
+			if(ks + 4 < KC) {
+				an = ar[ks + 4];
+#pragma unroll
+				for(int ct = 0; ct < NTB; ct++)
+					bn[ct] = br[(ks + 4) * CW + ct * 16];
+			}
+#pragma unroll
+			for(int ct = 0; ct < NTB; ct++)
+				acc[ct] = mfma16<T>::mma(a, bb[ct], acc[ct]);
+		}
+		__syncthreads();
+	}
+#pragma unroll
+	for(int ct = 0; ct < NTB; ct++) {
+		if(ct >= ncb)
+			break;
+		const int c = cb0 + ct * 16 + li;
 		for(int e = 0; e < 4; e++) {
-			const int s = s0 + mfma16<T>::row(lk, e);
+			const int s = s0 + 16 * wave + mfma16<T>::row(lk, e);
 			if(s < S)
-				partial[((size_t) blockIdx.z * S + s) * g.CT + c] = acc[u][e];
+				partial[((size_t) blockIdx.z * S + s) * g.CT + c] = acc[ct][e];
 		}
 	}
 }
 
-template <typename T, int XP_TPW>
+template <typename T, int XP_MS, int XP_TPW>
 static void launch_xproj_tpw(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st)
 {
 	const int KC = mfma16<T>::KC;
 	const size_t lds = ((size_t) g->nb * XP_MS * (KC + 1) + (size_t) KC * XP_NW * XP_TPW * 16) * sizeof(T);
 	const int ntile = g->CT / 16;
 	dim3 grid((S + XP_MS - 1) / XP_MS, (ntile + XP_NW * XP_TPW - 1) / (XP_NW * XP_TPW), nsplit);
-	hipLaunchKernelGGL((k_xproj<T, XP_TPW>), grid, dim3(NT), lds, st, spec, S, *g, W, tile_bl, k_per_split, partial);
+	/* the register staging of the A strip is sized for up to 6 baselines
+	 * (4 detectors); larger networks take the 16-baseline form */
+	if(g->nb <= 6)
+		hipLaunchKernelGGL((k_xproj<T, XP_MS, XP_TPW, 6>), grid, dim3(NT), lds, st, spec, S, *g, W, tile_bl, k_per_split, partial);
+	else
+		hipLaunchKernelGGL((k_xproj<T, XP_MS, XP_TPW, SPHR_MAX_BASELINES>), grid, dim3(NT), lds, st, spec, S, *g, W, tile_bl, k_per_split, partial);
 }
 
-/* tpw: 16-column tiles per wave (1, 2 or 4), the tile width the batch's
- * autotuner picks */
+/* tpw: 16-column tiles per wave (1, 2 or 4); ms: segments per workgroup
+ * (16 or 32); tpw 0: one baseline per workgroup (64 segments, up to 8
+ * column tiles per baseline); the shapes the batch's autotuner picks from */
 template <typename T>
-int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st, int tpw)
+int launch_xproj(const cplx<T> *spec, int S, const struct xproj_geom *g, const T *W, const int *tile_bl, int k_per_split, int nsplit, T *partial, hipStream_t st, int tpw, int ms)
 {
-	if(tpw == 1)
-		launch_xproj_tpw<T, 1>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
-	else if(tpw == 2)
-		launch_xproj_tpw<T, 2>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
-	else
-		launch_xproj_tpw<T, 4>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+	int maxcol = 0;
+	for(int b = 0; b < g->nb; b++) {
+		const int nc = (b + 1 < g->nb ? g->col_start[b + 1] : g->CT) - g->col_start[b];
+		maxcol = nc > maxcol ? nc : maxcol;
+	}
+	if(tpw == 0 && maxcol <= 8 * 16) {
+		dim3 grid((S + XPB_MS - 1) / XPB_MS, g->nb, nsplit);
+		if(maxcol <= 5 * 16)
+			hipLaunchKernelGGL((k_xproj_bl<T, 5>), grid, dim3(NT), 0, st, spec, S, *g, W, k_per_split, partial);
+		else
+			hipLaunchKernelGGL((k_xproj_bl<T, 8>), grid, dim3(NT), 0, st, spec, S, *g, W, k_per_split, partial);
+	} else if(ms == 32) {
+		if(tpw == 1)
+			launch_xproj_tpw<T, 32, 1>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+		else if(tpw == 2)
+			launch_xproj_tpw<T, 32, 2>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+		else
+			launch_xproj_tpw<T, 32, 4>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+	} else {
+		if(tpw == 1)
+			launch_xproj_tpw<T, 16, 1>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+		else if(tpw == 2)
+			launch_xproj_tpw<T, 16, 2>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+		else
+			launch_xproj_tpw<T, 16, 4>(spec, S, g, W, tile_bl, k_per_split, nsplit, partial, st);
+	}
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template int launch_xproj<double>(const cplx<double> *, int, const struct xproj_geom *, const double *, const int *, int, int, double *, hipStream_t, int);
-template int launch_xproj<float>(const cplx<float> *, int, const struct xproj_geom *, const float *, const int *, int, int, float *, hipStream_t, int);
+template int launch_xproj<double>(const cplx<double> *, int, const struct xproj_geom *, const double *, const int *, int, int, double *, hipStream_t, int, int);
+template int launch_xproj<float>(const cplx<float> *, int, const struct xproj_geom *, const float *, const int *, int, int, float *, hipStream_t, int, int);
 
 /* ------------------------------------------------------------------ */
 /* per-baseline power -> sky coefficients                              */

@@ -314,20 +314,21 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 		/* ---- partial sums over the K splits, rotate_z phases ---- */
 		if(act) {
 			for(int c = tid; c < xg.CT; c += SF_NT) {
-				/* four independent loads in flight */
+				/* eight independent loads in flight */
 				const T *p = partial + (size_t) s * xg.CT + c;
 				const size_t st = (size_t) S * xg.CT;
-				T v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+				T v[8];
+#pragma unroll
+				for(int u = 0; u < 8; u++)
+					v[u] = 0;
 				int sp = 0;
-				for(; sp + 4 <= nsplit; sp += 4) {
-					v0 += p[sp * st];
-					v1 += p[(sp + 1) * st];
-					v2 += p[(sp + 2) * st];
-					v3 += p[(sp + 3) * st];
-				}
+				for(; sp + 8 <= nsplit; sp += 8)
+#pragma unroll
+					for(int u = 0; u < 8; u++)
+						v[u] += p[(sp + u) * st];
 				for(; sp < nsplit; sp++)
-					v0 += p[sp * st];
-				pw[c] = (v0 + v1) + (v2 + v3);
+					v[sp & 7] += p[sp * st];
+				pw[c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
 			}
 			if(tid <= L) {
 				double sn = 0.0, cs = 1.0;
