@@ -302,6 +302,28 @@ def _lmoff(L, l, m):
     return base + l - abs(m)
 
 
+def bench_copy(S, args):
+    """A plain device-to-device copy of 256 MiB, five times: the reference
+    kernel of known size for scaling rocprofv3's FETCH_SIZE and WRITE_SIZE
+    counters to bytes (run under --pmc; prints the bytes per copy)."""
+    n = 256 << 20
+    a, b = S.DeviceBuffer(n), S.DeviceBuffer(n)
+    S.lib().sphr_memset_device(a.ptr, 1, n)
+    for _ in range(5):
+        if S.lib().sphr_memcpy_dtod(b.ptr, a.ptr, n):
+            raise RuntimeError(S.last_error())
+    S.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_per_copy": n, "copies": 5}))
+
+
+# HBM bytes per launch of the fused sky kernel (float32, l_max 32, 1024
+# segments, 4 K splits): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE medians
+# scaled by the counters' reading of a 256 MiB copy (FETCH_SIZE reads half
+# the copied bytes, WRITE_SIZE all of them); the summary with the commands
+# is profiles/three_site_l32_traffic.md
+SKY_FUSED_F32_TRAFFIC = 4759 * 1024 * 2 + 49837 * 1024
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,7 +333,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-budget", type=float, default=3.0)
-    ap.add_argument("--workload", choices=["sky", "localization", "jobs"], default="sky",
+    ap.add_argument("--workload", choices=["sky", "localization", "jobs", "copy"], default="sky",
                     help="sky: the three-site cross-power maps (default); localization: the log-likelihood skies; "
                          "jobs: the uneven batch with a plan per job")
     ap.add_argument("--events", type=int, default=8192)
@@ -333,6 +355,9 @@ def main():
         return
     if args.workload == "jobs":
         bench_jobs(S, args)
+        return
+    if args.workload == "copy":
+        bench_copy(S, args)
         return
 
     L, N, nseg = 32, 1024, args.segments
@@ -394,7 +419,9 @@ def main():
         "stages": names,
         "tuned": {"mixed": rmx["tuned"], "fp64": r64["tuned"], "info": rmx["info"]},
         "roofline": {"bound": "hbm", "kernel": names[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": SKY_FUSED_F32_TRAFFIC if names[k] == "sky_fused" and nseg == 1024 else None,
+                     "algorithmic_bytes": sb[k]},
         "plan_build_s": round(plan_s, 2),
     }
     if not args.no_cpu:

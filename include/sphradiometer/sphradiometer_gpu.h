@@ -25,8 +25,11 @@ int sphr_device_info(char *name, int name_len, int *compute_units, size_t *lds_p
 
 void *sphr_device_alloc(size_t bytes);
 void sphr_device_free(void *p);
+/* synchronous copies and fill, ordered after the work queued on the
+ * library's stream (sphr_stream) */
 int sphr_memcpy_htod(void *dst, const void *src, size_t bytes);
 int sphr_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int sphr_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int sphr_memset_device(void *dst, int value, size_t bytes);
 
 /* the stream the synchronous entry points use; any stream may be passed

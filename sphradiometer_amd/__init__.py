@@ -186,6 +186,7 @@ _PROTOS = {
     "sphr_device_free": (None, [c_void_p]),
     "sphr_memcpy_htod": (c_int, [c_void_p, c_void_p, c_size_t]),
     "sphr_memcpy_dtoh": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "sphr_memcpy_dtod": (c_int, [c_void_p, c_void_p, c_size_t]),
     "sphr_memset_device": (c_int, [c_void_p, c_int, c_size_t]),
     "sphr_stream": (c_void_p, []),
     "sphr_stream_synchronize": (c_int, [c_void_p]),

@@ -181,25 +181,37 @@ extern "C" void sphr_device_free(void *p)
 		hipFree(p);
 }
 
+/* The copies and the fill are ordered after the work already queued on
+ * the library's stream (a non-blocking stream, which the null-stream
+ * copies would not wait for): a host read sees the results of that work,
+ * and a host write cannot overtake a kernel still using the buffer. */
 extern "C" int sphr_memcpy_htod(void *dst, const void *src, size_t bytes)
 {
 	if(sphr_gpu_ensure())
 		return -1;
-	return sphr_rt_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy HtoD");
+	return sphr_rt_check(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize") || sphr_rt_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy HtoD") ? -1 : 0;
 }
 
 extern "C" int sphr_memcpy_dtoh(void *dst, const void *src, size_t bytes)
 {
 	if(sphr_gpu_ensure())
 		return -1;
-	return sphr_rt_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy DtoH");
+	return sphr_rt_check(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize") || sphr_rt_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy DtoH") ? -1 : 0;
+}
+
+extern "C" int sphr_memcpy_dtod(void *dst, const void *src, size_t bytes)
+{
+	if(sphr_gpu_ensure())
+		return -1;
+	return sphr_rt_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx.stream), "hipMemcpy DtoD") || sphr_rt_check(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize") ? -1 : 0;
 }
 
 extern "C" int sphr_memset_device(void *dst, int value, size_t bytes)
 {
 	if(sphr_gpu_ensure())
 		return -1;
-	return sphr_rt_check(hipMemset(dst, value, bytes), "hipMemset");
+	return sphr_rt_check(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize") || sphr_rt_check(hipMemset(dst, value, bytes), "hipMemset")
+		       || sphr_rt_check(hipDeviceSynchronize(), "hipMemset") ? -1 : 0;
 }
 
 extern "C" void *sphr_stream(void)

@@ -127,9 +127,11 @@ def test_gpu_mlp_training_reduces_loss(P):
     t = (t / np.linalg.norm(t, axis=1, keepdims=True)).astype(np.float32)
     m = P.MLP(*shp, max_batch=512, bf16=True, seed=2)
     dx, dt = S.DeviceBuffer.from_numpy(x), S.DeviceBuffer.from_numpy(t)
+    l0 = P.mean_loss(m, dx, dt, rows)
     hist = P.train(m, dx, dt, rows, epochs=6, batch=512, lr=3e-3)
-    print("loss per epoch", hist, flush=True)
-    assert hist[-1] < 0.5 * hist[0]
+    l1 = P.mean_loss(m, dx, dt, rows)
+    print("loss before %.4f after %.4f; per epoch %s" % (l0, l1, hist), flush=True)
+    assert l1 < 0.7 * l0
 
 
 @pytest.mark.gpu
@@ -183,11 +185,18 @@ def test_gpu_proposal_small_ab(P):
     for off in range(0, n_train, 4096):
         fac.make(4096, 100 + off, d_x, d_t, feat_offset=off)
     m = P.MLP(max_batch=1024, bf16=True, seed=1)
-    hist = P.train(m, d_x, d_t, n_train, epochs=4, batch=512, lr=1e-3)
+    # the last 2048 skies are held out of training for the loss check
+    n_fit = n_train - 2048
+    d_xv = S.DeviceBuffer.from_numpy(d_x.to_numpy(np.float32, (n_train, ncoef))[n_fit:])
+    d_tv = S.DeviceBuffer.from_numpy(d_t.to_numpy(np.float32, (n_train, 3))[n_fit:])
+    l0 = P.mean_loss(m, d_xv, d_tv, 2048)
+    hist = P.train(m, d_x, d_t, n_fit, epochs=4, batch=512, lr=1e-3)
+    l1 = P.mean_loss(m, d_xv, d_tv, 2048)
     res = P.evaluate(m, fac, 2048, seed=999)
     summ = P.summary(res)
-    print("loss", hist, "A/B", summ, flush=True)
-    assert hist[-1] < hist[0]
+    print("held-out loss before %.4f after %.4f; A/B %s" % (l0, l1, summ), flush=True)
+    assert l1 < l0
     # the fallback makes the final peak the plain one whenever the
-    # proposals miss by more than the value tolerance
-    assert summ["final"]["within_0.10_deg"] >= 0.9
+    # proposals do not reach the plain peak's value, so the final peak is
+    # the plain one or a maximum at least as high (a mirror image)
+    assert summ["final"]["within_0.10_deg_or_not_lower"] >= 0.99
