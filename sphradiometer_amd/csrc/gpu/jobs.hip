@@ -414,7 +414,10 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st)
 			sphr_mesh_rule((unsigned) B.a, &x, &w);
 			memcpy(hnodes + B.off_nt, x, B.nt * sizeof(double));
 			memcpy(hnodes + B.off_nt + B.nt, w, B.nt * sizeof(double));
-			sphr_gl_rule(B.nq, hnodes + B.off_nq, hnodes + B.off_nq + B.nq);
+			const double *gx, *gw;
+			sphr_gl_rule_cached(B.nq, &gx, &gw);
+			memcpy(hnodes + B.off_nq, gx, B.nq * sizeof(double));
+			memcpy(hnodes + B.off_nq + B.nq, gw, B.nq * sizeof(double));
 		}
 		if(sphr_rt_check(hipMemcpyAsync(nodes, hnodes, nsz * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpy")
 		   || sphr_rt_check(hipMemcpyAsync(J->d_bl + b0, J->bl + b0, (size_t) (b1 - b0) * sizeof(struct jb_bl), hipMemcpyHostToDevice, st), "hipMemcpy"))

@@ -64,6 +64,29 @@ struct mesh_rule {
 static struct mesh_rule *mesh_rules;
 static pthread_mutex_t mesh_lock = PTHREAD_MUTEX_INITIALIZER;
 
+static struct mesh_rule *gl_rules;	/* keyed by the node count in l_max */
+
+void sphr_gl_rule_cached(int n, const double **x, const double **w)
+{
+	pthread_mutex_lock(&mesh_lock);
+	struct mesh_rule *r;
+	for(r = gl_rules; r; r = r->next)
+		if(r->l_max == (unsigned) n)
+			break;
+	if(!r) {
+		r = malloc(sizeof(*r));
+		r->x = malloc(n * sizeof(*r->x));
+		r->w = malloc(n * sizeof(*r->w));
+		sphr_gl_rule(n, r->x, r->w);
+		r->l_max = (unsigned) n;
+		r->next = gl_rules;
+		gl_rules = r;
+	}
+	pthread_mutex_unlock(&mesh_lock);
+	*x = r->x;
+	*w = r->w;
+}
+
 void sphr_mesh_rule(unsigned l_max, const double **cos_theta, const double **weights)
 {
 	pthread_mutex_lock(&mesh_lock);
