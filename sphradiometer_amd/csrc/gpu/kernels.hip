@@ -18,6 +18,7 @@
  */
 #include <hip/hip_runtime.h>
 #include "kernels.h"
+#include "runtime.h"
 
 #define NT 256
 
@@ -114,32 +115,7 @@ __global__ void __launch_bounds__(256) k_rfft4(const TI *__restrict__ in, const 
 	static_assert(NS > 0, "N must be 1024 or 4096");
 	__shared__ cplx<T> buf[2][N];
 	const int tid = threadIdx.x;
-	const int sx = 2 * blockIdx.x, sy = sx + 1;
-	const bool has_y = sy < nseries;
-	const TI *x = in + (size_t) sx * N;
-	const TI *y = in + (size_t) (has_y ? sy : sx) * N;
-	cplx<T> a[NQ][4];
-	/* stage 0 input: samples q + r N/4 */
-	{
-		TI xv[NQ][4], yv[NQ][4];
-		double wv[NQ][4];
-#pragma unroll
-		for(int u = 0; u < NQ; u++)
-#pragma unroll
-			for(int r = 0; r < 4; r++) {
-				const int i = tid + 256 * u + r * N4;
-				xv[u][r] = x[i];
-				yv[u][r] = y[i];
-				wv[u][r] = window ? window[i] : 1.0;
-			}
-#pragma unroll
-		for(int u = 0; u < NQ; u++)
-#pragma unroll
-			for(int r = 0; r < 4; r++) {
-				const T w = (T) wv[u][r];
-				a[u][r] = mk((T) xv[u][r] * w, has_y ? (T) yv[u][r] * w : (T) 0);
-			}
-	}
+	const int npairs = (nseries + 1) / 2;
 	/* twiddles of every stage: w_r = tw[r k N / 4p], k = q mod p */
 	cplx<T> wt[NS][NQ][3];
 #pragma unroll
@@ -153,49 +129,91 @@ __global__ void __launch_bounds__(256) k_rfft4(const TI *__restrict__ in, const 
 				wt[st][u][r - 1] = tw[r * k * (N / (4 * p))];
 		}
 	}
+	double wv[NQ][4];
 #pragma unroll
-	for(int st = 0; st < NS; st++) {
-		const int p = 1 << (2 * st);
-		cplx<T> *o = buf[st & 1];
-		if(st > 0) {
-			const cplx<T> *iv = buf[(st - 1) & 1];
+	for(int u = 0; u < NQ; u++)
 #pragma unroll
-			for(int u = 0; u < NQ; u++)
+		for(int r = 0; r < 4; r++)
+			wv[u][r] = window ? window[tid + 256 * u + r * N4] : 1.0;
+	/* samples q + r N/4 of pair pp into registers */
+	TI xv[NQ][4], yv[NQ][4];
+	auto load = [&](int pp) {
+		const int sx = 2 * pp, sy = sx + 1 < nseries ? sx + 1 : sx;
+		const TI *x = in + (size_t) sx * N, *y = in + (size_t) sy * N;
 #pragma unroll
-				for(int r = 0; r < 4; r++)
-					a[u][r] = iv[tid + 256 * u + r * N4];
-		}
+		for(int u = 0; u < NQ; u++)
 #pragma unroll
-		for(int u = 0; u < NQ; u++) {
-			const int q = tid + 256 * u, k = q & (p - 1);
-			cplx<T> u0 = a[u][0], u1 = a[u][1], u2 = a[u][2], u3 = a[u][3];
-			if(st > 0) {
-				u1 = u1 * wt[st][u][0];
-				u2 = u2 * wt[st][u][1];
-				u3 = u3 * wt[st][u][2];
+			for(int r = 0; r < 4; r++) {
+				const int i = tid + 256 * u + r * N4;
+				xv[u][r] = x[i];
+				yv[u][r] = y[i];
 			}
-			const cplx<T> s02 = u0 + u2, d02 = u0 - u2, s13 = u1 + u3, d13 = u1 - u3;
-			const cplx<T> jd13 = mk(d13.im, -d13.re);	/* -i (u1 - u3) */
-			const int base = (q - k) * 4 + k;
-			o[base] = s02 + s13;
-			o[base + p] = d02 + jd13;
-			o[base + 2 * p] = s02 - s13;
-			o[base + 3 * p] = d02 - jd13;
+	};
+	/* persistent over pairs; the next pair's samples are loaded while
+	 * this pair's stages run */
+	if(blockIdx.x < npairs)
+		load(blockIdx.x);
+	for(int pp = blockIdx.x; pp < npairs; pp += gridDim.x) {
+		const int sx = 2 * pp, sy = sx + 1;
+		const bool has_y = sy < nseries;
+		cplx<T> a[NQ][4];
+#pragma unroll
+		for(int u = 0; u < NQ; u++)
+#pragma unroll
+			for(int r = 0; r < 4; r++) {
+				const T w = (T) wv[u][r];
+				a[u][r] = mk((T) xv[u][r] * w, has_y ? (T) yv[u][r] * w : (T) 0);
+			}
+		if(pp + (int) gridDim.x < npairs)
+			load(pp + gridDim.x);
+#pragma unroll
+		for(int st = 0; st < NS; st++) {
+			const int p = 1 << (2 * st);
+			cplx<T> *o = buf[st & 1];
+			if(st > 0) {
+				const cplx<T> *iv = buf[(st - 1) & 1];
+#pragma unroll
+				for(int u = 0; u < NQ; u++)
+#pragma unroll
+					for(int r = 0; r < 4; r++)
+						a[u][r] = iv[tid + 256 * u + r * N4];
+			}
+#pragma unroll
+			for(int u = 0; u < NQ; u++) {
+				const int q = tid + 256 * u, k = q & (p - 1);
+				cplx<T> u0 = a[u][0], u1 = a[u][1], u2 = a[u][2], u3 = a[u][3];
+				if(st > 0) {
+					u1 = u1 * wt[st][u][0];
+					u2 = u2 * wt[st][u][1];
+					u3 = u3 * wt[st][u][2];
+				}
+				const cplx<T> s02 = u0 + u2, d02 = u0 - u2, s13 = u1 + u3, d13 = u1 - u3;
+				const cplx<T> jd13 = mk(d13.im, -d13.re);	/* -i (u1 - u3) */
+				const int base = (q - k) * 4 + k;
+				o[base] = s02 + s13;
+				o[base + p] = d02 + jd13;
+				o[base + 2 * p] = s02 - s13;
+				o[base + 3 * p] = d02 - jd13;
+			}
+			__syncthreads();
 		}
+		/* split the packed pair: X = (Z[f] + conj Z[N-f]) / 2,
+		 * Y = (Z[f] - conj Z[N-f]) / 2i */
+		const cplx<T> *z = buf[(NS - 1) & 1];
+		constexpr int Nh = N / 2 + 1;
+		cplx<T> *ox = spec + (size_t) sx * Nh;
+		cplx<T> *oy = spec + (size_t) sy * Nh;
+		for(int f = tid; f < Nh; f += 256) {
+			const cplx<T> zf = z[f & (N - 1)], zc = conjg(z[(N - f) & (N - 1)]);
+			const cplx<T> sum = zf + zc, dif = zf - zc;
+			ox[f] = mk(sum.re * (T) 0.5, sum.im * (T) 0.5);
+			if(has_y)
+				oy[f] = mk(dif.im * (T) 0.5, -dif.re * (T) 0.5);
+		}
+		/* the last stage's buffer is read above; the next pair's first
+		 * stage writes the other one, its second this one: a barrier
+		 * before that second write */
 		__syncthreads();
-	}
-	/* split the packed pair: X = (Z[f] + conj Z[N-f]) / 2,
-	 * Y = (Z[f] - conj Z[N-f]) / 2i */
-	const cplx<T> *z = buf[(NS - 1) & 1];
-	constexpr int Nh = N / 2 + 1;
-	cplx<T> *ox = spec + (size_t) sx * Nh;
-	cplx<T> *oy = spec + (size_t) sy * Nh;
-	for(int f = tid; f < Nh; f += 256) {
-		const cplx<T> zf = z[f & (N - 1)], zc = conjg(z[(N - f) & (N - 1)]);
-		const cplx<T> sum = zf + zc, dif = zf - zc;
-		ox[f] = mk(sum.re * (T) 0.5, sum.im * (T) 0.5);
-		if(has_y)
-			oy[f] = mk(dif.im * (T) 0.5, -dif.re * (T) 0.5);
 	}
 }
 
@@ -205,6 +223,9 @@ int launch_rfft_pairs(const void *in, int in_f32, const double *window, int nser
 	const size_t lds = (size_t) N * sizeof(cplx<T>);
 	dim3 grid((nseries + 1) / 2);
 	if(N == 1024 || (N == 4096 && 2 * N * sizeof(cplx<T>) <= 160 * 1024)) {
+		/* persistent: a few pairs per workgroup, 4 workgroups per CU */
+		const int npairs = (nseries + 1) / 2, cap = 4 * sphr_rt_cus();
+		grid = dim3(npairs < cap ? npairs : cap);
 		if(N == 1024) {
 			if(in_f32)
 				hipLaunchKernelGGL((k_rfft4<T, float, 1>), grid, dim3(256), 0, st, (const float *) in, window, nseries, tw, spec);
