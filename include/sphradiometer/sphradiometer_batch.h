@@ -177,6 +177,9 @@ void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map)
  * the projection (tiles per wave, segments per workgroup, K split), keeps
  * the fastest, returns its index (time per run in best_ms) or -1 */
 int sphr_batch_tune(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, double *best_ms);
+/* Select one of the autotuner's shapes by its index (0..27, as
+ * sphr_batch_tune returns it); 0, or -1 for an index out of range */
+int sphr_batch_set_shape(struct sphr_batch *, int index);
 
 /* Shape information: [0] detectors, [1] baselines, [2] N, [3] sky l_max,
  * [4] projection columns, [5] K splits, [6] precision, [7] K per split,

@@ -203,6 +203,7 @@ _PROTOS = {
     "sphr_batch_profile_read": (c_int, [c_void_p, P(c_double), P(c_int)]),
     "sphr_batch_info": (None, [c_void_p, P(c_int)]),
     "sphr_batch_set_fused": (None, [c_void_p, c_int]),
+    "sphr_batch_set_shape": (c_int, [c_void_p, c_int]),
     "sphr_batch_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sphr_batch_tune": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P(c_double)]),
     "sphr_jobs_new": (c_void_p, [c_int, P(c_int), P(c_double), P(c_int), P(c_int), c_double, c_int]),
@@ -490,6 +491,11 @@ class Batch:
         keys = ["detectors", "baselines", "N", "sky_l", "columns", "k_splits", "precision", "k_per_split", "fused",
                 "tiles_per_wave", "segments_per_workgroup"]
         return dict(zip(keys, list(a)))
+
+    def set_shape(self, index):
+        """One of the autotuner's projection shapes (0..27)."""
+        if lib().sphr_batch_set_shape(self.h, int(index)):
+            raise ValueError(last_error())
 
     def set_fused(self, on):
         """Sky stage as the one fused kernel (default, where compiled for

@@ -372,6 +372,25 @@ extern "C" int sphr_batch_peaks(struct sphr_batch *bt, const void *alm, const vo
 /* autotuner: time the projection's tile width (1, 2, 4 tiles per wave)
  * and K split (about 1, 2, 4 workgroups per CU) on the caller's data and
  * keep the fastest whole run; returns the chosen candidate */
+/* the projection shapes the autotuner tries: one baseline per workgroup
+ * (tpw 0), then tiles per wave x segments per workgroup; with the 4 K
+ * splits, shape index = 7 x split + shape */
+static const int tune_tpw[7] = { 0, 1, 2, 4, 1, 2, 4 }, tune_ms[7] = { 64, 16, 16, 16, 32, 32, 32 };
+#define SPHR_BATCH_SHAPES 28
+
+extern "C" int sphr_batch_set_shape(struct sphr_batch *bt, int index)
+{
+	if(!bt || index < 0 || index >= SPHR_BATCH_SHAPES) {
+		sphr_set_error("sphr_batch_set_shape: index %d outside 0..%d", index, SPHR_BATCH_SHAPES - 1);
+		return -1;
+	}
+	bt->k_per_split = bt->ksplit_opt[index / 7];
+	bt->nsplit = bt->nsplit_opt[index / 7];
+	bt->tpw = tune_tpw[index % 7];
+	bt->xms = tune_ms[index % 7];
+	return 0;
+}
+
 extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, int S, const double *window, const double *gmst, void *alm, void *map, double *best_ms)
 {
 	if(!bt || S < 1 || S > bt->max_segments || !ts || !alm)
@@ -380,9 +399,7 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	hipEvent_t e0, e1;
 	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
 		return -1;
-	/* shapes: one baseline per workgroup (tpw 0), then tiles per wave x
-	 * segments per workgroup */
-	static const int tpws[7] = { 0, 1, 2, 4, 1, 2, 4 }, mss[7] = { 64, 16, 16, 16, 32, 32, 32 };
+	const int *tpws = tune_tpw, *mss = tune_ms;
 	float best = 1e30f;
 	int bo = 1, bw = 0;
 	const int prof = bt->profile;

@@ -216,6 +216,30 @@ def test_batch_sky_stage_forms_agree(oracle, three_site, precision):
             assert rel(mp[s], mesh) < (tol if prec == S.FP64 else 5e-6)
 
 
+def test_batch_every_tuner_shape_matches_oracle(oracle, three_site):
+    # each projection shape and K split the autotuner can pick, in both
+    # precisions, against the oracle on the same segments
+    ts, _ = inject.three_site(3, 1024, seed=21)
+    onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
+    want = [onet.segment(ts[:, s], 32) for s in range(3)]
+    d_ts = S.DeviceBuffer.from_numpy(ts.astype(np.float64))
+    for prec, cdt, tol in ((S.FP64, np.complex128, TOL64), (S.MIXED, np.complex64, 1e-5)):
+        b = S.Batch(three_site, 32, prec, max_segments=3)
+        d_alm = S.DeviceBuffer(3 * 33 ** 2 * np.dtype(cdt).itemsize)
+        d_map = S.DeviceBuffer(3 * 66 ** 2 * np.dtype(cdt).itemsize)
+        for shape in range(28):
+            b.set_shape(shape)
+            b.run(d_ts.ptr, 3, d_alm.ptr, d_map.ptr)
+            S.synchronize()
+            alm = d_alm.to_numpy(cdt, (3, 33 ** 2))
+            mp = d_map.to_numpy(cdt, (3, 66, 66))
+            for s in range(3):
+                assert rel(alm[s], want[s][0]) < tol, (prec, shape, s)
+                assert rel(mp[s], want[s][1]) < (tol if prec == S.FP64 else 5e-6), (prec, shape, s)
+        with pytest.raises(ValueError):
+            b.set_shape(28)
+
+
 def test_batch_full_size_three_site_properties(three_site):
     # the 1024-segment workload: golden segments, linearity (x -> 2x gives
     # 4x power), and a real sky for real data
