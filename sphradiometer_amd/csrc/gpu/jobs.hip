@@ -29,6 +29,7 @@
  * class with gathered offsets.
  */
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -43,7 +44,7 @@ struct jb_bl {
 	int a, np, nt, nq, Lt, lags, L;
 	double r_dt, theta, phi, inv_nb, scale;	/* scale = 1 / N^2 */
 	long off_nt, off_nq;		/* node arrays (x then w) in the chunk's node buffer */
-	long off_Pt, off_Pq, off_C, off_T, off_K;	/* chunk workspace (doubles) */
+	long off_Pt, off_Pq, off_C, off_T, off_K, off_Y;	/* chunk workspace (doubles) */
 	long off_M, off_R, off_c, off_pw;	/* persistent: M (T), R (cplx T), c (T), power slot */
 };
 
@@ -99,81 +100,28 @@ __device__ inline void pbar_row(double x, int lmax, double *row)
 	}
 }
 
+/*
+ * Per baseline: Pbar_l0 at the tap nodes (Pt[j][l], l <= a) and at the
+ * product nodes (Pq[q][l], l <= max(a, Lt)); the delay table of the tap
+ * nodes: y_j = r_dt cos(acos x_j) (the delay of node j in samples,
+ * projection.c:99-107), sin(pi y_j), and the weight w_j 2 pi / nt.
+ */
 __global__ void k_pg_tables(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, double *__restrict__ ws)
 {
 	const struct jb_bl &B = bl[b0 + blockIdx.x];
 	const int lq = B.a > B.Lt ? B.a : B.Lt;
 	for(int i = threadIdx.x; i < B.nt + B.nq; i += blockDim.x) {
-		if(i < B.nt)
-			pbar_row(nodes[B.off_nt + i], B.a, ws + B.off_Pt + (size_t) i * (B.a + 1));
-		else {
+		if(i < B.nt) {
+			const double x = nodes[B.off_nt + i];
+			pbar_row(x, B.a, ws + B.off_Pt + (size_t) i * (B.a + 1));
+			const double y = B.r_dt * cos(acos(x));
+			ws[B.off_Y + i] = y;
+			ws[B.off_Y + B.nt + i] = sin(M_PI * y);
+			ws[B.off_Y + 2 * B.nt + i] = nodes[B.off_nt + B.nt + i] * (2. * M_PI / B.nt);
+		} else {
 			const int q = i - B.nt;
 			pbar_row(nodes[B.off_nq + q], lq, ws + B.off_Pq + (size_t) q * (lq + 1));
 		}
-	}
-}
-
-#define PG_TC 8		/* taps per workgroup */
-
-/* C[i][l] = sum_j (f_ij nt)(w_j 2 pi / nt) Pbar_l0(x_j), f_ij the delay
- * element of tap k = i - (np-1)/2 at node j (projection.c:99-107) */
-__global__ void k_pg_taps(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, double *__restrict__ ws)
-{
-	extern __shared__ double pg_sm[];
-	const struct jb_bl &B = bl[b0 + blockIdx.y];
-	const int i0 = blockIdx.x * PG_TC;
-	if(i0 >= B.np)
-		return;
-	double *F = pg_sm;	/* [PG_TC][nt] */
-	const double *x = nodes + B.off_nt, *w = x + B.nt;
-	for(int e = threadIdx.x; e < PG_TC * B.nt; e += blockDim.x) {
-		const int t = e / B.nt, j = e - t * B.nt;
-		const int i = i0 + t;
-		double v = 0.0;
-		if(i < B.np) {
-			const int k = -(B.np - 1) / 2 + i;
-			const double f = sinc_pi(0 - k - B.r_dt * cos(acos(x[j])));
-			v = (f * (double) B.nt) * (w[j] * (2. * M_PI / B.nt));
-		}
-		F[e] = v;
-	}
-	__syncthreads();
-	const double *Pt = ws + B.off_Pt;
-	for(int e = threadIdx.x; e < PG_TC * (B.a + 1); e += blockDim.x) {
-		const int t = e / (B.a + 1), l = e - t * (B.a + 1);
-		if(i0 + t >= B.np)
-			continue;
-		double acc = 0.0;
-		for(int j = 0; j < B.nt; j++)
-			acc += F[t * B.nt + j] * Pt[(size_t) j * (B.a + 1) + l];
-		ws[B.off_C + (size_t) (i0 + t) * (B.a + 1) + l] = acc;
-	}
-}
-
-/* T[i][q] = sum_{l <= a} C[i][l] Pbar_l0(x_q) */
-__global__ void k_pg_synth(const struct jb_bl *__restrict__ bl, int b0, double *__restrict__ ws)
-{
-	extern __shared__ double pg_sm[];
-	const struct jb_bl &B = bl[b0 + blockIdx.y];
-	const int i0 = blockIdx.x * PG_TC;
-	if(i0 >= B.np)
-		return;
-	const int lq = B.a > B.Lt ? B.a : B.Lt;
-	double *Cs = pg_sm;	/* [PG_TC][a+1] */
-	for(int e = threadIdx.x; e < PG_TC * (B.a + 1); e += blockDim.x) {
-		const int t = e / (B.a + 1);
-		Cs[e] = i0 + t < B.np ? ws[B.off_C + (size_t) i0 * (B.a + 1) + e] : 0.0;
-	}
-	__syncthreads();
-	const double *Pq = ws + B.off_Pq;
-	for(int e = threadIdx.x; e < PG_TC * B.nq; e += blockDim.x) {
-		const int t = e / B.nq, q = e - t * B.nq;
-		if(i0 + t >= B.np)
-			continue;
-		double acc = 0.0;
-		for(int l = 0; l <= B.a; l++)
-			acc += Cs[t * (B.a + 1) + l] * Pq[(size_t) q * (lq + 1) + l];
-		ws[B.off_T + (size_t) (i0 + t) * B.nq + q] = acc;
 	}
 }
 
@@ -194,21 +142,255 @@ __global__ void k_pg_lag(const struct jb_bl *__restrict__ bl, int b0, double *__
 	ws[B.off_K + e] = acc;
 }
 
-/* M[u][d] = scale sum_q (2 pi w_q) K[u][q] Pbar_d0(x_q), d <= Lt */
-template <typename T>
-__global__ void k_pg_M(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, const double *__restrict__ ws, T *__restrict__ M)
+/*
+ * The plan's three products as one grouped float64 GEMM on MFMA
+ * (v_mfma_f64_16x16x4f64) over all baselines of a chunk: workgroup = a
+ * 64 x 64 tile of one baseline's product (4 waves of 32 x 32), K in
+ * chunks of 16 staged in LDS through registers (the next chunk's loads in
+ * flight during this chunk's MFMAs).  The workgroup finds its baseline
+ * in the chunk's tile prefix sums.
+ *   PG_TAPS:  C[i][l] = sum_j F[i][j] Pt[j][l], F the weighted sinc
+ *             samples (generated in the loader), i < np, l <= a, j < nt;
+ *   PG_SYNTH: T[i][q] = sum_l C[i][l] Pq[q][l], i < np, q < nq, l <= a;
+ *   PG_M:     M[u][d] = scale sum_q K[u][q] (2 pi w_q) Pq[q][d],
+ *             u < lags, d <= Lt, q < nq.
+ * MFMA 16x16x4 f64 operand layout: A[i][k] in lane i + 16 k, B[k][j] in
+ * lane j + 16 k, accumulator element e of lane j + 16 q holds D[q + 4 e][j].
+ */
+enum { PG_TAPS, PG_SYNTH, PG_M };
+#define PG_TILE 64
+#define PG_KC 16
+
+__device__ inline void pg_dims(int mode, const struct jb_bl &B, int *M, int *N, int *K)
 {
-	const struct jb_bl &B = bl[b0 + blockIdx.y];
-	const long e = (long) blockIdx.x * blockDim.x + threadIdx.x;
-	if(e >= (long) B.lags * (B.Lt + 1))
-		return;
-	const int uu = (int) (e / (B.Lt + 1)), d = (int) (e - (long) uu * (B.Lt + 1));
+	if(mode == PG_TAPS) {
+		*M = B.np;
+		*N = B.a + 1;
+		*K = B.nt;
+	} else if(mode == PG_SYNTH) {
+		*M = B.np;
+		*N = B.nq;
+		*K = B.a + 1;
+	} else {
+		*M = B.lags;
+		*N = B.Lt + 1;
+		*K = B.nq;
+	}
+}
+
+template <int MODE, typename TO>
+__global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict__ bl, int b0, int nb, const int *__restrict__ tile_start, const double *__restrict__ nodes, double *__restrict__ ws, TO *__restrict__ Mout)
+{
+	typedef double v4 __attribute__((ext_vector_type(4)));
+	__shared__ double sA[PG_KC][PG_TILE + 4];	/* [k][row] */
+	__shared__ double sB[PG_KC][PG_TILE + 4];	/* [k][col] */
+	/* this workgroup's baseline: the last with tile_start <= blockIdx.x */
+	int lo = 0, hi = nb - 1;
+	while(lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if(tile_start[mid] <= (int) blockIdx.x)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	const struct jb_bl &B = bl[b0 + lo];
+	int M, N, K;
+	pg_dims(MODE, B, &M, &N, &K);
+	const int tiles_m = (M + PG_TILE - 1) / PG_TILE;
+	const int local = blockIdx.x - tile_start[lo];
+	const int m0 = (local % tiles_m) * PG_TILE, n0 = (local / tiles_m) * PG_TILE;
 	const int lq = B.a > B.Lt ? B.a : B.Lt;
-	const double *K = ws + B.off_K + (size_t) uu * B.nq, *Pq = ws + B.off_Pq, *w = nodes + B.off_nq + B.nq;
-	double acc = 0.0;
-	for(int q = 0; q < B.nq; q++)
-		acc += (2. * M_PI * w[q]) * K[q] * Pq[(size_t) q * (lq + 1) + d];
-	M[B.off_M + e] = (T) (acc * B.scale);
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int li = lane & 15, lk = lane >> 4;
+	const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+	/* element loaders (zero outside the product) */
+	auto Aval = [&](int i, int k) -> double {
+		if(i >= M || k >= K)
+			return 0.0;
+		if(MODE == PG_TAPS) {
+			/* delay element of tap kk = i - (np-1)/2 at node k,
+			 * sinc(-kk - y) = (-1)^kk sin(pi y) / (pi (kk + y)) */
+			const int kk = -(B.np - 1) / 2 + i;
+			const double y = ws[B.off_Y + k], sy = ws[B.off_Y + B.nt + k];
+			const double den = (double) kk + y;
+			const double f = den == 0.0 ? 1.0 : ((kk & 1) ? -sy : sy) / (M_PI * den);
+			return (f * (double) B.nt) * ws[B.off_Y + 2 * B.nt + k];
+		}
+		if(MODE == PG_SYNTH)
+			return ws[B.off_C + (size_t) i * (B.a + 1) + k];
+		return ws[B.off_K + (size_t) i * B.nq + k];
+	};
+	auto Bval = [&](int k, int j) -> double {
+		if(k >= K || j >= N)
+			return 0.0;
+		if(MODE == PG_TAPS)
+			return ws[B.off_Pt + (size_t) k * (B.a + 1) + j];
+		if(MODE == PG_SYNTH)
+			return ws[B.off_Pq + (size_t) j * (lq + 1) + k];
+		return (2. * M_PI * nodes[B.off_nq + B.nq + k]) * ws[B.off_Pq + (size_t) k * (lq + 1) + j];
+	};
+	/* staging: 4 A and 4 B values per thread per chunk; A read along k
+	 * (rows of 16), B along the columns */
+	double ra[4], rb[4];
+	auto load = [&](int k0) {
+#pragma unroll
+		for(int r = 0; r < 4; r++) {
+			const int e = tid + 256 * r;
+			ra[r] = Aval(m0 + e / PG_KC, k0 + e % PG_KC);
+			rb[r] = Bval(k0 + e / PG_TILE, n0 + e % PG_TILE);
+		}
+	};
+	auto store = [&]() {
+#pragma unroll
+		for(int r = 0; r < 4; r++) {
+			const int e = tid + 256 * r;
+			sA[e % PG_KC][e / PG_KC] = ra[r];
+			sB[e / PG_TILE][e % PG_TILE] = rb[r];
+		}
+	};
+	v4 acc[2][2];
+#pragma unroll
+	for(int a = 0; a < 2; a++)
+#pragma unroll
+		for(int b = 0; b < 2; b++)
+			acc[a][b] = (v4) 0;
+	load(0);
+	for(int k0 = 0; k0 < K; k0 += PG_KC) {
+		store();
+		__syncthreads();
+		if(k0 + PG_KC < K)
+			load(k0 + PG_KC);
+#pragma unroll
+		for(int ks = 0; ks < PG_KC; ks += 4) {
+			double af[2], bf[2];
+#pragma unroll
+			for(int a = 0; a < 2; a++)
+				af[a] = sA[ks + lk][wm + 16 * a + li];
+#pragma unroll
+			for(int b = 0; b < 2; b++)
+				bf[b] = sB[ks + lk][wn + 16 * b + li];
+#pragma unroll
+			for(int a = 0; a < 2; a++)
+#pragma unroll
+				for(int b = 0; b < 2; b++)
+					acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+		}
+		__syncthreads();
+	}
+#pragma unroll
+	for(int a = 0; a < 2; a++)
+#pragma unroll
+		for(int b = 0; b < 2; b++) {
+			const int j = n0 + wn + 16 * b + li;
+			if(j >= N)
+				continue;
+#pragma unroll
+			for(int e = 0; e < 4; e++) {
+				const int i = m0 + wm + 16 * a + lk + 4 * e;
+				if(i >= M)
+					continue;
+				const double v = acc[a][b][e];
+				if(MODE == PG_TAPS)
+					ws[B.off_C + (size_t) i * (B.a + 1) + j] = v;
+				else if(MODE == PG_SYNTH)
+					ws[B.off_T + (size_t) i * B.nq + j] = v;
+				else
+					Mout[B.off_M + (size_t) i * (B.Lt + 1) + j] = (TO) (v * B.scale);
+			}
+		}
+}
+
+/*
+ * K[u][q] = sum_i T[i][q] T[i+u][nq-1-q], tiled: workgroup = one baseline
+ * x lq columns q (lq a power of two <= 16); the column blocks of T and of
+ * its mirror in LDS, zero-padded so that no bound is tested in the inner
+ * loop (Ta rows 0..np+6, Tb rows -7..np+13); each thread runs PG_LR = 8
+ * consecutive lags of one column over a register window of the mirror
+ * column: 16 LDS reads per 64 multiply-adds.
+ */
+#define PG_LR 8
+#define PG_LPA (PG_LR - 1)		/* Ta rows past np */
+#define PG_LPB (2 * PG_LR - 2)		/* Tb rows past np (7 before) */
+
+__host__ __device__ inline size_t pg_lag_lds(int np, int lq)
+{
+	return (size_t) ((np + PG_LPA) + (PG_LR - 1 + np + PG_LPB)) * lq * sizeof(double);
+}
+
+__global__ void __launch_bounds__(256) k_pg_lag2(const struct jb_bl *__restrict__ bl, int b0, int nb, const int *__restrict__ qblk_start, int lq, double *__restrict__ ws)
+{
+	extern __shared__ double lg_sm[];
+	int lo = 0, hi = nb - 1;
+	while(lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if(qblk_start[mid] <= (int) blockIdx.x)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	const struct jb_bl &B = bl[b0 + lo];
+	const int q0 = (blockIdx.x - qblk_start[lo]) * lq;
+	const int np = B.np, nq = B.nq;
+	const int na = np + PG_LPA, nbr = PG_LR - 1 + np + PG_LPB;
+	double *Ta = lg_sm;			/* [na][lq]: T[i][q0 + c] */
+	double *Tb = lg_sm + na * lq;		/* [nbr][lq]: T[i - 7][nq - 1 - (q0 + c)] */
+	const double *T = ws + B.off_T;
+	for(int e = threadIdx.x; e < nbr * lq; e += 256) {
+		const int r = e / lq, c = e - r * lq;
+		const int q = q0 + c;
+		if(r < na)
+			Ta[e] = q < nq && r < np ? T[(size_t) r * nq + q] : 0.0;
+		const int rb = r - (PG_LR - 1);
+		Tb[e] = q < nq && rb >= 0 && rb < np ? T[(size_t) rb * nq + (nq - 1 - q)] : 0.0;
+	}
+	__syncthreads();
+	const int c = threadIdx.x & (lq - 1), g = threadIdx.x / lq, ng = 256 / lq;
+	const int q = q0 + c;
+	if(q >= nq)
+		return;
+	double *Ko = ws + B.off_K;
+	const int nblk = (B.lags + PG_LR - 1) / PG_LR;
+	for(int blk = g; blk < nblk; blk += ng) {
+		const int uu0 = PG_LR * blk, u0 = uu0 - (np - 1);
+		/* rows i in [0, np) with 0 <= i + u < np for some lag u of the block */
+		int ilo = -u0 - (PG_LR - 1);
+		ilo = ilo < 0 ? 0 : ilo;
+		int ihi = np - u0;
+		ihi = ihi > np ? np : ihi;
+		double acc[PG_LR];
+#pragma unroll
+		for(int r = 0; r < PG_LR; r++)
+			acc[r] = 0.0;
+		/* win[t] = T'[ib + u0 + t], T' the mirror column; rows past the
+		 * range read the zero padding */
+		const double *tb = Tb + (size_t) (u0 + PG_LR - 1) * lq + c;
+		const double *ta = Ta + c;
+		double win[2 * PG_LR - 1];
+#pragma unroll
+		for(int t = 0; t < PG_LR - 1; t++)
+			win[PG_LR + t] = ilo < ihi ? tb[(ilo + t) * lq] : 0.0;
+		for(int ib = ilo; ib < ihi; ib += PG_LR) {
+#pragma unroll
+			for(int t = 0; t < PG_LR - 1; t++)
+				win[t] = win[PG_LR + t];
+#pragma unroll
+			for(int t = 0; t < PG_LR; t++)
+				win[PG_LR - 1 + t] = tb[(ib + PG_LR - 1 + t) * lq];
+			double tv[PG_LR];
+#pragma unroll
+			for(int s = 0; s < PG_LR; s++)
+				tv[s] = ta[(ib + s) * lq];
+#pragma unroll
+			for(int s = 0; s < PG_LR; s++)
+#pragma unroll
+				for(int r = 0; r < PG_LR; r++)
+					acc[r] = fma(tv[s], win[s + r], acc[r]);
+		}
+#pragma unroll
+		for(int r = 0; r < PG_LR; r++)
+			if(uu0 + r < B.lags)
+				Ko[(size_t) (uu0 + r) * nq + q] = acc[r];
+	}
 }
 
 /* R[(l, m)] = D^l_{0m} / nb = sqrt(4 pi / (2l+1)) conj(Y_lm(theta, phi)) / nb
@@ -362,7 +544,8 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st)
 	/* plan-generation workspace in chunks of baselines */
 	const size_t budget = (size_t) 1 << 30;
 	double *ws = NULL, *nodes = NULL, *hnodes = NULL;
-	size_t ws_cap = 0, nd_cap = 0;
+	int *d_ts = NULL, *h_ts = NULL;	/* per-baseline tile prefix sums of the four plan launches */
+	size_t ws_cap = 0, nd_cap = 0, ts_cap = 0;
 	int rc = -1;
 	int b0 = 0;
 	while(b0 < J->n_bl) {
@@ -372,7 +555,7 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st)
 		while(b1 < J->n_bl) {
 			struct jb_bl &B = J->bl[b1];
 			const int lq = B.a > B.Lt ? B.a : B.Lt;
-			const size_t need = (size_t) B.nt * (B.a + 1) + (size_t) B.nq * (lq + 1) + (size_t) B.np * (B.a + 1) + (size_t) B.np * B.nq + (size_t) B.lags * B.nq;
+			const size_t need = (size_t) B.nt * (B.a + 1) + (size_t) B.nq * (lq + 1) + (size_t) B.np * (B.a + 1) + (size_t) B.np * B.nq + (size_t) B.lags * B.nq + 3 * (size_t) B.nt;
 			if(b1 > b0 && (wsz + need) * sizeof(double) > budget)
 				break;
 			B.off_Pt = wsz;
@@ -380,6 +563,7 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st)
 			B.off_C = B.off_Pq + (long) B.nq * (lq + 1);
 			B.off_T = B.off_C + (long) B.np * (B.a + 1);
 			B.off_K = B.off_T + (long) B.np * B.nq;
+			B.off_Y = B.off_K + (long) B.lags * B.nq;
 			wsz += need;
 			B.off_nt = nsz;
 			nsz += 2 * (size_t) B.nt;
@@ -419,26 +603,62 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st)
 			memcpy(hnodes + B.off_nq, gx, B.nq * sizeof(double));
 			memcpy(hnodes + B.off_nq + B.nq, gw, B.nq * sizeof(double));
 		}
-		if(sphr_rt_check(hipMemcpyAsync(nodes, hnodes, nsz * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpy")
-		   || sphr_rt_check(hipMemcpyAsync(J->d_bl + b0, J->bl + b0, (size_t) (b1 - b0) * sizeof(struct jb_bl), hipMemcpyHostToDevice, st), "hipMemcpy"))
-			goto done;
 		{
 			const int nb = b1 - b0;
-			int np_max = 0, a_max = 0, nt_max = 0, lags_nq_max = 0, lags_lt_max = 0;
+			int np_max = 0, lags_nq_max = 0;
 			for(int b = b0; b < b1; b++) {
 				const struct jb_bl &B = J->bl[b];
 				np_max = B.np > np_max ? B.np : np_max;
-				a_max = B.a > a_max ? B.a : a_max;
-				nt_max = B.nt > nt_max ? B.nt : nt_max;
 				lags_nq_max = B.lags * B.nq > lags_nq_max ? B.lags * B.nq : lags_nq_max;
-				lags_lt_max = B.lags * (B.Lt + 1) > lags_lt_max ? B.lags * (B.Lt + 1) : lags_lt_max;
 			}
-			const int ntap = (np_max + PG_TC - 1) / PG_TC;
+			/* lag tile: the widest column block whose two LDS column
+			 * blocks stay within 48 KiB (four workgroups per CU); past
+			 * 160 KiB at one column the untiled kernel runs */
+			int lq = 16;
+			while(lq > 1 && pg_lag_lds(np_max, lq) > 48 * 1024)
+				lq >>= 1;
+			const bool lag_tiled = pg_lag_lds(np_max, lq) <= 160 * 1024;
+			if((size_t) 4 * nb > ts_cap) {
+				if(d_ts)
+					hipFree(d_ts);
+				d_ts = NULL;
+				free(h_ts);
+				h_ts = (int *) malloc((size_t) 4 * nb * sizeof(int));
+				if(!h_ts || sphr_rt_check(hipMalloc(&d_ts, (size_t) 4 * nb * sizeof(int)), "hipMalloc"))
+					goto done;
+				ts_cap = (size_t) 4 * nb;
+			}
+			long tot[4] = { 0, 0, 0, 0 };
+			for(int b = b0; b < b1; b++) {
+				const struct jb_bl &B = J->bl[b];
+				const int t = PG_TILE;
+				const long cnt[4] = {
+					(long) ((B.np + t - 1) / t) * ((B.a + 1 + t - 1) / t),
+					(long) ((B.np + t - 1) / t) * ((B.nq + t - 1) / t),
+					(long) ((B.lags + t - 1) / t) * ((B.Lt + 1 + t - 1) / t),
+					(long) ((B.nq + lq - 1) / lq),
+				};
+				for(int k = 0; k < 4; k++) {
+					h_ts[k * nb + (b - b0)] = (int) tot[k];
+					tot[k] += cnt[k];
+				}
+			}
+			if(tot[0] > INT_MAX || tot[1] > INT_MAX || tot[2] > INT_MAX || tot[3] > INT_MAX) {
+				sphr_set_error("sphr_jobs_new: plan chunk too large");
+				goto done;
+			}
+			if(sphr_rt_check(hipMemcpyAsync(nodes, hnodes, nsz * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpy")
+			   || sphr_rt_check(hipMemcpyAsync(J->d_bl + b0, J->bl + b0, (size_t) nb * sizeof(struct jb_bl), hipMemcpyHostToDevice, st), "hipMemcpy")
+			   || sphr_rt_check(hipMemcpyAsync(d_ts, h_ts, (size_t) 4 * nb * sizeof(int), hipMemcpyHostToDevice, st), "hipMemcpy"))
+				goto done;
 			hipLaunchKernelGGL(k_pg_tables, dim3(nb), dim3(256), 0, st, J->d_bl, b0, nodes, ws);
-			hipLaunchKernelGGL(k_pg_taps, dim3(ntap, nb), dim3(256), PG_TC * nt_max * sizeof(double), st, J->d_bl, b0, nodes, ws);
-			hipLaunchKernelGGL(k_pg_synth, dim3(ntap, nb), dim3(256), PG_TC * (a_max + 1) * sizeof(double), st, J->d_bl, b0, ws);
-			hipLaunchKernelGGL(k_pg_lag, dim3((lags_nq_max + 255) / 256, nb), dim3(256), 0, st, J->d_bl, b0, ws);
-			hipLaunchKernelGGL((k_pg_M<T>), dim3((lags_lt_max + 255) / 256, nb), dim3(256), 0, st, J->d_bl, b0, nodes, ws, (T *) J->M);
+			hipLaunchKernelGGL((k_pg_gemm<PG_TAPS, double>), dim3(tot[0]), dim3(256), 0, st, J->d_bl, b0, nb, d_ts, nodes, ws, (double *) NULL);
+			hipLaunchKernelGGL((k_pg_gemm<PG_SYNTH, double>), dim3(tot[1]), dim3(256), 0, st, J->d_bl, b0, nb, d_ts + nb, nodes, ws, (double *) NULL);
+			if(lag_tiled)
+				hipLaunchKernelGGL(k_pg_lag2, dim3(tot[3]), dim3(256), pg_lag_lds(np_max, lq), st, J->d_bl, b0, nb, d_ts + 3 * nb, lq, ws);
+			else
+				hipLaunchKernelGGL(k_pg_lag, dim3((lags_nq_max + 255) / 256, nb), dim3(256), 0, st, J->d_bl, b0, ws);
+			hipLaunchKernelGGL((k_pg_gemm<PG_M, T>), dim3(tot[2]), dim3(256), 0, st, J->d_bl, b0, nb, d_ts + 2 * nb, nodes, ws, (T *) J->M);
 			hipLaunchKernelGGL((k_pg_rot<T>), dim3(nb), dim3(128), 0, st, J->d_bl, b0, (cplx<T> *) J->R);
 			if(sphr_rt_check(hipGetLastError(), "plan kernels") || sphr_rt_check(hipStreamSynchronize(st), "plan"))
 				goto done;
@@ -451,7 +671,10 @@ done:
 		hipFree(ws);
 	if(nodes)
 		hipFree(nodes);
+	if(d_ts)
+		hipFree(d_ts);
 	free(hnodes);
+	free(h_ts);
 	return rc;
 }
 
