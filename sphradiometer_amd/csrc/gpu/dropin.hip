@@ -61,58 +61,50 @@ int synth_tables_build(int L, struct synth_dev *t)
 	}
 	const double *x;
 	sphr_mesh_rule(L, &x, NULL);
-	double *hx = (double *) malloc(B * sizeof(double));
-	double *hs = (double *) malloc((size_t) B * (L + 1) * sizeof(double));
+	/* every table in one device allocation, filled by one copy:
+	 * x [B], Pbar_mm [B][L+1], alpha [nal], twiddles [2n], cm [L+1],
+	 * the expanded phi-DFT matrix [2 (2L+1)][2 n] in float64 then float32 */
 	const size_t nal = (size_t) (L + 1) * (L + 2) / 2;
-	double *ha = (double *) calloc(nal, sizeof(double));
-	double *ht = (double *) malloc(2 * (size_t) n * sizeof(double));
+	const int nm = 2 * L + 1;
+	const size_t ne = (size_t) 2 * nm * 2 * n;
+	auto up = [](size_t b) { return (b + 255) & ~(size_t) 255; };
+	const size_t o_x = 0, o_s = up(o_x + B * sizeof(double)), o_a = up(o_s + (size_t) B * (L + 1) * sizeof(double));
+	const size_t o_t = up(o_a + nal * sizeof(double)), o_c = up(o_t + 2 * (size_t) n * sizeof(double));
+	const size_t o_e = up(o_c + (L + 1) * sizeof(double)), o_f = up(o_e + ne * sizeof(double));
+	const size_t total = o_f + ne * sizeof(float);
+	char *hb = (char *) calloc(total, 1);
 	double *tmp = (double *) malloc((L + 1) * sizeof(double));
 	int rc = -1;
-	if(!hx || !hs || !ha || !ht || !tmp)
+	if(!hb || !tmp) {
+		sphr_set_error("synthesis tables: out of memory");
 		goto done;
-	for(int j = 0; j < B; j++) {
-		hx[j] = x[j];
-		for(int m = 0; m <= L; m++) {
-			sphr_plm(m, m, x[j], tmp);
-			hs[(size_t) j * (L + 1) + m] = tmp[0];
-		}
 	}
-	/* alpha_lm = sqrt((4l^2 - 1) / (l^2 - m^2)) at the (l, m >= 0) slot of
-	 * the coefficient layout */
-	for(int m = 0; m <= L; m++)
-		for(int l = m + 1; l <= L; l++)
-			ha[lm_offset(L, l, m)] = sqrt((4.0 * l * l - 1.0) / ((double) (l - m) * (l + m)));
-	sphr_twiddles(n, +1, ht);
-	if(sphr_rt_check(hipMalloc(&t->x, B * sizeof(double)), "hipMalloc")
-	   || sphr_rt_check(hipMalloc(&t->smm, (size_t) B * (L + 1) * sizeof(double)), "hipMalloc")
-	   || sphr_rt_check(hipMalloc(&t->alpha, nal * sizeof(double)), "hipMalloc")
-	   || sphr_rt_check(hipMalloc(&t->twp, 2 * (size_t) n * sizeof(double)), "hipMalloc"))
-		goto done;
-	if(sphr_rt_check(hipMemcpy(t->x, hx, B * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
-	   || sphr_rt_check(hipMemcpy(t->smm, hs, (size_t) B * (L + 1) * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
-	   || sphr_rt_check(hipMemcpy(t->alpha, ha, nal * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
-	   || sphr_rt_check(hipMemcpy(t->twp, ht, 2 * (size_t) n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
-		goto done;
-	/* Pbar_mm at the equator is its sectoral constant */
-	for(int m = 0; m <= L; m++) {
-		double one;
-		sphr_plm(m, m, 0.0, &one);
-		tmp[m] = one;
-	}
-	if(sphr_rt_check(hipMalloc(&t->cm, (L + 1) * sizeof(double)), "hipMalloc") || sphr_rt_check(hipMemcpy(t->cm, tmp, (L + 1) * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
-		goto done;
-	t->tb.x = t->x;
-	t->tb.smm = t->smm;
-	t->tb.alpha = t->alpha;
-	t->tb.twp = t->twp;
 	{
-		/* the expanded phi-DFT matrix [2 (2L+1)][2 n] */
-		const int nm = 2 * L + 1;
-		const size_t ne = (size_t) 2 * nm * 2 * n;
-		double *e64 = (double *) malloc(ne * sizeof(double));
-		float *e32 = (float *) malloc(ne * sizeof(float));
-		int bad = !e64 || !e32;
-		for(int i = 0; i < nm && !bad; i++) {
+		double *hx = (double *) (hb + o_x), *hs = (double *) (hb + o_s), *ha = (double *) (hb + o_a);
+		double *ht = (double *) (hb + o_t), *hc = (double *) (hb + o_c), *e64 = (double *) (hb + o_e);
+		float *e32 = (float *) (hb + o_f);
+		for(int j = 0; j < B; j++) {
+			hx[j] = x[j];
+			for(int m = 0; m <= L; m++) {
+				sphr_plm(m, m, x[j], tmp);
+				hs[(size_t) j * (L + 1) + m] = tmp[0];
+			}
+		}
+		/* alpha_lm = sqrt((4l^2 - 1) / (l^2 - m^2)) at the (l, m >= 0)
+		 * slot of the coefficient layout */
+		for(int m = 0; m <= L; m++)
+			for(int l = m + 1; l <= L; l++)
+				ha[lm_offset(L, l, m)] = sqrt((4.0 * l * l - 1.0) / ((double) (l - m) * (l + m)));
+		sphr_twiddles(n, +1, ht);
+		/* Pbar_mm at the equator is its sectoral constant */
+		for(int m = 0; m <= L; m++) {
+			double one;
+			sphr_plm(m, m, 0.0, &one);
+			hc[m] = one;
+		}
+		/* rows 2 i, 2 i + 1 of the phi DFT for column i (m = i for i <= L,
+		 * i - (2L+1) above) */
+		for(int i = 0; i < nm; i++) {
 			const int m = i <= L ? i : i - nm;
 			for(int k = 0; k < n; k++) {
 				const long r = (((long) k * m) % n + n) % n;
@@ -124,18 +116,27 @@ int synth_tables_build(int L, struct synth_dev *t)
 				r1[1] = er;
 			}
 		}
-		for(size_t q = 0; q < ne && !bad; q++)
+		for(size_t q = 0; q < ne; q++)
 			e32[q] = (float) e64[q];
-		bad = bad || sphr_rt_check(hipMalloc(&t->E64, ne * sizeof(double)), "hipMalloc") || sphr_rt_check(hipMalloc(&t->E32, ne * sizeof(float)), "hipMalloc")
-		      || sphr_rt_check(hipMemcpy(t->E64, e64, ne * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")
-		      || sphr_rt_check(hipMemcpy(t->E32, e32, ne * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
-		free(e64);
-		free(e32);
-		if(bad)
-			goto done;
-		t->tb.E64 = t->E64;
-		t->tb.E32 = t->E32;
 	}
+	if(sphr_rt_check(hipMalloc(&t->blob, total), "hipMalloc") || sphr_rt_check(hipMemcpy(t->blob, hb, total, hipMemcpyHostToDevice), "hipMemcpy"))
+		goto done;
+	{
+		char *d = (char *) t->blob;
+		t->x = (double *) (d + o_x);
+		t->smm = (double *) (d + o_s);
+		t->alpha = (double *) (d + o_a);
+		t->twp = (double *) (d + o_t);
+		t->cm = (double *) (d + o_c);
+		t->E64 = (double *) (d + o_e);
+		t->E32 = (float *) (d + o_f);
+	}
+	t->tb.x = t->x;
+	t->tb.smm = t->smm;
+	t->tb.alpha = t->alpha;
+	t->tb.twp = t->twp;
+	t->tb.E64 = t->E64;
+	t->tb.E32 = t->E32;
 	if(sky_fused_order(L)) {
 		if(sky_fused_tables(L, &t->tb, &t->fz))
 			goto done;
@@ -146,10 +147,7 @@ int synth_tables_build(int L, struct synth_dev *t)
 	}
 	rc = 0;
 done:
-	free(hx);
-	free(hs);
-	free(ha);
-	free(ht);
+	free(hb);
 	free(tmp);
 	if(rc)
 		synth_tables_free(t);
@@ -158,20 +156,8 @@ done:
 
 void synth_tables_free(struct synth_dev *t)
 {
-	if(t->x)
-		hipFree(t->x);
-	if(t->smm)
-		hipFree(t->smm);
-	if(t->alpha)
-		hipFree(t->alpha);
-	if(t->twp)
-		hipFree(t->twp);
-	if(t->cm)
-		hipFree(t->cm);
-	if(t->E64)
-		hipFree(t->E64);
-	if(t->E32)
-		hipFree(t->E32);
+	if(t->blob)
+		hipFree(t->blob);
 	if(t->fz.bf64)
 		hipFree(t->fz.bf64);
 	if(t->fz.bf32)

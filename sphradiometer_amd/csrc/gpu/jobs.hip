@@ -85,54 +85,64 @@ __device__ inline double sinc_pi(double x)
 	return px == 0.0 ? 1.0 : sin(px) / px;
 }
 
-/* normalised Legendre Pbar_l0(x), l = 0..lmax, into row[0..lmax] */
-__device__ inline void pbar_row(double x, int lmax, double *row)
+/* normalised Legendre Pbar_l0(x), l = 0..lmax, into row[0..lmax]:
+ * Pbar_l0 = a_l x Pbar_{l-1,0} - (a_l / a_{l-1}) Pbar_{l-2,0},
+ * a_l = sqrt((4 l^2 - 1) / l^2), with lc[2l] = a_l, lc[2l+1] = a_l / a_{l-1} */
+__device__ inline void pbar_row(double x, int lmax, const double *__restrict__ lc, double *row)
 {
-	double p0 = 0.28209479177387814347, p1 = 0.0, am1 = 1.0;
+	double p0 = 0.28209479177387814347, p1 = lc[2] * x * p0;
 	row[0] = p0;
-	for(int l = 1; l <= lmax; l++) {
-		const double al = sqrt((4.0 * l * l - 1.0) / ((double) l * l));
-		const double pn = l == 1 ? al * x * p0 : al * (x * p0 - p1 / am1);
-		p1 = p0;
-		p0 = pn;
-		am1 = al;
+	if(lmax >= 1)
+		row[1] = p1;
+	for(int l = 2; l <= lmax; l++) {
+		const double pn = lc[2 * l] * x * p1 - lc[2 * l + 1] * p0;
+		p0 = p1;
+		p1 = pn;
 		row[l] = pn;
 	}
 }
 
 /*
- * Per baseline: Pbar_l0 at the tap nodes (Pt[j][l], l <= a) and at the
- * product nodes (Pq[q][l], l <= max(a, Lt)); the delay table of the tap
- * nodes: y_j = r_dt cos(acos x_j) (the delay of node j in samples,
- * projection.c:99-107), sin(pi y_j), and the weight w_j 2 pi / nt.
+ * Per baseline, over the first half of each (mirror-symmetric) rule:
+ * Pbar_l0 at the tap nodes (Pt[j][l], j < nt/2, l <= a) and at the
+ * product nodes (Pq[q][l], q < (nq+1)/2, l <= max(a, Lt)); the delay
+ * table of the tap nodes: y_j = r_dt cos(acos x_j) (the delay of node j
+ * in samples, projection.c:99-107), g_j = sin(pi y_j) nt (2 pi w_j / nt)
+ * / pi and the weight 2 pi w_j / nt; the product-rule weights.
+ * Y = [y | g | weight] (nt/2 each) then (nq+1)/2 product weights.
  */
-__global__ void k_pg_tables(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, double *__restrict__ ws)
+__global__ void k_pg_tables(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, const double *__restrict__ lc, double *__restrict__ ws)
 {
 	const struct jb_bl &B = bl[b0 + blockIdx.x];
 	const int lq = B.a > B.Lt ? B.a : B.Lt;
-	for(int i = threadIdx.x; i < B.nt + B.nq; i += blockDim.x) {
-		if(i < B.nt) {
+	const int nth = B.nt / 2, nqh = (B.nq + 1) / 2;
+	for(int i = threadIdx.x; i < nth + nqh; i += blockDim.x) {
+		if(i < nth) {
 			const double x = nodes[B.off_nt + i];
-			pbar_row(x, B.a, ws + B.off_Pt + (size_t) i * (B.a + 1));
+			pbar_row(x, B.a, lc, ws + B.off_Pt + (size_t) i * (B.a + 1));
 			const double y = B.r_dt * cos(acos(x));
+			const double ww = nodes[B.off_nt + B.nt + i] * (2. * M_PI / B.nt);
 			ws[B.off_Y + i] = y;
-			ws[B.off_Y + B.nt + i] = sin(M_PI * y);
-			ws[B.off_Y + 2 * B.nt + i] = nodes[B.off_nt + B.nt + i] * (2. * M_PI / B.nt);
+			ws[B.off_Y + nth + i] = sin(M_PI * y) / M_PI * (double) B.nt * ww;
+			ws[B.off_Y + 2 * nth + i] = ww;
 		} else {
-			const int q = i - B.nt;
-			pbar_row(nodes[B.off_nq + q], lq, ws + B.off_Pq + (size_t) q * (lq + 1));
+			const int q = i - nth;
+			pbar_row(nodes[B.off_nq + q], lq, lc, ws + B.off_Pq + (size_t) q * (lq + 1));
+			ws[B.off_Y + 3 * nth + q] = nodes[B.off_nq + B.nq + q];
 		}
 	}
 }
 
-/* K[u][q] = sum_i T[i][q] T[i+u][nq-1-q], u = uu - (np-1) */
+/* K[u][q] = sum_i T[i][q] T[i+u][nq-1-q], u = uu - (np-1), q < (nq+1)/2,
+ * one thread per entry: for taps too many for the tiled kernel's LDS */
 __global__ void k_pg_lag(const struct jb_bl *__restrict__ bl, int b0, double *__restrict__ ws)
 {
 	const struct jb_bl &B = bl[b0 + blockIdx.y];
+	const int nqh = (B.nq + 1) / 2;
 	const long e = (long) blockIdx.x * blockDim.x + threadIdx.x;
-	if(e >= (long) B.lags * B.nq)
+	if(e >= (long) B.lags * nqh)
 		return;
-	const int uu = (int) (e / B.nq), q = (int) (e - (long) uu * B.nq);
+	const int uu = (int) (e / nqh), q = (int) (e - (long) uu * nqh);
 	const int u = uu - (B.np - 1);
 	const double *T = ws + B.off_T;
 	const int lo = u < 0 ? -u : 0, hi = u > 0 ? B.np - u : B.np;
@@ -149,11 +159,20 @@ __global__ void k_pg_lag(const struct jb_bl *__restrict__ bl, int b0, double *__
  * chunks of 16 staged in LDS through registers (the next chunk's loads in
  * flight during this chunk's MFMAs).  The workgroup finds its baseline
  * in the chunk's tile prefix sums.
- *   PG_TAPS:  C[i][l] = sum_j F[i][j] Pt[j][l], F the weighted sinc
- *             samples (generated in the loader), i < np, l <= a, j < nt;
- *   PG_SYNTH: T[i][q] = sum_l C[i][l] Pq[q][l], i < np, q < nq, l <= a;
- *   PG_M:     M[u][d] = scale sum_q K[u][q] (2 pi w_q) Pq[q][d],
- *             u < lags, d <= Lt, q < nq.
+ *
+ * Both quadrature rules are mirror-symmetric: node n-1-j is -x_j with the
+ * same weight, and Pbar_l0(-x) = (-1)^l Pbar_l0(x).  Each product folds
+ * over the node pairs, which halves its arithmetic:
+ *   PG_TAPS:  C[i][l] = sum_{j < nt/2} (F[i][j] + (-1)^l F[i][j']) Pt[j][l],
+ *             F the weighted sinc samples (generated in the loader); a
+ *             tile's 64 columns are 32 even l (A = F + F') and 32 odd l
+ *             (A = F - F'), l = 64 t + 2c (+1);
+ *   PG_SYNTH: E, O = sum over even, odd l of C[i][l] Pq[q][l] for
+ *             q < nqh = (nq+1)/2; T[i][q] = E + O, T[i][nq-1-q] = E - O;
+ *             each K chunk of 16 holds 8 even l then 8 odd l;
+ *   PG_M:     M[u][d] = scale sum_{q < nqh} (K[u][q] + (-1)^d K[u][q'])
+ *             (2 pi w_q) Pq[q][d] (the middle node once), with
+ *             K[u][q'] = K[-u][q]; columns as in PG_TAPS.
  * MFMA 16x16x4 f64 operand layout: A[i][k] in lane i + 16 k, B[k][j] in
  * lane j + 16 k, accumulator element e of lane j + 16 q holds D[q + 4 e][j].
  */
@@ -161,28 +180,12 @@ enum { PG_TAPS, PG_SYNTH, PG_M };
 #define PG_TILE 64
 #define PG_KC 16
 
-__device__ inline void pg_dims(int mode, const struct jb_bl &B, int *M, int *N, int *K)
-{
-	if(mode == PG_TAPS) {
-		*M = B.np;
-		*N = B.a + 1;
-		*K = B.nt;
-	} else if(mode == PG_SYNTH) {
-		*M = B.np;
-		*N = B.nq;
-		*K = B.a + 1;
-	} else {
-		*M = B.lags;
-		*N = B.Lt + 1;
-		*K = B.nq;
-	}
-}
-
 template <int MODE, typename TO>
-__global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict__ bl, int b0, int nb, const int *__restrict__ tile_start, const double *__restrict__ nodes, double *__restrict__ ws, TO *__restrict__ Mout)
+__global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict__ bl, int b0, int nb, const int *__restrict__ tile_start, double *__restrict__ ws, TO *__restrict__ Mout)
 {
 	typedef double v4 __attribute__((ext_vector_type(4)));
-	__shared__ double sA[PG_KC][PG_TILE + 4];	/* [k][row] */
+	constexpr int NA = MODE == PG_SYNTH ? 1 : 2;	/* A tiles: pair sum and difference */
+	__shared__ double sA[NA][PG_KC][PG_TILE + 4];	/* [k][row] */
 	__shared__ double sB[PG_KC][PG_TILE + 4];	/* [k][col] */
 	/* this workgroup's baseline: the last with tile_start <= blockIdx.x */
 	int lo = 0, hi = nb - 1;
@@ -194,8 +197,10 @@ __global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict_
 			hi = mid - 1;
 	}
 	const struct jb_bl &B = bl[b0 + lo];
-	int M, N, K;
-	pg_dims(MODE, B, &M, &N, &K);
+	const int nqh = (B.nq + 1) / 2;
+	const int M = MODE == PG_M ? B.lags : B.np;
+	const int K = MODE == PG_TAPS ? B.nt / 2 : MODE == PG_SYNTH ? B.a + 1 : nqh;
+	const int ncol = MODE == PG_TAPS ? B.a + 1 : MODE == PG_SYNTH ? nqh : B.Lt + 1;
 	const int tiles_m = (M + PG_TILE - 1) / PG_TILE;
 	const int local = blockIdx.x - tile_start[lo];
 	const int m0 = (local % tiles_m) * PG_TILE, n0 = (local / tiles_m) * PG_TILE;
@@ -203,57 +208,94 @@ __global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict_
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int li = lane & 15, lk = lane >> 4;
 	const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-	/* element loaders (zero outside the product) */
-	auto Aval = [&](int i, int k) -> double {
-		if(i >= M || k >= K)
-			return 0.0;
-		if(MODE == PG_TAPS) {
-			/* delay element of tap kk = i - (np-1)/2 at node k,
-			 * sinc(-kk - y) = (-1)^kk sin(pi y) / (pi (kk + y)) */
-			const int kk = -(B.np - 1) / 2 + i;
-			const double y = ws[B.off_Y + k], sy = ws[B.off_Y + B.nt + k];
-			const double den = (double) kk + y;
-			const double f = den == 0.0 ? 1.0 : ((kk & 1) ? -sy : sy) / (M_PI * den);
-			return (f * (double) B.nt) * ws[B.off_Y + 2 * B.nt + k];
+	const int sel = NA == 2 && wn ? 1 : 0;	/* odd columns: the difference tile */
+	/* tile column c -> output column */
+	auto col = [&](int c) -> int { return MODE == PG_SYNTH ? n0 + c : n0 + (c < 32 ? 2 * c : 2 * (c - 32) + 1); };
+	/* K chunk entry -> l (synthesis: 8 even l, then 8 odd l) */
+	auto kl = [&](int k0, int k) -> int { return MODE == PG_SYNTH ? k0 + (k < 8 ? 2 * k : 2 * (k - 8) + 1) : k0 + k; };
+	const int nth = B.nt / 2;
+	const double *Y = ws + B.off_Y;
+	/* the pair (F[i][j], F[i][j']) of weighted delay elements of tap
+	 * kk = i - (np-1)/2 at tap nodes j and j' = nt-1-j (y_j' = -y_j):
+	 * F = (nt w 2 pi / nt) sinc(-kk - y), sinc(-kk - y) = (-1)^kk sin(pi y)
+	 * / (pi (kk + y)), so with g = sin(pi y) nt (w 2 pi / nt) / pi,
+	 * F + F' = (-1)^kk g (-2 y) / ((kk + y)(kk - y)) and
+	 * F - F' = (-1)^kk g (2 kk) / ((kk + y)(kk - y)): one division */
+	auto fpair = [&](int i, int j, double *sum, double *dif) {
+		const int kk = -(B.np - 1) / 2 + i;
+		const double y = Y[j], g = Y[nth + j];
+		const double dp = (double) kk + y, dm = (double) kk - y;
+		const double pr = dp * dm;
+		if(pr != 0.0) {
+			const double r = ((kk & 1) ? -g : g) / pr;
+			*sum = r * (-2.0 * y);
+			*dif = r * (2.0 * (double) kk);
+		} else {
+			/* a delay on the sample grid: sinc(0) = 1 there */
+			const double one = (double) B.nt * Y[2 * nth + j];
+			const double f1 = dp == 0.0 ? one : ((kk & 1) ? -g : g) / dp;
+			const double f2 = dm == 0.0 ? one : ((kk & 1) ? g : -g) / dm;
+			*sum = f1 + f2;
+			*dif = f1 - f2;
 		}
-		if(MODE == PG_SYNTH)
-			return ws[B.off_C + (size_t) i * (B.a + 1) + k];
-		return ws[B.off_K + (size_t) i * B.nq + k];
 	};
-	auto Bval = [&](int k, int j) -> double {
-		if(k >= K || j >= N)
-			return 0.0;
-		if(MODE == PG_TAPS)
-			return ws[B.off_Pt + (size_t) k * (B.a + 1) + j];
-		if(MODE == PG_SYNTH)
-			return ws[B.off_Pq + (size_t) j * (lq + 1) + k];
-		return (2. * M_PI * nodes[B.off_nq + B.nq + k]) * ws[B.off_Pq + (size_t) k * (lq + 1) + j];
-	};
-	/* staging: 4 A and 4 B values per thread per chunk; A read along k
-	 * (rows of 16), B along the columns */
-	double ra[4], rb[4];
+	double ra[NA][4], rb[4];
 	auto load = [&](int k0) {
 #pragma unroll
 		for(int r = 0; r < 4; r++) {
 			const int e = tid + 256 * r;
-			ra[r] = Aval(m0 + e / PG_KC, k0 + e % PG_KC);
-			rb[r] = Bval(k0 + e / PG_TILE, n0 + e % PG_TILE);
+			const int i = m0 + e / PG_KC, k = kl(k0, e % PG_KC);
+			double v0 = 0.0, v1 = 0.0;
+			if(i < M && k < K) {
+				if(MODE == PG_TAPS)
+					fpair(i, k, &v0, &v1);
+				else if(MODE == PG_SYNTH)
+					v0 = ws[B.off_C + (size_t) i * (B.a + 1) + k];
+				else {
+					const double kp = ws[B.off_K + (size_t) i * nqh + k];
+					if(2 * k == B.nq - 1)
+						v0 = kp;
+					else {
+						const double km = ws[B.off_K + (size_t) (B.lags - 1 - i) * nqh + k];
+						v0 = kp + km;
+						v1 = kp - km;
+					}
+				}
+			}
+			ra[0][r] = v0;
+			if(NA == 2)
+				ra[NA - 1][r] = v1;
+			const int kb = kl(k0, e / PG_TILE), j = col(e % PG_TILE);
+			double vb = 0.0;
+			if(kb < K && j < ncol) {
+				if(MODE == PG_TAPS)
+					vb = ws[B.off_Pt + (size_t) kb * (B.a + 1) + j];
+				else if(MODE == PG_SYNTH)
+					vb = ws[B.off_Pq + (size_t) j * (lq + 1) + kb];
+				else
+					vb = (2. * M_PI * Y[3 * nth + kb]) * ws[B.off_Pq + (size_t) kb * (lq + 1) + j];
+			}
+			rb[r] = vb;
 		}
 	};
 	auto store = [&]() {
 #pragma unroll
 		for(int r = 0; r < 4; r++) {
 			const int e = tid + 256 * r;
-			sA[e % PG_KC][e / PG_KC] = ra[r];
+#pragma unroll
+			for(int h = 0; h < NA; h++)
+				sA[h][e % PG_KC][e / PG_KC] = ra[h][r];
 			sB[e / PG_TILE][e % PG_TILE] = rb[r];
 		}
 	};
-	v4 acc[2][2];
+	v4 acc[2][2][2];	/* [parity of l (synthesis)][row block][column block] */
 #pragma unroll
-	for(int a = 0; a < 2; a++)
+	for(int p = 0; p < 2; p++)
 #pragma unroll
-		for(int b = 0; b < 2; b++)
-			acc[a][b] = (v4) 0;
+		for(int a = 0; a < 2; a++)
+#pragma unroll
+			for(int b = 0; b < 2; b++)
+				acc[p][a][b] = (v4) 0;
 	load(0);
 	for(int k0 = 0; k0 < K; k0 += PG_KC) {
 		store();
@@ -262,10 +304,11 @@ __global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict_
 			load(k0 + PG_KC);
 #pragma unroll
 		for(int ks = 0; ks < PG_KC; ks += 4) {
+			const int p = MODE == PG_SYNTH && ks >= 8 ? 1 : 0;
 			double af[2], bf[2];
 #pragma unroll
 			for(int a = 0; a < 2; a++)
-				af[a] = sA[ks + lk][wm + 16 * a + li];
+				af[a] = sA[sel][ks + lk][wm + 16 * a + li];
 #pragma unroll
 			for(int b = 0; b < 2; b++)
 				bf[b] = sB[ks + lk][wn + 16 * b + li];
@@ -273,7 +316,7 @@ __global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict_
 			for(int a = 0; a < 2; a++)
 #pragma unroll
 				for(int b = 0; b < 2; b++)
-					acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+					acc[p][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[p][a][b], 0, 0, 0);
 		}
 		__syncthreads();
 	}
@@ -281,27 +324,33 @@ __global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict_
 	for(int a = 0; a < 2; a++)
 #pragma unroll
 		for(int b = 0; b < 2; b++) {
-			const int j = n0 + wn + 16 * b + li;
-			if(j >= N)
+			const int j = col(wn + 16 * b + li);
+			if(j >= ncol)
 				continue;
 #pragma unroll
 			for(int e = 0; e < 4; e++) {
 				const int i = m0 + wm + 16 * a + lk + 4 * e;
 				if(i >= M)
 					continue;
-				const double v = acc[a][b][e];
+				const double v = acc[0][a][b][e];
 				if(MODE == PG_TAPS)
 					ws[B.off_C + (size_t) i * (B.a + 1) + j] = v;
-				else if(MODE == PG_SYNTH)
-					ws[B.off_T + (size_t) i * B.nq + j] = v;
-				else
+				else if(MODE == PG_SYNTH) {
+					const double o = acc[1][a][b][e];
+					double *Ti = ws + B.off_T + (size_t) i * B.nq;
+					Ti[j] = v + o;
+					if(2 * j != B.nq - 1)
+						Ti[B.nq - 1 - j] = v - o;
+				} else
 					Mout[B.off_M + (size_t) i * (B.Lt + 1) + j] = (TO) (v * B.scale);
 			}
 		}
 }
 
 /*
- * K[u][q] = sum_i T[i][q] T[i+u][nq-1-q], tiled: workgroup = one baseline
+ * K[u][q] = sum_i T[i][q] T[i+u][nq-1-q] for q < nqh = (nq+1)/2 (the
+ * other half is K[u][nq-1-q] = K[-u][q]), stored [lags][nqh].
+ * Tiled: workgroup = one baseline
  * x lq columns q (lq a power of two <= 16); the column blocks of T and of
  * its mirror in LDS, zero-padded so that no bound is tested in the inner
  * loop (Ta rows 0..np+6, Tb rows -7..np+13); each thread runs PG_LR = 8
@@ -345,8 +394,8 @@ __global__ void __launch_bounds__(256) k_pg_lag2(const struct jb_bl *__restrict_
 	}
 	__syncthreads();
 	const int c = threadIdx.x & (lq - 1), g = threadIdx.x / lq, ng = 256 / lq;
-	const int q = q0 + c;
-	if(q >= nq)
+	const int q = q0 + c, nqh = (nq + 1) / 2;
+	if(q >= nqh)
 		return;
 	double *Ko = ws + B.off_K;
 	const int nblk = (B.lags + PG_LR - 1) / PG_LR;
@@ -389,7 +438,7 @@ __global__ void __launch_bounds__(256) k_pg_lag2(const struct jb_bl *__restrict_
 #pragma unroll
 		for(int r = 0; r < PG_LR; r++)
 			if(uu0 + r < B.lags)
-				Ko[(size_t) (uu0 + r) * nq + q] = acc[r];
+				Ko[(size_t) (uu0 + r) * nqh + q] = acc[r];
 	}
 }
 
@@ -538,135 +587,202 @@ extern "C" void sphr_jobs_free(struct sphr_jobs *J)
 	free(J);
 }
 
+/*
+ * All plans, in chunks of baselines that share one float64 workspace of at
+ * most 1 GiB.  Everything the kernels read besides the workspace is laid
+ * out and uploaded once: the baseline table with each chunk's workspace
+ * offsets, every distinct quadrature rule once, and the per-chunk tile
+ * prefix sums; the chunks then run back to back on the stream (each
+ * reuses the workspace in stream order) with one synchronisation at the
+ * end.
+ */
 template <typename T>
 static int generate_plans(struct sphr_jobs *J, hipStream_t st)
 {
-	/* plan-generation workspace in chunks of baselines */
 	const size_t budget = (size_t) 1 << 30;
+	const int nbl = J->n_bl;
 	double *ws = NULL, *nodes = NULL, *hnodes = NULL;
-	int *d_ts = NULL, *h_ts = NULL;	/* per-baseline tile prefix sums of the four plan launches */
-	size_t ws_cap = 0, nd_cap = 0, ts_cap = 0;
-	int rc = -1;
-	int b0 = 0;
-	while(b0 < J->n_bl) {
-		/* choose the chunk and lay out its workspace */
-		size_t wsz = 0, nsz = 0;
+	int *d_ts = NULL, *h_ts = NULL;	/* per chunk: tile prefix sums of its four plan launches */
+	long *rule_t = NULL, *rule_q = NULL;
+	struct pg_chunk {
+		int b0, b1, lq, lag_tiled, np_max, lags_nq_max;
+		long tot[4];
+	} *ch = NULL;
+	int nch = 0, rc = -1;
+	size_t wsz_max = 0, nsz = 0, lc_off = 0;
+	int lc_n = 0;
+	int a_max = 0, nq_max = 0;
+	for(int b = 0; b < nbl; b++) {
+		a_max = J->bl[b].a > a_max ? J->bl[b].a : a_max;
+		nq_max = J->bl[b].nq > nq_max ? J->bl[b].nq : nq_max;
+	}
+	rule_t = (long *) malloc((a_max + 1) * sizeof(long));
+	rule_q = (long *) malloc((nq_max + 1) * sizeof(long));
+	ch = (struct pg_chunk *) calloc(nbl, sizeof(*ch));
+	h_ts = (int *) malloc((size_t) 4 * nbl * sizeof(int));
+	if(!rule_t || !rule_q || !ch || !h_ts) {
+		sphr_set_error("sphr_jobs_new: out of memory");
+		goto done;
+	}
+	/* distinct rules: the tap mesh of order a lists cos theta descending
+	 * (sh_series_trans.c:149-196); the product rule is any exact rule */
+	for(int i = 0; i <= a_max; i++)
+		rule_t[i] = -1;
+	for(int i = 0; i <= nq_max; i++)
+		rule_q[i] = -1;
+	for(int b = 0; b < nbl; b++) {
+		struct jb_bl &B = J->bl[b];
+		if(rule_t[B.a] < 0) {
+			rule_t[B.a] = nsz;
+			nsz += 2 * (size_t) B.nt;
+		}
+		if(rule_q[B.nq] < 0) {
+			rule_q[B.nq] = nsz;
+			nsz += 2 * (size_t) B.nq;
+		}
+		B.off_nt = rule_t[B.a];
+		B.off_nq = rule_q[B.nq];
+	}
+	{
+		int lq_max = 0;
+		for(int b = 0; b < nbl; b++) {
+			const int lq = J->bl[b].a > J->bl[b].Lt ? J->bl[b].a : J->bl[b].Lt;
+			lq_max = lq > lq_max ? lq : lq_max;
+		}
+		lc_off = nsz;
+		lc_n = lq_max + 1;
+		nsz += 2 * (size_t) lc_n;
+	}
+	hnodes = (double *) malloc(nsz * sizeof(double));
+	if(!hnodes) {
+		sphr_set_error("sphr_jobs_new: out of memory");
+		goto done;
+	}
+	{
+		/* compute the rules not cached yet, in parallel */
+		int *ord = (int *) malloc((a_max + nq_max + 2) * sizeof(int)), n_t = 0, n_q = 0;
+		if(!ord) {
+			sphr_set_error("sphr_jobs_new: out of memory");
+			goto done;
+		}
+		for(int i = 0; i <= a_max; i++)
+			if(rule_t[i] >= 0)
+				ord[n_t++] = i;
+		for(int i = 0; i <= nq_max; i++)
+			if(rule_q[i] >= 0)
+				ord[n_t + n_q++] = i;
+		sphr_rules_prefetch(ord, n_t, ord + n_t, n_q);
+		free(ord);
+	}
+	for(int i = 0; i <= a_max; i++)
+		if(rule_t[i] >= 0) {
+			const int nt = 2 * (i + 1);
+			const double *x, *w;
+			sphr_mesh_rule((unsigned) i, &x, &w);
+			memcpy(hnodes + rule_t[i], x, nt * sizeof(double));
+			memcpy(hnodes + rule_t[i] + nt, w, nt * sizeof(double));
+		}
+	for(int i = 0; i <= nq_max; i++)
+		if(rule_q[i] >= 0) {
+			const double *x, *w;
+			sphr_gl_rule_cached(i, &x, &w);
+			memcpy(hnodes + rule_q[i], x, i * sizeof(double));
+			memcpy(hnodes + rule_q[i] + i, w, i * sizeof(double));
+		}
+	/* Legendre recurrence coefficients a_l, a_l / a_{l-1} */
+	hnodes[lc_off] = hnodes[lc_off + 1] = 0.0;
+	for(int l = 1; l < lc_n; l++) {
+		const double al = sqrt((4.0 * l * l - 1.0) / ((double) l * l));
+		const double am1 = l == 1 ? 1.0 : sqrt((4.0 * (l - 1) * (l - 1) - 1.0) / ((double) (l - 1) * (l - 1)));
+		hnodes[lc_off + 2 * l] = al;
+		hnodes[lc_off + 2 * l + 1] = al / am1;
+	}
+	/* chunks, their workspace layout and tile counts */
+	for(int b0 = 0; b0 < nbl; nch++) {
+		struct pg_chunk &C = ch[nch];
+		size_t wsz = 0;
 		int b1 = b0;
-		while(b1 < J->n_bl) {
+		C.np_max = C.lags_nq_max = 0;
+		while(b1 < nbl) {
 			struct jb_bl &B = J->bl[b1];
 			const int lq = B.a > B.Lt ? B.a : B.Lt;
-			const size_t need = (size_t) B.nt * (B.a + 1) + (size_t) B.nq * (lq + 1) + (size_t) B.np * (B.a + 1) + (size_t) B.np * B.nq + (size_t) B.lags * B.nq + 3 * (size_t) B.nt;
+			const int nth = B.nt / 2, nqh = (B.nq + 1) / 2;
+			const size_t need = (size_t) nth * (B.a + 1) + (size_t) nqh * (lq + 1) + (size_t) B.np * (B.a + 1) + (size_t) B.np * B.nq + (size_t) B.lags * nqh + 3 * (size_t) nth + nqh;
 			if(b1 > b0 && (wsz + need) * sizeof(double) > budget)
 				break;
 			B.off_Pt = wsz;
-			B.off_Pq = B.off_Pt + (long) B.nt * (B.a + 1);
-			B.off_C = B.off_Pq + (long) B.nq * (lq + 1);
+			B.off_Pq = B.off_Pt + (long) nth * (B.a + 1);
+			B.off_C = B.off_Pq + (long) nqh * (lq + 1);
 			B.off_T = B.off_C + (long) B.np * (B.a + 1);
 			B.off_K = B.off_T + (long) B.np * B.nq;
-			B.off_Y = B.off_K + (long) B.lags * B.nq;
+			B.off_Y = B.off_K + (long) B.lags * nqh;
 			wsz += need;
-			B.off_nt = nsz;
-			nsz += 2 * (size_t) B.nt;
-			B.off_nq = nsz;
-			nsz += 2 * (size_t) B.nq;
+			C.np_max = B.np > C.np_max ? B.np : C.np_max;
+			C.lags_nq_max = B.lags * B.nq > C.lags_nq_max ? B.lags * B.nq : C.lags_nq_max;	/* bounds lags * nqh */
 			b1++;
 		}
-		if(wsz > ws_cap) {
-			if(ws)
-				hipFree(ws);
-			ws = NULL;
-			if(sphr_rt_check(hipMalloc(&ws, wsz * sizeof(double)), "hipMalloc"))
-				goto done;
-			ws_cap = wsz;
-		}
-		if(nsz > nd_cap) {
-			if(nodes)
-				hipFree(nodes);
-			nodes = NULL;
-			free(hnodes);
-			hnodes = (double *) malloc(nsz * sizeof(double));
-			if(!hnodes || sphr_rt_check(hipMalloc(&nodes, nsz * sizeof(double)), "hipMalloc"))
-				goto done;
-			nd_cap = nsz;
-		}
-		/* Gauss-Legendre rules: the tap mesh of order a lists cos theta
-		 * descending (sh_series_trans.c:149-196); the product rule is
-		 * any exact rule */
+		C.b0 = b0;
+		C.b1 = b1;
+		wsz_max = wsz > wsz_max ? wsz : wsz_max;
+		/* lag tile: the widest column block whose two LDS column blocks
+		 * stay within 48 KiB (four workgroups per CU); past 160 KiB at
+		 * one column the untiled kernel runs */
+		C.lq = 16;
+		while(C.lq > 1 && pg_lag_lds(C.np_max, C.lq) > 48 * 1024)
+			C.lq >>= 1;
+		C.lag_tiled = pg_lag_lds(C.np_max, C.lq) <= 160 * 1024;
+		const int nb = b1 - b0;
+		for(int k = 0; k < 4; k++)
+			C.tot[k] = 0;
 		for(int b = b0; b < b1; b++) {
 			const struct jb_bl &B = J->bl[b];
-			const double *x, *w;
-			sphr_mesh_rule((unsigned) B.a, &x, &w);
-			memcpy(hnodes + B.off_nt, x, B.nt * sizeof(double));
-			memcpy(hnodes + B.off_nt + B.nt, w, B.nt * sizeof(double));
-			const double *gx, *gw;
-			sphr_gl_rule_cached(B.nq, &gx, &gw);
-			memcpy(hnodes + B.off_nq, gx, B.nq * sizeof(double));
-			memcpy(hnodes + B.off_nq + B.nq, gw, B.nq * sizeof(double));
+			const int t = PG_TILE;
+			const long cnt[4] = {
+				(long) ((B.np + t - 1) / t) * ((B.a + 1 + t - 1) / t),
+				(long) ((B.np + t - 1) / t) * (((B.nq + 1) / 2 + t - 1) / t),
+				(long) ((B.lags + t - 1) / t) * ((B.Lt + 1 + t - 1) / t),
+				(long) (((B.nq + 1) / 2 + C.lq - 1) / C.lq),
+			};
+			for(int k = 0; k < 4; k++) {
+				h_ts[4 * (size_t) b0 + k * nb + (b - b0)] = (int) C.tot[k];
+				C.tot[k] += cnt[k];
+			}
 		}
-		{
-			const int nb = b1 - b0;
-			int np_max = 0, lags_nq_max = 0;
-			for(int b = b0; b < b1; b++) {
-				const struct jb_bl &B = J->bl[b];
-				np_max = B.np > np_max ? B.np : np_max;
-				lags_nq_max = B.lags * B.nq > lags_nq_max ? B.lags * B.nq : lags_nq_max;
-			}
-			/* lag tile: the widest column block whose two LDS column
-			 * blocks stay within 48 KiB (four workgroups per CU); past
-			 * 160 KiB at one column the untiled kernel runs */
-			int lq = 16;
-			while(lq > 1 && pg_lag_lds(np_max, lq) > 48 * 1024)
-				lq >>= 1;
-			const bool lag_tiled = pg_lag_lds(np_max, lq) <= 160 * 1024;
-			if((size_t) 4 * nb > ts_cap) {
-				if(d_ts)
-					hipFree(d_ts);
-				d_ts = NULL;
-				free(h_ts);
-				h_ts = (int *) malloc((size_t) 4 * nb * sizeof(int));
-				if(!h_ts || sphr_rt_check(hipMalloc(&d_ts, (size_t) 4 * nb * sizeof(int)), "hipMalloc"))
-					goto done;
-				ts_cap = (size_t) 4 * nb;
-			}
-			long tot[4] = { 0, 0, 0, 0 };
-			for(int b = b0; b < b1; b++) {
-				const struct jb_bl &B = J->bl[b];
-				const int t = PG_TILE;
-				const long cnt[4] = {
-					(long) ((B.np + t - 1) / t) * ((B.a + 1 + t - 1) / t),
-					(long) ((B.np + t - 1) / t) * ((B.nq + t - 1) / t),
-					(long) ((B.lags + t - 1) / t) * ((B.Lt + 1 + t - 1) / t),
-					(long) ((B.nq + lq - 1) / lq),
-				};
-				for(int k = 0; k < 4; k++) {
-					h_ts[k * nb + (b - b0)] = (int) tot[k];
-					tot[k] += cnt[k];
-				}
-			}
-			if(tot[0] > INT_MAX || tot[1] > INT_MAX || tot[2] > INT_MAX || tot[3] > INT_MAX) {
+		for(int k = 0; k < 4; k++)
+			if(C.tot[k] > INT_MAX) {
 				sphr_set_error("sphr_jobs_new: plan chunk too large");
 				goto done;
 			}
-			if(sphr_rt_check(hipMemcpyAsync(nodes, hnodes, nsz * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpy")
-			   || sphr_rt_check(hipMemcpyAsync(J->d_bl + b0, J->bl + b0, (size_t) nb * sizeof(struct jb_bl), hipMemcpyHostToDevice, st), "hipMemcpy")
-			   || sphr_rt_check(hipMemcpyAsync(d_ts, h_ts, (size_t) 4 * nb * sizeof(int), hipMemcpyHostToDevice, st), "hipMemcpy"))
-				goto done;
-			hipLaunchKernelGGL(k_pg_tables, dim3(nb), dim3(256), 0, st, J->d_bl, b0, nodes, ws);
-			hipLaunchKernelGGL((k_pg_gemm<PG_TAPS, double>), dim3(tot[0]), dim3(256), 0, st, J->d_bl, b0, nb, d_ts, nodes, ws, (double *) NULL);
-			hipLaunchKernelGGL((k_pg_gemm<PG_SYNTH, double>), dim3(tot[1]), dim3(256), 0, st, J->d_bl, b0, nb, d_ts + nb, nodes, ws, (double *) NULL);
-			if(lag_tiled)
-				hipLaunchKernelGGL(k_pg_lag2, dim3(tot[3]), dim3(256), pg_lag_lds(np_max, lq), st, J->d_bl, b0, nb, d_ts + 3 * nb, lq, ws);
-			else
-				hipLaunchKernelGGL(k_pg_lag, dim3((lags_nq_max + 255) / 256, nb), dim3(256), 0, st, J->d_bl, b0, ws);
-			hipLaunchKernelGGL((k_pg_gemm<PG_M, T>), dim3(tot[2]), dim3(256), 0, st, J->d_bl, b0, nb, d_ts + 2 * nb, nodes, ws, (T *) J->M);
-			hipLaunchKernelGGL((k_pg_rot<T>), dim3(nb), dim3(128), 0, st, J->d_bl, b0, (cplx<T> *) J->R);
-			if(sphr_rt_check(hipGetLastError(), "plan kernels") || sphr_rt_check(hipStreamSynchronize(st), "plan"))
-				goto done;
-		}
 		b0 = b1;
 	}
+	if(sphr_rt_check(hipMalloc(&ws, wsz_max * sizeof(double)), "hipMalloc") || sphr_rt_check(hipMalloc(&nodes, nsz * sizeof(double)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc(&d_ts, (size_t) 4 * nbl * sizeof(int)), "hipMalloc")
+	   || sphr_rt_check(hipMemcpyAsync(nodes, hnodes, nsz * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpy")
+	   || sphr_rt_check(hipMemcpyAsync(J->d_bl, J->bl, (size_t) nbl * sizeof(struct jb_bl), hipMemcpyHostToDevice, st), "hipMemcpy")
+	   || sphr_rt_check(hipMemcpyAsync(d_ts, h_ts, (size_t) 4 * nbl * sizeof(int), hipMemcpyHostToDevice, st), "hipMemcpy"))
+		goto done;
+	for(int c = 0; c < nch; c++) {
+		const struct pg_chunk &C = ch[c];
+		const int b0 = C.b0, nb = C.b1 - C.b0;
+		const int *ts = d_ts + 4 * (size_t) b0;
+		hipLaunchKernelGGL(k_pg_tables, dim3(nb), dim3(256), 0, st, J->d_bl, b0, nodes, nodes + lc_off, ws);
+		hipLaunchKernelGGL((k_pg_gemm<PG_TAPS, double>), dim3(C.tot[0]), dim3(256), 0, st, J->d_bl, b0, nb, ts, ws, (double *) NULL);
+		hipLaunchKernelGGL((k_pg_gemm<PG_SYNTH, double>), dim3(C.tot[1]), dim3(256), 0, st, J->d_bl, b0, nb, ts + nb, ws, (double *) NULL);
+		if(C.lag_tiled)
+			hipLaunchKernelGGL(k_pg_lag2, dim3(C.tot[3]), dim3(256), pg_lag_lds(C.np_max, C.lq), st, J->d_bl, b0, nb, ts + 3 * nb, C.lq, ws);
+		else
+			hipLaunchKernelGGL(k_pg_lag, dim3((C.lags_nq_max + 255) / 256, nb), dim3(256), 0, st, J->d_bl, b0, ws);
+		hipLaunchKernelGGL((k_pg_gemm<PG_M, T>), dim3(C.tot[2]), dim3(256), 0, st, J->d_bl, b0, nb, ts + 2 * nb, ws, (T *) J->M);
+		hipLaunchKernelGGL((k_pg_rot<T>), dim3(nb), dim3(128), 0, st, J->d_bl, b0, (cplx<T> *) J->R);
+		if(sphr_rt_check(hipGetLastError(), "plan kernels"))
+			goto done;
+	}
+	if(sphr_rt_check(hipStreamSynchronize(st), "plan"))
+		goto done;
 	rc = 0;
 done:
+	if(rc)
+		hipStreamSynchronize(st);
 	if(ws)
 		hipFree(ws);
 	if(nodes)
@@ -675,6 +791,9 @@ done:
 		hipFree(d_ts);
 	free(hnodes);
 	free(h_ts);
+	free(rule_t);
+	free(rule_q);
+	free(ch);
 	return rc;
 }
 
@@ -787,11 +906,6 @@ extern "C" struct sphr_jobs *sphr_jobs_new(int n_jobs, const int *n_det, const d
 		return NULL;
 	}
 	if((precision == SPHR_MIXED ? generate_plans<float>(J, st) : generate_plans<double>(J, st))) {
-		sphr_jobs_free(J);
-		return NULL;
-	}
-	/* the device copy of the baseline table now holds the final layout */
-	if(sphr_rt_check(hipMemcpy(J->d_bl, J->bl, nbl * sizeof(struct jb_bl), hipMemcpyHostToDevice), "hipMemcpy")) {
 		sphr_jobs_free(J);
 		return NULL;
 	}

@@ -20,6 +20,7 @@ struct synth_dev {
 	double *E64;
 	float *E32;
 	struct sky_fused_dev fz;	/* fused sky kernel tables (orders it is compiled for) */
+	void *blob;	/* the one device allocation holding x ... E32 */
 };
 
 /* the l_seed low-pass mesh of the extremizer's seed search */

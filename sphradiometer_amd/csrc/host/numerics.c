@@ -66,23 +66,49 @@ static pthread_mutex_t mesh_lock = PTHREAD_MUTEX_INITIALIZER;
 
 static struct mesh_rule *gl_rules;	/* keyed by the node count in l_max */
 
+/* find key in list under the lock; NULL if absent */
+static struct mesh_rule *rule_find(struct mesh_rule *list, unsigned key)
+{
+	for(struct mesh_rule *r = list; r; r = r->next)
+		if(r->l_max == key)
+			return r;
+	return NULL;
+}
+
+/* insert a rule computed outside the lock, unless another thread has
+ * inserted the same key meanwhile (then the copy is dropped); returns the
+ * cached entry */
+static struct mesh_rule *rule_insert(struct mesh_rule **list, struct mesh_rule *mine)
+{
+	pthread_mutex_lock(&mesh_lock);
+	struct mesh_rule *r = rule_find(*list, mine->l_max);
+	if(!r) {
+		mine->next = *list;
+		*list = mine;
+		r = mine;
+	}
+	pthread_mutex_unlock(&mesh_lock);
+	if(r != mine) {
+		free(mine->x);
+		free(mine->w);
+		free(mine);
+	}
+	return r;
+}
+
 void sphr_gl_rule_cached(int n, const double **x, const double **w)
 {
 	pthread_mutex_lock(&mesh_lock);
-	struct mesh_rule *r;
-	for(r = gl_rules; r; r = r->next)
-		if(r->l_max == (unsigned) n)
-			break;
+	struct mesh_rule *r = rule_find(gl_rules, (unsigned) n);
+	pthread_mutex_unlock(&mesh_lock);
 	if(!r) {
 		r = malloc(sizeof(*r));
 		r->x = malloc(n * sizeof(*r->x));
 		r->w = malloc(n * sizeof(*r->w));
 		sphr_gl_rule(n, r->x, r->w);
 		r->l_max = (unsigned) n;
-		r->next = gl_rules;
-		gl_rules = r;
+		r = rule_insert(&gl_rules, r);
 	}
-	pthread_mutex_unlock(&mesh_lock);
 	*x = r->x;
 	*w = r->w;
 }
@@ -90,10 +116,8 @@ void sphr_gl_rule_cached(int n, const double **x, const double **w)
 void sphr_mesh_rule(unsigned l_max, const double **cos_theta, const double **weights)
 {
 	pthread_mutex_lock(&mesh_lock);
-	struct mesh_rule *r;
-	for(r = mesh_rules; r; r = r->next)
-		if(r->l_max == l_max)
-			break;
+	struct mesh_rule *r = rule_find(mesh_rules, l_max);
+	pthread_mutex_unlock(&mesh_lock);
 	if(!r) {
 		const int n = 2 * ((int) l_max + 1);
 		double *xa = malloc(n * sizeof(*xa)), *wa = malloc(n * sizeof(*wa));
@@ -108,13 +132,25 @@ void sphr_mesh_rule(unsigned l_max, const double **cos_theta, const double **wei
 		free(xa);
 		free(wa);
 		r->l_max = l_max;
-		r->next = mesh_rules;
-		mesh_rules = r;
+		r = rule_insert(&mesh_rules, r);
 	}
-	pthread_mutex_unlock(&mesh_lock);
 	*cos_theta = r->x;
 	if(weights)
 		*weights = r->w;
+}
+
+/* fill both rule caches for a set of tap-mesh orders and product-rule
+ * node counts, the missing rules computed in parallel */
+void sphr_rules_prefetch(const int *mesh_orders, int n_mesh, const int *gl_counts, int n_gl)
+{
+	#pragma omp parallel for schedule(dynamic, 1)
+	for(int i = 0; i < n_mesh + n_gl; i++) {
+		const double *x, *w;
+		if(i < n_mesh)
+			sphr_mesh_rule((unsigned) mesh_orders[i], &x, &w);
+		else
+			sphr_gl_rule_cached(gl_counts[i - n_mesh], &x, &w);
+	}
 }
 
 /*

@@ -19,6 +19,7 @@ void sphr_gl_rule(int n, double *x, double *w);
 /* the rule of the (2(l_max+1))^2 pixel mesh: cos(theta) descending (theta
  * ascending), cached per l_max; returned arrays are owned by the cache */
 void sphr_mesh_rule(unsigned l_max, const double **cos_theta, const double **weights);
+void sphr_rules_prefetch(const int *mesh_orders, int n_mesh, const int *gl_counts, int n_gl);
 /* the n-point Gauss-Legendre rule of sphr_gl_rule, computed once per n
  * and kept (thread-safe); the arrays belong to the cache */
 void sphr_gl_rule_cached(int n, const double **x, const double **w);
