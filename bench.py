@@ -317,11 +317,12 @@ def bench_copy(S, args):
 
 
 # HBM bytes per launch of the fused sky kernel (float32, l_max 32, 1024
-# segments, 4 K splits): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE medians
-# scaled by the counters' reading of a 256 MiB copy (FETCH_SIZE reads half
-# the copied bytes, WRITE_SIZE all of them); the summary with the commands
-# is profiles/three_site_l32_traffic.md
-SKY_FUSED_F32_TRAFFIC = 4759 * 1024 * 2 + 49837 * 1024
+# segments, the K split the autotuner picks for the timed steps):
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE medians over the timed
+# dispatches, scaled by the counters' reading of a 256 MiB copy
+# (FETCH_SIZE reads half the copied bytes, WRITE_SIZE all of them); the
+# summary with the commands is profiles/three_site_l32_traffic.md
+SKY_FUSED_F32_TRAFFIC = 3344 * 1024 * 2 + 48738 * 1024
 
 
 def main():

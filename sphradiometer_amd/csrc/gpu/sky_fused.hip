@@ -24,9 +24,11 @@
  * the table (L2-resident) in the loop, behind the longer f64 MFMA.
  *
  * Legendre stage.  F[r][m] = sum_l a_lm Pbar_l|m|(x_r) for ring pairs
- * (x_j, -x_j) at once through the parity (-1)^(l+|m|).  Item (j, a)
- * handles |m| = a and |m| = L - a, so every item runs L + 2 recurrence
- * steps (balanced lanes), each step serving m = +|m| and -|m|.  The
+ * (x_j, -x_j) at once through the parity (-1)^(l+|m|).  Item (ring
+ * group, a) handles |m| = a and |m| = L - a for 2 or 3 rings (3 where
+ * items of 2 would not fit one round of the 256 threads: 187 items instead
+ * of 289 for L = 32), so every item runs L + 2 recurrence steps (balanced
+ * lanes), each step serving m = +|m| and -|m|.  The
  * recurrence Pbar_l = alpha_l x Pbar_{l-1} - (alpha_l / alpha_{l-1})
  * Pbar_{l-2} uses a division-free coefficient table built once per
  * workgroup, and runs two steps (one even, one odd l - |m|) per
@@ -128,8 +130,11 @@ __global__ void k_sky_bfrag(const double *__restrict__ twp, T *__restrict__ out)
 	out[idx] = (T) sf_bval<L>(twp, odd ? G::MO0 : G::ME0, odd ? G::NO : G::NE, 4 * ks + lk, 16 * ct + li);
 }
 
-/* rings per Legendre item, recurrence steps per loaded batch (even) */
-#define SF_RG 2
+/* rings per Legendre item: 2, or 3 where items of 2 rings would not fit
+ * one round of the sub-group; recurrence steps per loaded batch (even) */
+template <int L> struct sf_rg {
+	enum { V = ((L + 2) / 2) * (L / 2 + 1) > 256 ? 3 : 2 };
+};
 #define SF_US 4
 
 /* baselines handled with all rotation-row loads issued first; networks
@@ -247,8 +252,7 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 	cplx<T> *sR = reinterpret_cast<cplx<T> *>(sf_dyn);
 
 	const int gtid = threadIdx.x, sub = gtid / SF_NT, tid = gtid % SF_NT;
-	const int lane = tid & 63, wave = tid >> 6;
-	const int li = lane & 15, lk = lane >> 4;
+	const int wave = tid >> 6;
 	T *Fe = Fe_[sub], *Fo = Fo_[sub];
 	cplx<T> *sa = sa_[sub], *rot = rot_[sub];
 	T *pw = reinterpret_cast<T *>(sf_dyn + (rlds ? (size_t) xg.nb * G::ncoef * sizeof(cplx<T>) : 0)) + (size_t) sub * xg.CT;
@@ -296,11 +300,13 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 	}
 	__syncthreads();
 
-	/* B fragments of this lane, [k-step][column tile] (float32: kept in
-	 * registers) */
+	/* B fragments of this lane, [k-step][column tile], from the table in
+	 * MFMA lane order: float32 keeps them in registers for the life of the
+	 * workgroup, float64 reads them in the loop */
 	const T *__restrict__ btab = sf_mf<T>::btab(tb);
 	T bf[sf_mf<T>::BREG ? G::KSE + G::KSO : 1][G::NCT];
 	if(sf_mf<T>::BREG) {
+		const int lane = tid & 63;
 #pragma unroll
 		for(int k = 0; k < (sf_mf<T>::BREG ? G::KSE + G::KSO : 1); k++)
 #pragma unroll
@@ -309,11 +315,17 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 	}
 
 	for(int base = blockIdx.x * NSUB; base < S; base += gridDim.x * NSUB) {
+		/* the thread index through an empty asm, so that the per-thread
+		 * index arithmetic and table loads of the phases below stay inside
+		 * the loop instead of being hoisted and held live across it */
+		int otid = tid;
+		asm volatile("" : "+v"(otid));
+		const int olane = otid & 63, oli = olane & 15, olk = olane >> 4;
 		const int s = base + sub;
 		const bool act = s < S;
 		/* ---- partial sums over the K splits, rotate_z phases ---- */
 		if(act) {
-			for(int c = tid; c < xg.CT; c += SF_NT) {
+			for(int c = otid; c < xg.CT; c += SF_NT) {
 				/* eight independent loads in flight */
 				const T *p = partial + (size_t) s * xg.CT + c;
 				const size_t st = (size_t) S * xg.CT;
@@ -330,11 +342,11 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 					v[sp & 7] += p[sp * st];
 				pw[c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
 			}
-			if(tid <= L) {
+			if(otid <= L) {
 				double sn = 0.0, cs = 1.0;
-				if(gmst && tid)
-					sincos(-(double) tid * gmst[s], &sn, &cs);
-				rot[tid] = mk((T) cs, (T) sn);
+				if(gmst && otid)
+					sincos(-(double) otid * gmst[s], &sn, &cs);
+				rot[otid] = mk((T) cs, (T) sn);
 			}
 		}
 		__syncthreads();
@@ -342,9 +354,9 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 		/* ---- a_lm: rotation rows over the baselines, rotate_z ---- */
 		if(act) {
 			if(rlds)
-				sf_alm<T, L>(sR, pw, slm, rot, gmst != NULL, xg, csb, lbb, tid, sa, alm + (size_t) s * G::ncoef);
+				sf_alm<T, L>(sR, pw, slm, rot, gmst != NULL, xg, csb, lbb, otid, sa, alm + (size_t) s * G::ncoef);
 			else
-				sf_alm<T, L>(R, pw, slm, rot, gmst != NULL, xg, csb, lbb, tid, sa, alm + (size_t) s * G::ncoef);
+				sf_alm<T, L>(R, pw, slm, rot, gmst != NULL, xg, csb, lbb, otid, sa, alm + (size_t) s * G::ncoef);
 		}
 		__syncthreads();
 		if(!map)
@@ -354,8 +366,9 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 		 * runs the recurrences of SF_RG rings at once, so each
 		 * coefficient and a_lm read from LDS serves SF_RG rings ---- */
 		if(act) {
+			constexpr int SF_RG = sf_rg<L>::V;
 			constexpr int NG = (G::B + SF_RG - 1) / SF_RG;
-			for(int it = tid; it < NG * G::NA; it += SF_NT) {
+			for(int it = otid; it < NG * G::NA; it += SF_NT) {
 				const int tg = it % NG, a = it / NG;
 				T x[SF_RG];
 #pragma unroll
@@ -475,13 +488,13 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 					ae[ct] = (v4) 0;
 					ao[ct] = (v4) 0;
 				}
-				const T *fe = Fe + (16 * rt + li) * G::FS + lk, *fo = Fo + (16 * rt + li) * G::FS + lk;
+				const T *fe = Fe + (16 * rt + oli) * G::FS + olk, *fo = Fo + (16 * rt + oli) * G::FS + olk;
 #pragma unroll
 				for(int ks = 0; ks < G::KSE; ks++) {
 					const T av = fe[4 * ks];
 #pragma unroll
 					for(int ct = 0; ct < G::NCT; ct++) {
-						const T bv = sf_mf<T>::BREG ? bf[sf_mf<T>::BREG ? ks : 0][ct] : btab[(ks * G::NCT + ct) * 64 + lane];
+						const T bv = sf_mf<T>::BREG ? bf[sf_mf<T>::BREG ? ks : 0][ct] : btab[(ks * G::NCT + ct) * 64 + olane];
 						ae[ct] = sf_mf<T>::mma(av, bv, ae[ct]);
 					}
 				}
@@ -490,7 +503,7 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 					const T av = fo[4 * ks];
 #pragma unroll
 					for(int ct = 0; ct < G::NCT; ct++) {
-						const T bv = sf_mf<T>::BREG ? bf[sf_mf<T>::BREG ? G::KSE + ks : 0][ct] : btab[((G::KSE + ks) * G::NCT + ct) * 64 + lane];
+						const T bv = sf_mf<T>::BREG ? bf[sf_mf<T>::BREG ? G::KSE + ks : 0][ct] : btab[((G::KSE + ks) * G::NCT + ct) * 64 + olane];
 						ao[ct] = sf_mf<T>::mma(av, bv, ao[ct]);
 					}
 				}
@@ -498,7 +511,7 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 				for(int ct = 0; ct < G::NCT; ct++)
 #pragma unroll
 					for(int e = 0; e < 4; e++) {
-						const int r = 16 * rt + sf_mf<T>::row(lk, e), c = 16 * ct + li;
+						const int r = 16 * rt + sf_mf<T>::row(olk, e), c = 16 * ct + oli;
 						mr[(size_t) r * 2 * G::n + c] = ae[ct][e] + ao[ct][e];
 						mr[(size_t) r * 2 * G::n + G::n + c] = ae[ct][e] - ao[ct][e];
 					}
@@ -509,7 +522,7 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 			 * units in reverse thread order so a partial last round
 			 * lands on the last wave ---- */
 			const int ntask = G::XR * G::B + 16 * G::NRT * G::XP;
-			for(int u = SF_NT - 1 - tid; u < 2 * ntask; u += SF_NT) {
+			for(int u = SF_NT - 1 - otid; u < 2 * ntask; u += SF_NT) {
 				const int task = u >> 1, par = u & 1;
 				int r, k;
 				if(task < G::XR * G::B) {
