@@ -145,6 +145,18 @@ struct sh_series *sh_series_scale(struct sh_series *, double complex);
 struct sh_series *sh_series_neg(struct sh_series *);
 struct sh_series *sh_series_conj(struct sh_series *);
 struct sh_series *sh_series_real(struct sh_series *);
+
+/*
+ * HEALPix FITS output (sh_series_healpix.cc:131-286, declared at
+ * sh_series.h:380-382 of the reference): the a_lm table of the real part
+ * of the series, and the RING-ordered map of Re f at nside =
+ * ceilpow2(ceil(l_max / 2 + 0.75)), computed on the GPU.  The files are
+ * written directly in the healpix binary-table layout (no healpix_cxx).
+ * Return 0 / the series on success, -1 / NULL on failure.
+ */
+int sh_series_write_healpix_map(const struct sh_series *, const char *);
+int sh_series_write_healpix_alm(const struct sh_series *, const char *);
+struct sh_series *sh_series_read_healpix_alm(const char *);
 struct sh_series *sh_series_imag(struct sh_series *);
 struct sh_series *sh_series_clip(struct sh_series *, double);
 struct sh_series *sh_series_scale_power_l(struct sh_series *, double, double);

@@ -103,6 +103,29 @@ int sphr_loc_peaks(struct sphr_loc *, const void *alm, const void *map, int n_ev
 void sphr_loc_info(const struct sphr_loc *, int info[8]);
 
 /*
+ * HEALPix maps of many series on the GPU (the pixel-domain output of
+ * sh_series_healpix.cc:247-286 and healpix.py:141-171, and the posterior
+ * of RapidLocalization.py:498-530).  A plan holds the Legendre tables of
+ * order l_max for nside (any nside up to 1024; sphr_healpix_nside gives the
+ * reference's lmax2nside rule).  Layouts (device, row-major):
+ *   alm   [n][(l_max+1)^2] complex (float64 pairs for SPHR_FP64, float32
+ *         pairs for SPHR_MIXED), the reference coefficient order
+ *   map   [n][12 nside^2] real: Re f at the RING pixel centres
+ *   maps  [2][n_events][12 nside^2], the beta = +1 and -1 skies (as
+ *         sphr_loc_run's alm through sphr_healpix_maps with n = 2 n_events)
+ *   prob  [n_events][12 nside^2]: (e_+ / sum e_+ + e_- / sum e_-) / 2,
+ *         e = exp(map - max map)
+ * Calls are asynchronous on stream (NULL = the library stream).
+ */
+struct sphr_healpix;
+int sphr_healpix_nside(unsigned l_max);
+struct sphr_healpix *sphr_healpix_new(unsigned l_max, int nside, int precision);
+void sphr_healpix_free(struct sphr_healpix *);
+long sphr_healpix_npix(const struct sphr_healpix *);
+int sphr_healpix_maps(const struct sphr_healpix *, const void *alm, int n, void *map, void *stream);
+int sphr_healpix_posterior(const struct sphr_healpix *, const void *maps, int n_events, void *prob, void *stream);
+
+/*
  * Learned peak proposal (no counterpart in the reference): a multilayer
  * perceptron from the (L+1)^2 real-form SH coefficients of a sky
  * (sphr_features) to K candidate directions, trained and evaluated with

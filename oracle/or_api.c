@@ -273,3 +273,31 @@ int orx_array_set_l(int n, unsigned l_old, int polar, const double complex *in, 
 	or_array_free(a);
 	return 0;
 }
+
+/* HEALPix output (or_healpix.c) */
+int orx_healpix_nside_c(unsigned l)
+{
+	return or_healpix_nside_c(l);
+}
+
+int orx_healpix_nside_py(unsigned l)
+{
+	return or_healpix_nside_py(l);
+}
+
+void orx_healpix_pix2ang(int nside, long npix, double *theta, double *phi)
+{
+	for(long p = 0; p < npix; p++)
+		or_healpix_pix2ang(nside, p, theta + p, phi + p);
+}
+
+void orx_healpix_map(unsigned l, int polar, const double complex *coeff, int nside, double *map)
+{
+	struct or_series s = { l, polar, (double complex *) coeff };
+	or_healpix_map(&s, nside, map);
+}
+
+void orx_healpix_posterior(const double *map_p, const double *map_n, long npix, double *prob)
+{
+	or_healpix_posterior(map_p, map_n, npix, prob);
+}

@@ -132,6 +132,12 @@ void or_dft(double complex *data, int n, int sign);    /* in place, sign=-1 forw
 void or_mesh_nodes(unsigned l_max, int *ntheta, int *nphi, double *cos_theta, double *weights);
 double complex or_Y(unsigned l, int m, double theta, double phi);
 double complex or_eval(const struct or_series *, double theta, double phi);
+/* HEALPix output (or_healpix.c) */
+int or_healpix_nside_c(unsigned l_max);
+int or_healpix_nside_py(unsigned l_max);
+void or_healpix_pix2ang(int nside, long pix, double *theta, double *phi);
+void or_healpix_map(const struct or_series *, int nside, double *map);
+void or_healpix_posterior(const double *map_p, const double *map_n, long npix, double *prob);
 struct or_series *or_from_mesh(struct or_series *, const double complex *mesh);
 struct or_series *or_from_realmesh(struct or_series *, const double *mesh);
 double complex *or_to_mesh(const struct or_series *, int *ntheta, int *nphi);

@@ -82,6 +82,11 @@ def lib():
         L.orx_precalc_length.argtypes = [ctypes.c_long, c_double]
         L.orx_preprocess_snr.argtypes = [c_void_p, c_int, c_int, c_double, c_void_p]
         L.orx_localize.argtypes = [c_void_p, c_void_p, c_int, c_double, c_double, c_void_p, c_uint, c_double, c_double, c_void_p]
+        L.orx_healpix_nside_c.argtypes = [c_uint]
+        L.orx_healpix_nside_py.argtypes = [c_uint]
+        L.orx_healpix_pix2ang.argtypes = [c_int, ctypes.c_long, c_void_p, c_void_p]
+        L.orx_healpix_map.argtypes = [c_uint, c_int, c_void_p, c_int, c_void_p]
+        L.orx_healpix_posterior.argtypes = [c_void_p, c_void_p, ctypes.c_long, c_void_p]
         _LIB = L
     return _LIB
 
@@ -454,4 +459,36 @@ def preprocess_snr(series, delta_t):
     n, ln = x.shape
     out = np.zeros((n, precalc_length(ln, delta_t)), dtype=np.complex128)
     lib().orx_preprocess_snr(_ptr(x), n, ln, delta_t, _ptr(out))
+    return out
+
+
+# HEALPix output (or_healpix.c; sh_series_healpix.cc, healpix.py,
+# RapidLocalization.py:498-530)
+def healpix_nside_c(l_max):
+    return lib().orx_healpix_nside_c(l_max)
+
+
+def healpix_nside_py(l_max):
+    return lib().orx_healpix_nside_py(l_max)
+
+
+def healpix_pix2ang(nside):
+    n = 12 * nside * nside
+    th, ph = np.zeros(n), np.zeros(n)
+    lib().orx_healpix_pix2ang(nside, n, _ptr(th), _ptr(ph))
+    return th, ph
+
+
+def healpix_map(coeff, l, nside, polar=0):
+    c = np.ascontiguousarray(coeff, dtype=np.complex128)
+    out = np.zeros(12 * nside * nside)
+    lib().orx_healpix_map(l, polar, _ptr(c), nside, _ptr(out))
+    return out
+
+
+def healpix_posterior(map_p, map_n):
+    a = np.ascontiguousarray(map_p, dtype=np.float64)
+    b = np.ascontiguousarray(map_n, dtype=np.float64)
+    out = np.zeros(a.size)
+    lib().orx_healpix_posterior(_ptr(a), _ptr(b), a.size, _ptr(out))
     return out

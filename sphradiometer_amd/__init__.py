@@ -221,6 +221,15 @@ _PROTOS = {
     "sphr_loc_free": (None, [c_void_p]),
     "sphr_loc_run": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sphr_loc_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sh_series_write_healpix_map": (c_int, [P(ShSeries), ctypes.c_char_p]),
+    "sh_series_write_healpix_alm": (c_int, [P(ShSeries), ctypes.c_char_p]),
+    "sh_series_read_healpix_alm": (P(ShSeries), [ctypes.c_char_p]),
+    "sphr_healpix_nside": (c_int, [c_uint]),
+    "sphr_healpix_new": (c_void_p, [c_uint, c_int, c_int]),
+    "sphr_healpix_free": (None, [c_void_p]),
+    "sphr_healpix_npix": (ctypes.c_long, [c_void_p]),
+    "sphr_healpix_maps": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "sphr_healpix_posterior": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "sphr_loc_info": (None, [c_void_p, P(c_int)]),
 }
 
@@ -616,6 +625,39 @@ class Localizer:
         try:
             if getattr(self, "h", None):
                 lib().sphr_loc_free(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+
+class Healpix:
+    """sphradiometer_batch.h sphr_healpix_*: RING-ordered HEALPix maps of
+    many series on the GPU (Re f at the pixel centres), and the posterior
+    of the beta = +1 / -1 skies of localization events."""
+
+    def __init__(self, l_max, nside=None, precision=FP64):
+        self.l_max = int(l_max)
+        self.nside = int(nside) if nside is not None else lib().sphr_healpix_nside(self.l_max)
+        self.precision = precision
+        self.h = lib().sphr_healpix_new(self.l_max, self.nside, precision)
+        if not self.h:
+            raise RuntimeError("sphr_healpix_new failed: " + last_error())
+        self.npix = lib().sphr_healpix_npix(self.h)
+
+    def maps(self, d_alm, n, d_map, stream=None):
+        """d_alm [n][(l_max+1)^2] complex -> d_map [n][npix] real (device)."""
+        if lib().sphr_healpix_maps(self.h, d_alm, n, d_map, stream):
+            raise RuntimeError("sphr_healpix_maps failed: " + last_error())
+
+    def posterior(self, d_maps, n_events, d_prob, stream=None):
+        """d_maps [2][n_events][npix] -> d_prob [n_events][npix] (device)."""
+        if lib().sphr_healpix_posterior(self.h, d_maps, n_events, d_prob, stream):
+            raise RuntimeError("sphr_healpix_posterior failed: " + last_error())
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().sphr_healpix_free(self.h)
         except Exception:
             pass
         self.h = None

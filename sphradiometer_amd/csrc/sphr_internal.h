@@ -26,6 +26,8 @@ void sphr_gl_rule_cached(int n, const double **x, const double **w);
 /* orthonormal associated Legendre functions (Condon-Shortley phase),
  * out[l - m] for l = m .. l_max */
 void sphr_plm(unsigned l_max, int m, double x, double *out);
+/* HEALPix map FITS file (healpix_fits.c) */
+int sphr_fits_write_healpix_map(const char *filename, const double *map, int nside);
 /* all 3j (j1 j2 j; m1 m2 -m1-m2) for j = jmin .. j1+j2 into out[j - jmin];
  * returns jmin, or -1 when every symbol vanishes */
 int sphr_3j_range(int j1, int j2, int m1, int m2, double *out);
