@@ -145,6 +145,79 @@ FP64_MFMA_TFLOPS = 78.6
 FP32_MFMA_TFLOPS = 157.3
 
 
+def bench_healpix(S, args):
+    """The HEALPix output of the localization workload (RapidLocalization.py:
+    498-530): per event the beta = +1 and -1 skies at l_max 50 as RING
+    maps at nside 32 (healpy_alm_to_map's rule, healpix.py:163) and their
+    posterior.  One step = the maps of 2 x 8192 skies + the posterior of
+    8192 events; value = events/s (float32 maps); HBM roofline of the map
+    kernel (a_lm read + maps written)."""
+    L, E = 50, args.events
+    nside = 1 << max(0, math.ceil(math.log2(L / 2.0)))
+    ncoef = (L + 1) ** 2
+    rng = np.random.default_rng(0)
+    lsc = np.array([1.0 / (1.0 + l) for l, m in _lm_list(L)])
+    alm = (rng.standard_normal((2, E, ncoef)) + 1j * rng.standard_normal((2, E, ncoef))) * lsc * 8.0
+    res = {}
+    for name, prec, cdt, rdt in (("fp64", S.FP64, np.complex128, np.float64), ("mixed", S.MIXED, np.complex64, np.float32)):
+        hp = S.Healpix(L, nside, prec)
+        d_alm = S.DeviceBuffer.from_numpy(alm.astype(cdt))
+        d_map = S.DeviceBuffer(2 * E * hp.npix * np.dtype(rdt).itemsize)
+        d_prob = S.DeviceBuffer(E * hp.npix * np.dtype(rdt).itemsize)
+        st = S.lib().sphr_stream()
+        for _ in range(args.warmup):
+            hp.maps(d_alm.ptr, 2 * E, d_map.ptr, stream=st)
+            hp.posterior(d_map.ptr, E, d_prob.ptr, stream=st)
+        S.lib().sphr_stream_synchronize(st)
+        ev = [S.lib().sphr_event_new() for _ in range(3)]
+        S.lib().sphr_event_record(ev[0], st)
+        for _ in range(args.steps):
+            hp.maps(d_alm.ptr, 2 * E, d_map.ptr, stream=st)
+            hp.posterior(d_map.ptr, E, d_prob.ptr, stream=st)
+        S.lib().sphr_event_record(ev[1], st)
+        for _ in range(args.steps):
+            hp.maps(d_alm.ptr, 2 * E, d_map.ptr, stream=st)
+        S.lib().sphr_event_record(ev[2], st)
+        S.lib().sphr_event_synchronize(ev[2])
+        ms = S.lib().sphr_event_elapsed_ms(ev[0], ev[1]) / args.steps
+        ms_maps = S.lib().sphr_event_elapsed_ms(ev[1], ev[2]) / args.steps
+        res[name] = dict(ms=ms, ms_maps=ms_maps, prob=d_prob.to_numpy(rdt, (E, hp.npix)), npix=hp.npix,
+                         esz=np.dtype(cdt).itemsize, rsz=np.dtype(rdt).itemsize)
+        print("%s: %.3f ms per step (maps %.3f ms), %d events, nside %d" % (name, ms, ms_maps, E, nside), file=sys.stderr, flush=True)
+    perr = np.linalg.norm(res["mixed"]["prob"] - res["fp64"]["prob"], axis=1) / np.linalg.norm(res["fp64"]["prob"], axis=1)
+    mx = res["mixed"]
+    map_bytes = 2 * E * ncoef * mx["esz"] + 2 * E * mx["npix"] * mx["rsz"]
+    ach = map_bytes / (mx["ms_maps"] * 1e-3) / 1e9
+    out = {
+        "metric": "HEALPix posterior events/s, l_max=50 beta = +1 / -1 skies -> nside 32 RING maps + posterior",
+        "value": round(E / (mx["ms"] * 1e-3), 1), "unit": "events/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(mx["ms"], 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: complex Gaussian a_lm scaled by 8 / (l + 1) (seed 0)",
+        "config": {"workload": "%d events x 2 skies, l_max %d, nside %d (%d pixels)" % (E, L, nside, mx["npix"]),
+                   "events": E, "l_max": L, "nside": nside},
+        "fp64": {"value": round(E / (res["fp64"]["ms"] * 1e-3), 1), "ms_per_step": round(res["fp64"]["ms"], 4),
+                 "maps_ms": round(res["fp64"]["ms_maps"], 4)},
+        "mixed_vs_fp64": {"prob_rel_err_max": float(perr.max())},
+        "roofline": {"bound": "hbm", "kernel": "k_healpix", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": map_bytes,
+                     "kernel_ms": round(mx["ms_maps"], 4)},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        # the FP64 oracle on a bounded sample: the two skies of a few events
+        t0, n = time.time(), 0
+        while n < 64 and (n < 2 or time.time() - t0 < args.cpu_budget):
+            a = alm[0, n].astype(np.complex128)
+            mp = O.healpix_map(a, L, nside)
+            mn = O.healpix_map(alm[1, n].astype(np.complex128), L, nside)
+            O.healpix_posterior(mp, mn)
+            n += 1
+        dt = time.time() - t0
+        out["cpu_baseline"] = {"value": round(n / dt, 3), "unit": "events/s", "cores": 1, "kind": "port",
+                               "sample": "%d events (2 maps + posterior each), FP64 C oracle, 1 thread" % n}
+    print(json.dumps(out), flush=True)
+
+
 def bench_localization(S, args):
     """The localization workload: three sites, complex SNR series of 410
     samples at 4096 Hz (+-50 ms, padded to 584), beta = +1 / -1 plans
@@ -334,9 +407,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-budget", type=float, default=3.0)
-    ap.add_argument("--workload", choices=["sky", "localization", "jobs", "copy"], default="sky",
+    ap.add_argument("--workload", choices=["sky", "localization", "jobs", "healpix", "copy"], default="sky",
                     help="sky: the three-site cross-power maps (default); localization: the log-likelihood skies; "
-                         "jobs: the uneven batch with a plan per job")
+                         "jobs: the uneven batch with a plan per job; healpix: RING maps + posterior of localization skies")
     ap.add_argument("--events", type=int, default=8192)
     ap.add_argument("--jobs", type=int, default=4096)
     args = ap.parse_args()
@@ -356,6 +429,9 @@ def main():
         return
     if args.workload == "jobs":
         bench_jobs(S, args)
+        return
+    if args.workload == "healpix":
+        bench_healpix(S, args)
         return
     if args.workload == "copy":
         bench_copy(S, args)

@@ -16,16 +16,22 @@
  *   south cap: the mirror of the north cap,
  * pixels numbered ring by ring from the north pole, phi ascending.
  *
- * Kernel: workgroup = one ring x HP_SPW series.  Per series and |m| one
- * thread runs the Legendre recurrence at the ring's z (float64, from
- * Pbar_mm = cm[m] sin^m theta, the division-free coefficient pairs of the
- * plan), giving F_m and F_-m; with G_m = F_m exp(i m phi_0) the ring's
- * values are
- *   f(phi_0 + 2 pi j / nphi) = Re sum_m G_m exp(2 pi i m j / nphi)
- *     = G_0.re + sum_{m>0} A_m cos(2 pi m j / nphi) - B_m sin(2 pi m j / nphi),
+ * Kernel: workgroup = one series, its coefficients staged in LDS once;
+ * the northern rings (through the equator) in chunks of HP_RC, each with
+ * its mirror ring 4 nside - i (same nphi and phi_0, z -> -z).  Per chunk,
+ * task (|m|, ring) - rings fastest, so that the lanes of a wave share m
+ * and read the same coefficients - runs the Legendre recurrence at the
+ * ring's z once for the ring and its mirror (even and odd l - m sums,
+ * Pbar_lm(-z) = (-1)^(l+m) Pbar_lm(z)) (float64,
+ * from Pbar_mm = cm[m] sin^m theta and the division-free coefficient
+ * pairs of the plan), giving F_m and F_-m; with G_m = F_m exp(i m phi_0)
+ * the ring's values are
+ *   f(phi_0 + 2 pi j / nphi) = Re sum_m G_m exp(i m phi'),  phi' = 2 pi j / nphi,
+ *     = G_0.re + sum_{m>0} A_m cos(m phi') - B_m sin(m phi'),
  *   A_m = Re (G_m + G_-m), B_m = Im (G_m - G_-m),
- * one thread per (series, pixel), the twiddles of the ring in LDS at
- * exact integer indices (m j mod nphi).
+ * kept in LDS per (ring, m); then one thread per four pixels of a ring
+ * (a quarter turn apart) sums over m with exp(i m phi') advanced by one
+ * complex rotation per m.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -39,8 +45,7 @@
 #include "runtime.h"
 
 #define HP_NT 256
-#define HP_SPW 4		/* series per workgroup */
-#define HP_NSIDE_MAX 1024	/* ring twiddles (4 nside) in LDS */
+#define HP_NSIDE_MAX 4096	/* the per-ring form keeps 4 nside twiddles in LDS */
 
 struct sphr_healpix {
 	int L, nside, precision;
@@ -77,8 +82,190 @@ __device__ inline void hp_ring(int nside, int i, double *z, double *st, int *nph
 	}
 }
 
+#define HP_RC 16		/* rings per chunk */
+#define HP_SPW 4		/* series per workgroup of the per-ring form */
+
 template <typename T>
 __global__ void __launch_bounds__(HP_NT) k_healpix(const cplx<T> *__restrict__ alm, int n, int L, int nside, const double *__restrict__ cf, const double *__restrict__ cm, T *__restrict__ map)
+{
+	extern __shared__ __align__(16) unsigned char hp_sm[];
+	const int s = blockIdx.x, L1 = L + 1, N2 = 2 * nside;
+	const int ncoef = L1 * L1;
+	const long npix = 12L * nside * nside;
+	cplx<T> *a = reinterpret_cast<cplx<T> *>(hp_sm);		/* [ncoef] */
+	T *AB = reinterpret_cast<T *>(a + ncoef);			/* [2 HP_RC][L + 1][2]: north rows, then their mirrors */
+	/* the chunk's rings: first pixel, pixel count, AB row, prefix sums of
+	 * the quarter rings */
+	__shared__ long es[2 * HP_RC];
+	__shared__ int en[2 * HP_RC], ea[2 * HP_RC], eq[2 * HP_RC + 1], ne_s;
+	for(int t = threadIdx.x; t < ncoef; t += HP_NT)
+		a[t] = alm[(size_t) s * ncoef + t];
+	__syncthreads();
+	T *out = map + (size_t) s * npix;
+	for(int r0 = 1; r0 <= N2; r0 += HP_RC) {
+		const int nr = N2 - r0 + 1 < HP_RC ? N2 - r0 + 1 : HP_RC;
+		if(threadIdx.x == 0) {
+			int ne = 0;
+			eq[0] = 0;
+			for(int rr = 0; rr < nr; rr++)
+				for(int h = 0; h < 2; h++) {
+					/* ring r0 + rr and its mirror 4 nside - r0 - rr (the
+					 * equator ring is its own mirror) */
+					const int ring = h ? 4 * nside - r0 - rr : r0 + rr;
+					if(h && ring == r0 + rr)
+						continue;
+					double z, st, phi0;
+					int nphi;
+					long start;
+					hp_ring(nside, ring, &z, &st, &nphi, &start, &phi0);
+					es[ne] = start;
+					en[ne] = nphi;
+					ea[ne] = h ? HP_RC + rr : rr;
+					eq[ne + 1] = eq[ne] + nphi / 4;
+					ne++;
+				}
+			ne_s = ne;
+		}
+		/* Legendre sums: task (|m|, northern ring of the chunk), split by
+		 * the parity of l - m: Pbar_lm(-z) = (-1)^(l+m) Pbar_lm(z), so the
+		 * ring gets E + O and its mirror E - O */
+		for(int t = threadIdx.x; t < HP_RC * L1; t += HP_NT) {
+			const int m = t / HP_RC, rr = t - m * HP_RC;
+			double arn = 0., bin = 0., ars = 0., bis = 0.;
+			if(rr < nr) {
+				double z, st, phi0;
+				int nphi;
+				long start;
+				hp_ring(nside, r0 + rr, &z, &st, &nphi, &start, &phi0);
+				/* Pbar_mm = cm[m] sin^m theta by repeated products */
+				double p2 = cm[m];
+				for(int k = 0; k < m; k++)
+					p2 *= st;
+				const cplx<T> *ap = a + lm_offset(L, m, m);
+				const cplx<T> *an = m ? a + lm_offset(L, m, -m) : ap;
+				/* even (e) and odd (o) l - m sums of +m (p) and -m (n) */
+				double epr = (double) ap[0].re * p2, epi = (double) ap[0].im * p2;
+				double enr = (double) an[0].re * p2, eni = (double) an[0].im * p2;
+				double opr = 0., opi = 0., onr = 0., oni = 0.;
+				const double *c = cf + 2 * lm_offset(L, m, m);
+				double p1 = 0.;
+				if(m < L) {
+					p1 = c[2] * z * p2;
+					opr = (double) ap[1].re * p1;
+					opi = (double) ap[1].im * p1;
+					onr = (double) an[1].re * p1;
+					oni = (double) an[1].im * p1;
+				}
+				int d = 2;
+				for(; d + 1 <= L - m; d += 2) {
+					const double pe = c[2 * d] * z * p1 - c[2 * d + 1] * p2;
+					const double po = c[2 * d + 2] * z * pe - c[2 * d + 3] * p1;
+					epr = fma((double) ap[d].re, pe, epr);
+					epi = fma((double) ap[d].im, pe, epi);
+					enr = fma((double) an[d].re, pe, enr);
+					eni = fma((double) an[d].im, pe, eni);
+					opr = fma((double) ap[d + 1].re, po, opr);
+					opi = fma((double) ap[d + 1].im, po, opi);
+					onr = fma((double) an[d + 1].re, po, onr);
+					oni = fma((double) an[d + 1].im, po, oni);
+					p2 = pe;
+					p1 = po;
+				}
+				if(d <= L - m) {
+					const double pe = c[2 * d] * z * p1 - c[2 * d + 1] * p2;
+					epr = fma((double) ap[d].re, pe, epr);
+					epi = fma((double) ap[d].im, pe, epi);
+					enr = fma((double) an[d].re, pe, enr);
+					eni = fma((double) an[d].im, pe, eni);
+				}
+				double sn, cs;
+				sincos(m * phi0, &sn, &cs);
+				/* the ring (h = 0) and its mirror (h = 1): F = E +- O;
+				 * Pbar_l,-m = (-1)^m Pbar_lm; G_+-m = F_+-m exp(+-i m phi_0) */
+				for(int h = 0; h < 2; h++) {
+					const double sg = h ? -1. : 1.;
+					const double fpr = epr + sg * opr, fpi = epi + sg * opi;
+					double fnr = enr + sg * onr, fni = eni + sg * oni;
+					if(m & 1) {
+						fnr = -fnr;
+						fni = -fni;
+					}
+					const double gpr = fpr * cs - fpi * sn, gpi = fpr * sn + fpi * cs;
+					const double gnr = fnr * cs + fni * sn, gni = -fnr * sn + fni * cs;
+					const double ar = m ? gpr + gnr : gpr, bi = m ? gpi - gni : 0.;
+					if(h) {
+						ars = ar;
+						bis = bi;
+					} else {
+						arn = ar;
+						bin = bi;
+					}
+				}
+			}
+			AB[2 * (rr * L1 + m)] = (T) arn;
+			AB[2 * (rr * L1 + m) + 1] = (T) bin;
+			AB[2 * ((HP_RC + rr) * L1 + m)] = (T) ars;
+			AB[2 * ((HP_RC + rr) * L1 + m) + 1] = (T) bis;
+		}
+		__syncthreads();
+		/* pixels of the chunk's rings, four at once: nphi is a multiple of
+		 * 4, and the pixels j + q nphi / 4 (q = 0..3) sit at phi' + q pi / 2,
+		 * where exp(i m (phi' + q pi / 2)) = i^(m q) exp(i m phi').  With
+		 * C_m = A_m cos(m phi') - B_m sin(m phi'),
+		 * D_m = A_m sin(m phi') + B_m cos(m phi') and S_k, T_k the sums
+		 * of C_m, D_m over m = k mod 4:
+		 *   q = 0: S0 + S1 + S2 + S3,   q = 1: S0 - T1 - S2 + T3,
+		 *   q = 2: S0 - S1 + S2 - S3,   q = 3: S0 + T1 - S2 - T3 */
+		const int ne = ne_s;
+		for(int u = threadIdx.x; u < eq[ne]; u += HP_NT) {
+			int k = 0;
+			while(k + 1 < ne && u >= eq[k + 1])
+				k++;
+			const int j = u - eq[k], n4 = en[k] / 4;
+			double sn, cs;
+			sincospi(2. * j / en[k], &sn, &cs);
+			const T *ab = AB + 2 * ea[k] * L1;
+			const T er = (T) cs, ei = (T) sn;
+			T cr = er, ci = ei;	/* exp(i m phi') for m = 1 */
+			T s0 = ab[0], s1 = 0, s2 = 0, s3 = 0, t1 = 0, t3 = 0;
+			for(int m = 1; m <= L; m++) {
+				const T A = ab[2 * m], B = ab[2 * m + 1];
+				const T c = fma(A, cr, -B * ci);
+				switch(m & 3) {
+				case 0:
+					s0 += c;
+					break;
+				case 1:
+					s1 += c;
+					t1 = fma(A, ci, fma(B, cr, t1));
+					break;
+				case 2:
+					s2 += c;
+					break;
+				default:
+					s3 += c;
+					t3 = fma(A, ci, fma(B, cr, t3));
+					break;
+				}
+				const T nr_ = cr * er - ci * ei;
+				ci = fma(cr, ei, ci * er);
+				cr = nr_;
+			}
+			T *o = out + es[k] + j;
+			o[0] = (s0 + s2) + (s1 + s3);
+			o[n4] = (s0 - s2) - (t1 - t3);
+			o[2 * n4] = (s0 + s2) - (s1 + s3);
+			o[3 * n4] = (s0 - s2) + (t1 - t3);
+		}
+		__syncthreads();
+	}
+}
+
+/* orders whose coefficients do not fit in LDS: workgroup = one ring x
+ * HP_SPW series, the coefficients read from global memory, the ring's
+ * twiddles in LDS at exact integer indices (m j mod nphi) */
+template <typename T>
+__global__ void __launch_bounds__(HP_NT) k_healpix_ring(const cplx<T> *__restrict__ alm, int n, int L, int nside, const double *__restrict__ cf, const double *__restrict__ cm, T *__restrict__ map)
 {
 	extern __shared__ __align__(16) unsigned char hp_sm[];
 	const int ring = blockIdx.x + 1, s0 = blockIdx.y * HP_SPW;
@@ -309,18 +496,27 @@ extern "C" int sphr_healpix_maps(const struct sphr_healpix *h, const void *alm, 
 	if(!n)
 		return 0;
 	hipStream_t st = stream ? (hipStream_t) stream : sphr_rt_stream();
-	const int nring = 4 * h->nside - 1;
 	const size_t esz = h->precision == SPHR_MIXED ? sizeof(float) : sizeof(double);
-	const size_t lds = (2 * HP_SPW * (size_t) (h->L + 1) + 2 * 4 * (size_t) h->nside) * esz;
-	if(lds > 160 * 1024) {
-		sphr_set_error("sphr_healpix_maps: l_max %d with nside %d needs %zu bytes of LDS", h->L, h->nside, lds);
-		return -1;
+	const size_t lds = (2 * (size_t) (h->L + 1) * (h->L + 1) + 4 * HP_RC * (size_t) (h->L + 1)) * esz;
+	if(lds <= 150 * 1024) {
+		/* one workgroup per series, its coefficients in LDS */
+		if(h->precision == SPHR_MIXED)
+			hipLaunchKernelGGL(k_healpix<float>, dim3(n), dim3(HP_NT), lds, st, (const cplx<float> *) alm, n, h->L, h->nside, h->d_cf, h->d_cm, (float *) map);
+		else
+			hipLaunchKernelGGL(k_healpix<double>, dim3(n), dim3(HP_NT), lds, st, (const cplx<double> *) alm, n, h->L, h->nside, h->d_cf, h->d_cm, (double *) map);
+	} else {
+		/* one workgroup per (ring, HP_SPW series) */
+		const size_t lr = (2 * HP_SPW * (size_t) (h->L + 1) + 2 * 4 * (size_t) h->nside) * esz;
+		if(lr > 160 * 1024) {
+			sphr_set_error("sphr_healpix_maps: l_max %d with nside %d needs %zu bytes of LDS", h->L, h->nside, lr);
+			return -1;
+		}
+		const dim3 grid(4 * h->nside - 1, (n + HP_SPW - 1) / HP_SPW);
+		if(h->precision == SPHR_MIXED)
+			hipLaunchKernelGGL(k_healpix_ring<float>, grid, dim3(HP_NT), lr, st, (const cplx<float> *) alm, n, h->L, h->nside, h->d_cf, h->d_cm, (float *) map);
+		else
+			hipLaunchKernelGGL(k_healpix_ring<double>, grid, dim3(HP_NT), lr, st, (const cplx<double> *) alm, n, h->L, h->nside, h->d_cf, h->d_cm, (double *) map);
 	}
-	const dim3 grid(nring, (n + HP_SPW - 1) / HP_SPW);
-	if(h->precision == SPHR_MIXED)
-		hipLaunchKernelGGL(k_healpix<float>, grid, dim3(HP_NT), lds, st, (const cplx<float> *) alm, n, h->L, h->nside, h->d_cf, h->d_cm, (float *) map);
-	else
-		hipLaunchKernelGGL(k_healpix<double>, grid, dim3(HP_NT), lds, st, (const cplx<double> *) alm, n, h->L, h->nside, h->d_cf, h->d_cm, (double *) map);
 	return sphr_rt_check(hipGetLastError(), "k_healpix") ? -1 : 0;
 }
 

@@ -221,8 +221,10 @@ def _gpu_maps(series, L, nside, prec):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("L,nside", [(0, 1), (1, 1), (7, 2), (7, 8), (25, 16), (32, 16), (32, 32), (50, 32)])
+@pytest.mark.parametrize("L,nside", [(0, 1), (1, 1), (7, 2), (7, 8), (25, 16), (32, 16), (32, 32), (50, 32), (100, 8), (130, 4)])
 def test_gpu_healpix_maps_match_oracle(oracle, L, nside):
+    # l_max 100 (float64) and 130 (both) exceed the LDS of the
+    # one-workgroup-per-series form and run the per-ring form
     print("healpix maps L", L, "nside", nside, flush=True)
     rng = np.random.default_rng(L * 100 + nside)
     series = [rand_series(rng, L) for _ in range(5)]
