@@ -5,10 +5,8 @@
  * 190-272).
  *
  * correlator_baseline_integrate_power_fd, correlator_network_integrate_
- * power_fd and the FFT helpers run on the GPU and are synchronous.  Plan
- * construction runs in host code.  The time-domain correlator is not part
- * of this library; its plan struct is declared because the reference
- * headers expose it.
+ * power_fd, the time-domain integrators and the FFT helpers run on the GPU
+ * and are synchronous.  Plan construction runs in host code.
  */
 #ifndef __RADIOMETER_CORRELATOR_H__
 #define __RADIOMETER_CORRELATOR_H__
@@ -101,6 +99,19 @@ struct correlator_network_baselines *correlator_network_baselines_new(const stru
 void correlator_network_baselines_free(struct correlator_network_baselines *);
 struct correlator_network_baselines *correlator_network_baselines_copy(const struct correlator_network_baselines *);
 unsigned int correlator_network_l_max(struct correlator_network_baselines *, double);
+
+/*
+ * Time-domain correlator (correlator.c:325-400, 611-673, 820-858, 967-984):
+ * the integration runs on the GPU and is synchronous; the plan's struct is
+ * the reference's, allocated inside a larger one that also holds the
+ * device tables, so it is created and freed only by these functions.
+ */
+struct correlator_plan_td *correlator_plan_td_new(const struct correlator_baseline *, double);
+void correlator_plan_td_free(struct correlator_plan_td *);
+struct sh_series *correlator_baseline_integrate_power_td(const double *, const double *, const double *, int, struct correlator_plan_td *);
+struct correlator_network_plan_td *correlator_network_plan_td_new(struct correlator_network_baselines *, double);
+void correlator_network_plan_td_free(struct correlator_network_plan_td *);
+struct sh_series *correlator_network_integrate_power_td(struct sh_series *, double **, int, double **, struct correlator_network_plan_td *);
 
 struct correlator_plan_fd *correlator_plan_fd_new(const struct correlator_baseline *, int, double);
 void correlator_plan_fd_free(struct correlator_plan_fd *);

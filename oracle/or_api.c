@@ -301,3 +301,61 @@ void orx_healpix_posterior(const double *map_p, const double *map_n, long npix, 
 {
 	or_healpix_posterior(map_p, map_n, npix, prob);
 }
+
+/* time-domain correlator (or_td.c) */
+void *orx_plan_td_new(const double *ra, const double *rb, double delta_t)
+{
+	return or_plan_td_new(ra, rb, delta_t);
+}
+
+void orx_plan_td_free(void *p)
+{
+	or_plan_td_free(p);
+}
+
+/* info: [0] transient, [1] power l_max */
+void orx_plan_td_info(const void *p, int *info)
+{
+	info[0] = or_plan_td_transient(p);
+	info[1] = (int) or_plan_td_l_max(p);
+}
+
+/* power_2d (full layout of order l_max) and power_1d (l_max + 1) out */
+int orx_baseline_integrate_power_td(const double *ta, const double *tb, const double *window, int n, void *p, double complex *power_2d, double complex *power_1d)
+{
+	struct or_series *r = or_baseline_integrate_power_td(ta, tb, window, n, p);
+	if(!r)
+		return -1;
+	const unsigned L = r->l_max;
+	memcpy(power_2d, r->coeff, (size_t) (L + 1) * (L + 1) * sizeof(*power_2d));
+	memcpy(power_1d, or_plan_td_power_1d(p), (size_t) (L + 1) * sizeof(*power_1d));
+	or_series_free(r);
+	return 0;
+}
+
+void *orx_network_td_new(int n_instruments, const double *pos, double delta_t)
+{
+	return or_network_td_new(n_instruments, pos, delta_t);
+}
+
+void orx_network_td_free(void *net)
+{
+	or_network_td_free(net);
+}
+
+/* ts [n_instruments][n], windows [n_baselines][n - 2 transient_k] packed
+ * with stride wstride; sky of order sky_l out (full layout) */
+int orx_network_integrate_power_td(void *net, const double *ts, int n, const double *windows, int wstride, unsigned sky_l, double complex *sky)
+{
+	struct or_network_td *nt = net;
+	double **t = malloc(nt->n_instruments * sizeof(*t)), **w = malloc(nt->n_baselines * sizeof(*w));
+	for(int i = 0; i < nt->n_instruments; i++)
+		t[i] = (double *) ts + (size_t) i * n;
+	for(int k = 0; k < nt->n_baselines; k++)
+		w[k] = (double *) windows + (size_t) k * wstride;
+	struct or_series s = { sky_l, 0, sky };
+	struct or_series *r = or_network_integrate_power_td(&s, t, n, w, nt);
+	free(t);
+	free(w);
+	return r ? 0 : -1;
+}

@@ -132,6 +132,23 @@ void or_dft(double complex *data, int n, int sign);    /* in place, sign=-1 forw
 void or_mesh_nodes(unsigned l_max, int *ntheta, int *nphi, double *cos_theta, double *weights);
 double complex or_Y(unsigned l, int m, double theta, double phi);
 double complex or_eval(const struct or_series *, double theta, double phi);
+/* time-domain correlator (or_td.c) */
+struct or_plan_td;
+struct or_network_td {
+	int n_instruments, n_baselines;
+	int *index_a, *index_b;
+	struct or_plan_td **plans;
+};
+struct or_plan_td *or_plan_td_new(const double ra[3], const double rb[3], double delta_t);
+void or_plan_td_free(struct or_plan_td *);
+struct or_series *or_baseline_integrate_power_td(const double *ta, const double *tb, const double *window, int n, struct or_plan_td *);
+struct or_network_td *or_network_td_new(int n_instruments, const double *pos, double delta_t);
+void or_network_td_free(struct or_network_td *);
+struct or_series *or_network_integrate_power_td(struct or_series *sky, double **ts, int n, double **windows, struct or_network_td *);
+int or_plan_td_transient(const struct or_plan_td *);
+unsigned or_plan_td_l_max(const struct or_plan_td *);
+const double complex *or_plan_td_power_1d(const struct or_plan_td *);
+
 /* HEALPix output (or_healpix.c) */
 int or_healpix_nside_c(unsigned l_max);
 int or_healpix_nside_py(unsigned l_max);

@@ -87,6 +87,15 @@ def lib():
         L.orx_healpix_pix2ang.argtypes = [c_int, ctypes.c_long, c_void_p, c_void_p]
         L.orx_healpix_map.argtypes = [c_uint, c_int, c_void_p, c_int, c_void_p]
         L.orx_healpix_posterior.argtypes = [c_void_p, c_void_p, ctypes.c_long, c_void_p]
+        L.orx_plan_td_new.restype = c_void_p
+        L.orx_plan_td_new.argtypes = [c_void_p, c_void_p, c_double]
+        L.orx_plan_td_free.argtypes = [c_void_p]
+        L.orx_plan_td_info.argtypes = [c_void_p, c_void_p]
+        L.orx_baseline_integrate_power_td.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
+        L.orx_network_td_new.restype = c_void_p
+        L.orx_network_td_new.argtypes = [c_int, c_void_p, c_double]
+        L.orx_network_td_free.argtypes = [c_void_p]
+        L.orx_network_integrate_power_td.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_uint, c_void_p]
         _LIB = L
     return _LIB
 
@@ -492,3 +501,53 @@ def healpix_posterior(map_p, map_n):
     out = np.zeros(a.size)
     lib().orx_healpix_posterior(_ptr(a), _ptr(b), a.size, _ptr(out))
     return out
+
+
+# time-domain correlator (or_td.c; correlator.c:325-400, 648-673, 820-858,
+# 967-984)
+class PlanTD:
+    def __init__(self, ra, rb, delta_t):
+        a = np.ascontiguousarray(ra, dtype=np.float64)
+        b = np.ascontiguousarray(rb, dtype=np.float64)
+        self.h = lib().orx_plan_td_new(_ptr(a), _ptr(b), delta_t)
+        info = np.zeros(2, dtype=np.int32)
+        lib().orx_plan_td_info(self.h, _ptr(info))
+        self.transient, self.l_max = int(info[0]), int(info[1])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orx_plan_td_free(self.h)
+            self.h = None
+
+    def integrate(self, ta, tb, window):
+        """(power_2d full layout, power_1d) of the baseline"""
+        a = np.ascontiguousarray(ta, dtype=np.float64)
+        b = np.ascontiguousarray(tb, dtype=np.float64)
+        w = np.ascontiguousarray(window, dtype=np.float64)
+        L = self.l_max
+        p2 = np.zeros((L + 1) ** 2, dtype=np.complex128)
+        p1 = np.zeros(L + 1, dtype=np.complex128)
+        if lib().orx_baseline_integrate_power_td(_ptr(a), _ptr(b), _ptr(w), a.size, self.h, _ptr(p2), _ptr(p1)):
+            raise RuntimeError("integrate_power_td failed")
+        return p2, p1
+
+
+class NetworkTD:
+    def __init__(self, positions, delta_t):
+        p = np.ascontiguousarray(positions, dtype=np.float64)
+        self.n_instruments = p.shape[0]
+        self.h = lib().orx_network_td_new(p.shape[0], _ptr(p), delta_t)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orx_network_td_free(self.h)
+            self.h = None
+
+    def integrate(self, ts, windows, sky_l):
+        """sky of order sky_l; ts [n_instruments][n], windows [n_baselines][stride]"""
+        t = np.ascontiguousarray(ts, dtype=np.float64)
+        w = np.ascontiguousarray(windows, dtype=np.float64)
+        sky = np.zeros((sky_l + 1) ** 2, dtype=np.complex128)
+        if lib().orx_network_integrate_power_td(self.h, _ptr(t), t.shape[1], _ptr(w), w.shape[1], sky_l, _ptr(sky)):
+            raise RuntimeError("network_integrate_power_td failed")
+        return sky

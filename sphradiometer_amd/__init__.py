@@ -80,8 +80,19 @@ class PlanFD(ctypes.Structure):
                 ("power_1d", P(ShSeries)), ("rotation_plan", P(RotationPlan))]
 
 
+class PlanTD(ctypes.Structure):
+    _fields_ = [("baseline", P(Baseline)), ("delta_t", c_double), ("transient", c_int),
+                ("proj_a", P(ShSeriesArray)), ("proj_b", P(ShSeriesArray)),
+                ("sample_a", P(ShSeries)), ("sample_b", P(ShSeries)), ("product", P(ShSeries)),
+                ("power_1d", P(ShSeries)), ("product_plan", c_void_p), ("rotation_plan", P(RotationPlan))]
+
+
 class NetworkBaselines(ctypes.Structure):
     _fields_ = [("baselines", P(P(Baseline))), ("n_baselines", c_int)]
+
+
+class NetworkPlanTD(ctypes.Structure):
+    _fields_ = [("baselines", P(NetworkBaselines)), ("plans", P(P(PlanTD)))]
 
 
 class NetworkPlanFD(ctypes.Structure):
@@ -170,6 +181,12 @@ _PROTOS = {
     "correlator_network_plan_fd_copy": (P(NetworkPlanFD), [P(NetworkPlanFD)]),
     "correlator_network_plan_fd_set_l": (P(NetworkPlanFD), [P(NetworkPlanFD), c_uint]),
     "correlator_baseline_integrate_power_fd": (P(ShSeries), [c_void_p, c_void_p, P(PlanFD)]),
+    "correlator_plan_td_new": (P(PlanTD), [P(Baseline), c_double]),
+    "correlator_plan_td_free": (None, [P(PlanTD)]),
+    "correlator_baseline_integrate_power_td": (P(ShSeries), [c_void_p, c_void_p, c_void_p, c_int, P(PlanTD)]),
+    "correlator_network_plan_td_new": (P(NetworkPlanTD), [P(NetworkBaselines), c_double]),
+    "correlator_network_plan_td_free": (None, [P(NetworkPlanTD)]),
+    "correlator_network_integrate_power_td": (P(ShSeries), [P(ShSeries), P(c_void_p), c_int, P(c_void_p), P(NetworkPlanTD)]),
     "correlator_network_integrate_power_fd": (P(ShSeries), [P(ShSeries), P(c_void_p), P(NetworkPlanFD)]),
     "correlator_tseries_to_fseries_plan": (c_void_p, [c_void_p, c_void_p, c_int]),
     "correlator_tseries_to_fseries": (None, [c_void_p, c_void_p, c_int, c_void_p]),

@@ -1,0 +1,421 @@
+/*
+ * Time-domain correlator (correlator.c:325-400, 611-673, 820-858,
+ * 967-984 of the reference) with the integration on the GPU.
+ *
+ * The reference integrates sample by sample: for each output sample t it
+ * dots the two delay-tap projections with the series (sample_a(t) =
+ * sum_k proj_a[k] a[t + k], sample_b likewise), multiplies the two polar
+ * series (sh_series_product) and adds window[t] times the product to
+ * power_1d.  As functions of x = cos theta, sample_a(t)(x) =
+ * sum_k T^a_k(x) a[t + k] with T^a_k the synthesis of proj_a[k], and the
+ * product's coefficient d is 2 pi int sample_a sample_b Pbar_d0 dx, which a
+ * Gauss-Legendre rule of n_q = (a + b + P) / 2 + 1 nodes integrates
+ * exactly.  Summed over t:
+ *   power_1d[d] = sum_q 2 pi w_q Pbar_d0(x_q) U_q,
+ *   U_q = sum_{k,k'} T^a_k(x_q) C[k][k'] T^b_k'(x_q),
+ *   C[k][k'] = sum_t window[t] a[t + k] b[t + k'],
+ * the same arithmetic up to rounding, with the per-sample work folded into
+ * the windowed correlation matrix C (n_p x n_p, a GEMM over t with the
+ * Hankel operands generated from the series in the tile loader, float64
+ * MFMA), then Y = C T^b (the library GEMM) and the bilinear and d sums.
+ * The rotation to the baseline direction is the drop-in sh_series_rotate.
+ *
+ * The plan keeps the reference's struct (proj_a, proj_b, sample_a,
+ * sample_b, product, power_1d, product_plan, rotation_plan) as the first
+ * member of a larger allocation that holds the device tables.  After an
+ * integration power_1d holds the axially symmetric power, as in the
+ * reference; sample_a, sample_b and product are not updated.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sphradiometer/correlator.h>
+#include <sphradiometer/projection.h>
+#include <sphradiometer/sh_series.h>
+#include <sphradiometer/sphradiometer_gpu.h>
+#include "../sphr_internal.h"
+#include "kernels.h"
+#include "runtime.h"
+
+struct td_plan {
+	struct correlator_plan_td pub;	/* the reference layout, first */
+	int np, nq, P;
+	double *d_Ta, *d_Tb;		/* [np][nq]: tap projections at the product nodes */
+	double *d_WP;			/* [nq][P + 1]: 2 pi w_q Pbar_d0(x_q) */
+	/* per-call device buffers, grown as needed */
+	size_t cap_n, cap_part;
+	double *d_a, *d_b, *d_w, *d_part, *d_C, *d_Y, *d_pow;
+};
+
+template <typename T, typename TA>
+int launch_gemm(int M, int N, int K, const TA *A, long lda, const T *B, long ldb, T *C, long ldc, hipStream_t st);
+
+namespace {
+
+/*
+ * partial[z][k][k'] = sum_{t in split z} w[t] a[t + k] b[t + k'],
+ * k, k' < np: 64 x 64 tiles of 4 waves (32 x 32 each) on
+ * v_mfma_f64_16x16x4f64, t in chunks of 16 through LDS, the operands read
+ * from the series (Hankel structure) with the next chunk's loads in flight
+ * during the current chunk's MFMAs.  MFMA layout: A[i][k] in lane
+ * i + 16 k, B[k][j] in lane j + 16 k, accumulator element e of lane
+ * j + 16 q holds D[q + 4 e][j].
+ */
+#define TD_TILE 64
+#define TD_KC 16
+
+__global__ void __launch_bounds__(256) k_td_corr(const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ w, int nt, int np, int tpz, int tiles, double *__restrict__ part)
+{
+	typedef double v4 __attribute__((ext_vector_type(4)));
+	__shared__ double sA[TD_KC][TD_TILE + 4];	/* [t][k] */
+	__shared__ double sB[TD_KC][TD_TILE + 4];	/* [t][k'] */
+	const int tile = blockIdx.x % (tiles * tiles), z = blockIdx.x / (tiles * tiles);
+	const int m0 = (tile % tiles) * TD_TILE, n0 = (tile / tiles) * TD_TILE;
+	const int t0 = z * tpz, t1 = t0 + tpz < nt ? t0 + tpz : nt;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int li = lane & 15, lk = lane >> 4;
+	const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+	double ra[4], rb[4];
+	auto load = [&](int c0) {
+#pragma unroll
+		for(int r = 0; r < 4; r++) {
+			const int e = tid + 256 * r;
+			const int t = c0 + e / TD_TILE, k = e % TD_TILE;
+			const bool ok = t < t1;
+			ra[r] = ok && m0 + k < np ? w[t] * a[t + m0 + k] : 0.0;
+			rb[r] = ok && n0 + k < np ? b[t + n0 + k] : 0.0;
+		}
+	};
+	v4 acc[2][2];
+#pragma unroll
+	for(int i = 0; i < 2; i++)
+#pragma unroll
+		for(int j = 0; j < 2; j++)
+			acc[i][j] = (v4) 0;
+	if(t0 < t1)
+		load(t0);
+	for(int c0 = t0; c0 < t1; c0 += TD_KC) {
+#pragma unroll
+		for(int r = 0; r < 4; r++) {
+			const int e = tid + 256 * r;
+			sA[e / TD_TILE][e % TD_TILE] = ra[r];
+			sB[e / TD_TILE][e % TD_TILE] = rb[r];
+		}
+		__syncthreads();
+		if(c0 + TD_KC < t1)
+			load(c0 + TD_KC);
+#pragma unroll
+		for(int ks = 0; ks < TD_KC; ks += 4) {
+			double af[2], bf[2];
+#pragma unroll
+			for(int i = 0; i < 2; i++)
+				af[i] = sA[ks + lk][wm + 16 * i + li];
+#pragma unroll
+			for(int j = 0; j < 2; j++)
+				bf[j] = sB[ks + lk][wn + 16 * j + li];
+#pragma unroll
+			for(int i = 0; i < 2; i++)
+#pragma unroll
+				for(int j = 0; j < 2; j++)
+					acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+		}
+		__syncthreads();
+	}
+	double *P = part + (size_t) z * np * np;
+#pragma unroll
+	for(int i = 0; i < 2; i++)
+#pragma unroll
+		for(int j = 0; j < 2; j++) {
+			const int c = n0 + wn + 16 * j + li;
+			if(c >= np)
+				continue;
+#pragma unroll
+			for(int e = 0; e < 4; e++) {
+				const int r = m0 + wm + 16 * i + lk + 4 * e;
+				if(r < np)
+					P[(size_t) r * np + c] = acc[i][j][e];
+			}
+		}
+}
+
+/* C = sum_z partial[z] */
+__global__ void k_td_reduce(const double *__restrict__ part, int nz, long n, double *__restrict__ C)
+{
+	const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= n)
+		return;
+	double s = 0.0;
+	for(int z = 0; z < nz; z++)
+		s += part[(size_t) z * n + i];
+	C[i] = s;
+}
+
+/* U_q = sum_k Ta[k][q] Y[k][q]; power[d] = sum_q U_q WP[q][d] */
+__global__ void __launch_bounds__(256) k_td_power(const double *__restrict__ Ta, const double *__restrict__ Y, const double *__restrict__ WP, int np, int nq, int P, double *__restrict__ power)
+{
+	extern __shared__ double td_U[];
+	for(int q = threadIdx.x; q < nq; q += blockDim.x) {
+		double u = 0.0;
+		for(int k = 0; k < np; k++)
+			u = fma(Ta[(size_t) k * nq + q], Y[(size_t) k * nq + q], u);
+		td_U[q] = u;
+	}
+	__syncthreads();
+	for(int d = threadIdx.x; d <= P; d += blockDim.x) {
+		double s = 0.0;
+		for(int q = 0; q < nq; q++)
+			s = fma(td_U[q], WP[(size_t) q * (P + 1) + d], s);
+		power[d] = s;
+	}
+}
+
+}	/* namespace */
+
+/* ------------------------------------------------------------- the plan */
+
+static void td_dev_free(struct td_plan *p)
+{
+	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_Y, p->d_pow };
+	for(unsigned i = 0; i < sizeof(b) / sizeof(b[0]); i++)
+		if(b[i])
+			(void) hipFree(b[i]);
+}
+
+/* correlator_plan_td_new (correlator.c:325-385) */
+extern "C" struct correlator_plan_td *correlator_plan_td_new(const struct correlator_baseline *baseline, double delta_t)
+{
+	if(!baseline || delta_t <= 0. || sphr_gpu_ensure())
+		return NULL;
+	struct td_plan *p = (struct td_plan *) calloc(1, sizeof(*p));
+	gsl_vector *dp = gsl_vector_alloc(3);
+	double *R = sh_series_rot_matrix(baseline->theta, baseline->phi);
+	double *Ta = NULL, *Tb = NULL, *WP = NULL;
+	if(!p || !dp || !R)
+		goto fail;
+	{
+		struct correlator_plan_td *t = &p->pub;
+		const double D = vector_magnitude(baseline->d);
+		const unsigned a_l = projection_matrix_l_max(D / 2, delta_t), b_l = projection_matrix_l_max(D / 2, delta_t);
+		unsigned P = correlator_baseline_power_l_max(baseline, delta_t);
+		if(P > a_l + b_l)
+			P = a_l + b_l;
+		t->sample_a = sh_series_new(a_l, 1);
+		t->sample_b = sh_series_new(b_l, 1);
+		t->product = sh_series_new(P, 1);
+		t->power_1d = sh_series_new_zero(P, 1);
+		if(!t->sample_a || !t->sample_b || !t->product || !t->power_1d)
+			goto fail;
+		/* d' = +D/2 z and -D/2 z */
+		gsl_vector_set(dp, 0, 0);
+		gsl_vector_set(dp, 1, 0);
+		gsl_vector_set(dp, 2, +D / 2.);
+		t->proj_a = projection_matrix_delay(projection_matrix_n_elements(vector_magnitude(dp), delta_t), a_l, dp, delta_t);
+		gsl_vector_set(dp, 2, -D / 2.);
+		t->proj_b = projection_matrix_delay(projection_matrix_n_elements(vector_magnitude(dp), delta_t), b_l, dp, delta_t);
+		if(!t->proj_a || !t->proj_b)
+			goto fail;
+		t->product_plan = sh_series_product_plan_new(t->power_1d, t->sample_a, t->sample_b);
+		t->rotation_plan = sh_series_rotation_plan_new(t->power_1d, R);
+		if(!t->product_plan || !t->rotation_plan)
+			goto fail;
+		t->baseline = baseline;
+		t->delta_t = delta_t;
+		t->transient = correlator_transient(t->proj_a, t->proj_b);
+		/* device tables on an exact product rule: degree a + b + P */
+		/* the correlation kernel reads a[t + k] for t < n - 2 transient,
+		 * k < n_p: in range when n_p = 2 transient + 1 (odd tap counts,
+		 * projection.c's 2 (int) (60 + r / dt + 1) + 1) */
+		if(t->proj_a->n != t->proj_b->n || t->proj_a->n != 2 * t->transient + 1) {
+			sphr_set_error("correlator_plan_td_new: tap counts %d, %d with transient %d", t->proj_a->n, t->proj_b->n, t->transient);
+			goto fail;
+		}
+		const int np = t->proj_a->n, nq = (int) ((a_l + b_l + P) / 2 + 1);
+		p->np = np;
+		p->nq = nq;
+		p->P = (int) P;
+		Ta = (double *) malloc((size_t) np * nq * sizeof(double));
+		Tb = (double *) malloc((size_t) np * nq * sizeof(double));
+		WP = (double *) malloc((size_t) nq * (P + 1) * sizeof(double));
+		if(!Ta || !Tb || !WP || sphr_td_tables(t->proj_a, t->proj_b, nq, (int) P, Ta, Tb, WP))
+			goto fail;
+		if(sphr_rt_check(hipMalloc(&p->d_Ta, (size_t) np * nq * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_Tb, (size_t) np * nq * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_WP, (size_t) nq * (P + 1) * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_C, (size_t) np * np * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_Y, (size_t) np * nq * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_pow, (size_t) (P + 1) * sizeof(double)), "hipMalloc")
+		   || sphr_memcpy_htod(p->d_Ta, Ta, (size_t) np * nq * sizeof(double))
+		   || sphr_memcpy_htod(p->d_Tb, Tb, (size_t) np * nq * sizeof(double))
+		   || sphr_memcpy_htod(p->d_WP, WP, (size_t) nq * (P + 1) * sizeof(double)))
+			goto fail;
+	}
+	free(Ta);
+	free(Tb);
+	free(WP);
+	free(R);
+	gsl_vector_free(dp);
+	return &p->pub;
+fail:
+	free(Ta);
+	free(Tb);
+	free(WP);
+	free(R);
+	if(dp)
+		gsl_vector_free(dp);
+	if(p)
+		correlator_plan_td_free(&p->pub);
+	return NULL;
+}
+
+extern "C" void correlator_plan_td_free(struct correlator_plan_td *plan)
+{
+	if(!plan)
+		return;
+	struct td_plan *p = (struct td_plan *) plan;
+	sh_series_rotation_plan_free(plan->rotation_plan);
+	sh_series_product_plan_free(plan->product_plan);
+	sh_series_free(plan->power_1d);
+	sh_series_free(plan->product);
+	sh_series_free(plan->sample_b);
+	sh_series_free(plan->sample_a);
+	sh_series_array_free(plan->proj_b);
+	sh_series_array_free(plan->proj_a);
+	td_dev_free(p);
+	free(p);
+}
+
+static int grow(double **buf, size_t *cap, size_t need)
+{
+	if(need <= *cap)
+		return 0;
+	if(*buf)
+		(void) hipFree(*buf);
+	*buf = NULL;
+	if(sphr_rt_check(hipMalloc(buf, need * sizeof(double)), "hipMalloc"))
+		return -1;
+	*cap = need;
+	return 0;
+}
+
+/* power_1d of the plan (float64, host) from the series on the GPU */
+static int td_power_1d(const double *ta, const double *tb, const double *window, int n, struct td_plan *p, double *power)
+{
+	const int np = p->np, nq = p->nq, nt = n - 2 * p->pub.transient;
+	hipStream_t st = sphr_rt_stream();
+	size_t cap_a = p->cap_n, cap_b = p->cap_n, cap_w = p->cap_n;
+	/* K splits of about 2048 samples: enough workgroups on a short series,
+	 * partial sums that stay small on a long one */
+	const int tiles = (np + TD_TILE - 1) / TD_TILE;
+	int nz = (nt + 2047) / 2048;
+	if(nz < 1)
+		nz = 1;
+	const int tpz = ((nt + nz - 1) / nz + TD_KC - 1) / TD_KC * TD_KC;
+	nz = nt > 0 ? (nt + tpz - 1) / tpz : 1;
+	if(grow(&p->d_a, &cap_a, (size_t) n) || grow(&p->d_b, &cap_b, (size_t) n) || grow(&p->d_w, &cap_w, (size_t) (nt > 0 ? nt : 1)))
+		return -1;
+	p->cap_n = cap_a < cap_b ? (cap_a < cap_w ? cap_a : cap_w) : (cap_b < cap_w ? cap_b : cap_w);
+	if(grow(&p->d_part, &p->cap_part, (size_t) nz * np * np))
+		return -1;
+	if(sphr_memcpy_htod(p->d_a, ta, (size_t) n * sizeof(double)) || sphr_memcpy_htod(p->d_b, tb, (size_t) n * sizeof(double))
+	   || (nt > 0 && sphr_memcpy_htod(p->d_w, window, (size_t) nt * sizeof(double))))
+		return -1;
+	if(nt > 0)
+		hipLaunchKernelGGL(k_td_corr, dim3(tiles * tiles * nz), dim3(256), 0, st, p->d_a, p->d_b, p->d_w, nt, np, tpz, tiles, p->d_part);
+	else
+		(void) hipMemsetAsync(p->d_part, 0, (size_t) np * np * sizeof(double), st);
+	const long nc = (long) np * np;
+	hipLaunchKernelGGL(k_td_reduce, dim3((nc + 255) / 256), dim3(256), 0, st, p->d_part, nt > 0 ? nz : 1, nc, p->d_C);
+	if(launch_gemm<double, double>(np, nq, np, p->d_C, np, p->d_Tb, nq, p->d_Y, nq, st))
+		return -1;
+	hipLaunchKernelGGL(k_td_power, dim3(1), dim3(256), (size_t) nq * sizeof(double), st, p->d_Ta, p->d_Y, p->d_WP, np, nq, p->P, p->d_pow);
+	if(sphr_rt_check(hipGetLastError(), "time-domain correlator"))
+		return -1;
+	return sphr_memcpy_dtoh(power, p->d_pow, (size_t) (p->P + 1) * sizeof(double));
+}
+
+/* correlator_baseline_integrate_power_td (correlator.c:648-673) */
+extern "C" struct sh_series *correlator_baseline_integrate_power_td(const double *time_series_a, const double *time_series_b, const double *window, int n, struct correlator_plan_td *plan)
+{
+	if(!plan || !time_series_a || !time_series_b || !window)
+		return NULL;
+	if(n - 2 * plan->transient < 0)
+		return NULL;
+	struct td_plan *p = (struct td_plan *) plan;
+	double *power = (double *) malloc((size_t) (p->P + 1) * sizeof(double));
+	if(!power)
+		return NULL;
+	pthread_mutex_lock(&sphr_dropin_lock);
+	const int rc = td_power_1d(time_series_a, time_series_b, window, n, p, power);
+	pthread_mutex_unlock(&sphr_dropin_lock);
+	if(rc) {
+		free(power);
+		return NULL;
+	}
+	for(int d = 0; d <= p->P; d++)
+		sh_series_set(plan->power_1d, d, 0, power[d]);
+	free(power);
+	struct sh_series *power_2d = sh_series_new(plan->power_1d->l_max, 0);
+	if(!power_2d || !sh_series_rotate(power_2d, plan->power_1d, plan->rotation_plan)) {
+		sh_series_free(power_2d);
+		return NULL;
+	}
+	return power_2d;
+}
+
+/* correlator_network_plan_td_new / _free (correlator.c:820-858) */
+extern "C" struct correlator_network_plan_td *correlator_network_plan_td_new(struct correlator_network_baselines *baselines, double delta_t)
+{
+	struct correlator_network_plan_td *n = (struct correlator_network_plan_td *) malloc(sizeof(*n));
+	struct correlator_plan_td **plans = baselines ? (struct correlator_plan_td **) calloc(baselines->n_baselines, sizeof(*plans)) : NULL;
+	if(!n || !plans) {
+		free(n);
+		free(plans);
+		return NULL;
+	}
+	for(int i = 0; i < baselines->n_baselines; i++) {
+		plans[i] = correlator_plan_td_new(baselines->baselines[i], delta_t);
+		if(!plans[i]) {
+			while(--i >= 0)
+				correlator_plan_td_free(plans[i]);
+			free(n);
+			free(plans);
+			return NULL;
+		}
+	}
+	n->baselines = baselines;
+	n->plans = plans;
+	return n;
+}
+
+extern "C" void correlator_network_plan_td_free(struct correlator_network_plan_td *plan)
+{
+	if(plan) {
+		for(int i = 0; i < plan->baselines->n_baselines; i++)
+			correlator_plan_td_free(plan->plans[i]);
+		free(plan->plans);
+	}
+	free(plan);
+}
+
+/* correlator_network_integrate_power_td (correlator.c:967-984) */
+extern "C" struct sh_series *correlator_network_integrate_power_td(struct sh_series *sky, double **tseries, int tseries_length, double **windows, struct correlator_network_plan_td *plan)
+{
+	if(!sky || !tseries || !windows || !plan)
+		return NULL;
+	sh_series_zero(sky);
+	int k = 0;
+	const int n_inst = instrument_array_len(plan->baselines->baselines[0]->instruments);
+	for(int i = 1; i < n_inst; i++)
+		for(int j = 0; j < i; j++, k++) {
+			struct sh_series *power_2d = correlator_baseline_integrate_power_td(tseries[i], tseries[j], windows[k], tseries_length, plan->plans[k]);
+			if(!power_2d)
+				return NULL;
+			if(!sh_series_add(sky, 1.0 / plan->baselines->n_baselines, power_2d)) {
+				sh_series_free(power_2d);
+				return NULL;
+			}
+			sh_series_free(power_2d);
+		}
+	return sky;
+}

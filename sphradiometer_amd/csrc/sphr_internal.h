@@ -26,6 +26,9 @@ void sphr_gl_rule_cached(int n, const double **x, const double **w);
 /* orthonormal associated Legendre functions (Condon-Shortley phase),
  * out[l - m] for l = m .. l_max */
 void sphr_plm(unsigned l_max, int m, double x, double *out);
+/* tables of the GPU time-domain correlator (td_tables.c) */
+struct sh_series_array;
+int sphr_td_tables(const struct sh_series_array *pa, const struct sh_series_array *pb, int nq, int P, double *Ta, double *Tb, double *WP);
 /* HEALPix map FITS file (healpix_fits.c) */
 int sphr_fits_write_healpix_map(const char *filename, const double *map, int nside);
 /* all 3j (j1 j2 j; m1 m2 -m1-m2) for j = jmin .. j1+j2 into out[j - jmin];
