@@ -218,6 +218,81 @@ def bench_healpix(S, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_td(S, args):
+    """The time-domain correlator (correlator.c:611-673, 967-984) over a
+    long stream: three sites, 4096 Hz, a 16 s stream integrated with a
+    square window in one network call (three baselines).  value = stream
+    samples/s; MFMA roofline of the windowed Hankel correlation (2 n_p^2
+    flops per output sample and baseline, float64)."""
+    import ctypes
+    from sphradiometer_amd import inject
+    L = S.lib()
+    dt, n = 1.0 / 4096, 16 * 4096
+    pos = inject.positions("ABC")
+    inst = L.instrument_array_new(0)
+    for x, y, z in pos:
+        L.instrument_array_append(inst, L.instrument_new(float(x), float(y), float(z), None))
+    bl = L.correlator_network_baselines_new(inst)
+    t0 = time.time()
+    plan = L.correlator_network_plan_td_new(bl, dt)
+    if not plan:
+        raise RuntimeError(S.last_error())
+    plan_s = time.time() - t0
+    nb = bl.contents.n_baselines
+    pl = [plan.contents.plans[k].contents for k in range(nb)]
+    trs = [p.transient for p in pl]
+    nps = [2 * t + 1 for t in trs]
+    lmax = max(p.power_1d.contents.l_max for p in pl)
+    print("td plans built in %.1f s: taps %s, power l_max %s" % (plan_s, nps, [p.power_1d.contents.l_max for p in pl]), file=sys.stderr, flush=True)
+    rng = np.random.default_rng(0)
+    ts = rng.standard_normal((3, n))
+    wins = [np.full(n - 2 * t, 1.0 / (n - 2 * t)) for t in trs]
+    tp = (ctypes.c_void_p * 3)(*[ts[i].ctypes.data for i in range(3)])
+    wp = (ctypes.c_void_p * nb)(*[wins[k].ctypes.data for k in range(nb)])
+    sky = L.sh_series_new(lmax, 0)
+    for _ in range(args.warmup):
+        if not L.correlator_network_integrate_power_td(sky, tp, n, wp, plan):
+            raise RuntimeError(S.last_error())
+    t0 = time.time()
+    for _ in range(args.steps):
+        L.correlator_network_integrate_power_td(sky, tp, n, wp, plan)
+    sec = (time.time() - t0) / args.steps
+    flops = sum(2.0 * npk * npk * (n - 2 * t) for npk, t in zip(nps, trs))
+    ach = flops / sec / 1e12
+    out = {
+        "metric": "TD network cross-power, 3 sites at 4096 Hz: stream samples/s (16 s stream, one call)",
+        "value": round(n / sec, 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: white N(0,1) float64 (seed 0), square window",
+        "config": {"workload": "three sites, %d samples at 4096 Hz, power l_max %d, sky l_max %d" % (n, lmax, lmax),
+                   "samples": n, "taps": nps, "sky_l": lmax},
+        "roofline": {"bound": "mfma", "kernel": "k_td_corr (whole call)", "achieved": round(ach, 3), "peak": FP64_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(ach / FP64_MFMA_TFLOPS, 4), "traffic": None},
+        "plan_build_s": round(plan_s, 1),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        # the sample-by-sample FP64 oracle on every baseline, a bounded
+        # number of output samples each (after one call that builds the
+        # product tables); network samples/s = 1 / sum of per-sample costs
+        cost, m = 0.0, 256
+        k = 0
+        for i in range(1, 3):
+            for j in range(i):
+                td = O.PlanTD(pos[i], pos[j], dt)
+                a = rng.standard_normal(2 * td.transient + m)
+                td.integrate(a[: 2 * td.transient + 1], a[: 2 * td.transient + 1], np.ones(1))
+                t0 = time.time()
+                td.integrate(a, a, np.full(m, 1.0 / m))
+                cost += (time.time() - t0) / m
+                k += 1
+        out["cpu_baseline"] = {"value": round(1.0 / cost, 3), "unit": "samples/s", "cores": 1, "kind": "port",
+                               "sample": "%d output samples of each of the %d baselines (sh_series_array_dot x 2 + product per sample), FP64 C oracle, 1 thread" % (m, k)}
+    L.sh_series_free(sky)
+    L.correlator_network_plan_td_free(plan)
+    print(json.dumps(out), flush=True)
+
+
 def bench_localization(S, args):
     """The localization workload: three sites, complex SNR series of 410
     samples at 4096 Hz (+-50 ms, padded to 584), beta = +1 / -1 plans
@@ -407,9 +482,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-budget", type=float, default=3.0)
-    ap.add_argument("--workload", choices=["sky", "localization", "jobs", "healpix", "copy"], default="sky",
+    ap.add_argument("--workload", choices=["sky", "localization", "jobs", "healpix", "td", "copy"], default="sky",
                     help="sky: the three-site cross-power maps (default); localization: the log-likelihood skies; "
-                         "jobs: the uneven batch with a plan per job; healpix: RING maps + posterior of localization skies")
+                         "jobs: the uneven batch with a plan per job; healpix: RING maps + posterior of localization skies; "
+                         "td: the time-domain correlator over a long stream")
     ap.add_argument("--events", type=int, default=8192)
     ap.add_argument("--jobs", type=int, default=4096)
     args = ap.parse_args()
@@ -432,6 +508,9 @@ def main():
         return
     if args.workload == "healpix":
         bench_healpix(S, args)
+        return
+    if args.workload == "td":
+        bench_td(S, args)
         return
     if args.workload == "copy":
         bench_copy(S, args)

@@ -45,7 +45,10 @@ struct td_plan {
 	double *d_WP;			/* [nq][P + 1]: 2 pi w_q Pbar_d0(x_q) */
 	/* per-call device buffers, grown as needed */
 	size_t cap_n, cap_part;
-	double *d_a, *d_b, *d_w, *d_part, *d_C, *d_Y, *d_pow;
+	double *d_a, *d_b, *d_w, *d_part, *d_C, *d_Y, *d_U, *d_pow;
+	/* a network's series and windows (kept on its first plan) */
+	size_t cap_nts, cap_nws;
+	double *d_nts, *d_nws;
 };
 
 template <typename T, typename TA>
@@ -151,23 +154,36 @@ __global__ void k_td_reduce(const double *__restrict__ part, int nz, long n, dou
 	C[i] = s;
 }
 
-/* U_q = sum_k Ta[k][q] Y[k][q]; power[d] = sum_q U_q WP[q][d] */
-__global__ void __launch_bounds__(256) k_td_power(const double *__restrict__ Ta, const double *__restrict__ Y, const double *__restrict__ WP, int np, int nq, int P, double *__restrict__ power)
+/* U_q = sum_k Ta[k][q] Y[k][q]: 64 q per workgroup, the k sum split
+ * four ways and reduced in LDS */
+__global__ void __launch_bounds__(256) k_td_u(const double *__restrict__ Ta, const double *__restrict__ Y, int np, int nq, double *__restrict__ U)
 {
-	extern __shared__ double td_U[];
-	for(int q = threadIdx.x; q < nq; q += blockDim.x) {
-		double u = 0.0;
-		for(int k = 0; k < np; k++)
+	__shared__ double red[4][64];
+	const int qi = threadIdx.x & 63, ks = threadIdx.x >> 6, q = blockIdx.x * 64 + qi;
+	double u = 0.0;
+	if(q < nq)
+		for(int k = ks; k < np; k += 4)
 			u = fma(Ta[(size_t) k * nq + q], Y[(size_t) k * nq + q], u);
-		td_U[q] = u;
-	}
+	red[ks][qi] = u;
 	__syncthreads();
-	for(int d = threadIdx.x; d <= P; d += blockDim.x) {
-		double s = 0.0;
-		for(int q = 0; q < nq; q++)
-			s = fma(td_U[q], WP[(size_t) q * (P + 1) + d], s);
-		power[d] = s;
-	}
+	if(ks == 0 && q < nq)
+		U[q] = (red[0][qi] + red[1][qi]) + (red[2][qi] + red[3][qi]);
+}
+
+/* power[d] = sum_q U_q WP[q][d]: 64 d per workgroup, the q sum split
+ * four ways and reduced in LDS */
+__global__ void __launch_bounds__(256) k_td_pw(const double *__restrict__ U, const double *__restrict__ WP, int nq, int P, double *__restrict__ power)
+{
+	__shared__ double red[4][64];
+	const int di = threadIdx.x & 63, qs = threadIdx.x >> 6, d = blockIdx.x * 64 + di;
+	double s = 0.0;
+	if(d <= P)
+		for(int q = qs; q < nq; q += 4)
+			s = fma(U[q], WP[(size_t) q * (P + 1) + d], s);
+	red[qs][di] = s;
+	__syncthreads();
+	if(qs == 0 && d <= P)
+		power[d] = (red[0][di] + red[1][di]) + (red[2][di] + red[3][di]);
 }
 
 }	/* namespace */
@@ -176,7 +192,7 @@ __global__ void __launch_bounds__(256) k_td_power(const double *__restrict__ Ta,
 
 static void td_dev_free(struct td_plan *p)
 {
-	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_Y, p->d_pow };
+	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_Y, p->d_U, p->d_pow, p->d_nts, p->d_nws };
 	for(unsigned i = 0; i < sizeof(b) / sizeof(b[0]); i++)
 		if(b[i])
 			(void) hipFree(b[i]);
@@ -244,6 +260,7 @@ extern "C" struct correlator_plan_td *correlator_plan_td_new(const struct correl
 		   || sphr_rt_check(hipMalloc(&p->d_WP, (size_t) nq * (P + 1) * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_C, (size_t) np * np * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_Y, (size_t) np * nq * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_U, (size_t) nq * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_pow, (size_t) (P + 1) * sizeof(double)), "hipMalloc")
 		   || sphr_memcpy_htod(p->d_Ta, Ta, (size_t) np * nq * sizeof(double))
 		   || sphr_memcpy_htod(p->d_Tb, Tb, (size_t) np * nq * sizeof(double))
@@ -298,12 +315,12 @@ static int grow(double **buf, size_t *cap, size_t need)
 	return 0;
 }
 
-/* power_1d of the plan (float64, host) from the series on the GPU */
-static int td_power_1d(const double *ta, const double *tb, const double *window, int n, struct td_plan *p, double *power)
+/* power_1d of the plan (float64, host) from device copies of the series
+ * a, b (n samples) and window (n - 2 transient) */
+static int td_power_dev(const double *a, const double *b, const double *w, int n, struct td_plan *p, double *power)
 {
 	const int np = p->np, nq = p->nq, nt = n - 2 * p->pub.transient;
 	hipStream_t st = sphr_rt_stream();
-	size_t cap_a = p->cap_n, cap_b = p->cap_n, cap_w = p->cap_n;
 	/* K splits of about 2048 samples: enough workgroups on a short series,
 	 * partial sums that stay small on a long one */
 	const int tiles = (np + TD_TILE - 1) / TD_TILE;
@@ -312,26 +329,49 @@ static int td_power_1d(const double *ta, const double *tb, const double *window,
 		nz = 1;
 	const int tpz = ((nt + nz - 1) / nz + TD_KC - 1) / TD_KC * TD_KC;
 	nz = nt > 0 ? (nt + tpz - 1) / tpz : 1;
-	if(grow(&p->d_a, &cap_a, (size_t) n) || grow(&p->d_b, &cap_b, (size_t) n) || grow(&p->d_w, &cap_w, (size_t) (nt > 0 ? nt : 1)))
-		return -1;
-	p->cap_n = cap_a < cap_b ? (cap_a < cap_w ? cap_a : cap_w) : (cap_b < cap_w ? cap_b : cap_w);
 	if(grow(&p->d_part, &p->cap_part, (size_t) nz * np * np))
 		return -1;
-	if(sphr_memcpy_htod(p->d_a, ta, (size_t) n * sizeof(double)) || sphr_memcpy_htod(p->d_b, tb, (size_t) n * sizeof(double))
-	   || (nt > 0 && sphr_memcpy_htod(p->d_w, window, (size_t) nt * sizeof(double))))
-		return -1;
 	if(nt > 0)
-		hipLaunchKernelGGL(k_td_corr, dim3(tiles * tiles * nz), dim3(256), 0, st, p->d_a, p->d_b, p->d_w, nt, np, tpz, tiles, p->d_part);
+		hipLaunchKernelGGL(k_td_corr, dim3(tiles * tiles * nz), dim3(256), 0, st, a, b, w, nt, np, tpz, tiles, p->d_part);
 	else
 		(void) hipMemsetAsync(p->d_part, 0, (size_t) np * np * sizeof(double), st);
 	const long nc = (long) np * np;
 	hipLaunchKernelGGL(k_td_reduce, dim3((nc + 255) / 256), dim3(256), 0, st, p->d_part, nt > 0 ? nz : 1, nc, p->d_C);
 	if(launch_gemm<double, double>(np, nq, np, p->d_C, np, p->d_Tb, nq, p->d_Y, nq, st))
 		return -1;
-	hipLaunchKernelGGL(k_td_power, dim3(1), dim3(256), (size_t) nq * sizeof(double), st, p->d_Ta, p->d_Y, p->d_WP, np, nq, p->P, p->d_pow);
+	hipLaunchKernelGGL(k_td_u, dim3((nq + 63) / 64), dim3(256), 0, st, p->d_Ta, p->d_Y, np, nq, p->d_U);
+	hipLaunchKernelGGL(k_td_pw, dim3((p->P + 64) / 64), dim3(256), 0, st, p->d_U, p->d_WP, nq, p->P, p->d_pow);
 	if(sphr_rt_check(hipGetLastError(), "time-domain correlator"))
 		return -1;
 	return sphr_memcpy_dtoh(power, p->d_pow, (size_t) (p->P + 1) * sizeof(double));
+}
+
+/* the same from host series: copied into the plan's buffers first */
+static int td_power_1d(const double *ta, const double *tb, const double *window, int n, struct td_plan *p, double *power)
+{
+	const int nt = n - 2 * p->pub.transient;
+	size_t cap_a = p->cap_n, cap_b = p->cap_n, cap_w = p->cap_n;
+	if(grow(&p->d_a, &cap_a, (size_t) n) || grow(&p->d_b, &cap_b, (size_t) n) || grow(&p->d_w, &cap_w, (size_t) (nt > 0 ? nt : 1)))
+		return -1;
+	p->cap_n = cap_a < cap_b ? (cap_a < cap_w ? cap_a : cap_w) : (cap_b < cap_w ? cap_b : cap_w);
+	if(sphr_memcpy_htod(p->d_a, ta, (size_t) n * sizeof(double)) || sphr_memcpy_htod(p->d_b, tb, (size_t) n * sizeof(double))
+	   || (nt > 0 && sphr_memcpy_htod(p->d_w, window, (size_t) nt * sizeof(double))))
+		return -1;
+	return td_power_dev(p->d_a, p->d_b, p->d_w, n, p, power);
+}
+
+/* the baseline's sky from its power_1d: power_1d set on the plan, rotated
+ * to the baseline direction */
+static struct sh_series *td_finish(struct td_plan *p, const double *power)
+{
+	for(int d = 0; d <= p->P; d++)
+		sh_series_set(p->pub.power_1d, d, 0, power[d]);
+	struct sh_series *power_2d = sh_series_new(p->pub.power_1d->l_max, 0);
+	if(!power_2d || !sh_series_rotate(power_2d, p->pub.power_1d, p->pub.rotation_plan)) {
+		sh_series_free(power_2d);
+		return NULL;
+	}
+	return power_2d;
 }
 
 /* correlator_baseline_integrate_power_td (correlator.c:648-673) */
@@ -348,18 +388,8 @@ extern "C" struct sh_series *correlator_baseline_integrate_power_td(const double
 	pthread_mutex_lock(&sphr_dropin_lock);
 	const int rc = td_power_1d(time_series_a, time_series_b, window, n, p, power);
 	pthread_mutex_unlock(&sphr_dropin_lock);
-	if(rc) {
-		free(power);
-		return NULL;
-	}
-	for(int d = 0; d <= p->P; d++)
-		sh_series_set(plan->power_1d, d, 0, power[d]);
+	struct sh_series *power_2d = rc ? NULL : td_finish(p, power);
 	free(power);
-	struct sh_series *power_2d = sh_series_new(plan->power_1d->l_max, 0);
-	if(!power_2d || !sh_series_rotate(power_2d, plan->power_1d, plan->rotation_plan)) {
-		sh_series_free(power_2d);
-		return NULL;
-	}
 	return power_2d;
 }
 
@@ -398,24 +428,51 @@ extern "C" void correlator_network_plan_td_free(struct correlator_network_plan_t
 	free(plan);
 }
 
-/* correlator_network_integrate_power_td (correlator.c:967-984) */
+/* correlator_network_integrate_power_td (correlator.c:967-984); the
+ * series go to the device once for all baselines, in the first plan's
+ * buffers */
 extern "C" struct sh_series *correlator_network_integrate_power_td(struct sh_series *sky, double **tseries, int tseries_length, double **windows, struct correlator_network_plan_td *plan)
 {
-	if(!sky || !tseries || !windows || !plan)
+	if(!sky || !tseries || !windows || !plan || !plan->baselines->n_baselines)
 		return NULL;
-	sh_series_zero(sky);
-	int k = 0;
+	const int nb = plan->baselines->n_baselines;
 	const int n_inst = instrument_array_len(plan->baselines->baselines[0]->instruments);
-	for(int i = 1; i < n_inst; i++)
-		for(int j = 0; j < i; j++, k++) {
-			struct sh_series *power_2d = correlator_baseline_integrate_power_td(tseries[i], tseries[j], windows[k], tseries_length, plan->plans[k]);
-			if(!power_2d)
-				return NULL;
-			if(!sh_series_add(sky, 1.0 / plan->baselines->n_baselines, power_2d)) {
-				sh_series_free(power_2d);
-				return NULL;
-			}
+	int tmax = 0, Pmax = 0;
+	for(int k = 0; k < nb; k++) {
+		const struct td_plan *p = (const struct td_plan *) plan->plans[k];
+		tmax = p->pub.transient > tmax ? p->pub.transient : tmax;
+		Pmax = p->P > Pmax ? p->P : Pmax;
+	}
+	sh_series_zero(sky);
+	if(tseries_length - 2 * tmax < 0)
+		return NULL;
+	double *power = (double *) malloc((size_t) (Pmax + 1) * sizeof(double));
+	struct td_plan *p0 = (struct td_plan *) plan->plans[0];
+	struct sh_series *result = sky;
+	if(!power)
+		return NULL;
+	pthread_mutex_lock(&sphr_dropin_lock);
+	int rc = grow(&p0->d_nts, &p0->cap_nts, (size_t) n_inst * tseries_length) || grow(&p0->d_nws, &p0->cap_nws, (size_t) nb * tseries_length);
+	double *dts = p0->d_nts, *dws = p0->d_nws;
+	for(int i = 0; i < n_inst && !rc; i++)
+		rc = sphr_memcpy_htod(dts + (size_t) i * tseries_length, tseries[i], (size_t) tseries_length * sizeof(double));
+	for(int k = 0; k < nb && !rc; k++) {
+		const int nt = tseries_length - 2 * plan->plans[k]->transient;
+		rc = nt > 0 && sphr_memcpy_htod(dws + (size_t) k * tseries_length, windows[k], (size_t) nt * sizeof(double));
+	}
+	pthread_mutex_unlock(&sphr_dropin_lock);
+	int k = 0;
+	for(int i = 1; i < n_inst && !rc; i++)
+		for(int j = 0; j < i && !rc; j++, k++) {
+			struct td_plan *p = (struct td_plan *) plan->plans[k];
+			pthread_mutex_lock(&sphr_dropin_lock);
+			rc = td_power_dev(dts + (size_t) i * tseries_length, dts + (size_t) j * tseries_length, dws + (size_t) k * tseries_length, tseries_length, p, power);
+			pthread_mutex_unlock(&sphr_dropin_lock);
+			struct sh_series *power_2d = rc ? NULL : td_finish(p, power);
+			if(!power_2d || !sh_series_add(sky, 1.0 / nb, power_2d))
+				rc = -1;
 			sh_series_free(power_2d);
 		}
-	return sky;
+	free(power);
+	return rc ? NULL : result;
 }
