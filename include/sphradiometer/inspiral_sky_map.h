@@ -55,6 +55,33 @@ int preprocess_SNR_arrays(double complex *const *series, int n_series, int lengt
  * logprior's l_max.  0 on success, -1 on failure. */
 int generate_alm_skys_arrays(struct sh_series **skyp, struct sh_series **skyn, struct correlator_network_plan_fd *plansp, struct correlator_network_plan_fd *plansn, double complex *const *series, int length, double gmst, const struct sh_series *logprior);
 
+/*
+ * The precalc tree (inspiral_sky_map.c:1113-1811, declared at
+ * inspiral_sky_map.h:59-69 of the reference): the same folder layout and
+ * the same raw field order in each .dat file, FITS a_lm tables for the
+ * log prior and the autocorrelator projections.  0 / non-NULL on success;
+ * on failure -1 / NULL with a message for sphr_last_error().
+ */
+
+/* inspiral_sky_map.c:1458-1553; per-baseline folders for every baseline
+ * of n_ifo detectors (the reference makes n_ifo of them) */
+int make_precalc_directories(char *precalc_path, int n_ifo);
+int write_precalc_time_series_length(unsigned int length, char *parent_dir);
+int read_precalc_time_series_length(unsigned int *length, char *parent_dir);
+int write_precalc_logprior(const struct sh_series *series, char *parent_dir);
+struct sh_series *read_precalc_logprior(char *parent_dir);
+/* the two polarisations' plans share baselines and rotation plan */
+int write_precalc_correlator_network_plan_fd(const struct correlator_network_plan_fd *fdplansp, const struct correlator_network_plan_fd *fdplansn, char *parent_dir);
+/* the baselines are rebuilt from instruments and must reproduce the stored
+ * directions exactly */
+struct correlator_network_baselines *read_precalc_correlator_network_baselines(const struct instrument_array *instruments, char *parent_dir);
+/* fills caller-owned structures; on failure their plans are NULL */
+int read_precalc_correlator_network_plan_fd(struct correlator_network_plan_fd *fdplansp, struct correlator_network_plan_fd *fdplansn, const struct correlator_network_baselines *baselines, int tseries_length, char *parent_dir);
+int write_precalc_autocorrelator_network_plan_fd(const struct autocorrelator_network_plan_fd *fdplansp, const struct autocorrelator_network_plan_fd *fdplansn, char *parent_dir);
+int read_precalc_autocorrelator_network_plan_fd(struct autocorrelator_network_plan_fd *fdplansp, struct autocorrelator_network_plan_fd *fdplansn, const struct instrument_array *instruments, int tseries_length, char *parent_dir);
+/* inspiral_sky_map.c:678-691 */
+void autocorrelator_network_plan_fd_free(struct autocorrelator_network_plan_fd *plan);
+
 #ifdef __cplusplus
 #undef complex
 }

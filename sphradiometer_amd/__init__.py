@@ -99,6 +99,10 @@ class NetworkPlanFD(ctypes.Structure):
     _fields_ = [("baselines", P(NetworkBaselines)), ("plans", P(P(PlanFD)))]
 
 
+class AutoPlanFD(ctypes.Structure):
+    _fields_ = [("instruments", P(InstrumentArray)), ("projections", P(P(P(ShSeries)))), ("length", c_int)]
+
+
 class Detector(ctypes.Structure):
     """struct sphr_detector: LALDetector's leading fields."""
     _fields_ = [("location", c_double * 3), ("response", (ctypes.c_float * 3) * 3)]
@@ -238,6 +242,17 @@ _PROTOS = {
     "sphr_loc_free": (None, [c_void_p]),
     "sphr_loc_run": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sphr_loc_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "make_precalc_directories": (c_int, [ctypes.c_char_p, c_int]),
+    "write_precalc_time_series_length": (c_int, [c_uint, ctypes.c_char_p]),
+    "read_precalc_time_series_length": (c_int, [P(c_uint), ctypes.c_char_p]),
+    "write_precalc_logprior": (c_int, [P(ShSeries), ctypes.c_char_p]),
+    "read_precalc_logprior": (P(ShSeries), [ctypes.c_char_p]),
+    "write_precalc_correlator_network_plan_fd": (c_int, [P(NetworkPlanFD), P(NetworkPlanFD), ctypes.c_char_p]),
+    "read_precalc_correlator_network_baselines": (P(NetworkBaselines), [P(InstrumentArray), ctypes.c_char_p]),
+    "read_precalc_correlator_network_plan_fd": (c_int, [P(NetworkPlanFD), P(NetworkPlanFD), P(NetworkBaselines), c_int, ctypes.c_char_p]),
+    "write_precalc_autocorrelator_network_plan_fd": (c_int, [P(AutoPlanFD), P(AutoPlanFD), ctypes.c_char_p]),
+    "read_precalc_autocorrelator_network_plan_fd": (c_int, [P(AutoPlanFD), P(AutoPlanFD), P(InstrumentArray), c_int, ctypes.c_char_p]),
+    "autocorrelator_network_plan_fd_free": (None, [P(AutoPlanFD)]),
     "sh_series_write_healpix_map": (c_int, [P(ShSeries), ctypes.c_char_p]),
     "sh_series_write_healpix_alm": (c_int, [P(ShSeries), ctypes.c_char_p]),
     "sh_series_read_healpix_alm": (P(ShSeries), [ctypes.c_char_p]),

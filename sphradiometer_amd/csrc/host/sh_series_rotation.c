@@ -149,6 +149,11 @@ static struct sh_series_rotation_plan *plan_alloc(unsigned l_max)
 	return p;
 }
 
+struct sh_series_rotation_plan *sphr_rotation_plan_alloc(unsigned l_max)
+{
+	return plan_alloc(l_max);
+}
+
 static double complex *plan_base(const struct sh_series_rotation_plan *p)
 {
 	return p->l_max ? p->D[1] - 4 : NULL;

@@ -45,6 +45,10 @@ void sphr_tukey_window(int n, double beta, double *w);
 struct sh_series_rotation_plan;
 const double *sphr_rotation_blocks(const struct sh_series_rotation_plan *);
 
+/* a rotation plan of order l_max with its D^l blocks allocated (contents
+ * undefined), as sh_series_rotation_plan_free releases them */
+struct sh_series_rotation_plan *sphr_rotation_plan_alloc(unsigned l_max);
+
 /* ---- GPU side (runtime.hip, dropin.hip) ---- */
 
 /* 0 on success; on failure -1 with a message for sphr_last_error() */

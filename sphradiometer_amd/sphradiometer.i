@@ -48,6 +48,7 @@
 
 %pointer_functions(unsigned int, uintp);
 %pointer_functions(struct correlator_network_plan_fd, correlator_network_plan_fdp);
+%pointer_functions(struct autocorrelator_network_plan_fd, autocorrelator_network_plan_fdp);
 %pointer_functions(struct sh_series, sh_seriesp);
 %pointer_functions(struct sh_series *, sh_seriespp);
 

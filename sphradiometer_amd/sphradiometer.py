@@ -9,7 +9,7 @@ their fields read like the SWIG proxies' (``plan.contents.delay_product``).
 """
 import ctypes
 
-from sphradiometer_amd import (CDouble, NetworkPlanFD, PlanFD, ShSeries, _PROTOS, lib)
+from sphradiometer_amd import (AutoPlanFD, CDouble, NetworkPlanFD, PlanFD, ShSeries, _PROTOS, lib)
 
 _L = lib()
 
@@ -51,15 +51,28 @@ _array_functions(CDouble, "double_complex_array")
 _array_functions(ctypes.POINTER(CDouble), "double_complexp_array")
 
 
+# SWIG's %pointer_functions allocate with calloc and release with free, so
+# a structure made by new_*p can be handed to the library's _free functions
+_libc = ctypes.CDLL(None)
+_libc.calloc.restype = ctypes.c_void_p
+_libc.calloc.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
+_libc.free.argtypes = [ctypes.c_void_p]
+
+
 def _pointer_functions(ctype, name):
     def new():
-        return ctypes.pointer(ctype())
+        m = _libc.calloc(1, ctypes.sizeof(ctype))
+        if not m:
+            raise MemoryError("new_" + name)
+        return ctypes.cast(m, ctypes.POINTER(ctype))
 
     def copy(v):
-        return ctypes.pointer(ctype(v) if not isinstance(v, ctype) else v)
+        p = new()
+        p[0] = v if isinstance(v, ctype) or not issubclass(ctype, ctypes.Structure) else ctype(v)
+        return p
 
     def delete(p):
-        return None
+        _libc.free(ctypes.cast(p, ctypes.c_void_p))
 
     def assign(p, v):
         p[0] = v
@@ -76,5 +89,6 @@ def _pointer_functions(ctype, name):
 
 _pointer_functions(ctypes.c_uint, "uintp")
 _pointer_functions(NetworkPlanFD, "correlator_network_plan_fdp")
+_pointer_functions(AutoPlanFD, "autocorrelator_network_plan_fdp")
 _pointer_functions(ShSeries, "sh_seriesp")
 _pointer_functions(ctypes.POINTER(ShSeries), "sh_seriespp")
