@@ -229,7 +229,7 @@ static size_t block_offset_host(int l)
 }
 
 /* D^l_{0 m'} rows of packed blocks, in the coefficient layout of order L */
-static void polar_rows(int L, const double *blocks, double *rows)
+void sphr_polar_rows(int L, const double *blocks, double *rows)
 {
 	rows[0] = 1.0;
 	rows[1] = 0.0;
@@ -323,7 +323,7 @@ extern "C" int sphr_gpu_rotate(unsigned l_max, int polar, const double *coeff, c
 		rows = (double *) malloc(2 * nc * sizeof(double));
 		if(!rows)
 			goto fail;
-		polar_rows(L, blocks, rows);
+		sphr_polar_rows(L, blocks, rows);
 		CK(hipMemcpyAsync(dbl, rows, nc * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy rotation rows");
 	} else if(nbl && L > 0) {
 		CK(hipMemcpyAsync(dbl, blocks, nbl * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy rotation blocks");
@@ -439,7 +439,7 @@ extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int
 				goto fail;
 			rows = r;
 			if(Lb > 0)
-				polar_rows(Lb, bl[b].blocks, rows);
+				sphr_polar_rows(Lb, bl[b].blocks, rows);
 			else {
 				rows[0] = 1.0;
 				rows[1] = 0.0;

@@ -120,6 +120,10 @@ int launch_gemm(int M, int N, int K, const TA *A, long lda, const T *B, long ldb
 int launch_xspec(const cplx<double> *const *fs_dev, int n, int nb, const int *ia, const int *ib, cplx<double> *fsp, hipStream_t st);
 int launch_dotc(const cplx<double> *coeff, int n, int stride, const cplx<double> *vec, const double *vec_real, double scale, cplx<double> *partial, int nsplit, cplx<double> *out, hipStream_t st);
 int launch_rotsum(int nb, const cplx<double> *const *power, const int *Lb, const int *polar, const cplx<double> *const *blocks, int L, double scale, cplx<double> *sky, hipStream_t st);
+/* the rows D^l_{0 m'} of packed rotation blocks (l = 1 first, each
+ * (2l+1)^2) in the coefficient layout of order L, as k_rotsum reads them
+ * for a polar power; rows holds 2 (L + 1)^2 doubles */
+void sphr_polar_rows(int L, const double *blocks, double *rows);
 int launch_rotate_z(int L, const cplx<double> *in, double phi, cplx<double> *out, hipStream_t st);
 int launch_fft_single(const double *in, int real_input, int N, const cplx<double> *tw, cplx<double> *out, hipStream_t st);
 int launch_dft_direct(const double *in, int real_input, int N, const cplx<double> *tw, cplx<double> *out, hipStream_t st);

@@ -18,7 +18,10 @@
  * the windowed correlation matrix C (n_p x n_p, a GEMM over t with the
  * Hankel operands generated from the series in the tile loader, float64
  * MFMA), then Y = C T^b (the library GEMM) and the bilinear and d sums.
- * The rotation to the baseline direction is the drop-in sh_series_rotate.
+ * The rotation to the baseline direction and the network average run on
+ * the device (k_rotsum) from the rows D^l_{0 m'} of the plan's rotation
+ * plan, uploaded once when the plan is made; a call copies back only
+ * power_1d and the rotated (or summed) series.
  *
  * The plan keeps the reference's struct (proj_a, proj_b, sample_a,
  * sample_b, product, power_1d, product_plan, rotation_plan) as the first
@@ -45,7 +48,11 @@ struct td_plan {
 	double *d_WP;			/* [nq][P + 1]: 2 pi w_q Pbar_d0(x_q) */
 	/* per-call device buffers, grown as needed */
 	size_t cap_n, cap_part;
-	double *d_a, *d_b, *d_w, *d_part, *d_C, *d_Y, *d_U, *d_pow;
+	double *d_a, *d_b, *d_w, *d_part, *d_C, *d_Y, *d_U;
+	double *d_powc;			/* power_1d as P + 1 complex (imaginary parts 0) */
+	double *d_rows;			/* (P + 1)^2 complex: D^l_{0 m'} of the rotation plan */
+	size_t cap_out;
+	double *d_out;			/* rotated series (or a network's partial sums) */
 	/* a network's series and windows (kept on its first plan) */
 	size_t cap_nts, cap_nws;
 	double *d_nts, *d_nws;
@@ -170,8 +177,8 @@ __global__ void __launch_bounds__(256) k_td_u(const double *__restrict__ Ta, con
 		U[q] = (red[0][qi] + red[1][qi]) + (red[2][qi] + red[3][qi]);
 }
 
-/* power[d] = sum_q U_q WP[q][d]: 64 d per workgroup, the q sum split
- * four ways and reduced in LDS */
+/* power[d] = sum_q U_q WP[q][d] (stored as the complex power[d] + 0 i):
+ * 64 d per workgroup, the q sum split four ways and reduced in LDS */
 __global__ void __launch_bounds__(256) k_td_pw(const double *__restrict__ U, const double *__restrict__ WP, int nq, int P, double *__restrict__ power)
 {
 	__shared__ double red[4][64];
@@ -182,8 +189,10 @@ __global__ void __launch_bounds__(256) k_td_pw(const double *__restrict__ U, con
 			s = fma(U[q], WP[(size_t) q * (P + 1) + d], s);
 	red[qs][di] = s;
 	__syncthreads();
-	if(qs == 0 && d <= P)
-		power[d] = (red[0][di] + red[1][di]) + (red[2][di] + red[3][di]);
+	if(qs == 0 && d <= P) {
+		power[2 * d] = (red[0][di] + red[1][di]) + (red[2][di] + red[3][di]);
+		power[2 * d + 1] = 0.0;
+	}
 }
 
 }	/* namespace */
@@ -192,7 +201,7 @@ __global__ void __launch_bounds__(256) k_td_pw(const double *__restrict__ U, con
 
 static void td_dev_free(struct td_plan *p)
 {
-	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_Y, p->d_U, p->d_pow, p->d_nts, p->d_nws };
+	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_Y, p->d_U, p->d_powc, p->d_rows, p->d_out, p->d_nts, p->d_nws };
 	for(unsigned i = 0; i < sizeof(b) / sizeof(b[0]); i++)
 		if(b[i])
 			(void) hipFree(b[i]);
@@ -206,7 +215,7 @@ extern "C" struct correlator_plan_td *correlator_plan_td_new(const struct correl
 	struct td_plan *p = (struct td_plan *) calloc(1, sizeof(*p));
 	gsl_vector *dp = gsl_vector_alloc(3);
 	double *R = sh_series_rot_matrix(baseline->theta, baseline->phi);
-	double *Ta = NULL, *Tb = NULL, *WP = NULL;
+	double *Ta = NULL, *Tb = NULL, *WP = NULL, *rows = NULL;
 	if(!p || !dp || !R)
 		goto fail;
 	{
@@ -253,15 +262,19 @@ extern "C" struct correlator_plan_td *correlator_plan_td_new(const struct correl
 		Ta = (double *) malloc((size_t) np * nq * sizeof(double));
 		Tb = (double *) malloc((size_t) np * nq * sizeof(double));
 		WP = (double *) malloc((size_t) nq * (P + 1) * sizeof(double));
-		if(!Ta || !Tb || !WP || sphr_td_tables(t->proj_a, t->proj_b, nq, (int) P, Ta, Tb, WP))
+		rows = (double *) malloc(2 * (size_t) (P + 1) * (P + 1) * sizeof(double));
+		if(!Ta || !Tb || !WP || !rows || sphr_td_tables(t->proj_a, t->proj_b, nq, (int) P, Ta, Tb, WP))
 			goto fail;
+		sphr_polar_rows((int) P, sphr_rotation_blocks(t->rotation_plan), rows);
 		if(sphr_rt_check(hipMalloc(&p->d_Ta, (size_t) np * nq * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_Tb, (size_t) np * nq * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_WP, (size_t) nq * (P + 1) * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_C, (size_t) np * np * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_Y, (size_t) np * nq * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_U, (size_t) nq * sizeof(double)), "hipMalloc")
-		   || sphr_rt_check(hipMalloc(&p->d_pow, (size_t) (P + 1) * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_powc, 2 * (size_t) (P + 1) * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_rows, 2 * (size_t) (P + 1) * (P + 1) * sizeof(double)), "hipMalloc")
+		   || sphr_memcpy_htod(p->d_rows, rows, 2 * (size_t) (P + 1) * (P + 1) * sizeof(double))
 		   || sphr_memcpy_htod(p->d_Ta, Ta, (size_t) np * nq * sizeof(double))
 		   || sphr_memcpy_htod(p->d_Tb, Tb, (size_t) np * nq * sizeof(double))
 		   || sphr_memcpy_htod(p->d_WP, WP, (size_t) nq * (P + 1) * sizeof(double)))
@@ -270,6 +283,7 @@ extern "C" struct correlator_plan_td *correlator_plan_td_new(const struct correl
 	free(Ta);
 	free(Tb);
 	free(WP);
+	free(rows);
 	free(R);
 	gsl_vector_free(dp);
 	return &p->pub;
@@ -277,6 +291,7 @@ fail:
 	free(Ta);
 	free(Tb);
 	free(WP);
+	free(rows);
 	free(R);
 	if(dp)
 		gsl_vector_free(dp);
@@ -315,9 +330,9 @@ static int grow(double **buf, size_t *cap, size_t need)
 	return 0;
 }
 
-/* power_1d of the plan (float64, host) from device copies of the series
- * a, b (n samples) and window (n - 2 transient) */
-static int td_power_dev(const double *a, const double *b, const double *w, int n, struct td_plan *p, double *power)
+/* the plan's power (complex, in d_powc) from device copies of the series
+ * a, b (n samples) and window (n - 2 transient); launches only */
+static int td_power_dev(const double *a, const double *b, const double *w, int n, struct td_plan *p)
 {
 	const int np = p->np, nq = p->nq, nt = n - 2 * p->pub.transient;
 	hipStream_t st = sphr_rt_stream();
@@ -340,14 +355,12 @@ static int td_power_dev(const double *a, const double *b, const double *w, int n
 	if(launch_gemm<double, double>(np, nq, np, p->d_C, np, p->d_Tb, nq, p->d_Y, nq, st))
 		return -1;
 	hipLaunchKernelGGL(k_td_u, dim3((nq + 63) / 64), dim3(256), 0, st, p->d_Ta, p->d_Y, np, nq, p->d_U);
-	hipLaunchKernelGGL(k_td_pw, dim3((p->P + 64) / 64), dim3(256), 0, st, p->d_U, p->d_WP, nq, p->P, p->d_pow);
-	if(sphr_rt_check(hipGetLastError(), "time-domain correlator"))
-		return -1;
-	return sphr_memcpy_dtoh(power, p->d_pow, (size_t) (p->P + 1) * sizeof(double));
+	hipLaunchKernelGGL(k_td_pw, dim3((p->P + 64) / 64), dim3(256), 0, st, p->d_U, p->d_WP, nq, p->P, p->d_powc);
+	return sphr_rt_check(hipGetLastError(), "time-domain correlator") ? -1 : 0;
 }
 
 /* the same from host series: copied into the plan's buffers first */
-static int td_power_1d(const double *ta, const double *tb, const double *window, int n, struct td_plan *p, double *power)
+static int td_power_1d(const double *ta, const double *tb, const double *window, int n, struct td_plan *p)
 {
 	const int nt = n - 2 * p->pub.transient;
 	size_t cap_a = p->cap_n, cap_b = p->cap_n, cap_w = p->cap_n;
@@ -357,21 +370,18 @@ static int td_power_1d(const double *ta, const double *tb, const double *window,
 	if(sphr_memcpy_htod(p->d_a, ta, (size_t) n * sizeof(double)) || sphr_memcpy_htod(p->d_b, tb, (size_t) n * sizeof(double))
 	   || (nt > 0 && sphr_memcpy_htod(p->d_w, window, (size_t) nt * sizeof(double))))
 		return -1;
-	return td_power_dev(p->d_a, p->d_b, p->d_w, n, p, power);
+	return td_power_dev(p->d_a, p->d_b, p->d_w, n, p);
 }
 
-/* the baseline's sky from its power_1d: power_1d set on the plan, rotated
- * to the baseline direction */
-static struct sh_series *td_finish(struct td_plan *p, const double *power)
+/* power_1d of the plan set from the device power; buf holds 2 (P + 1)
+ * doubles */
+static int td_fetch_power(struct td_plan *p, double *buf)
 {
+	if(sphr_memcpy_dtoh(buf, p->d_powc, 2 * (size_t) (p->P + 1) * sizeof(double)))
+		return -1;
 	for(int d = 0; d <= p->P; d++)
-		sh_series_set(p->pub.power_1d, d, 0, power[d]);
-	struct sh_series *power_2d = sh_series_new(p->pub.power_1d->l_max, 0);
-	if(!power_2d || !sh_series_rotate(power_2d, p->pub.power_1d, p->pub.rotation_plan)) {
-		sh_series_free(power_2d);
-		return NULL;
-	}
-	return power_2d;
+		sh_series_set(p->pub.power_1d, d, 0, buf[2 * d]);
+	return 0;
 }
 
 /* correlator_baseline_integrate_power_td (correlator.c:648-673) */
@@ -382,14 +392,29 @@ extern "C" struct sh_series *correlator_baseline_integrate_power_td(const double
 	if(n - 2 * plan->transient < 0)
 		return NULL;
 	struct td_plan *p = (struct td_plan *) plan;
-	double *power = (double *) malloc((size_t) (p->P + 1) * sizeof(double));
-	if(!power)
-		return NULL;
+	const int P = p->P;
+	const size_t nsky = (size_t) (P + 1) * (P + 1);
+	double *buf = (double *) malloc(2 * (size_t) (P + 1) * sizeof(double));
+	struct sh_series *power_2d = sh_series_new(P, 0);
+	int rc = !buf || !power_2d;
 	pthread_mutex_lock(&sphr_dropin_lock);
-	const int rc = td_power_1d(time_series_a, time_series_b, window, n, p, power);
+	if(!rc)
+		rc = td_power_1d(time_series_a, time_series_b, window, n, p) || grow(&p->d_out, &p->cap_out, 2 * nsky);
+	if(!rc) {
+		const cplx<double> *pw[1] = { (const cplx<double> *) p->d_powc }, *bk[1] = { (const cplx<double> *) p->d_rows };
+		const int Lb[1] = { P }, pol[1] = { 1 };
+		rc = launch_rotsum(1, pw, Lb, pol, bk, P, 1.0, (cplx<double> *) p->d_out, sphr_rt_stream());
+		if(rc)
+			sphr_set_error("rotation launch failed");
+	}
+	if(!rc)
+		rc = td_fetch_power(p, buf) || sphr_memcpy_dtoh(power_2d->coeff, p->d_out, 2 * nsky * sizeof(double));
 	pthread_mutex_unlock(&sphr_dropin_lock);
-	struct sh_series *power_2d = rc ? NULL : td_finish(p, power);
-	free(power);
+	free(buf);
+	if(rc) {
+		sh_series_free(power_2d);
+		return NULL;
+	}
 	return power_2d;
 }
 
@@ -428,9 +453,12 @@ extern "C" void correlator_network_plan_td_free(struct correlator_network_plan_t
 	free(plan);
 }
 
-/* correlator_network_integrate_power_td (correlator.c:967-984); the
+/* correlator_network_integrate_power_td (correlator.c:967-984).  The
  * series go to the device once for all baselines, in the first plan's
- * buffers */
+ * buffers; each baseline's power stays on the device, and one k_rotsum
+ * launch per SPHR_MAX_BASELINES baselines rotates and averages them.  As
+ * in the reference (sh_series_add), a sky of lower order than a baseline's
+ * power, or a polar sky, fails after the powers are computed. */
 extern "C" struct sh_series *correlator_network_integrate_power_td(struct sh_series *sky, double **tseries, int tseries_length, double **windows, struct correlator_network_plan_td *plan)
 {
 	if(!sky || !tseries || !windows || !plan || !plan->baselines->n_baselines)
@@ -446,13 +474,16 @@ extern "C" struct sh_series *correlator_network_integrate_power_td(struct sh_ser
 	sh_series_zero(sky);
 	if(tseries_length - 2 * tmax < 0)
 		return NULL;
-	double *power = (double *) malloc((size_t) (Pmax + 1) * sizeof(double));
+	const int L = (int) sky->l_max;
+	const size_t nsky = (size_t) (L + 1) * (L + 1);
+	const int nchunk = (nb + SPHR_MAX_BASELINES - 1) / SPHR_MAX_BASELINES;
+	double *buf = (double *) malloc(2 * (size_t) (Pmax + 1) * sizeof(double));
+	double *part = nchunk > 1 ? (double *) malloc(2 * nsky * sizeof(double)) : NULL;
 	struct td_plan *p0 = (struct td_plan *) plan->plans[0];
-	struct sh_series *result = sky;
-	if(!power)
-		return NULL;
+	int rc = !buf || (nchunk > 1 && !part);
 	pthread_mutex_lock(&sphr_dropin_lock);
-	int rc = grow(&p0->d_nts, &p0->cap_nts, (size_t) n_inst * tseries_length) || grow(&p0->d_nws, &p0->cap_nws, (size_t) nb * tseries_length);
+	if(!rc)
+		rc = grow(&p0->d_nts, &p0->cap_nts, (size_t) n_inst * tseries_length) || grow(&p0->d_nws, &p0->cap_nws, (size_t) nb * tseries_length);
 	double *dts = p0->d_nts, *dws = p0->d_nws;
 	for(int i = 0; i < n_inst && !rc; i++)
 		rc = sphr_memcpy_htod(dts + (size_t) i * tseries_length, tseries[i], (size_t) tseries_length * sizeof(double));
@@ -460,19 +491,42 @@ extern "C" struct sh_series *correlator_network_integrate_power_td(struct sh_ser
 		const int nt = tseries_length - 2 * plan->plans[k]->transient;
 		rc = nt > 0 && sphr_memcpy_htod(dws + (size_t) k * tseries_length, windows[k], (size_t) nt * sizeof(double));
 	}
-	pthread_mutex_unlock(&sphr_dropin_lock);
 	int k = 0;
 	for(int i = 1; i < n_inst && !rc; i++)
-		for(int j = 0; j < i && !rc; j++, k++) {
-			struct td_plan *p = (struct td_plan *) plan->plans[k];
-			pthread_mutex_lock(&sphr_dropin_lock);
-			rc = td_power_dev(dts + (size_t) i * tseries_length, dts + (size_t) j * tseries_length, dws + (size_t) k * tseries_length, tseries_length, p, power);
-			pthread_mutex_unlock(&sphr_dropin_lock);
-			struct sh_series *power_2d = rc ? NULL : td_finish(p, power);
-			if(!power_2d || !sh_series_add(sky, 1.0 / nb, power_2d))
-				rc = -1;
-			sh_series_free(power_2d);
+		for(int j = 0; j < i && !rc; j++, k++)
+			rc = td_power_dev(dts + (size_t) i * tseries_length, dts + (size_t) j * tseries_length, dws + (size_t) k * tseries_length, tseries_length, (struct td_plan *) plan->plans[k]);
+	for(k = 0; k < nb && !rc; k++)
+		rc = td_fetch_power((struct td_plan *) plan->plans[k], buf);
+	if(!rc && (sky->polar || L < Pmax))
+		rc = -1;
+	if(!rc)
+		rc = grow(&p0->d_out, &p0->cap_out, 2 * nsky * nchunk);
+	for(int c = 0; c < nchunk && !rc; c++) {
+		const cplx<double> *pw[SPHR_MAX_BASELINES], *bk[SPHR_MAX_BASELINES];
+		int Lb[SPHR_MAX_BASELINES], pol[SPHR_MAX_BASELINES], m = 0;
+		for(k = c * SPHR_MAX_BASELINES; k < nb && m < SPHR_MAX_BASELINES; k++, m++) {
+			const struct td_plan *p = (const struct td_plan *) plan->plans[k];
+			pw[m] = (const cplx<double> *) p->d_powc;
+			bk[m] = (const cplx<double> *) p->d_rows;
+			Lb[m] = p->P;
+			pol[m] = 1;
 		}
-	free(power);
-	return rc ? NULL : result;
+		rc = launch_rotsum(m, pw, Lb, pol, bk, L, 1.0 / nb, (cplx<double> *) p0->d_out + nsky * c, sphr_rt_stream());
+		if(rc)
+			sphr_set_error("rotation launch failed");
+	}
+	for(int c = 0; c < nchunk && !rc; c++) {
+		double *dst = c ? part : (double *) sky->coeff;
+		rc = sphr_memcpy_dtoh(dst, p0->d_out + 2 * nsky * c, 2 * nsky * sizeof(double));
+		for(size_t i = 0; c && !rc && i < 2 * nsky; i++)
+			((double *) sky->coeff)[i] += part[i];
+	}
+	pthread_mutex_unlock(&sphr_dropin_lock);
+	free(buf);
+	free(part);
+	if(rc) {
+		sh_series_zero(sky);
+		return NULL;
+	}
+	return sky;
 }
