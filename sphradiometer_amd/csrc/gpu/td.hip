@@ -17,7 +17,8 @@
  * the same arithmetic up to rounding, with the per-sample work folded into
  * the windowed correlation matrix C (n_p x n_p, a GEMM over t with the
  * Hankel operands generated from the series in the tile loader, float64
- * MFMA), then Y = C T^b (the library GEMM) and the bilinear and d sums.
+ * MFMA), then Y = C T^b and its bilinear form with T^a in one pass
+ * (k_td_yu) and the d sums (k_td_pw).
  * The rotation to the baseline direction and the network average run on
  * the device (k_rotsum) from the rows D^l_{0 m'} of the plan's rotation
  * plan, uploaded once when the plan is made; a call copies back only
@@ -48,7 +49,8 @@ struct td_plan {
 	double *d_WP;			/* [nq][P + 1]: 2 pi w_q Pbar_d0(x_q) */
 	/* per-call device buffers, grown as needed */
 	size_t cap_n, cap_part;
-	double *d_a, *d_b, *d_w, *d_part, *d_C, *d_Y, *d_U;
+	double *d_a, *d_b, *d_w, *d_part, *d_C;
+	double *d_U;			/* [ceil(np / 16)][nq]: partial bilinear forms */
 	double *d_powc;			/* power_1d as P + 1 complex (imaginary parts 0) */
 	double *d_rows;			/* (P + 1)^2 complex: D^l_{0 m'} of the rotation plan */
 	size_t cap_out;
@@ -57,9 +59,6 @@ struct td_plan {
 	size_t cap_nts, cap_nws;
 	double *d_nts, *d_nws;
 };
-
-template <typename T, typename TA>
-int launch_gemm(int M, int N, int K, const TA *A, long lda, const T *B, long ldb, T *C, long ldc, hipStream_t st);
 
 namespace {
 
@@ -161,36 +160,80 @@ __global__ void k_td_reduce(const double *__restrict__ part, int nz, long n, dou
 	C[i] = s;
 }
 
-/* U_q = sum_k Ta[k][q] Y[k][q]: 64 q per workgroup, the k sum split
- * four ways and reduced in LDS */
-__global__ void __launch_bounds__(256) k_td_u(const double *__restrict__ Ta, const double *__restrict__ Y, int np, int nq, double *__restrict__ U)
+/* Upart[kb][q] = sum_{k in block kb} Ta[k][q] sum_k' C[k][k'] Tb[k'][q]:
+ * Y = C Tb and its bilinear form with Ta in one pass, for 16 k (block kb)
+ * by 64 q per workgroup; each wave takes 4 k, each lane one q.  The 16 C
+ * rows go through LDS in chunks of TD_YU_KC (every lane of a wave reads
+ * the same element: a broadcast), the Tb row reads are coalesced over q. */
+#define TD_YU_KC 256
+/* the product rule's node count that k_td_pw's LDS holds */
+#define TD_NQ_MAX 7680
+__global__ void __launch_bounds__(256) k_td_yu(const double *__restrict__ C, const double *__restrict__ Ta, const double *__restrict__ Tb, int np, int nq, double *__restrict__ Upart)
 {
+	__shared__ double Cs[16][TD_YU_KC];
 	__shared__ double red[4][64];
-	const int qi = threadIdx.x & 63, ks = threadIdx.x >> 6, q = blockIdx.x * 64 + qi;
+	const int qi = threadIdx.x & 63, kw = threadIdx.x >> 6;
+	const int q = blockIdx.x * 64 + qi, kb = blockIdx.y, k0 = kb * 16 + kw * 4;
+	double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+	for(int c0 = 0; c0 < np; c0 += TD_YU_KC) {
+		const int nc = np - c0 < TD_YU_KC ? np - c0 : TD_YU_KC;
+		__syncthreads();
+		for(int e = threadIdx.x; e < 16 * TD_YU_KC; e += 256) {
+			const int r = e / TD_YU_KC, c = e % TD_YU_KC, k = kb * 16 + r;
+			Cs[r][c] = (k < np && c < nc) ? C[(size_t) k * np + c0 + c] : 0.0;
+		}
+		__syncthreads();
+		if(q < nq) {
+			const double *tb = Tb + (size_t) c0 * nq + q;
+#pragma unroll 4
+			for(int c = 0; c < nc; c++) {
+				const double b = tb[(size_t) c * nq];
+				y0 = fma(Cs[kw * 4 + 0][c], b, y0);
+				y1 = fma(Cs[kw * 4 + 1][c], b, y1);
+				y2 = fma(Cs[kw * 4 + 2][c], b, y2);
+				y3 = fma(Cs[kw * 4 + 3][c], b, y3);
+			}
+		}
+	}
 	double u = 0.0;
-	if(q < nq)
-		for(int k = ks; k < np; k += 4)
-			u = fma(Ta[(size_t) k * nq + q], Y[(size_t) k * nq + q], u);
-	red[ks][qi] = u;
+	if(q < nq) {
+		const double y[4] = { y0, y1, y2, y3 };
+		for(int j = 0; j < 4; j++)
+			if(k0 + j < np)
+				u = fma(Ta[(size_t) (k0 + j) * nq + q], y[j], u);
+	}
+	red[kw][qi] = u;
 	__syncthreads();
-	if(ks == 0 && q < nq)
-		U[q] = (red[0][qi] + red[1][qi]) + (red[2][qi] + red[3][qi]);
+	if(kw == 0 && q < nq)
+		Upart[(size_t) kb * nq + q] = (red[0][qi] + red[1][qi]) + (red[2][qi] + red[3][qi]);
 }
 
-/* power[d] = sum_q U_q WP[q][d] (stored as the complex power[d] + 0 i):
- * 64 d per workgroup, the q sum split four ways and reduced in LDS */
-__global__ void __launch_bounds__(256) k_td_pw(const double *__restrict__ U, const double *__restrict__ WP, int nq, int P, double *__restrict__ power)
+/* power[d] = sum_q U_q WP[q][d] with U_q = sum_kb Upart[kb][q] (stored as
+ * the complex power[d] + 0 i): 16 d per workgroup, the q sum split 16
+ * ways; every workgroup first sums U into LDS (nq doubles, dynamic) */
+__global__ void __launch_bounds__(256) k_td_pw(const double *__restrict__ Upart, int nkb, const double *__restrict__ WP, int nq, int P, double *__restrict__ power)
 {
-	__shared__ double red[4][64];
-	const int di = threadIdx.x & 63, qs = threadIdx.x >> 6, d = blockIdx.x * 64 + di;
+	extern __shared__ double Us[];
+	__shared__ double red[16][16];
+	for(int q = threadIdx.x; q < nq; q += 256) {
+		double u = 0.0;
+		for(int kb = 0; kb < nkb; kb++)
+			u += Upart[(size_t) kb * nq + q];
+		Us[q] = u;
+	}
+	__syncthreads();
+	const int di = threadIdx.x & 15, qs = threadIdx.x >> 4, d = blockIdx.x * 16 + di;
 	double s = 0.0;
 	if(d <= P)
-		for(int q = qs; q < nq; q += 4)
-			s = fma(U[q], WP[(size_t) q * (P + 1) + d], s);
+		for(int q = qs; q < nq; q += 16)
+			s = fma(Us[q], WP[(size_t) q * (P + 1) + d], s);
 	red[qs][di] = s;
 	__syncthreads();
 	if(qs == 0 && d <= P) {
-		power[2 * d] = (red[0][di] + red[1][di]) + (red[2][di] + red[3][di]);
+		double t = 0.0;
+		for(int g = 0; g < 16; g++)
+			t += red[g][di];
+		power[2 * d] = t;
 		power[2 * d + 1] = 0.0;
 	}
 }
@@ -201,7 +244,7 @@ __global__ void __launch_bounds__(256) k_td_pw(const double *__restrict__ U, con
 
 static void td_dev_free(struct td_plan *p)
 {
-	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_Y, p->d_U, p->d_powc, p->d_rows, p->d_out, p->d_nts, p->d_nws };
+	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_U, p->d_powc, p->d_rows, p->d_out, p->d_nts, p->d_nws };
 	for(unsigned i = 0; i < sizeof(b) / sizeof(b[0]); i++)
 		if(b[i])
 			(void) hipFree(b[i]);
@@ -256,6 +299,11 @@ extern "C" struct correlator_plan_td *correlator_plan_td_new(const struct correl
 			goto fail;
 		}
 		const int np = t->proj_a->n, nq = (int) ((a_l + b_l + P) / 2 + 1);
+		/* k_td_pw holds the nq bilinear forms in LDS */
+		if(nq > TD_NQ_MAX) {
+			sphr_set_error("correlator_plan_td_new: %d product nodes exceed %d", nq, TD_NQ_MAX);
+			goto fail;
+		}
 		p->np = np;
 		p->nq = nq;
 		p->P = (int) P;
@@ -270,8 +318,7 @@ extern "C" struct correlator_plan_td *correlator_plan_td_new(const struct correl
 		   || sphr_rt_check(hipMalloc(&p->d_Tb, (size_t) np * nq * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_WP, (size_t) nq * (P + 1) * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_C, (size_t) np * np * sizeof(double)), "hipMalloc")
-		   || sphr_rt_check(hipMalloc(&p->d_Y, (size_t) np * nq * sizeof(double)), "hipMalloc")
-		   || sphr_rt_check(hipMalloc(&p->d_U, (size_t) nq * sizeof(double)), "hipMalloc")
+		   || sphr_rt_check(hipMalloc(&p->d_U, (size_t) ((np + 15) / 16) * nq * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_powc, 2 * (size_t) (P + 1) * sizeof(double)), "hipMalloc")
 		   || sphr_rt_check(hipMalloc(&p->d_rows, 2 * (size_t) (P + 1) * (P + 1) * sizeof(double)), "hipMalloc")
 		   || sphr_memcpy_htod(p->d_rows, rows, 2 * (size_t) (P + 1) * (P + 1) * sizeof(double))
@@ -352,10 +399,9 @@ static int td_power_dev(const double *a, const double *b, const double *w, int n
 		(void) hipMemsetAsync(p->d_part, 0, (size_t) np * np * sizeof(double), st);
 	const long nc = (long) np * np;
 	hipLaunchKernelGGL(k_td_reduce, dim3((nc + 255) / 256), dim3(256), 0, st, p->d_part, nt > 0 ? nz : 1, nc, p->d_C);
-	if(launch_gemm<double, double>(np, nq, np, p->d_C, np, p->d_Tb, nq, p->d_Y, nq, st))
-		return -1;
-	hipLaunchKernelGGL(k_td_u, dim3((nq + 63) / 64), dim3(256), 0, st, p->d_Ta, p->d_Y, np, nq, p->d_U);
-	hipLaunchKernelGGL(k_td_pw, dim3((p->P + 64) / 64), dim3(256), 0, st, p->d_U, p->d_WP, nq, p->P, p->d_powc);
+	const int nkb = (np + 15) / 16;
+	hipLaunchKernelGGL(k_td_yu, dim3((nq + 63) / 64, nkb), dim3(256), 0, st, p->d_C, p->d_Ta, p->d_Tb, np, nq, p->d_U);
+	hipLaunchKernelGGL(k_td_pw, dim3((p->P + 16) / 16), dim3(256), (size_t) nq * sizeof(double), st, p->d_U, nkb, p->d_WP, nq, p->P, p->d_powc);
 	return sphr_rt_check(hipGetLastError(), "time-domain correlator") ? -1 : 0;
 }
 
