@@ -65,14 +65,18 @@ namespace {
 /*
  * partial[z][k][k'] = sum_{t in split z} w[t] a[t + k] b[t + k'],
  * k, k' < np: 64 x 64 tiles of 4 waves (32 x 32 each) on
- * v_mfma_f64_16x16x4f64, t in chunks of 16 through LDS, the operands read
+ * v_mfma_f64_16x16x4f64, t in chunks of TD_KC through LDS, the operands read
  * from the series (Hankel structure) with the next chunk's loads in flight
  * during the current chunk's MFMAs.  MFMA layout: A[i][k] in lane
  * i + 16 k, B[k][j] in lane j + 16 k, accumulator element e of lane
  * j + 16 q holds D[q + 4 e][j].
  */
 #define TD_TILE 64
-#define TD_KC 16
+#define TD_KC 32
+/* operand elements per thread per chunk */
+#define TD_LR (TD_KC * TD_TILE / 256)
+/* output samples per K split */
+#define TD_TSPLIT 1024
 
 __global__ void __launch_bounds__(256) k_td_corr(const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ w, int nt, int np, int tpz, int tiles, double *__restrict__ part)
 {
@@ -85,10 +89,10 @@ __global__ void __launch_bounds__(256) k_td_corr(const double *__restrict__ a, c
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int li = lane & 15, lk = lane >> 4;
 	const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-	double ra[4], rb[4];
+	double ra[TD_LR], rb[TD_LR];
 	auto load = [&](int c0) {
 #pragma unroll
-		for(int r = 0; r < 4; r++) {
+		for(int r = 0; r < TD_LR; r++) {
 			const int e = tid + 256 * r;
 			const int t = c0 + e / TD_TILE, k = e % TD_TILE;
 			const bool ok = t < t1;
@@ -106,7 +110,7 @@ __global__ void __launch_bounds__(256) k_td_corr(const double *__restrict__ a, c
 		load(t0);
 	for(int c0 = t0; c0 < t1; c0 += TD_KC) {
 #pragma unroll
-		for(int r = 0; r < 4; r++) {
+		for(int r = 0; r < TD_LR; r++) {
 			const int e = tid + 256 * r;
 			sA[e / TD_TILE][e % TD_TILE] = ra[r];
 			sB[e / TD_TILE][e % TD_TILE] = rb[r];
@@ -383,10 +387,10 @@ static int td_power_dev(const double *a, const double *b, const double *w, int n
 {
 	const int np = p->np, nq = p->nq, nt = n - 2 * p->pub.transient;
 	hipStream_t st = sphr_rt_stream();
-	/* K splits of about 2048 samples: enough workgroups on a short series,
-	 * partial sums that stay small on a long one */
+	/* K splits of about TD_TSPLIT samples: enough workgroups on a short
+	 * series, partial sums that stay small on a long one */
 	const int tiles = (np + TD_TILE - 1) / TD_TILE;
-	int nz = (nt + 2047) / 2048;
+	int nz = (nt + TD_TSPLIT - 1) / TD_TSPLIT;
 	if(nz < 1)
 		nz = 1;
 	const int tpz = ((nt + nz - 1) / nz + TD_KC - 1) / TD_KC * TD_KC;
