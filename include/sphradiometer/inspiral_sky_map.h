@@ -79,6 +79,14 @@ struct correlator_network_baselines *read_precalc_correlator_network_baselines(c
 int read_precalc_correlator_network_plan_fd(struct correlator_network_plan_fd *fdplansp, struct correlator_network_plan_fd *fdplansn, const struct correlator_network_baselines *baselines, int tseries_length, char *parent_dir);
 int write_precalc_autocorrelator_network_plan_fd(const struct autocorrelator_network_plan_fd *fdplansp, const struct autocorrelator_network_plan_fd *fdplansn, char *parent_dir);
 int read_precalc_autocorrelator_network_plan_fd(struct autocorrelator_network_plan_fd *fdplansp, struct autocorrelator_network_plan_fd *fdplansn, const struct instrument_array *instruments, int tseries_length, char *parent_dir);
+
+/* inspiral_sky_map.c:561-660: projections[i][f] = the (i, i) element of
+ * the projection operator of polarisation beta for PSD values psd[f]
+ * (n_instruments each), expanded at order l_max; the detectors are the
+ * instruments' data pointers (struct sphr_detector) */
+struct autocorrelator_network_plan_fd *autocorrelator_network_plan_fd_new(const struct instrument_array *instruments, double beta, double psi, double **psd, int length, int l_max);
+/* inspiral_sky_map.c:663-675: every projection resized to l_max */
+struct autocorrelator_network_plan_fd *autocorrelator_network_plan_fd_set_l(struct autocorrelator_network_plan_fd *plan, unsigned l_max);
 /* inspiral_sky_map.c:678-691 */
 void autocorrelator_network_plan_fd_free(struct autocorrelator_network_plan_fd *plan);
 

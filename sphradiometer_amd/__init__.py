@@ -252,6 +252,8 @@ _PROTOS = {
     "read_precalc_correlator_network_plan_fd": (c_int, [P(NetworkPlanFD), P(NetworkPlanFD), P(NetworkBaselines), c_int, ctypes.c_char_p]),
     "write_precalc_autocorrelator_network_plan_fd": (c_int, [P(AutoPlanFD), P(AutoPlanFD), ctypes.c_char_p]),
     "read_precalc_autocorrelator_network_plan_fd": (c_int, [P(AutoPlanFD), P(AutoPlanFD), P(InstrumentArray), c_int, ctypes.c_char_p]),
+    "autocorrelator_network_plan_fd_new": (P(AutoPlanFD), [P(InstrumentArray), c_double, c_double, c_void_p, c_int, c_int]),
+    "autocorrelator_network_plan_fd_set_l": (P(AutoPlanFD), [P(AutoPlanFD), c_uint]),
     "autocorrelator_network_plan_fd_free": (None, [P(AutoPlanFD)]),
     "sh_series_write_healpix_map": (c_int, [P(ShSeries), ctypes.c_char_p]),
     "sh_series_write_healpix_alm": (c_int, [P(ShSeries), ctypes.c_char_p]),

@@ -492,3 +492,99 @@ done:
 	free(tw);
 	return res;
 }
+
+/* ------------------------------------------------------------------ */
+/* autocorrelator plans (the diagonal of the projection operator)      */
+/* ------------------------------------------------------------------ */
+
+/*
+ * inspiral_sky_map.c:561-660 (diagonal_projections,
+ * autocorrelator_network_plan_fd_new): projections[i][f] is the (i, i)
+ * element of the projection operator of polarisation beta for the PSD
+ * values psd[f], expanded in the Earth frame at order l_max.  The
+ * detectors are the instruments' data pointers, as for
+ * correlator_network_plan_mult_by_projection.
+ */
+struct autocorrelator_network_plan_fd *autocorrelator_network_plan_fd_new(const struct instrument_array *instruments, double beta, double psi, double **psd, int length, int l_max)
+{
+	if(!instruments || !psd || length < 0 || l_max < 0) {
+		sphr_set_error("autocorrelator_network_plan_fd_new: bad arguments");
+		return NULL;
+	}
+	const int n = instrument_array_len(instruments);
+	if(n > 64) {
+		sphr_set_error("autocorrelator_network_plan_fd_new: %d instruments exceed 64", n);
+		return NULL;
+	}
+	const struct sphr_detector *det[64];
+	for(int k = 0; k < n; k++) {
+		det[k] = instrument_array_get(instruments, k)->data;
+		if(!det[k]) {
+			sphr_set_error("instrument %d has no detector response (data == NULL)", k);
+			return NULL;
+		}
+	}
+	struct autocorrelator_network_plan_fd *plan = calloc(1, sizeof(*plan));
+	struct sh_series ***pr = calloc(n > 0 ? n : 1, sizeof(*pr));
+	if(!plan || !pr) {
+		free(plan);
+		free(pr);
+		sphr_set_error("autocorrelator_network_plan_fd_new: out of memory");
+		return NULL;
+	}
+	plan->instruments = instruments;
+	plan->length = length;
+	plan->projections = pr;
+	int fail = 0;
+	for(int i = 0; i < n && !fail; i++) {
+		pr[i] = calloc(length > 0 ? length : 1, sizeof(**pr));
+		fail = !pr[i];
+		for(int f = 0; f < length && !fail; f++)
+			fail = !(pr[i][f] = sh_series_new(l_max, 0));
+	}
+	if(!fail) {
+#pragma omp parallel for collapse(2) schedule(dynamic)
+		for(int i = 0; i < n; i++)
+			for(int f = 0; f < length; f++) {
+				struct cbc_args args = { i, i, n, det, beta, psi, psd[f] };
+				if(!sh_series_from_func(pr[i][f], cbc_projection, &args)) {
+#pragma omp atomic write
+					fail = 1;
+				}
+			}
+	}
+	if(fail) {
+		sphr_set_error("autocorrelator_network_plan_fd_new: projection expansion failed");
+		autocorrelator_network_plan_fd_free(plan);
+		return NULL;
+	}
+	return plan;
+}
+
+/* inspiral_sky_map.c:663-675 */
+struct autocorrelator_network_plan_fd *autocorrelator_network_plan_fd_set_l(struct autocorrelator_network_plan_fd *plan, unsigned l_max)
+{
+	for(int i = 0; i < instrument_array_len(plan->instruments); i++)
+		for(int f = 0; f < plan->length; f++)
+			if(!sh_series_resize(plan->projections[i][f], l_max))
+				return NULL;
+	return plan;
+}
+
+/* inspiral_sky_map.c:678-691; also releases the partial plans the
+ * constructor and the precalc reader leave on failure */
+void autocorrelator_network_plan_fd_free(struct autocorrelator_network_plan_fd *plan)
+{
+	if(!plan)
+		return;
+	if(plan->projections) {
+		for(int i = 0; i < instrument_array_len(plan->instruments); i++) {
+			if(plan->projections[i])
+				for(int f = 0; f < plan->length; f++)
+					sh_series_free(plan->projections[i][f]);
+			free(plan->projections[i]);
+		}
+		free(plan->projections);
+	}
+	free(plan);
+}

@@ -599,23 +599,6 @@ int write_precalc_autocorrelator_network_plan_fd(const struct autocorrelator_net
 	return 0;
 }
 
-/* inspiral_sky_map.c:678-691 */
-void autocorrelator_network_plan_fd_free(struct autocorrelator_network_plan_fd *plan)
-{
-	if(!plan)
-		return;
-	if(plan->projections) {
-		for(int i = 0; i < instrument_array_len(plan->instruments); i++) {
-			if(plan->projections[i])
-				for(int k = 0; k < plan->length; k++)
-					sh_series_free(plan->projections[i][k]);
-			free(plan->projections[i]);
-		}
-		free(plan->projections);
-	}
-	free(plan);
-}
-
 /* the projections of one polarisation; NULL on failure */
 static struct sh_series ***read_projections(const struct instrument_array *instruments, int length, const char *sign, const char *parent)
 {
