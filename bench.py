@@ -465,12 +465,19 @@ def bench_copy(S, args):
 
 
 # HBM bytes per launch of the fused sky kernel (float32, l_max 32, 1024
-# segments, the K split the autotuner picks for the timed steps):
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE medians over the timed
-# dispatches, scaled by the counters' reading of a 256 MiB copy
-# (FETCH_SIZE reads half the copied bytes, WRITE_SIZE all of them); the
-# summary with the commands is profiles/three_site_l32_traffic.md
-SKY_FUSED_F32_TRAFFIC = 3344 * 1024 * 2 + 48738 * 1024
+# segments): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE medians over the
+# timed dispatches at 7 K splits, scaled by the counters' reading of a
+# 256 MiB copy (FETCH_SIZE reads half the copied bytes, WRITE_SIZE all of
+# them); the summary with the commands is
+# profiles/three_site_l32_traffic.md.  The reads are the K-split partial
+# sums (6.85 MB measured, 6.88 MB of partials at 7 splits), so for the
+# split the autotuner picks in a run they are rescaled by the split count.
+SKY_FUSED_F32_WRITE = 48738 * 1024
+SKY_FUSED_F32_READ_7 = 3344 * 1024 * 2
+
+
+def sky_fused_traffic(k_splits, segments, columns):
+    return SKY_FUSED_F32_WRITE + SKY_FUSED_F32_READ_7 + (k_splits - 7) * segments * columns * 4
 
 
 def main():
@@ -576,7 +583,9 @@ def main():
         "tuned": {"mixed": rmx["tuned"], "fp64": r64["tuned"], "info": rmx["info"]},
         "roofline": {"bound": "hbm", "kernel": names[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": SKY_FUSED_F32_TRAFFIC if names[k] == "sky_fused" and nseg == 1024 else None,
+                     "traffic": (sky_fused_traffic(rmx["info"]["k_splits"], nseg, rmx["info"]["columns"])
+                                 if names[k] == "sky_fused" and nseg == 1024 and L == 32 else None),
+                     "traffic_basis": "rocprofv3 FETCH_SIZE/WRITE_SIZE at 7 K splits, partial-sum reads rescaled to the tuned split",
                      "algorithmic_bytes": sb[k]},
         "plan_build_s": round(plan_s, 2),
     }
