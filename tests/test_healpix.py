@@ -256,13 +256,15 @@ def test_gpu_write_healpix_map_file(tmp_path, oracle):
 
 
 @pytest.mark.gpu
-def test_gpu_posterior_matches_oracle(oracle):
+@pytest.mark.parametrize("L", [12, 40, 100])
+def test_gpu_posterior_matches_oracle(oracle, L):
+    # nside 8, 32 and 64 (768 to 49152 pixels per map)
     rng = np.random.default_rng(13)
-    L, E = 12, 3
+    E = 3
     N = oracle.healpix_nside_py(L)
     series = [rand_series(rng, L) * 3 for _ in range(2 * E)]
     for prec, tol in ((S.FP64, TOL64), (S.MIXED, TOL_MIXED)):
-        print("posterior precision", prec, flush=True)
+        print("posterior L", L, "precision", prec, flush=True)
         cdt = np.complex128 if prec == S.FP64 else np.complex64
         rdt = np.float64 if prec == S.FP64 else np.float32
         hp = S.Healpix(L, N, prec)
