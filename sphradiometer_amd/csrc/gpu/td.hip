@@ -53,7 +53,9 @@ struct td_plan {
 	double *d_U;			/* [ceil(np / 16)][nq]: partial bilinear forms */
 	double *d_powc;			/* power_1d as P + 1 complex (imaginary parts 0) */
 	double *d_rows;			/* (P + 1)^2 complex: D^l_{0 m'} of the rotation plan */
-	size_t cap_out;
+	size_t cap_out, cap_sv, cap_edge;
+	double *d_sv;			/* window differences on their support: values, then offsets */
+	double *d_edge;			/* first row and column of C */
 	double *d_out;			/* rotated series (or a network's partial sums) */
 	/* a network's series and windows (kept on its first plan) */
 	size_t cap_nts, cap_nws;
@@ -63,8 +65,9 @@ struct td_plan {
 namespace {
 
 /*
- * partial[z][k][k'] = sum_{t in split z} w[t] a[t + k] b[t + k'],
- * k, k' < np: 64 x 64 tiles of 4 waves (32 x 32 each) on
+ * partial[z][k][k'] = sum_{t in split z} w[t] a[s_t + k] b[s_t + k'],
+ * k, k' < np, with s_t = t or, for a gathered support, s_t = sidx[t]
+ * (reads at or past na, the series length, count as 0): 64 x 64 tiles of 4 waves (32 x 32 each) on
  * v_mfma_f64_16x16x4f64, t in chunks of TD_KC through LDS, the operands read
  * from the series (Hankel structure) with the next chunk's loads in flight
  * during the current chunk's MFMAs.  MFMA layout: A[i][k] in lane
@@ -78,7 +81,7 @@ namespace {
 /* output samples per K split */
 #define TD_TSPLIT 1024
 
-__global__ void __launch_bounds__(256) k_td_corr(const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ w, int nt, int np, int tpz, int tiles, double *__restrict__ part)
+__global__ void __launch_bounds__(256) k_td_corr(const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ w, const double *__restrict__ sidx, int na, int nt, int np, int tpz, int tiles, double *__restrict__ part)
 {
 	typedef double v4 __attribute__((ext_vector_type(4)));
 	__shared__ double sA[TD_KC][TD_TILE + 4];	/* [t][k] */
@@ -96,8 +99,11 @@ __global__ void __launch_bounds__(256) k_td_corr(const double *__restrict__ a, c
 			const int e = tid + 256 * r;
 			const int t = c0 + e / TD_TILE, k = e % TD_TILE;
 			const bool ok = t < t1;
-			ra[r] = ok && m0 + k < np ? w[t] * a[t + m0 + k] : 0.0;
-			rb[r] = ok && n0 + k < np ? b[t + n0 + k] : 0.0;
+			/* sample offset of term t: t itself, or the t-th entry of a
+			 * gathered support */
+			const int st = sidx && ok ? (int) sidx[t] : t;
+			ra[r] = ok && m0 + k < np && st + m0 + k < na ? w[t] * a[st + m0 + k] : 0.0;
+			rb[r] = ok && n0 + k < np && st + n0 + k < na ? b[st + n0 + k] : 0.0;
 		}
 	};
 	v4 acc[2][2];
@@ -162,6 +168,72 @@ __global__ void k_td_reduce(const double *__restrict__ part, int nz, long n, dou
 	for(int z = 0; z < nz; z++)
 		s += part[(size_t) z * n + i];
 	C[i] = s;
+}
+
+/* part[c][j] = sum_{t in chunk c} w[t] a[t] b[t + j] and part[c][np + j]
+ * = sum_{t in chunk c} w[t] a[t + j] b[t], j < np (the first row and
+ * column of C, summed over the chunks by k_td_reduce): TD_EC samples per
+ * workgroup, staged in LDS with the np - 1 samples after them, each
+ * thread one or more j of both */
+#define TD_EC 1024
+#define TD_EDGE_NP_MAX 1024
+__global__ void __launch_bounds__(256) k_td_edge(const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ w, int nt, int n, int np, double *__restrict__ part)
+{
+	__shared__ double wa[TD_EC], wb[TD_EC], ya[TD_EC + TD_EDGE_NP_MAX], yb[TD_EC + TD_EDGE_NP_MAX];
+	const int t0 = blockIdx.x * TD_EC;
+	for(int t = threadIdx.x; t < TD_EC; t += 256) {
+		const bool ok = t0 + t < nt;
+		wa[t] = ok ? w[t0 + t] * a[t0 + t] : 0.0;
+		wb[t] = ok ? w[t0 + t] * b[t0 + t] : 0.0;
+	}
+	for(int t = threadIdx.x; t < TD_EC + np - 1; t += 256) {
+		const bool ok = t0 + t < n;
+		ya[t] = ok ? a[t0 + t] : 0.0;
+		yb[t] = ok ? b[t0 + t] : 0.0;
+	}
+	__syncthreads();
+	for(int jj = threadIdx.x; jj < np; jj += 256) {
+		double r = 0.0, c = 0.0;
+		for(int t = 0; t < TD_EC; t++) {
+			r = fma(wa[t], yb[t + jj], r);
+			c = fma(wb[t], ya[t + jj], c);
+		}
+		part[(size_t) blockIdx.x * 2 * np + jj] = r;
+		part[(size_t) blockIdx.x * 2 * np + np + jj] = c;
+	}
+}
+
+/* C from its first row and column (edge) and the diagonal differences D
+ * (in X on entry, C on return): C[k + 1][k' + 1] = C[k][k'] + D[k][k'].
+ * One wave per diagonal k' - k: each lane sums a segment of D, a scan
+ * over the lanes gives each segment's starting value, and the lanes then
+ * write their segments. */
+__global__ void __launch_bounds__(256) k_td_diag(const double *__restrict__ edge, int np, double *__restrict__ X)
+{
+	const int lane = threadIdx.x & 63;
+	const int d = (int) (blockIdx.x * 4 + (threadIdx.x >> 6)) - (np - 1);
+	if(d > np - 1)
+		return;
+	const int k0 = d >= 0 ? 0 : -d, kp0 = d >= 0 ? d : 0;
+	const int len = np - (d >= 0 ? d : -d);
+	const int seg = (len + 63) / 64, b0 = lane * seg, b1 = b0 + seg < len ? b0 + seg : len;
+	const size_t step = (size_t) np + 1, base = (size_t) k0 * np + kp0;
+	double sum = 0.0;
+	for(int e = b0; e < b1; e++)
+		sum += X[base + e * step];
+	/* exclusive scan of the segment sums over the wave */
+	double incl = sum;
+	for(int o = 1; o < 64; o <<= 1) {
+		const double u = __shfl_up(incl, o);
+		if(lane >= o)
+			incl += u;
+	}
+	double c = (d >= 0 ? edge[d] : edge[np - d]) + (incl - sum);
+	for(int e = b0; e < b1; e++) {
+		const double x = X[base + e * step];
+		X[base + e * step] = c;
+		c += x;
+	}
 }
 
 /* Upart[kb][q] = sum_{k in block kb} Ta[k][q] sum_k' C[k][k'] Tb[k'][q]:
@@ -248,7 +320,7 @@ __global__ void __launch_bounds__(256) k_td_pw(const double *__restrict__ Upart,
 
 static void td_dev_free(struct td_plan *p)
 {
-	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_U, p->d_powc, p->d_rows, p->d_out, p->d_nts, p->d_nws };
+	double *b[] = { p->d_Ta, p->d_Tb, p->d_WP, p->d_a, p->d_b, p->d_w, p->d_part, p->d_C, p->d_U, p->d_powc, p->d_rows, p->d_out, p->d_sv, p->d_edge, p->d_nts, p->d_nws };
 	for(unsigned i = 0; i < sizeof(b) / sizeof(b[0]); i++)
 		if(b[i])
 			(void) hipFree(b[i]);
@@ -382,27 +454,71 @@ static int grow(double **buf, size_t *cap, size_t need)
 }
 
 /* the plan's power (complex, in d_powc) from device copies of the series
- * a, b (n samples) and window (n - 2 transient); launches only */
-static int td_power_dev(const double *a, const double *b, const double *w, int n, struct td_plan *p)
+ * a, b (n samples) and window w (n - 2 transient; hw: the same window in
+ * host memory, or NULL); launches only.
+ *
+ * With v[s] = w[s - 1] - w[s] for s = 0..nt (w = 0 outside [0, nt)),
+ * C[k + 1][k' + 1] - C[k][k'] = sum_s v[s] a[s + k] b[s + k'] exactly.  For
+ * windows that are constant over most of the series (the square and
+ * Tukey windows of the reference's stitching) v has few nonzeros: the
+ * correlation kernel then runs over that support only, and C follows from
+ * its first row and column along the diagonals. */
+static int td_power_dev(const double *a, const double *b, const double *w, const double *hw, int n, struct td_plan *p)
 {
 	const int np = p->np, nq = p->nq, nt = n - 2 * p->pub.transient;
 	hipStream_t st = sphr_rt_stream();
-	/* K splits of about TD_TSPLIT samples: enough workgroups on a short
+	int nnz = 0;
+	if(hw && nt >= 2 && np >= 2)
+		for(int s = 0; s <= nt; s++)
+			nnz += (s > 0 ? hw[s - 1] : 0.0) - (s < nt ? hw[s] : 0.0) != 0.0;
+	const int sparse = hw && nt >= 2 && np >= 2 && np <= TD_EDGE_NP_MAX && 2 * nnz <= nt;
+	const double *cw = w, *sidx = NULL;
+	int nk = nt;
+	if(sparse) {
+		double *hv = (double *) malloc(2 * (size_t) (nnz > 0 ? nnz : 1) * sizeof(double));
+		if(!hv)
+			return -1;
+		for(int s = 0, e = 0; s <= nt; s++) {
+			const double v = (s > 0 ? hw[s - 1] : 0.0) - (s < nt ? hw[s] : 0.0);
+			if(v != 0.0) {
+				hv[e] = v;
+				hv[nnz + e++] = (double) s;
+			}
+		}
+		const int rc = grow(&p->d_sv, &p->cap_sv, 2 * (size_t) (nnz > 0 ? nnz : 1)) || (nnz > 0 && sphr_memcpy_htod(p->d_sv, hv, 2 * (size_t) nnz * sizeof(double)));
+		free(hv);
+		if(rc || grow(&p->d_edge, &p->cap_edge, 2 * (size_t) np))
+			return -1;
+		cw = p->d_sv;
+		sidx = p->d_sv + nnz;
+		nk = nnz;
+	}
+	/* K splits of about TD_TSPLIT terms: enough workgroups on a short
 	 * series, partial sums that stay small on a long one */
 	const int tiles = (np + TD_TILE - 1) / TD_TILE;
-	int nz = (nt + TD_TSPLIT - 1) / TD_TSPLIT;
+	int nz = (nk + TD_TSPLIT - 1) / TD_TSPLIT;
 	if(nz < 1)
 		nz = 1;
-	const int tpz = ((nt + nz - 1) / nz + TD_KC - 1) / TD_KC * TD_KC;
-	nz = nt > 0 ? (nt + tpz - 1) / tpz : 1;
-	if(grow(&p->d_part, &p->cap_part, (size_t) nz * np * np))
+	const int tpz = ((nk + nz - 1) / nz + TD_KC - 1) / TD_KC * TD_KC;
+	nz = nk > 0 ? (nk + tpz - 1) / tpz : 1;
+	/* d_part holds the K-split partials of C, then (sparse windows) the
+	 * chunk sums of its first row and column: sized once for both, before
+	 * any launch */
+	const int nch = (nt + TD_EC - 1) / TD_EC;
+	const size_t need = (size_t) nz * np * np > (size_t) nch * 2 * np ? (size_t) nz * np * np : (size_t) nch * 2 * np;
+	if(grow(&p->d_part, &p->cap_part, need))
 		return -1;
-	if(nt > 0)
-		hipLaunchKernelGGL(k_td_corr, dim3(tiles * tiles * nz), dim3(256), 0, st, a, b, w, nt, np, tpz, tiles, p->d_part);
+	if(nk > 0)
+		hipLaunchKernelGGL(k_td_corr, dim3(tiles * tiles * nz), dim3(256), 0, st, a, b, cw, sidx, n, nk, np, tpz, tiles, p->d_part);
 	else
 		(void) hipMemsetAsync(p->d_part, 0, (size_t) np * np * sizeof(double), st);
 	const long nc = (long) np * np;
-	hipLaunchKernelGGL(k_td_reduce, dim3((nc + 255) / 256), dim3(256), 0, st, p->d_part, nt > 0 ? nz : 1, nc, p->d_C);
+	hipLaunchKernelGGL(k_td_reduce, dim3((nc + 255) / 256), dim3(256), 0, st, p->d_part, nk > 0 ? nz : 1, nc, p->d_C);
+	if(sparse) {
+		hipLaunchKernelGGL(k_td_edge, dim3(nch), dim3(256), 0, st, a, b, w, nt, n, np, p->d_part);
+		hipLaunchKernelGGL(k_td_reduce, dim3((2 * np + 255) / 256), dim3(256), 0, st, p->d_part, nch, (long) 2 * np, p->d_edge);
+		hipLaunchKernelGGL(k_td_diag, dim3((2 * np - 1 + 3) / 4), dim3(256), 0, st, p->d_edge, np, p->d_C);
+	}
 	const int nkb = (np + 15) / 16;
 	hipLaunchKernelGGL(k_td_yu, dim3((nq + 63) / 64, nkb), dim3(256), 0, st, p->d_C, p->d_Ta, p->d_Tb, np, nq, p->d_U);
 	hipLaunchKernelGGL(k_td_pw, dim3((p->P + 16) / 16), dim3(256), (size_t) nq * sizeof(double), st, p->d_U, nkb, p->d_WP, nq, p->P, p->d_powc);
@@ -420,7 +536,7 @@ static int td_power_1d(const double *ta, const double *tb, const double *window,
 	if(sphr_memcpy_htod(p->d_a, ta, (size_t) n * sizeof(double)) || sphr_memcpy_htod(p->d_b, tb, (size_t) n * sizeof(double))
 	   || (nt > 0 && sphr_memcpy_htod(p->d_w, window, (size_t) nt * sizeof(double))))
 		return -1;
-	return td_power_dev(p->d_a, p->d_b, p->d_w, n, p);
+	return td_power_dev(p->d_a, p->d_b, p->d_w, window, n, p);
 }
 
 /* power_1d of the plan set from the device power; buf holds 2 (P + 1)
@@ -544,7 +660,7 @@ extern "C" struct sh_series *correlator_network_integrate_power_td(struct sh_ser
 	int k = 0;
 	for(int i = 1; i < n_inst && !rc; i++)
 		for(int j = 0; j < i && !rc; j++, k++)
-			rc = td_power_dev(dts + (size_t) i * tseries_length, dts + (size_t) j * tseries_length, dws + (size_t) k * tseries_length, tseries_length, (struct td_plan *) plan->plans[k]);
+			rc = td_power_dev(dts + (size_t) i * tseries_length, dts + (size_t) j * tseries_length, dws + (size_t) k * tseries_length, windows[k], tseries_length, (struct td_plan *) plan->plans[k]);
 	for(k = 0; k < nb && !rc; k++)
 		rc = td_fetch_power((struct td_plan *) plan->plans[k], buf);
 	if(!rc && (sky->polar || L < Pmax))

@@ -12,7 +12,9 @@ the same taps; the routes share no code).  The stitching identity is exact
 algebra: two overlapping segments integrated with their own windows sum
 to the whole series integrated with the two windows added at their
 offsets.  The library's GPU integrators are compared with the oracle at
-1e-12 relative.
+1e-12 relative, with random windows (the correlation over every sample)
+and with square and Tukey windows (the correlation over the samples where
+the window changes, the rest recovered along the diagonals of C).
 """
 import ctypes
 
@@ -122,6 +124,32 @@ def test_gpu_td_baseline_matches_oracle(oracle, extra):
     short = np.zeros(2 * tr - 1)
     assert not S.lib().correlator_baseline_integrate_power_td(short.ctypes.data, short.ctypes.data, w.ctypes.data, short.size, plan)
     S.lib().correlator_plan_td_free(plan)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["square", "tukey"])
+@pytest.mark.parametrize("extra", [600, 5000])
+def test_gpu_td_flat_windows_match_oracle(oracle, kind, extra):
+    """windows flat over most of the series (square, Tukey): the library
+    correlates only where the window changes and recovers the rest along
+    the diagonals of C"""
+    print("td window", kind, "output samples", extra, flush=True)
+    net = _Net(POS[:2])
+    L = S.lib()
+    plan = L.correlator_plan_td_new(net.baseline(0), DT)
+    td = oracle.PlanTD(POS[1], POS[0], DT)
+    tr = plan.contents.transient
+    rng = np.random.default_rng(extra + len(kind))
+    n = 2 * tr + extra
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    if kind == "square":
+        w = np.full(extra, 1.0 / extra)
+    else:
+        w = np.ctypeslib.as_array(L.correlator_tukey_window_new(extra, 0, extra // 8, 1.0 / extra), (extra,)).copy()
+    got = _integrate(plan, a, b, w)
+    want, _ = td.integrate(a, b, w)
+    assert rel(got, want) < TOL64, rel(got, want)
+    L.correlator_plan_td_free(plan)
 
 
 @pytest.mark.gpu
