@@ -18,7 +18,10 @@
  * the windowed correlation matrix C (n_p x n_p, a GEMM over t with the
  * Hankel operands generated from the series in the tile loader, float64
  * MFMA), then Y = C T^b and its bilinear form with T^a in one pass
- * (k_td_yu) and the d sums (k_td_pw).
+ * (k_td_yu) and the d sums (k_td_pw).  For a window that is flat over
+ * most of the series, C comes from the samples where the window changes
+ * (the same GEMM over that support gives the diagonal differences of C),
+ * its first row and column, and a sum along each diagonal.
  * The rotation to the baseline direction and the network average run on
  * the device (k_rotsum) from the rows D^l_{0 m'} of the plan's rotation
  * plan, uploaded once when the plan is made; a call copies back only
