@@ -21,6 +21,8 @@
  * with i = 4 q + e (f32) or i = q + 4 e (f64).
  */
 #include <hip/hip_runtime.h>
+#include <pthread.h>
+#include <rocblas/rocblas.h>
 #include "kernels.h"
 
 namespace {
@@ -124,3 +126,53 @@ template int launch_gemm<double, double>(int, int, int, const double *, long, co
 template int launch_gemm<double, float>(int, int, int, const float *, long, const double *, long, double *, long, hipStream_t);
 template int launch_gemm<float, float>(int, int, int, const float *, long, const float *, long, float *, long, hipStream_t);
 template int launch_gemm<float, double>(int, int, int, const double *, long, const float *, long, float *, long, hipStream_t);
+
+/*
+ * The same row-major product through rocBLAS (column-major): C^T = B^T A^T,
+ * i.e. gemm(N, M, K) with B as the first operand.  For the plain
+ * float32 / float64 products whose shape the library kernel above does
+ * not fill well.  One handle per device, created on first use; the
+ * stream is set under a lock, so calls from several threads each run on
+ * their own stream.
+ */
+static pthread_mutex_t blas_lock = PTHREAD_MUTEX_INITIALIZER;
+static rocblas_handle blas_handles[64];
+
+static rocblas_handle blas_handle(void)
+{
+	int dev = 0;
+	if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+		return NULL;
+	if(!blas_handles[dev] && rocblas_create_handle(&blas_handles[dev]) != rocblas_status_success)
+		blas_handles[dev] = NULL;
+	return blas_handles[dev];
+}
+
+static rocblas_status blas_call(int M, int N, int K, const float *A, long lda, const float *B, long ldb, float *C, long ldc, rocblas_handle h)
+{
+	const float one = 1.0f, zero = 0.0f;
+	return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, B, (rocblas_int) ldb, A, (rocblas_int) lda, &zero, C, (rocblas_int) ldc);
+}
+
+static rocblas_status blas_call(int M, int N, int K, const double *A, long lda, const double *B, long ldb, double *C, long ldc, rocblas_handle h)
+{
+	const double one = 1.0, zero = 0.0;
+	return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, B, (rocblas_int) ldb, A, (rocblas_int) lda, &zero, C, (rocblas_int) ldc);
+}
+
+template <typename T>
+int launch_gemm_blas(int M, int N, int K, const T *A, long lda, const T *B, long ldb, T *C, long ldc, hipStream_t st)
+{
+	if(M <= 0 || N <= 0 || K <= 0)
+		return 0;
+	pthread_mutex_lock(&blas_lock);
+	rocblas_handle h = blas_handle();
+	rocblas_status r = h ? rocblas_set_stream(h, st) : rocblas_status_invalid_handle;
+	if(r == rocblas_status_success)
+		r = blas_call(M, N, K, A, lda, B, ldb, C, ldc, h);
+	pthread_mutex_unlock(&blas_lock);
+	return r == rocblas_status_success ? 0 : -1;
+}
+
+template int launch_gemm_blas<float>(int, int, int, const float *, long, const float *, long, float *, long, hipStream_t);
+template int launch_gemm_blas<double>(int, int, int, const double *, long, const double *, long, double *, long, hipStream_t);

@@ -115,6 +115,9 @@ int launch_extremize_seeded(const cplx<TI> *alm, int S, int alm_div, int L, int 
  * T is the arithmetic type, TA the storage type of A (widened on load) */
 template <typename T, typename TA>
 int launch_gemm(int M, int N, int K, const TA *A, long lda, const T *B, long ldb, T *C, long ldc, hipStream_t st);
+/* the same product (A in T) through rocBLAS */
+template <typename T>
+int launch_gemm_blas(int M, int N, int K, const T *A, long lda, const T *B, long ldb, T *C, long ldc, hipStream_t st);
 
 /* drop-in helpers (FP64) */
 int launch_xspec(const cplx<double> *const *fs_dev, int n, int nb, const int *ia, const int *ib, cplx<double> *fsp, hipStream_t st);
