@@ -477,7 +477,8 @@ __global__ void k_pg_rot(const struct jb_bl *__restrict__ bl, int b0, cplx<T> *_
 /* ---------------------------------------------------------- data path */
 
 /* c[u] = N sum_s x_a[s] x_b[(s+u) mod N] for the baseline's lags; LDS
- * tiles of sc samples, nt lags per workgroup */
+ * tiles of sc samples, up to nt lags per workgroup (nt a power of two, at
+ * least 32, and at most sc) */
 template <typename T>
 __global__ void k_lagcorr(const struct jb_job *__restrict__ jobs, const struct jb_bl *__restrict__ bl, const float *__restrict__ in, int sc, T *__restrict__ c)
 {
@@ -493,6 +494,17 @@ __global__ void k_lagcorr(const struct jb_job *__restrict__ jobs, const struct j
 	const int N = J.N;
 	const int u0 = uu0 - (B.np - 1);
 	const float *xa = in + J.off_in + (size_t) B.ia * N, *xb = in + J.off_in + (size_t) B.ib * N;
+	/* float64: the nl lags of this workgroup take LR lanes each (a power
+	 * of two, at least 32), and the other lanes of the workgroup split the
+	 * samples, part q of P summing t = q, q + P, ... (float64 run 13.0 ->
+	 * 10.0 ms at 4096 jobs).  float32 keeps one lane per lag: the split
+	 * measured slower there (4.33 -> 4.57 ms). */
+	const int nl = B.lags - uu0 < nt ? B.lags - uu0 : nt;
+	int LR = sizeof(T) == 8 ? 32 : nt;
+	while(LR < nl)
+		LR <<= 1;
+	const int P = nt / LR, li = threadIdx.x % LR, part = threadIdx.x / LR;
+	const T *sbl = sb + li;
 	T acc = 0;
 	for(int s0 = 0; s0 < N; s0 += sc) {
 		const int len = N - s0 < sc ? N - s0 : sc;
@@ -506,13 +518,29 @@ __global__ void k_lagcorr(const struct jb_job *__restrict__ jobs, const struct j
 			sb[t] = (T) xb[idx];
 		}
 		__syncthreads();
-		for(int t = 0; t < len; t++)
-			acc += sa[t] * sb[t + threadIdx.x];
+		if(sizeof(T) == 4)
+			for(int t = 0; t < len; t++)
+				acc += sa[t] * sb[t + threadIdx.x];
+		else
+			for(int t = part; t < len; t += P)
+				acc += sa[t] * sbl[t];
 		__syncthreads();
 	}
-	const int uu = uu0 + threadIdx.x;
-	if(uu < B.lags)
-		c[B.off_c + uu] = acc * (T) N;
+	if(sizeof(T) == 4) {
+		const int uu = uu0 + threadIdx.x;
+		if(uu < B.lags)
+			c[B.off_c + uu] = acc * (T) N;
+		return;
+	}
+	/* sum the parts of each lag (sa is free after the last barrier) */
+	sa[threadIdx.x] = acc;
+	__syncthreads();
+	if(part == 0 && li < nl) {
+		T sum = 0;
+		for(int q = 0; q < P; q++)
+			sum += sa[q * LR + li];
+		c[B.off_c + uu0 + li] = sum * (T) N;
+	}
 }
 
 /* per job: power_1d_b = c_b M_b, a_lm = sum_b R_b power_1d_b[l] */
