@@ -206,6 +206,25 @@ __global__ void __launch_bounds__(256) k_td_edge(const double *__restrict__ a, c
 	}
 }
 
+/* w[t] = -sum_{s <= t} v[s], t < nt: the window rebuilt from its first
+ * differences on their support (offsets sidx, running sums cum), so a
+ * flat window need not be copied to the device */
+__global__ void k_td_wfill(const double *__restrict__ sidx, const double *__restrict__ cum, int nnz, int nt, double *__restrict__ w)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if(t >= nt)
+		return;
+	int lo = 0, hi = nnz;
+	while(lo < hi) {
+		const int mid = (lo + hi) >> 1;
+		if(sidx[mid] <= (double) t)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	w[t] = lo ? -cum[lo - 1] : 0.0;
+}
+
 /* C from its first row and column (edge) and the diagonal differences D
  * (in X on entry, C on return): C[k + 1][k' + 1] = C[k][k'] + D[k][k'].
  * One wave per diagonal k' - k: each lane sums a segment of D, a scan
@@ -458,7 +477,9 @@ static int grow(double **buf, size_t *cap, size_t need)
 
 /* the plan's power (complex, in d_powc) from device copies of the series
  * a, b (n samples) and window w (n - 2 transient; hw: the same window in
- * host memory, or NULL); launches only.
+ * host memory, or NULL); launches only.  nnz is td_window_nnz of the
+ * window: when it is not -1 the window is not read from w but rebuilt
+ * there (w is then a device buffer of nt doubles to fill).
  *
  * With v[s] = w[s - 1] - w[s] for s = 0..nt (w = 0 outside [0, nt)),
  * C[k + 1][k' + 1] - C[k][k'] = sum_s v[s] a[s + k] b[s + k'] exactly.  For
@@ -466,35 +487,51 @@ static int grow(double **buf, size_t *cap, size_t need)
  * Tukey windows of the reference's stitching) v has few nonzeros: the
  * correlation kernel then runs over that support only, and C follows from
  * its first row and column along the diagonals. */
-static int td_power_dev(const double *a, const double *b, const double *w, const double *hw, int n, struct td_plan *p)
+/* the number of nonzero first differences of the host window hw (nt
+ * samples) when the flat-window route applies to a plan of np taps, else
+ * -1 */
+static int td_window_nnz(const double *hw, int nt, int np)
+{
+	if(!hw || nt < 2 || np < 2 || np > TD_EDGE_NP_MAX)
+		return -1;
+	int nnz = 0;
+	for(int s = 0; s <= nt; s++)
+		if((s > 0 ? hw[s - 1] : 0.0) - (s < nt ? hw[s] : 0.0) != 0.0 && 2 * ++nnz > nt)
+			return -1;
+	return nnz;
+}
+
+static int td_power_dev(const double *a, const double *b, double *w, const double *hw, int nnz, int n, struct td_plan *p)
 {
 	const int np = p->np, nq = p->nq, nt = n - 2 * p->pub.transient;
 	hipStream_t st = sphr_rt_stream();
-	int nnz = 0;
-	if(hw && nt >= 2 && np >= 2)
-		for(int s = 0; s <= nt; s++)
-			nnz += (s > 0 ? hw[s - 1] : 0.0) - (s < nt ? hw[s] : 0.0) != 0.0;
-	const int sparse = hw && nt >= 2 && np >= 2 && np <= TD_EDGE_NP_MAX && 2 * nnz <= nt;
+	const int sparse = nnz >= 0;
 	const double *cw = w, *sidx = NULL;
 	int nk = nt;
 	if(sparse) {
-		double *hv = (double *) malloc(2 * (size_t) (nnz > 0 ? nnz : 1) * sizeof(double));
+		/* values, offsets and running sums of the window's differences */
+		double *hv = (double *) malloc(3 * (size_t) (nnz > 0 ? nnz : 1) * sizeof(double));
 		if(!hv)
 			return -1;
+		double run = 0.0;
 		for(int s = 0, e = 0; s <= nt; s++) {
 			const double v = (s > 0 ? hw[s - 1] : 0.0) - (s < nt ? hw[s] : 0.0);
 			if(v != 0.0) {
+				run += v;
 				hv[e] = v;
-				hv[nnz + e++] = (double) s;
+				hv[nnz + e] = (double) s;
+				hv[2 * nnz + e++] = run;
 			}
 		}
-		const int rc = grow(&p->d_sv, &p->cap_sv, 2 * (size_t) (nnz > 0 ? nnz : 1)) || (nnz > 0 && sphr_memcpy_htod(p->d_sv, hv, 2 * (size_t) nnz * sizeof(double)));
+		const int rc = grow(&p->d_sv, &p->cap_sv, 3 * (size_t) (nnz > 0 ? nnz : 1)) || (nnz > 0 && sphr_memcpy_htod(p->d_sv, hv, 3 * (size_t) nnz * sizeof(double)));
 		free(hv);
 		if(rc || grow(&p->d_edge, &p->cap_edge, 2 * (size_t) np))
 			return -1;
 		cw = p->d_sv;
 		sidx = p->d_sv + nnz;
 		nk = nnz;
+		/* the window itself, for the first row and column */
+		hipLaunchKernelGGL(k_td_wfill, dim3((nt + 255) / 256), dim3(256), 0, st, sidx, p->d_sv + 2 * nnz, nnz, nt, w);
 	}
 	/* K splits of about TD_TSPLIT terms: enough workgroups on a short
 	 * series, partial sums that stay small on a long one */
@@ -536,10 +573,11 @@ static int td_power_1d(const double *ta, const double *tb, const double *window,
 	if(grow(&p->d_a, &cap_a, (size_t) n) || grow(&p->d_b, &cap_b, (size_t) n) || grow(&p->d_w, &cap_w, (size_t) (nt > 0 ? nt : 1)))
 		return -1;
 	p->cap_n = cap_a < cap_b ? (cap_a < cap_w ? cap_a : cap_w) : (cap_b < cap_w ? cap_b : cap_w);
+	const int nnz = td_window_nnz(window, nt, p->np);
 	if(sphr_memcpy_htod(p->d_a, ta, (size_t) n * sizeof(double)) || sphr_memcpy_htod(p->d_b, tb, (size_t) n * sizeof(double))
-	   || (nt > 0 && sphr_memcpy_htod(p->d_w, window, (size_t) nt * sizeof(double))))
+	   || (nt > 0 && nnz < 0 && sphr_memcpy_htod(p->d_w, window, (size_t) nt * sizeof(double))))
 		return -1;
-	return td_power_dev(p->d_a, p->d_b, p->d_w, window, n, p);
+	return td_power_dev(p->d_a, p->d_b, p->d_w, window, nnz, n, p);
 }
 
 /* power_1d of the plan set from the device power; buf holds 2 (P + 1)
@@ -656,14 +694,21 @@ extern "C" struct sh_series *correlator_network_integrate_power_td(struct sh_ser
 	double *dts = p0->d_nts, *dws = p0->d_nws;
 	for(int i = 0; i < n_inst && !rc; i++)
 		rc = sphr_memcpy_htod(dts + (size_t) i * tseries_length, tseries[i], (size_t) tseries_length * sizeof(double));
+	/* windows flat over most of the series are rebuilt on the device from
+	 * their differences; the others are copied */
+	int nnzs[SPHR_MAX_BASELINES];
 	for(int k = 0; k < nb && !rc; k++) {
 		const int nt = tseries_length - 2 * plan->plans[k]->transient;
-		rc = nt > 0 && sphr_memcpy_htod(dws + (size_t) k * tseries_length, windows[k], (size_t) nt * sizeof(double));
+		const int nnz = td_window_nnz(windows[k], nt, ((const struct td_plan *) plan->plans[k])->np);
+		if(k < SPHR_MAX_BASELINES)
+			nnzs[k] = nnz;
+		rc = nt > 0 && (k >= SPHR_MAX_BASELINES || nnz < 0) && sphr_memcpy_htod(dws + (size_t) k * tseries_length, windows[k], (size_t) nt * sizeof(double));
 	}
 	int k = 0;
 	for(int i = 1; i < n_inst && !rc; i++)
 		for(int j = 0; j < i && !rc; j++, k++)
-			rc = td_power_dev(dts + (size_t) i * tseries_length, dts + (size_t) j * tseries_length, dws + (size_t) k * tseries_length, windows[k], tseries_length, (struct td_plan *) plan->plans[k]);
+			rc = td_power_dev(dts + (size_t) i * tseries_length, dts + (size_t) j * tseries_length, dws + (size_t) k * tseries_length, windows[k],
+					  k < SPHR_MAX_BASELINES ? nnzs[k] : -1, tseries_length, (struct td_plan *) plan->plans[k]);
 	for(k = 0; k < nb && !rc; k++)
 		rc = td_fetch_power((struct td_plan *) plan->plans[k], buf);
 	if(!rc && (sky->polar || L < Pmax))
