@@ -191,6 +191,9 @@ int sphr_jobs_run(struct sphr_jobs *, const float *in, void *alm, void *map, voi
 /* autotuner: times the lag-correlation tiles on these buffers, keeps the
  * fastest; returns the chosen candidate index, its time per run */
 int sphr_jobs_tune(struct sphr_jobs *, const float *in, void *alm, double *best_ms);
+/* sets the lag tile to one of the tuner's candidates (512/1024/2048
+ * samples x 128/256 threads); -1 for any other */
+int sphr_jobs_set_lag_tile(struct sphr_jobs *, int samples, int threads);
 /* [0] input floats, [1] a_lm elements, [2] map elements, [3] baselines,
  * [4] l_max classes, [5..6] lag tile, [7] plan bytes */
 void sphr_jobs_info(const struct sphr_jobs *, long info[8]);

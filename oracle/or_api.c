@@ -333,6 +333,77 @@ int orx_baseline_integrate_power_td(const double *ta, const double *tb, const do
 	return 0;
 }
 
+/*
+ * CPU baseline: the per-segment pipeline (FFT of each detector, network
+ * integrate, to_mesh) over S segments, tseries [D][S][N], outputs
+ * sky [S][(L+1)^2] and mesh [S][2B][2B].
+ *   seg_threads = 1: segments in order on this thread, the ring loop of
+ *     to_mesh on mesh_threads threads (the reference's own split:
+ *     integrate on one core, to_mesh on 4, sh_series_trans.c:829);
+ *   seg_threads > 1: segments split over that many threads, each with its
+ *     own plan scratch (fseries_product, power_1d), as callers of the
+ *     reference do with correlator_network_plan_fd_copy (correlator.c:
+ *     908-935); to_mesh on one thread inside each.
+ * The delay products and rotation plans are read-only and shared.
+ */
+int orx_segments_run(void *vnet, const double *ts, int S, unsigned sky_l, int seg_threads, int mesh_threads, double complex *sky_out, double complex *mesh_out)
+{
+	struct or_network *net = vnet;
+	const int D = net->n_instruments, nb = net->n_baselines;
+	const int n = net->plans[0]->delay_product->n;
+	const size_t nsky = or_length(sky_l, 0);
+	const size_t nmesh = (size_t) (2 * (sky_l + 1)) * (2 * (sky_l + 1));
+	int rc = 0;
+	if(seg_threads < 1)
+		seg_threads = 1;
+#pragma omp parallel num_threads(seg_threads) reduction(| : rc)
+	{
+		/* this thread's copy of the network: shared tables, own scratch */
+		struct or_plan_fd *pl = malloc(nb * sizeof(*pl));
+		struct or_plan_fd **pp = malloc(nb * sizeof(*pp));
+		for(int k = 0; k < nb; k++) {
+			pl[k] = *net->plans[k];
+			pl[k].fseries_product = malloc((size_t) n * sizeof(double complex));
+			pl[k].power_1d = or_series_new(net->plans[k]->power_1d->l_max, net->plans[k]->power_1d->polar);
+			pp[k] = &pl[k];
+		}
+		struct or_network mine = *net;
+		mine.plans = pp;
+		double complex *fsb = malloc((size_t) D * n * sizeof(*fsb));
+		double complex **fs = malloc(D * sizeof(*fs));
+		double *tmp = malloc((size_t) n * sizeof(*tmp));
+		const int mt = seg_threads > 1 ? 1 : mesh_threads;
+#pragma omp for schedule(dynamic, 4)
+		for(int s = 0; s < S; s++) {
+			for(int i = 0; i < D; i++) {
+				memcpy(tmp, ts + ((size_t) i * S + s) * n, (size_t) n * sizeof(*tmp));
+				or_tseries_to_fseries(tmp, fsb + (size_t) i * n, n);
+				fs[i] = fsb + (size_t) i * n;
+			}
+			struct or_series sky = { sky_l, 0, sky_out + (size_t) s * nsky };
+			if(!or_network_integrate_power_fd(&sky, fs, &mine)) {
+				rc |= 1;
+				continue;
+			}
+			if(mesh_out) {
+				double complex *m = or_to_mesh_threads(&sky, NULL, NULL, mt);
+				memcpy(mesh_out + (size_t) s * nmesh, m, nmesh * sizeof(*m));
+				free(m);
+			}
+		}
+		for(int k = 0; k < nb; k++) {
+			free(pl[k].fseries_product);
+			or_series_free(pl[k].power_1d);
+		}
+		free(pl);
+		free(pp);
+		free(fsb);
+		free(fs);
+		free(tmp);
+	}
+	return rc ? -1 : 0;
+}
+
 void *orx_network_td_new(int n_instruments, const double *pos, double delta_t)
 {
 	return or_network_td_new(n_instruments, pos, delta_t);

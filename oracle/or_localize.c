@@ -107,6 +107,10 @@ int or_network_mult_by_projection(struct or_network *net, const double *resp, do
 		or_array_series(res, 0, &d0);
 		or_array_series(p->delay_product, 0, &a0);
 		struct or_product_plan *pp = or_product_plan_new(&d0, &a0, proj[0]);
+		/* the bins are independent; the general products hold no
+		 * shared state (pixel_product_polar's caches are not reached:
+		 * the projection is never polar) */
+#pragma omp parallel for schedule(dynamic)
 		for(int f = 0; f < N; f++) {
 			struct or_series dd, aa;
 			or_array_series(res, f, &dd);

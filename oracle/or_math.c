@@ -248,14 +248,22 @@ void or_dft(double complex *data, int n, int sign)
 		dft_bluestein(data, n, sign);
 		return;
 	}
-	double complex *tw = malloc((size_t) n * sizeof(*tw));
-	for(int t = 0; t < n; t++) {
-		double ang = 2. * M_PI * (double) t / n;
-		tw[t] = cos(ang) + I * (sign < 0 ? -sin(ang) : sin(ang));
+	/* the twiddles of the last length and sign, per thread: the
+	 * pipelines call the same transform many times */
+	static __thread double complex *tw;
+	static __thread int tw_n, tw_sign;
+	if(!tw || tw_n != n || tw_sign != sign) {
+		free(tw);
+		tw = malloc((size_t) n * sizeof(*tw));
+		for(int t = 0; t < n; t++) {
+			double ang = 2. * M_PI * (double) t / n;
+			tw[t] = cos(ang) + I * (sign < 0 ? -sin(ang) : sin(ang));
+		}
+		tw_n = n;
+		tw_sign = sign;
 	}
 	double complex *out = malloc((size_t) n * sizeof(*out));
 	dft_rec(data, 1, out, n, tw, n);
 	memcpy(data, out, (size_t) n * sizeof(*out));
 	free(out);
-	free(tw);
 }

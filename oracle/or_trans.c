@@ -143,7 +143,11 @@ struct or_series *or_from_realmesh(struct or_series *s, const double *mesh)
 /* sh_series_trans.c:796-871: Legendre sums per ring and m, -m stored at
  * column nphi - m with the (-1)^m factor, then an unnormalised backward
  * DFT along phi. */
-double complex *or_to_mesh(const struct or_series *s, int *ntheta, int *nphi)
+/* threads of the ring loop in or_to_mesh: the reference runs it on 4
+ * (sh_series_trans.c:829) */
+int or_mesh_threads = 4;
+
+double complex *or_to_mesh_threads(const struct or_series *s, int *ntheta, int *nphi, int threads)
 {
 	int nt, np;
 	const unsigned L = s->l_max;
@@ -151,8 +155,9 @@ double complex *or_to_mesh(const struct or_series *s, int *ntheta, int *nphi)
 	double *x = malloc(2 * (L + 1) * sizeof(*x));
 	or_mesh_nodes(L, &nt, &np, x, NULL);
 	double complex *F = calloc((size_t) nt * np, sizeof(*F));
-	double *P = malloc((L + 1) * sizeof(*P));
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(static)
 	for(int i = 0; i < nt; i++) {
+		double P[L + 1];
 		or_sphPlm_array(L, 0, x[i], P);
 		double complex acc = 0.;
 		for(unsigned l = 0; l <= L; l++)
@@ -171,13 +176,17 @@ double complex *or_to_mesh(const struct or_series *s, int *ntheta, int *nphi)
 	}
 	for(int i = 0; i < nt; i++)
 		or_dft(F + (size_t) i * np, np, +1);
-	free(P);
 	free(x);
 	if(ntheta)
 		*ntheta = nt;
 	if(nphi)
 		*nphi = np;
 	return F;
+}
+
+double complex *or_to_mesh(const struct or_series *s, int *ntheta, int *nphi)
+{
+	return or_to_mesh_threads(s, ntheta, nphi, or_mesh_threads);
 }
 
 /* sh_series_trans.c:538-565 + 777-787 */

@@ -158,6 +158,8 @@ void or_healpix_posterior(const double *map_p, const double *map_n, long npix, d
 struct or_series *or_from_mesh(struct or_series *, const double complex *mesh);
 struct or_series *or_from_realmesh(struct or_series *, const double *mesh);
 double complex *or_to_mesh(const struct or_series *, int *ntheta, int *nphi);
+double complex *or_to_mesh_threads(const struct or_series *, int *ntheta, int *nphi, int threads);
+extern int or_mesh_threads;
 struct or_series *or_from_realfunc(struct or_series *, double (*f)(double, double, void *), void *data);
 struct or_series *or_from_func(struct or_series *, double complex (*f)(double, double, void *), void *data);
 struct or_series *or_impulse(unsigned l_max, double theta, double phi);

@@ -231,6 +231,7 @@ _PROTOS = {
     "sphr_jobs_free": (None, [c_void_p]),
     "sphr_jobs_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sphr_jobs_tune": (c_int, [c_void_p, c_void_p, c_void_p, P(c_double)]),
+    "sphr_jobs_set_lag_tile": (c_int, [c_void_p, c_int, c_int]),
     "sphr_jobs_info": (None, [c_void_p, P(ctypes.c_long)]),
     "sphr_jobs_offsets": (None, [c_void_p, P(ctypes.c_long), P(ctypes.c_long), P(ctypes.c_long)]),
     "precalculated_TimeSeries_length": (ctypes.c_long, [ctypes.c_long, c_double]),
@@ -734,6 +735,10 @@ class Jobs:
         if k < 0:
             raise RuntimeError("sphr_jobs_tune failed: " + last_error())
         return k, ms.value
+
+    def set_lag_tile(self, samples, threads):
+        if lib().sphr_jobs_set_lag_tile(self.h, int(samples), int(threads)):
+            raise RuntimeError("sphr_jobs_set_lag_tile failed: " + last_error())
 
     def __del__(self):
         try:

@@ -1050,36 +1050,51 @@ extern "C" int sphr_jobs_run(struct sphr_jobs *J, const float *in, void *alm, vo
 }
 
 /* autotuner: time the lag-correlation tiles on the caller's data and keep
- * the fastest (the graph is re-captured on the next run) */
+ * the fastest (the graph is re-captured on the next run).  On any failure
+ * the tile in use before the call is restored and -1 returned. */
 extern "C" int sphr_jobs_tune(struct sphr_jobs *J, const float *in, void *alm, double *best_ms)
 {
 	static const int cand[][2] = { { 512, 128 }, { 1024, 128 }, { 2048, 128 }, { 512, 256 }, { 1024, 256 }, { 2048, 256 } };
-	hipStream_t st = sphr_rt_stream();
-	hipEvent_t e0, e1;
-	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
+	if(!J || !in || !alm) {
+		sphr_set_error("sphr_jobs_tune: bad arguments");
 		return -1;
+	}
+	hipStream_t st = sphr_rt_stream();
+	const int sc0 = J->lag_sc, nt0 = J->lag_nt;
+	hipEvent_t e0 = NULL, e1 = NULL;
 	float best = 1e30f;
-	int bi = 0;
+	int bi = -1;
+	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
+		goto done;
 	for(unsigned k = 0; k < sizeof(cand) / sizeof(cand[0]); k++) {
 		J->lag_sc = cand[k][0];
 		J->lag_nt = cand[k][1];
-		if(run_direct(J, in, alm, NULL, st))
-			return -1;
-		hipEventRecord(e0, st);
-		for(int r = 0; r < 3; r++)
-			run_direct(J, in, alm, NULL, st);
-		hipEventRecord(e1, st);
-		if(sphr_rt_check(hipEventSynchronize(e1), "tune"))
-			return -1;
+		int r = run_direct(J, in, alm, NULL, st);
+		r = r || sphr_rt_check(hipEventRecord(e0, st), "tune");
+		for(int i = 0; i < 3 && !r; i++)
+			r = run_direct(J, in, alm, NULL, st);
+		r = r || sphr_rt_check(hipEventRecord(e1, st), "tune") || sphr_rt_check(hipEventSynchronize(e1), "tune");
 		float ms = 0.f;
-		hipEventElapsedTime(&ms, e0, e1);
+		r = r || sphr_rt_check(hipEventElapsedTime(&ms, e0, e1), "tune");
+		if(r) {
+			bi = -1;
+			goto done;
+		}
 		if(ms < best) {
 			best = ms;
 			bi = (int) k;
 		}
 	}
-	hipEventDestroy(e0);
-	hipEventDestroy(e1);
+done:
+	if(e0)
+		hipEventDestroy(e0);
+	if(e1)
+		hipEventDestroy(e1);
+	if(bi < 0) {
+		J->lag_sc = sc0;
+		J->lag_nt = nt0;
+		return -1;
+	}
 	J->lag_sc = cand[bi][0];
 	J->lag_nt = cand[bi][1];
 	if(J->exec) {
@@ -1091,6 +1106,26 @@ extern "C" int sphr_jobs_tune(struct sphr_jobs *J, const float *in, void *alm, d
 	if(best_ms)
 		*best_ms = best / 3.0;
 	return bi;
+}
+
+/* choose a lag tile by hand: one of the tuner's candidates (samples per
+ * chunk 512, 1024 or 2048; 128 or 256 threads).  The captured graph is
+ * dropped so the next run uses it. */
+extern "C" int sphr_jobs_set_lag_tile(struct sphr_jobs *J, int samples, int threads)
+{
+	if(!J || (samples != 512 && samples != 1024 && samples != 2048) || (threads != 128 && threads != 256)) {
+		sphr_set_error("sphr_jobs_set_lag_tile: not a tuner tile");
+		return -1;
+	}
+	J->lag_sc = samples;
+	J->lag_nt = threads;
+	if(J->exec) {
+		hipGraphExecDestroy(J->exec);
+		J->exec = NULL;
+		hipGraphDestroy(J->graph);
+		J->graph = NULL;
+	}
+	return 0;
 }
 
 /* [0] input floats, [1] a_lm complex elements, [2] map complex elements,

@@ -115,3 +115,88 @@ def test_gpu_jobs_tune_and_uneven_workload():
     got = d_alm.to_numpy(np.complex64, (info["alm_elems"],))
     assert np.all(np.isfinite(got))
     assert rel(got, ref) < 1e-5
+
+
+def top_jobs():
+    """Jobs at the top of the uneven workload's range: five sites, 8192
+    samples, l_max 64 (ten baselines, natural orders up to ~548); an
+    antipodal pair (the longest baseline, |d| = 42.5 ms); a close pair
+    whose natural order is below the target l_max, so its power stays at
+    its own order."""
+    rng = np.random.default_rng(31)
+    r = inject.EARTH_RADIUS_SPHERE / inject.C_SI
+    v5 = rng.standard_normal((5, 3))
+    a = rng.standard_normal(3)
+    anti = np.stack([a, -a])
+    b = rng.standard_normal(3)
+    b /= np.linalg.norm(b)
+    t = np.cross(b, [0.0, 0.0, 1.0])
+    close = np.stack([b, b + 0.02 * t / np.linalg.norm(t)])
+    jobs = []
+    for v, N, L in ((v5, 8192, 64), (anti, 8192, 64), (close, 512, 64)):
+        jobs.append({"positions": r * v / np.linalg.norm(v, axis=1, keepdims=True), "N": N, "l_max": L})
+    data = [rng.standard_normal((j["positions"].shape[0], j["N"])).astype(np.float32) for j in jobs]
+    return jobs, data
+
+
+@pytest.fixture(scope="module")
+def top_case(oracle):
+    jobs, data = top_jobs()
+    want = []
+    for j, d in zip(jobs, data):
+        onet = oracle.Network(j["positions"], j["N"], DT, j["l_max"])
+        want.append(onet.segment(d.astype(np.float64), j["l_max"]))
+    # the close pair's natural order is below its target
+    onet = oracle.Network(jobs[2]["positions"], jobs[2]["N"], DT)
+    assert onet.info(0)["l_max"] < jobs[2]["l_max"]
+    return jobs, data, want
+
+
+def _check_jobs(J, jobs, want, d_in, cdt, tol):
+    info = J.info()
+    d_alm = S.DeviceBuffer(info["alm_elems"] * np.dtype(cdt).itemsize)
+    d_map = S.DeviceBuffer(info["map_elems"] * np.dtype(cdt).itemsize)
+    J.run(d_in.ptr, d_alm.ptr, d_map.ptr)
+    S.synchronize()
+    alm = d_alm.to_numpy(cdt, (info["alm_elems"],))
+    mp = d_map.to_numpy(cdt, (info["map_elems"],))
+    errs = []
+    for k, (j, (sky, mesh)) in enumerate(zip(jobs, want)):
+        L = j["l_max"]
+        a = alm[J.off_alm[k]:J.off_alm[k] + (L + 1) ** 2]
+        m = mp[J.off_map[k]:J.off_map[k] + (2 * L + 2) ** 2].reshape(2 * L + 2, 2 * L + 2)
+        errs.append((rel(a, sky), rel(m, mesh)))
+    print("errors", errs, flush=True)
+    for ea, em in errs:
+        assert ea < tol and em < tol, errs
+
+
+@pytest.mark.gpu
+def test_gpu_jobs_top_of_range_match_oracle(top_case):
+    jobs, data, want = top_case
+    d_in = S.DeviceBuffer.from_numpy(np.concatenate([d.ravel() for d in data]))
+    for prec, cdt, tol in ((S.FP64, np.complex128, 1e-12), (S.MIXED, np.complex64, 1e-5)):
+        print("top-of-range jobs", prec, flush=True)
+        J = S.Jobs(jobs, DT, precision=prec)
+        _check_jobs(J, jobs, want, d_in, cdt, tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [S.FP64, S.MIXED])
+def test_gpu_jobs_tuned_tiles_match_oracle(top_case, prec):
+    """every lag tile the tuner can keep gives the oracle's answer: the
+    float64 lag kernel splits samples over idle lanes differently per
+    tile"""
+    jobs, data, want = top_case
+    cdt, tol = (np.complex128, 1e-12) if prec == S.FP64 else (np.complex64, 1e-5)
+    d_in = S.DeviceBuffer.from_numpy(np.concatenate([d.ravel() for d in data]))
+    J = S.Jobs(jobs, DT, precision=prec)
+    info = J.info()
+    d_alm = S.DeviceBuffer(info["alm_elems"] * np.dtype(cdt).itemsize)
+    k, ms = J.tune(d_in.ptr, d_alm.ptr)
+    print("tuned", prec, k, "%.3f ms" % ms, J.info(), flush=True)
+    _check_jobs(J, jobs, want, d_in, cdt, tol)
+    for sc, nt in ((512, 128), (2048, 128), (1024, 256)):
+        J.set_lag_tile(sc, nt)
+        print("lag tile", sc, nt, flush=True)
+        _check_jobs(J, jobs, want, d_in, cdt, tol)

@@ -3,17 +3,22 @@ RapidLocalization.py:246-298): oracle pins on the CPU, then the library's
 projection multiply, per-event skies and the batched GPU path against the
 oracle.
 
-Pins: the Tukey window's closed form; the padded length at 4096 Hz (174
-samples of light-travel guard for 410 -> 584); the antenna response of a
-zenith source on a polar detector (F+ = -cos 2psi, Fx = sin 2psi) and the
-psi-invariance of F+^2 + Fx^2; the projection operator is a rank-one
+Pins: the Tukey window's closed form and scipy.signal.windows.tukey (the
+textbook definition, computed by code that shares nothing with the
+oracle); the padded length at 4096 Hz (174 samples of light-travel guard
+for 410 -> 584); the antenna response of a zenith source on a polar
+detector (F+ = -cos 2psi, Fx = sin 2psi), the textbook pattern of an
+L-shaped detector at the pole for sources anywhere on the sky, covariance
+under a rotation of the Earth frame about z (detector and source turned
+together) and the psi-invariance of F+^2 + Fx^2; the projection operator is a rank-one
 projector minus the identity (trace 1 - n, Hermitian, (p + 1)^2 = p + 1);
 the log prior is axisymmetric and even, with a_00 close to the analytic
 sqrt(4 pi) (log 2 - 1 - log 4 pi); a sky from zero data is the rotated,
 l-weighted prior alone, and the data part scales as the square of the data.
 
 The LAL pieces (antenna response formula, Tukey definition) have no
-reference test: parity unpinned for those beyond the closed forms above.
+reference test in the reference; they are pinned by the closed forms and
+independent implementations above.
 """
 import math
 
@@ -42,6 +47,18 @@ def test_oracle_tukey_closed_form(oracle):
     assert np.all(w[20:381] == 1.0)
 
 
+@pytest.mark.parametrize("n", [2, 3, 21, 410, 411, 584, 1000])
+def test_oracle_tukey_matches_scipy(oracle, n):
+    # scipy's Tukey window (sym=True) tapers n <= floor(beta (n-1)/2) with
+    # 0.5 (1 + cos(pi (-1 + 2 n / (beta (n-1))))), the textbook form; the
+    # oracle restates LAL's |y|-parametrised cos^2 taper.  A linear or
+    # half-width taper fails this at every n > 21.
+    from scipy.signal import windows
+    for beta in (0.1, 0.5, 1.0):
+        ref = windows.tukey(n, beta)
+        assert np.max(np.abs(oracle.tukey(n, beta) - ref)) < 1e-14, (n, beta)
+
+
 def test_oracle_precalc_length(oracle):
     assert oracle.precalc_length(410, 1.0 / 4096) == 584
     assert oracle.precalc_length(21, DT_SMALL) == N_SMALL
@@ -68,6 +85,60 @@ def test_oracle_antenna_response(oracle):
         a = oracle.det_response(Dr.ravel(), ra, dec, 0.0, 0.4)
         b = oracle.det_response(Dr.ravel(), ra, dec, 0.7, 0.4)
         assert abs(a[0] ** 2 + a[1] ** 2 - b[0] ** 2 - b[1] ** 2) < 1e-15
+
+
+def test_oracle_antenna_response_textbook_pattern(oracle):
+    # An L-shaped detector at the north pole with arms along x and y,
+    # D = (xx - yy)/2, sees a source at polar angle theta = pi/2 - dec and
+    # azimuth phi = ra - gmst with the textbook pattern (Thorne 1987,
+    # eqs. 104a-b; Schutz and Tinto 1987):
+    #   F+ = 1/2 (1 + cos^2 th) cos 2ph cos 2ps - cos th sin 2ph sin 2ps
+    #   Fx = 1/2 (1 + cos^2 th) cos 2ph sin 2ps + cos th sin 2ph cos 2ps
+    # whose azimuth is measured a quarter turn away from LAL's and whose
+    # polarisation angle turns the other way: ph = phi + pi/2, ps = -psi.
+    # One mapping has to fit both outputs at every sky position; a sign
+    # error in any component of the wave frame, swapped X and Y or gmst
+    # entering with the wrong sign fails it.  F+^2 + Fx^2 needs no mapping.
+    D = np.diag([0.5, -0.5, 0.0])
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        ra, dec = rng.uniform(0, 2 * math.pi), rng.uniform(-1.5, 1.5)
+        psi, gmst = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi)
+        fp, fx = oracle.det_response(D.ravel(), ra, dec, psi, gmst)
+        c = math.cos(math.pi / 2 - dec)
+        ph, ps = ra - gmst + math.pi / 2, -psi
+        tp = 0.5 * (1 + c * c) * math.cos(2 * ph) * math.cos(2 * ps) - c * math.sin(2 * ph) * math.sin(2 * ps)
+        tx = 0.5 * (1 + c * c) * math.cos(2 * ph) * math.sin(2 * ps) + c * math.sin(2 * ph) * math.cos(2 * ps)
+        assert abs(fp - tp) < 1e-15 and abs(fx - tx) < 1e-15
+        mod = 0.25 * (1 + c * c) ** 2 * math.cos(2 * (ra - gmst)) ** 2 + c * c * math.sin(2 * (ra - gmst)) ** 2
+        assert abs(fp * fp + fx * fx - mod) < 1e-15
+
+
+def test_oracle_antenna_response_earth_rotation(oracle):
+    # Turning the detector by gamma about the Earth's axis (D -> Rz D Rz^T)
+    # and the source by the same angle (ra -> ra + gamma) leaves the
+    # response unchanged: the polarisation frame is fixed to local north
+    # and east, which turn with them.  Turning the source the other way
+    # does not, so the check fixes the sign of the hour angle.
+    rng = np.random.default_rng(12)
+    for name in "ABC":
+        Dr = inject.responses(name)[0]
+        worst_wrong = 0.0
+        for _ in range(30):
+            g = rng.uniform(0, 2 * math.pi)
+            Rz = np.array([[math.cos(g), -math.sin(g), 0.0], [math.sin(g), math.cos(g), 0.0], [0.0, 0.0, 1.0]])
+            D2 = Rz @ Dr @ Rz.T
+            ra, dec = rng.uniform(0, 2 * math.pi), rng.uniform(-1.5, 1.5)
+            psi, gmst = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi)
+            a = oracle.det_response(Dr.ravel(), ra, dec, psi, gmst)
+            b = oracle.det_response(D2.ravel(), ra + g, dec, psi, gmst)
+            assert abs(a[0] - b[0]) < 1e-14 and abs(a[1] - b[1]) < 1e-14
+            # gmst and ra enter only through the hour angle gmst - ra
+            c = oracle.det_response(Dr.ravel(), ra + g, dec, psi, gmst + g)
+            assert abs(a[0] - c[0]) < 1e-14 and abs(a[1] - c[1]) < 1e-14
+            w = oracle.det_response(D2.ravel(), ra - g, dec, psi, gmst)
+            worst_wrong = max(worst_wrong, abs(a[0] - w[0]), abs(a[1] - w[1]))
+        assert worst_wrong > 0.1
 
 
 def test_oracle_projection_is_projector_minus_identity(oracle):
@@ -268,3 +339,49 @@ def test_gpu_batched_localization_matches_oracle(oracle, local_minimum):
                 t, p, v = (peaks["map"][q][bi, e] for q in range(3))
                 local_minimum(-want[bi], L, t, p, -v, tol, seed=False)
                 assert v >= mp[bi, e].real.max() - tol
+
+
+DT_1K, LEN_1K, L_CFG = 1.0 / 1024, 103, 50
+
+
+@pytest.fixture(scope="module")
+def plans_1k(oracle):
+    """Plans at the localization order l_max = 50: three sites at 1024 Hz,
+    +-50 ms series (103 samples, 147 bins after the light-travel padding),
+    natural orders 33, 88 and 85, so two baselines are truncated to 50 by
+    set_l after the projection multiply (pixel-domain products, as at
+    4096 Hz) and one stays below it (microcode products)."""
+    n = oracle.precalc_length(LEN_1K, DT_1K)
+    out = {}
+    for beta in (1.0, -1.0):
+        onet = oracle.Network(inject.positions("ABC"), n, DT_1K)
+        onet.mult_by_projection(det_responses("ABC"), beta, 0.0, np.ones((n, 3)))
+        assert sorted(onet.info(k)["l_max"] for k in range(3)) == [33, 85, 88]
+        onet.set_l(L_CFG)
+        net = lib_network(beta, n, DT_1K)
+        net.set_l(L_CFG)
+        out[beta] = (net, onet)
+    return n, out
+
+
+@pytest.mark.gpu
+def test_gpu_batched_localization_l50_matches_oracle(oracle, plans_1k):
+    n, nets = plans_1k
+    netp, onp = nets[1.0]
+    netn, onn = nets[-1.0]
+    for k in range(3):
+        assert rel(netp.delay_product(k), onp.delay_product(k)) < 1e-12, k
+        assert rel(netn.delay_product(k), onn.delay_product(k)) < 1e-12, k
+    L = L_CFG
+    lp = oracle.log_uniformsky_prior(L)
+    E = 4
+    x, meta = inject.snr_events(E, length=LEN_1K, delta_t=DT_1K, seed=21)
+    for prec, cdt, tol in ((S.FP64, np.complex128, 1e-12), (S.MIXED, np.complex64, 1e-5)):
+        loc = S.Localizer(netp, netn, lp, L, LEN_1K, DT_1K, precision=prec, max_events=E)
+        print("localizer l=50", prec, loc.info(), flush=True)
+        alm, mp, _ = _run_loc(loc, x, meta["gmst"], L, cdt)
+        for e in range(E):
+            want = [o.localize(x[:, e].astype(np.complex128), DT_1K, meta["gmst"][e], lp, L) for o in (onp, onn)]
+            for bi in range(2):
+                assert rel(alm[bi, e], want[bi]) < tol, (prec, e, bi, rel(alm[bi, e], want[bi]))
+                assert rel(mp[bi, e], oracle.to_mesh(want[bi], L)) < tol, (prec, e, bi)

@@ -116,6 +116,35 @@ def test_gpu_mlp_loss_and_gradients_match_oracle(P):
 
 
 @pytest.mark.gpu
+def test_gpu_mlp_full_size_gradients_match_oracle(P):
+    """the proposal network at its real size: 1089 -> 3 x 512 (GELU) -> 4
+    candidates, float32 operands on the MFMA path, against the float64
+    oracle"""
+    shp = (1089, 512, 3, 4)
+    rows = 192
+    rng = np.random.default_rng(15)
+    m = P.MLP(*shp, max_batch=256, bf16=False, seed=3)
+    p = m.params().astype(np.float64)
+    x = rng.standard_normal((rows, shp[0])).astype(np.float32)
+    t = rng.standard_normal((rows, 3))
+    t = (t / np.linalg.norm(t, axis=1, keepdims=True)).astype(np.float32)
+    dx, dt, dl = S.DeviceBuffer.from_numpy(x), S.DeviceBuffer.from_numpy(t), S.DeviceBuffer(rows * 4)
+    m.loss_grad(dx.ptr, dt.ptr, rows, 0.05, dl.ptr)
+    S.synchronize()
+    lo, go, lrow = OM.loss_grad(p, x.astype(np.float64), t.astype(np.float64), *shp, 0.05)
+    assert rel(dl.to_numpy(np.float32, (rows,)), lrow) < 1e-5
+    g = m.grads()
+    # per layer, so that a wrong block cannot hide behind a large one
+    off = 0
+    for din, dout in OM.shapes(*shp):
+        for n in (din * dout, dout):
+            e = rel(g[off:off + n], go[off:off + n])
+            assert e < 1e-4, (din, dout, n, e)
+            off += n
+    assert off == g.size
+
+
+@pytest.mark.gpu
 def test_gpu_mlp_training_reduces_loss(P):
     # targets a fixed smooth function of the inputs: Adam must fit it
     shp = (32, 64, 3, 2)

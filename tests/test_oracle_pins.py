@@ -204,16 +204,18 @@ def _random_series(rng, l, polar):
     return rng.uniform(-s, s, n) + 1j * rng.uniform(-s, s, n)
 
 
-@pytest.mark.parametrize("l1,p1,l2,p2", [(5, 0, 10, 0), (5, 1, 10, 1), (5, 1, 10, 0), (40, 1, 70, 1), (12, 0, 9, 0)])
+@pytest.mark.parametrize("l1,p1,l2,p2", [(5, 0, 10, 0), (5, 1, 10, 1), (5, 1, 10, 0), (10, 0, 5, 1), (7, 1, 3, 0),
+                                         (40, 1, 70, 1), (12, 0, 9, 0), (30, 1, 35, 0)])
 def test_product_pointwise(oracle, l1, p1, l2, p2):
     # product_validate.c:46-100: the product series evaluates to the product
-    # of the factors at random points (|err| / (l1^2 + l2^2) <= 1e-13)
+    # of the factors at random points (|err| / (l1^2 + l2^2) <= 1e-13).
+    # Mixed polar flags are the localization plan's case (a polar delay
+    # product times a non-polar projection, sh_series_product.c:123-162);
+    # below l = 60 they run the microcode, above it the pixel product.
     rng = np.random.default_rng(l1 * 100 + l2)
     a = _random_series(rng, l1, p1)
     b = _random_series(rng, l2, p2)
     dl, dp = l1 + l2, p1 and p2
-    if p1 != p2 and dl < 60:
-        pytest.skip("the microcode plan needs equal polar flags")
     prod = oracle.product(a, l1, p1, b, l2, p2, dl, dp)
     for _ in range(100):
         th, ph = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi)
@@ -223,13 +225,15 @@ def test_product_pointwise(oracle, l1, p1, l2, p2):
         assert abs(va * vb - vp) / (l1 * l1 + l2 * l2) < 1e-13
 
 
-@pytest.mark.parametrize("l1,l2,dl,polar", [(6, 9, 15, 1), (20, 25, 30, 1), (8, 7, 12, 0)])
-def test_microcode_equals_pixel_product(oracle, l1, l2, dl, polar):
-    rng = np.random.default_rng(dl)
-    a = _random_series(rng, l1, polar)
-    b = _random_series(rng, l2, polar)
-    pm = oracle.product(a, l1, polar, b, l2, polar, dl, polar, mode=1)
-    pp = oracle.product(a, l1, polar, b, l2, polar, dl, polar, mode=0)
+@pytest.mark.parametrize("l1,p1,l2,p2,dl", [(6, 1, 9, 1, 15), (20, 1, 25, 1, 30), (8, 0, 7, 0, 12),
+                                           (6, 1, 9, 0, 15), (9, 0, 6, 1, 12), (20, 1, 25, 0, 30)])
+def test_microcode_equals_pixel_product(oracle, l1, p1, l2, p2, dl):
+    rng = np.random.default_rng(dl + 7 * p2)
+    a = _random_series(rng, l1, p1)
+    b = _random_series(rng, l2, p2)
+    dp = p1 and p2
+    pm = oracle.product(a, l1, p1, b, l2, p2, dl, dp, mode=1)
+    pp = oracle.product(a, l1, p1, b, l2, p2, dl, dp, mode=0)
     assert np.max(np.abs(pm - pp)) < 1e-14 * (l1 + l2)
 
 

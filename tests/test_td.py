@@ -152,9 +152,21 @@ def test_gpu_td_flat_windows_match_oracle(oracle, kind, extra):
     L.correlator_plan_td_free(plan)
 
 
+def _window(L, kind, m, rng):
+    if kind == "square":
+        return np.full(m, 1.0 / m)
+    if kind == "tukey":
+        return np.ctypeslib.as_array(L.correlator_tukey_window_new(m, 0, m // 8, 1.0 / m), (m,)).copy()
+    return rng.uniform(0.5, 1.0, m) / m
+
+
 @pytest.mark.gpu
-def test_gpu_td_network_matches_oracle(oracle):
-    print("td network", flush=True)
+@pytest.mark.parametrize("kinds", [("random",) * 3, ("square",) * 3, ("tukey",) * 3, ("square", "random", "tukey")])
+def test_gpu_td_network_matches_oracle(oracle, kinds):
+    """the network integrator takes, per baseline, the window-difference
+    route for flat windows (square, Tukey) and the full correlation for
+    the others; every mix against the oracle"""
+    print("td network", kinds, flush=True)
     net = _Net(POS)
     L = S.lib()
     plan = L.correlator_network_plan_td_new(net.bl, DT)
@@ -164,7 +176,7 @@ def test_gpu_td_network_matches_oracle(oracle):
     rng = np.random.default_rng(7)
     n = 2 * max(trs) + 700
     ts = rng.standard_normal((3, n))
-    wins = [rng.uniform(0.5, 1.0, n - 2 * t) / (n - 2 * t) for t in trs]
+    wins = [_window(L, kind, n - 2 * t, rng) for kind, t in zip(kinds, trs)]
     stride = max(len(w) for w in wins)
     wpack = np.zeros((3, stride))
     for k, w in enumerate(wins):
