@@ -54,6 +54,21 @@ void sphr_batch_free(struct sphr_batch *);
  * when map is NULL too.  Returns 0 when the launches were queued. */
 int sphr_batch_run(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, void *stream);
 
+/* sphr_batch_run without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory while they run (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as sphr_batch_run; the spectra,
+ * K-split partials and ring sums go to a static 64 MiB device array that
+ * the library holds on each GPU it runs on until the process ends, so
+ * n_segments is bounded by that size instead of max_segments (at three
+ * sites, N = 1024, l_max 32: 1024 segments in float64).  Larger calls
+ * return -1 with the byte count in sphr_last_error().  No allocation, no
+ * profiling events.  Every call on one GPU shares the array: calls that
+ * overlap on one GPU (other streams, other threads) race; calls queued on
+ * one stream do not.  The batch object itself is made beforehand by
+ * sphr_batch_new. */
+int sphr_batch_run_static(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, void *stream);
+
 /* Per-stage timing: while enabled, every run records HIP events on its
  * stream around its stages (FFT, projection, a_lm + Legendre ring sums,
  * phi DFT); _read synchronises and returns the summed milliseconds per

@@ -220,6 +220,7 @@ _PROTOS = {
     "sphr_batch_new": (c_void_p, [P(NetworkPlanFD), c_uint, c_int, c_int]),
     "sphr_batch_free": (None, [c_void_p]),
     "sphr_batch_run": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sphr_batch_run_static": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sphr_batch_profile": (c_int, [c_void_p, c_int]),
     "sphr_batch_profile_read": (c_int, [c_void_p, P(c_double), P(c_int)]),
     "sphr_batch_info": (None, [c_void_p, P(c_int)]),
@@ -550,6 +551,13 @@ class Batch:
         r = lib().sphr_batch_run(self.h, d_tseries, int(f32_input), n_segments, d_window, d_gmst, d_alm, d_map, stream)
         if r:
             raise RuntimeError("sphr_batch_run failed: " + last_error())
+
+    def run_static(self, d_tseries, n_segments, d_alm, d_map=None, d_window=None, d_gmst=None, f32_input=False, stream=None):
+        """sphr_batch_run_static: the same run with the library's static
+        device scratch (no allocation)."""
+        r = lib().sphr_batch_run_static(self.h, d_tseries, int(f32_input), n_segments, d_window, d_gmst, d_alm, d_map, stream)
+        if r:
+            raise RuntimeError("sphr_batch_run_static failed: " + last_error())
 
     def peaks(self, d_alm, n_segments, d_theta, d_phi, d_value, d_map=None, find_max=True, stream=None):
         """Queue the per-segment peak search (sphr_batch_peaks); theta, phi

@@ -16,6 +16,7 @@
  *   synthesis tables of order L and the FFT twiddles
  */
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -238,34 +239,37 @@ fail:
 	return NULL;
 }
 
+/* one run on the given scratch: spectra, K-split partials and (for the
+ * two-kernel sky form) ring sums; F may be NULL, then the sky stage uses
+ * the fused kernel or the one-kernel form */
 template <typename T>
-static int run_t(struct sphr_batch *bt, const void *ts, int f32, int S, const double *window, const double *gmst, void *alm, void *map, hipStream_t st, hipEvent_t *ev)
+static int run_t(struct sphr_batch *bt, void *spec, void *partial, void *F, const void *ts, int f32, int S, const double *window, const double *gmst, void *alm, void *map, hipStream_t st, hipEvent_t *ev)
 {
 	if(ev)
 		hipEventRecord(ev[0], st);
-	if(launch_rfft_pairs<T>(ts, f32, window, bt->D * S, bt->N, (const cplx<T> *) bt->tw, (cplx<T> *) bt->spec, st))
+	if(launch_rfft_pairs<T>(ts, f32, window, bt->D * S, bt->N, (const cplx<T> *) bt->tw, (cplx<T> *) spec, st))
 		return -1;
 	if(ev)
 		hipEventRecord(ev[1], st);
-	if(launch_xproj<T>((const cplx<T> *) bt->spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) bt->partial, st, bt->tpw, bt->xms))
+	if(launch_xproj<T>((const cplx<T> *) spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) partial, st, bt->tpw, bt->xms))
 		return -1;
 	if(ev)
 		hipEventRecord(ev[2], st);
 	/* the fused sky kernel at the orders it is compiled for; else a_lm
 	 * and ring sums, then the phi DFT as a GEMM; the one-kernel form when
 	 * the coefficients do not fit the LDS budget */
-	if(bt->fused && launch_sky_fused<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st) == 0) {
+	if(bt->fused && launch_sky_fused<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st) == 0) {
 		if(ev)
 			hipEventRecord(ev[3], st);
-	} else if(launch_sky_lf<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) bt->F : (cplx<T> *) NULL, st) == 0) {
+	} else if((F || !map) && launch_sky_lf<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) F : (cplx<T> *) NULL, st) == 0) {
 		if(ev)
 			hipEventRecord(ev[3], st);
-		if(map && launch_phi<T>((const cplx<T> *) bt->F, S, &bt->syn.g, &bt->syn.tb, (cplx<T> *) map, st))
+		if(map && launch_phi<T>((const cplx<T> *) F, S, &bt->syn.g, &bt->syn.tb, (cplx<T> *) map, st))
 			return -1;
 	} else {
 		if(ev)
 			hipEventRecord(ev[3], st);
-		if(launch_sky<T>((const T *) bt->partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st))
+		if(launch_sky<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st))
 			return -1;
 	}
 	if(ev)
@@ -295,9 +299,81 @@ extern "C" int sphr_batch_run(struct sphr_batch *bt, const void *ts, int f32, in
 		}
 		ev = bt->ev[bt->ev_used++].e;
 	}
-	const int r = bt->precision == SPHR_MIXED ? run_t<float>(bt, ts, f32, S, window, gmst, alm, map, st, ev) : run_t<double>(bt, ts, f32, S, window, gmst, alm, map, st, ev);
+	const int r = bt->precision == SPHR_MIXED ? run_t<float>(bt, bt->spec, bt->partial, bt->F, ts, f32, S, window, gmst, alm, map, st, ev) : run_t<double>(bt, bt->spec, bt->partial, bt->F, ts, f32, S, window, gmst, alm, map, st, ev);
 	if(r)
 		sphr_set_error("sphr_batch_run: kernel launch failed (%s)", hipGetErrorString(hipGetLastError()));
+	return r;
+}
+
+/*
+ * The allocation-free run.  Its scratch (spectra, K-split partials, ring
+ * sums) lives in one file-scope device array instead of the batch
+ * object's buffers.  Each GPU holds its own copy of the array, at its own
+ * address, from the library's load until the process ends.  In host code
+ * the array's name is only a stand-in: its device address comes from
+ * hipGetSymbolAddress, looked up once per device and kept in
+ * scratch_dev[].  No kernel reads scratch it did not write in the same
+ * run, so nothing is cleared between calls.
+ */
+#define SPHR_SCRATCH_BYTES ((size_t) 64 << 20)
+#define SPHR_SCRATCH_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[SPHR_SCRATCH_BYTES];
+static void *scratch_dev[SPHR_SCRATCH_DEVICES];
+static pthread_mutex_t scratch_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static void *scratch_address(void)
+{
+	int dev = -1;
+	if(sphr_rt_check(hipGetDevice(&dev), "hipGetDevice"))
+		return NULL;
+	if(dev < 0 || dev >= SPHR_SCRATCH_DEVICES) {
+		sphr_set_error("sphr_batch_run_static: device %d beyond the %d the library serves", dev, SPHR_SCRATCH_DEVICES);
+		return NULL;
+	}
+	pthread_mutex_lock(&scratch_lock);
+	if(!scratch_dev[dev]) {
+		void *p = NULL;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess || !p)
+			sphr_set_error("sphr_batch_run_static: hipGetSymbolAddress of the scratch array failed (%s)", hipGetErrorString(e));
+		else
+			scratch_dev[dev] = p;
+	}
+	void *p = scratch_dev[dev];
+	pthread_mutex_unlock(&scratch_lock);
+	return p;
+}
+
+static size_t align256(size_t b)
+{
+	return (b + 255) & ~(size_t) 255;
+}
+
+extern "C" int sphr_batch_run_static(struct sphr_batch *bt, const void *ts, int f32, int S, const double *window, const double *gmst, void *alm, void *map, void *stream)
+{
+	if(!bt || S < 1 || !ts || !alm) {
+		sphr_set_error("sphr_batch_run_static: bad arguments");
+		return -1;
+	}
+	if(sphr_gpu_ensure())
+		return -1;
+	const size_t esz = bt->precision == SPHR_MIXED ? sizeof(float) : sizeof(double);
+	const int need_F = map && !(bt->fused && sky_fused_order(bt->L));
+	const size_t b_spec = align256((size_t) bt->D * S * bt->Nh * 2 * esz);
+	const size_t b_part = align256((size_t) bt->nsplit * S * bt->g.CT * esz);
+	const size_t b_F = need_F ? align256((size_t) S * 2 * (bt->L + 1) * (2 * bt->L + 1) * 2 * esz) : 0;
+	if(b_spec + b_part + b_F > SPHR_SCRATCH_BYTES) {
+		sphr_set_error("sphr_batch_run_static: %d segments need %zu bytes of scratch, more than the static %zu; run fewer segments per call", S, b_spec + b_part + b_F, SPHR_SCRATCH_BYTES);
+		return -1;
+	}
+	unsigned char *base = (unsigned char *) scratch_address();
+	if(!base)
+		return -1;
+	void *spec = base, *partial = base + b_spec, *F = need_F ? (void *) (base + b_spec + b_part) : NULL;
+	hipStream_t st = stream ? (hipStream_t) stream : sphr_rt_stream();
+	const int r = bt->precision == SPHR_MIXED ? run_t<float>(bt, spec, partial, F, ts, f32, S, window, gmst, alm, map, st, NULL) : run_t<double>(bt, spec, partial, F, ts, f32, S, window, gmst, alm, map, st, NULL);
+	if(r)
+		sphr_set_error("sphr_batch_run_static: kernel launch failed (%s)", hipGetErrorString(hipGetLastError()));
 	return r;
 }
 
