@@ -47,39 +47,46 @@ def stage_bytes(info, S, esz, in_bytes_per_sample=8):
     return [fft, xproj, lf, phi]
 
 
-def run_path(S, net, L, d_ts, nseg, precision, steps, warmup):
+def run_path(S, net, L, d_ts, nseg, precision, steps, warmup, d_window=None):
     cdt = np.complex128 if precision == S.FP64 else np.complex64
     esz = np.dtype(cdt).itemsize // 2
     b = S.Batch(net, L, precision, max_segments=nseg)
     d_alm = S.DeviceBuffer(nseg * (L + 1) ** 2 * 2 * esz)
     d_map = S.DeviceBuffer(nseg * (2 * L + 2) ** 2 * 2 * esz)
     st = S.lib().sphr_stream()
-    tuned, tuned_ms = b.tune(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr)
+    dw = d_window.ptr if d_window is not None else None
+    tuned, tuned_ms = b.tune(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, d_window=dw)
     for _ in range(warmup):
-        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, stream=st)
+        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, d_window=dw, stream=st)
     S.lib().sphr_stream_synchronize(st)
     e0, e1 = S.lib().sphr_event_new(), S.lib().sphr_event_new()
     S.lib().sphr_event_record(e0, st)
     for _ in range(steps):
-        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, stream=st)
+        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, d_window=dw, stream=st)
     S.lib().sphr_event_record(e1, st)
     S.lib().sphr_event_synchronize(e1)
     ms = S.lib().sphr_event_elapsed_ms(e0, e1)
     # per-stage times on the same stream (separate pass, events around each stage)
     b.profile(True)
     for _ in range(steps):
-        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, stream=st)
+        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, d_window=dw, stream=st)
     stage_ms, runs = b.profile_read()
     b.profile(False)
     alm = d_alm.to_numpy(cdt, (nseg, (L + 1) ** 2))
     mp = d_map.to_numpy(cdt, (nseg, 2 * L + 2, 2 * L + 2))
+    # each sky's peak: map seed + extremize on the series (sphr_batch_peaks)
+    d_pk = [S.DeviceBuffer(nseg * 8) for _ in range(3)]
+    b.peaks(d_alm.ptr, nseg, d_pk[0].ptr, d_pk[1].ptr, d_pk[2].ptr, d_map=d_map.ptr)
+    S.synchronize()
+    peaks = [d.to_numpy(np.float64, (nseg,)) for d in d_pk]
     info = b.info()
     S.lib().sphr_event_free(e0)
     S.lib().sphr_event_free(e1)
-    return dict(ms_per_step=ms / steps, stage_ms=[t / runs for t in stage_ms], alm=alm, map=mp, info=info, esz=esz, tuned=tuned)
+    return dict(ms_per_step=ms / steps, stage_ms=[t / runs for t in stage_ms], alm=alm, map=mp, info=info, esz=esz, tuned=tuned,
+                peaks=peaks)
 
 
-def plain_fp64(S, net, ts, L, budget_s, max_segments):
+def plain_fp64(S, net, ts, L, budget_s, max_segments, window=None):
     """The FP64 drop-in path as a reference caller drives it, one segment at
     a time through the C API: correlator_tseries_to_fseries per detector
     (plans made once), correlator_network_integrate_power_fd into a sky
@@ -100,7 +107,7 @@ def plain_fp64(S, net, ts, L, budget_s, max_segments):
 
     def one(s):
         for d in range(D):
-            tsb[d][:] = ts[d, s]
+            tsb[d][:] = ts[d, s] if window is None else ts[d, s] * window
             lib.correlator_tseries_to_fseries(tsb[d].ctypes.data, fsb[d].ctypes.data, N, plans[d])
         if not lib.correlator_network_integrate_power_fd(sky, fptrs, net.plan):
             raise RuntimeError("integrate failed: " + S.last_error())
@@ -123,22 +130,62 @@ def plain_fp64(S, net, ts, L, budget_s, max_segments):
     return dict(value=done / el, segments=done, last=done - 1, alm=alm)
 
 
-def cpu_baseline(ts, L, budget_s):
+def host_cpu():
+    """The host's usable cores and CPU model, for the CPU baseline."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
+
+
+def _timed_segments(net, ts, L, budget_s, seg_threads, mesh_threads, chunk):
+    """Runs chunks of segments through the oracle's pipeline until the
+    budget is spent; returns (segments, seconds)."""
+    done, el = 0, 0.0
+    S = ts.shape[1]
+    while done < S and el < budget_s:
+        k = min(chunk, S - done)
+        t0 = time.time()
+        net.segments_run(ts[:, done:done + k], L, seg_threads=seg_threads, mesh_threads=mesh_threads)
+        el += time.time() - t0
+        done += k
+    return done, el
+
+
+def cpu_baseline(ts, L, budget_s, positions=None, delta_t=1.0 / 4096, label="three-site"):
+    """The FP64 C oracle on this host in the two modes of the reference's
+    own threading: reference-faithful (integrate on one thread, the ring
+    loop of to_mesh on 4, sh_series_trans.c:829) and all host cores
+    (segments split over threads with per-thread plan scratch, as
+    correlator_network_plan_fd_copy callers do, correlator.c:908-935).
+    value = the all-cores rate."""
     from oracle import oracle as O
     from sphradiometer_amd import inject
-    pos = inject.positions("ABC")
+    pos = inject.positions("ABC") if positions is None else positions
+    ncpu, model = host_cpu()
+    threads = max(1, min(16, ncpu))
     t0 = time.time()
-    net = O.Network(pos, ts.shape[2], 1.0 / 4096, L)
+    net = O.Network(pos, ts.shape[2], delta_t, L)
     plan_s = time.time() - t0
-    done = 0
-    t0 = time.time()
-    while done < ts.shape[1] and time.time() - t0 < budget_s:
-        net.segment(ts[:, done], L)
-        done += 1
-    el = time.time() - t0
-    return dict(value=done / el, unit="sky maps/s", cores=1, kind="port",
-                sample="%d of the 1024 three-site segments (a_lm + 66x66 map each), FP64 C oracle, 1 thread; "
-                       "plan build %.2f s not included" % (done, plan_s))
+    net.segments_run(ts[:, :2], L, seg_threads=1, mesh_threads=4)
+    n1, e1 = _timed_segments(net, ts, L, budget_s / 2, 1, 4, 16)
+    nall, eall = _timed_segments(net, ts, L, budget_s / 2, threads, 1, 16 * threads)
+    return dict(value=nall / eall, unit="sky maps/s", cores=threads, kind="port",
+                sample="%d of the %d %s segments (a_lm + map each) over %d threads, FP64 C oracle (gcc -O3, AVX2, OpenMP); "
+                       "plan build %.2f s not included" % (nall, ts.shape[1], label, threads, plan_s),
+                reference_faithful={"value": round(n1 / e1, 2), "cores": 1, "mesh_threads": 4,
+                                    "sample": "%d segments in order, to_mesh's ring loop on 4 threads" % n1},
+                host={"nproc": ncpu, "cpu_model": model})
 
 
 FP64_MFMA_TFLOPS = 78.6
@@ -470,6 +517,58 @@ def _lmoff(L, l, m):
     return base + l - abs(m)
 
 
+def bench_two_site(S, args):
+    """The two-site workload: sites (46.45N 119.41W) and (30.56N 90.77W) on
+    a 6371 km sphere, white N(0,1) float64 at 4096 Hz from seed 0, 16
+    non-overlapping 1 s segments (4096 samples, symmetric Hann window),
+    sky l_max 25 (natural order 129 truncated by set_l).  One step = the
+    batched pipeline over the 16 segments (a_lm + 52 x 52 map each);
+    value = maps/s of the mixed-precision path, the float64 batch, the
+    float64 drop-in per segment and the CPU oracle beside it."""
+    from sphradiometer_amd import inject
+    L, N, nseg = 25, 4096, 16
+    pos = inject.positions("AB")
+    t0 = time.time()
+    net = S.Network(pos, N, 1.0 / 4096)
+    net.set_l(L)
+    plan_s = time.time() - t0
+    ts = inject.two_site(nseg, N, seed=0)
+    w = inject.hann(N)
+    d_ts = S.DeviceBuffer.from_numpy(ts)
+    d_w = S.DeviceBuffer.from_numpy(w)
+    r64 = run_path(S, net, L, d_ts, nseg, S.FP64, args.steps, args.warmup, d_window=d_w)
+    rmx = run_path(S, net, L, d_ts, nseg, S.MIXED, args.steps, args.warmup, d_window=d_w)
+    print("fp64 %.4f ms, mixed %.4f ms per step of %d segments" % (r64["ms_per_step"], rmx["ms_per_step"], nseg), file=sys.stderr, flush=True)
+    pl = plain_fp64(S, net, ts, L, args.plain_budget, nseg, window=w)
+    plain_err = float(np.linalg.norm(pl["alm"] - r64["alm"][pl["last"]]) / np.linalg.norm(r64["alm"][pl["last"]]))
+    alm_err = np.linalg.norm(rmx["alm"] - r64["alm"], axis=1) / np.linalg.norm(r64["alm"], axis=1)
+    map_err = float(np.linalg.norm(rmx["map"] - r64["map"]) / np.linalg.norm(r64["map"]))
+    value = nseg / (rmx["ms_per_step"] * 1e-3)
+    ncoef, nmesh = (L + 1) ** 2, (2 * L + 2) ** 2
+    b_map = 2 * N * 8 + (ncoef + nmesh) * 8 + (N + 2) * rmx["info"]["columns"] * 4 / nseg
+    gbs = b_map * nseg / (rmx["ms_per_step"] * 1e-3) / 1e9
+    out = {
+        "metric": "sky maps/s, 2-det l_max=25, 16 x 4096-sample Hann segments (batched mixed precision)",
+        "value": round(value, 1), "unit": "sky maps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(rmx["ms_per_step"], 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic: white N(0,1) float64 per detector (seed 0), Hann-windowed",
+        "config": {"workload": "two sites, 16 segments x 4096 samples at 4096 Hz, Hann window, sky l_max 25, a_lm + 52x52 map per segment",
+                   "segments": nseg, "samples": N, "l_max": L, "detectors": 2, "baselines": 1},
+        "fp64": {"value": round(nseg / (r64["ms_per_step"] * 1e-3), 1), "ms_per_step": round(r64["ms_per_step"], 4)},
+        "fp64_plain": {"value": round(pl["value"], 1), "segments": pl["segments"], "alm_rel_err_vs_batched_fp64": plain_err},
+        "mixed_vs_fp64": {"alm_rel_err_max": float(alm_err.max()), "map_rel_err": map_err},
+        "stage_ms": [round(t, 5) for t in rmx["stage_ms"]],
+        "roofline": {"bound": "hbm", "kernel": "whole step (16 segments: launch-latency bound)", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_map": round(b_map, 1)},
+        "plan_build_s": round(plan_s, 2),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ts * w, L, args.cpu_budget, positions=pos, label="two-site")
+        out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 2)
+    print(json.dumps(out))
+
+
 def bench_copy(S, args):
     """A plain device-to-device copy of 256 MiB, five times: the reference
     kernel of known size for scaling rocprofv3's FETCH_SIZE and WRITE_SIZE
@@ -506,11 +605,12 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--segments", type=int, default=1024)
-    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-budget", type=float, default=3.0)
-    ap.add_argument("--workload", choices=["sky", "localization", "jobs", "healpix", "td", "copy"], default="sky",
-                    help="sky: the three-site cross-power maps (default); localization: the log-likelihood skies; "
+    ap.add_argument("--workload", choices=["sky", "two_site", "localization", "jobs", "healpix", "td", "copy"], default="sky",
+                    help="sky: the three-site cross-power maps (default); two_site: 2 sites, l_max 25, 16 Hann segments; "
+                         "localization: the log-likelihood skies; "
                          "jobs: the uneven batch with a plan per job; healpix: RING maps + posterior of localization skies; "
                          "td: the time-domain correlator over a long stream")
     ap.add_argument("--events", type=int, default=8192)
@@ -542,6 +642,9 @@ def main():
     if args.workload == "copy":
         bench_copy(S, args)
         return
+    if args.workload == "two_site":
+        bench_two_site(S, args)
+        return
 
     L, N, nseg = 32, 1024, args.segments
     t0 = time.time()
@@ -570,9 +673,25 @@ def main():
     names = ["fft", "xproj", "sky_fused", "none"] if rmx["info"].get("fused") else ["fft", "xproj", "alm_legendre", "phi_gemm"]
     sb = stage_bytes(rmx["info"], nseg, rmx["esz"])
     k = int(np.argmax(rmx["stage_ms"]))
-    achieved = sb[k] / (rmx["stage_ms"][k] * 1e-3) / 1e9
     sb64 = stage_bytes(r64["info"], nseg, r64["esz"])
     k64 = int(np.argmax(r64["stage_ms"]))
+    # the dominant (fused sky) kernel's algorithmic bytes: the a_lm and maps
+    # it writes and the rotation rows it reads; the K-split partial sums it
+    # reads are an intermediate of the pipeline, reported beside
+    info, esz = rmx["info"], rmx["esz"]
+    ncoef, nmesh = (L + 1) ** 2, (2 * L + 2) ** 2
+    sky_out = nseg * (ncoef + nmesh) * 2 * esz + info["baselines"] * ncoef * 2 * esz
+    partial_rd = info["k_splits"] * nseg * info["columns"] * esz
+    t_dom = rmx["stage_ms"][k] * 1e-3
+    achieved = (sky_out if names[k] == "sky_fused" else sb[k]) / t_dom / 1e9
+    # the whole step against the per-map bytes of the path (series in,
+    # a_lm + map out, the delay-product table shared by the batch)
+    b_map = 3 * N * 8 + ncoef * 2 * esz + nmesh * 2 * esz + (N + 2) * info["columns"] * esz / nseg
+    step_gbs = b_map * nseg / (rmx["ms_per_step"] * 1e-3) / 1e9
+    # peak direction: mixed against float64 (both map-seeded + extremize)
+    sep = np.degrees(np.arccos(np.clip(np.cos(rmx["peaks"][0]) * np.cos(r64["peaks"][0]) + np.sin(rmx["peaks"][0]) * np.sin(r64["peaks"][0])
+                                       * np.cos(rmx["peaks"][1] - r64["peaks"][1]), -1.0, 1.0)))
+    pk_val_err = np.abs(rmx["peaks"][2] - r64["peaks"][2]) / np.abs(r64["peaks"][2])
 
     out = {
         "metric": "sky maps/s, 3-det l_max=32: mixed-prec batched vs FP64 plain at <=1e-5 a_lm err",
@@ -597,7 +716,10 @@ def main():
                        "alm_rel_err_vs_batched_fp64": plain_err},
         "mixed_vs_fp64_plain": round(value / pl["value"], 1),
         "mixed_vs_fp64": {"speedup": round(value / fp64_value, 3), "alm_rel_err_max": float(alm_err.max()),
-                          "alm_rel_err_median": float(np.median(alm_err)), "map_rel_err": map_err},
+                          "alm_rel_err_median": float(np.median(alm_err)), "map_rel_err": map_err,
+                          "peak_sep_deg_median": float(np.median(sep)), "peak_sep_deg_p99": float(np.quantile(sep, 0.99)),
+                          "peak_sep_deg_max": float(sep.max()), "peaks_within_0.01_deg": float(np.mean(sep <= 0.01)),
+                          "peak_value_rel_err_max": float(pk_val_err.max())},
         "stage_ms": [round(t, 5) for t in rmx["stage_ms"]],
         "stages": names,
         "tuned": {"mixed": rmx["tuned"], "fp64": r64["tuned"], "info": rmx["info"]},
@@ -606,12 +728,18 @@ def main():
                      "traffic": (sky_fused_traffic(rmx["info"]["k_splits"], nseg, rmx["info"]["columns"])
                                  if names[k] == "sky_fused" and nseg == 1024 and L == 32 else None),
                      "traffic_basis": "rocprofv3 FETCH_SIZE/WRITE_SIZE at 7 K splits, partial-sum reads rescaled to the tuned split",
-                     "algorithmic_bytes": sb[k]},
+                     "algorithmic_bytes": sky_out if names[k] == "sky_fused" else sb[k],
+                     "algorithmic_basis": "a_lm + maps written, rotation rows read",
+                     "intermediate_read_bytes": partial_rd if names[k] == "sky_fused" else None,
+                     "achieved_with_intermediates": round((sky_out + partial_rd) / t_dom / 1e9, 1) if names[k] == "sky_fused" else None},
+        "step_roofline": {"bytes_per_map": round(b_map, 1), "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4), "roofline_maps_per_s": round(HBM_PEAK_GBS * 1e9 / b_map, 1)},
         "plan_build_s": round(plan_s, 2),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ts, L, args.cpu_budget)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 2)
+        out["vs_cpu_all_cores"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out))
 
 

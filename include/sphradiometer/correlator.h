@@ -125,6 +125,13 @@ struct correlator_network_plan_fd *correlator_network_plan_fd_set_l(struct corre
 
 struct sh_series *correlator_baseline_integrate_power_fd(const double complex *, const double complex *, struct correlator_plan_fd *);
 struct sh_series *correlator_network_integrate_power_fd(struct sh_series *, double complex **, struct correlator_network_plan_fd *);
+/* No counterpart in the reference.  The FD integrators keep a device copy
+ * of each plan's delay product and rotation table from one call to the
+ * next; the plan functions above (free, set_l, mult_by_projection) drop
+ * it when they change the plan.  A caller that writes into a plan's
+ * delay_product or rotation_plan in place calls this before the next
+ * integrate. */
+void correlator_plan_fd_device_invalidate(struct correlator_plan_fd *);
 
 #ifdef __cplusplus
 #undef complex

@@ -180,6 +180,14 @@ int sphr_refine_peaks(const void *alm, int alm_f32, int n_series, int alm_div, i
 int sphr_inject_bursts(const float *params, int n_det, int n_segments, int n, double delta_t, double noise, unsigned long long seed, float *out, void *stream);
 /* test hook: C = op(A) op(B) of the network's GEMM (device float32) */
 int sphr_mlp_gemm_check(int bf16, int ta, int tb, int M, int N, int K, const float *A, const float *B, float *C);
+/* test hook of the split-FP16 product: C [M][N] = A [M][K] B[N][K]^T
+ * (float32 device arrays) through the fp16 pair split and three MFMA
+ * products; synchronous */
+int sphr_gemm_h3_check(int M, int N, int K, const float *A, const float *B, float *C);
+/* test and timing hook of the real GEMM: C [M][N] = A [M][K] B [K][N]
+ * (row-major device arrays, float64 when f64 else float32) on the
+ * library's kernel (impl 0) or rocBLAS (impl 1), queued on stream */
+int sphr_gemm_run(int impl, int f64, int M, int N, int K, const void *A, long lda, const void *B, long ldb, void *C, long ldc, void *stream);
 
 /*
  * Batches of independent jobs (no counterpart in the reference): job j

@@ -38,6 +38,7 @@ def lib():
         L.orx_plan_get_D.argtypes = [c_void_p, c_int, c_void_p]
         L.orx_network_integrate.argtypes = [c_void_p, c_void_p, c_uint, c_void_p]
         L.orx_segment_pipeline.argtypes = [c_void_p, c_void_p, c_void_p, c_uint, c_double, c_void_p, c_void_p]
+        L.orx_segments_run.argtypes = [c_void_p, c_void_p, c_int, c_uint, c_int, c_int, c_void_p, c_void_p]
         L.orx_tseries_to_fseries.argtypes = [c_void_p, c_void_p, c_int]
         L.orx_ctseries_to_fseries.argtypes = [c_void_p, c_void_p, c_int]
         L.orx_dft.argtypes = [c_void_p, c_int, c_int]
@@ -188,6 +189,21 @@ class Network:
         if lib().orx_localize(self._h, _ptr(x), x.shape[1], delta_t, gmst, _ptr(lp), sky_l, power, overall, _ptr(out)):
             raise RuntimeError("localize failed")
         return out
+
+    def segments_run(self, tseries, sky_l, seg_threads=1, mesh_threads=4, mesh=True):
+        """The per-segment pipeline over tseries [n_inst][S][N] in C:
+        seg_threads = 1 runs the segments in order with the ring loop of
+        to_mesh on mesh_threads threads (the reference's split);
+        seg_threads > 1 splits the segments over that many threads, each
+        with its own plan scratch.  Returns (sky [S][(L+1)^2], mesh)."""
+        ts = np.ascontiguousarray(tseries, dtype=np.float64)
+        D, S, N = ts.shape
+        sky = np.zeros((S, (sky_l + 1) ** 2), dtype=np.complex128)
+        B2 = 2 * (sky_l + 1)
+        m = np.zeros((S, B2, B2), dtype=np.complex128) if mesh else None
+        if lib().orx_segments_run(self._h, _ptr(ts), S, sky_l, seg_threads, mesh_threads, _ptr(sky), None if m is None else _ptr(m)):
+            raise RuntimeError("segments_run failed")
+        return sky, m
 
     def segment(self, tseries, sky_l, window=None, gmst=0.0, mesh=True):
         ts = np.ascontiguousarray(tseries, dtype=np.float64)

@@ -192,6 +192,9 @@ _PROTOS = {
     "correlator_network_plan_td_free": (None, [P(NetworkPlanTD)]),
     "correlator_network_integrate_power_td": (P(ShSeries), [P(ShSeries), P(c_void_p), c_int, P(c_void_p), P(NetworkPlanTD)]),
     "correlator_network_integrate_power_fd": (P(ShSeries), [P(ShSeries), P(c_void_p), P(NetworkPlanFD)]),
+    "correlator_plan_fd_device_invalidate": (None, [P(PlanFD)]),
+    "sphr_gemm_h3_check": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "sphr_gemm_run": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, ctypes.c_long, c_void_p, ctypes.c_long, c_void_p, ctypes.c_long, c_void_p]),
     "correlator_tseries_to_fseries_plan": (c_void_p, [c_void_p, c_void_p, c_int]),
     "correlator_tseries_to_fseries": (None, [c_void_p, c_void_p, c_int, c_void_p]),
     "correlator_ctseries_to_fseries_plan": (c_void_p, [c_void_p, c_void_p, c_int]),

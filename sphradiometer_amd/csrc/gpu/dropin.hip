@@ -373,6 +373,108 @@ fail:
 	return -1;
 }
 
+/*
+ * Device copies of FD plans, kept across calls: each plan's delay product
+ * and its rotation table (the D^l_{0 m'} rows for a polar power, the packed
+ * D blocks otherwise), keyed by the plan and checked against the host
+ * arrays and shapes they were made from.  The library's plan mutators
+ * (correlator_plan_fd_free, _set_l, mult_by_projection) drop a plan's
+ * entry through sphr_gpu_plan_forget; a caller that writes into a plan's
+ * delay_product or rotation_plan in place calls it too.  Callers hold the
+ * drop-in lock.
+ */
+#define PLAN_CACHE 64
+struct plan_dev {
+	const void *key;
+	const double *dp_host, *blocks_host;
+	int n, stride, polar;
+	unsigned l_max;
+	cplx<double> *d_dp, *d_rot;
+	unsigned long long used;
+};
+static struct plan_dev plan_cache[PLAN_CACHE];
+static unsigned long long plan_tick;
+
+static void plan_entry_clear(struct plan_dev *e)
+{
+	if(e->d_dp)
+		hipFree(e->d_dp);
+	if(e->d_rot)
+		hipFree(e->d_rot);
+	memset(e, 0, sizeof(*e));
+}
+
+static size_t rot_count(const struct sphr_fd_baseline *b)
+{
+	return b->polar ? (size_t) (b->l_max + 1) * (b->l_max + 1) : (b->l_max > 0 ? block_offset_host(b->l_max + 1) : 1);
+}
+
+static const struct plan_dev *plan_device(const struct sphr_fd_baseline *b, int n)
+{
+	struct plan_dev *e = NULL, *victim = &plan_cache[0];
+	for(int i = 0; i < PLAN_CACHE; i++) {
+		if(b->key && plan_cache[i].key == b->key) {
+			e = &plan_cache[i];
+			break;
+		}
+		if(plan_cache[i].used < victim->used)
+			victim = &plan_cache[i];
+	}
+	if(e && e->dp_host == b->dp && e->blocks_host == b->blocks && e->n == n && e->stride == b->stride && e->polar == b->polar && e->l_max == b->l_max) {
+		e->used = ++plan_tick;
+		return e;
+	}
+	if(!e)
+		e = victim;
+	plan_entry_clear(e);
+	const size_t ndp = (size_t) n * b->stride, nrot = rot_count(b);
+	double *rows = NULL;
+	if(sphr_rt_check(hipMalloc(&e->d_dp, ndp * sizeof(cplx<double>)), "hipMalloc plan") || sphr_rt_check(hipMalloc(&e->d_rot, nrot * sizeof(cplx<double>)), "hipMalloc plan"))
+		goto fail;
+	if(sphr_rt_check(hipMemcpy(e->d_dp, b->dp, ndp * sizeof(cplx<double>), hipMemcpyHostToDevice), "copy delay product"))
+		goto fail;
+	if(b->polar) {
+		rows = (double *) malloc(2 * nrot * sizeof(double));
+		if(!rows)
+			goto fail;
+		if(b->l_max > 0)
+			sphr_polar_rows((int) b->l_max, b->blocks, rows);
+		else {
+			rows[0] = 1.0;
+			rows[1] = 0.0;
+		}
+		if(sphr_rt_check(hipMemcpy(e->d_rot, rows, nrot * sizeof(cplx<double>), hipMemcpyHostToDevice), "copy rotation rows"))
+			goto fail;
+		free(rows);
+		rows = NULL;
+	} else if(b->l_max > 0 && sphr_rt_check(hipMemcpy(e->d_rot, b->blocks, nrot * sizeof(cplx<double>), hipMemcpyHostToDevice), "copy rotation blocks"))
+		goto fail;
+	e->key = b->key;
+	e->dp_host = b->dp;
+	e->blocks_host = b->blocks;
+	e->n = n;
+	e->stride = b->stride;
+	e->polar = b->polar;
+	e->l_max = b->l_max;
+	e->used = ++plan_tick;
+	return e;
+fail:
+	free(rows);
+	plan_entry_clear(e);
+	return NULL;
+}
+
+extern "C" void sphr_gpu_plan_forget(const void *key)
+{
+	if(!key)
+		return;
+	LOCK();
+	for(int i = 0; i < PLAN_CACHE; i++)
+		if(plan_cache[i].key == key)
+			plan_entry_clear(&plan_cache[i]);
+	UNLOCK();
+}
+
 extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int n, int nb, const struct sphr_fd_baseline *bl, unsigned sky_l, double scale, double *sky)
 {
 	if(sphr_gpu_ensure())
@@ -382,39 +484,41 @@ extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int
 		return -1;
 	}
 	const size_t nsky = (size_t) (sky_l + 1) * (sky_l + 1);
-	size_t dp_total = 0, pw_total = 0, rot_total = 0;
-	size_t dp_off[SPHR_MAX_BASELINES], pw_off[SPHR_MAX_BASELINES], rot_off[SPHR_MAX_BASELINES];
+	size_t pw_total = 0, pw_off[SPHR_MAX_BASELINES];
 	for(int b = 0; b < nb; b++) {
-		dp_off[b] = dp_total;
-		dp_total += (size_t) n * bl[b].stride;
 		pw_off[b] = pw_total;
 		pw_total += bl[b].stride;
-		rot_off[b] = rot_total;
-		rot_total += bl[b].polar ? (size_t) (bl[b].l_max + 1) * (bl[b].l_max + 1) : block_offset_host(bl[b].l_max + 1) + 1;
 	}
 	const int nsplit = n >= 64 * 256 ? 256 : (n + 63) / 64;
 	int maxstride = 1;
 	for(int b = 0; b < nb; b++)
 		maxstride = bl[b].stride > maxstride ? bl[b].stride : maxstride;
-	double *rows = NULL;
+	const struct plan_dev *pd[SPHR_MAX_BASELINES];
 	LOCK();
 	hipStream_t st = sphr_rt_stream();
-	cplx<double> *dfs = (cplx<double> *) sphr_rt_scratch(SL_A, (size_t) n_inst * n * sizeof(cplx<double>));
-	cplx<double> *dfsp = (cplx<double> *) sphr_rt_scratch(SL_B, (size_t) nb * n * sizeof(cplx<double>));
-	cplx<double> *ddp = (cplx<double> *) sphr_rt_scratch(SL_C, dp_total * sizeof(cplx<double>));
-	cplx<double> *dpw = (cplx<double> *) sphr_rt_scratch(SL_D, pw_total * sizeof(cplx<double>));
-	cplx<double> *drot = (cplx<double> *) sphr_rt_scratch(SL_E, rot_total * sizeof(cplx<double>));
+	/* staging of the spectra in (pinned) and of the sky, powers and
+	 * cross spectra out */
+	const size_t in_bytes = (size_t) n_inst * n * sizeof(cplx<double>);
+	const size_t out_bytes = (nsky + pw_total + (size_t) nb * n) * sizeof(cplx<double>);
+	cplx<double> *dfs = (cplx<double> *) sphr_rt_scratch(SL_A, in_bytes);
+	cplx<double> *dout = (cplx<double> *) sphr_rt_scratch(SL_B, out_bytes);
 	cplx<double> *dpart = (cplx<double> *) sphr_rt_scratch(SL_F, (size_t) nsplit * maxstride * sizeof(cplx<double>));
-	cplx<double> *dsky = (cplx<double> *) sphr_rt_scratch(SL_G, nsky * sizeof(cplx<double>));
-	if(!dfs || !dfsp || !ddp || !dpw || !drot || !dpart || !dsky)
+	cplx<double> *hin = (cplx<double> *) sphr_rt_pinned(0, in_bytes);
+	cplx<double> *hout = (cplx<double> *) sphr_rt_pinned(1, out_bytes);
+	cplx<double> *dsky = dout, *dpw = dout + nsky, *dfsp = dpw + pw_total;
+	if(!dfs || !dout || !dpart || !hin || !hout)
 		goto fail;
+	for(int b = 0; b < nb; b++)
+		if(!(pd[b] = plan_device(&bl[b], n)))
+			goto fail;
 	{
 		const cplx<double> *fsd[2 * SPHR_MAX_BASELINES];
 		int ia[SPHR_MAX_BASELINES], ib[SPHR_MAX_BASELINES];
 		for(int i = 0; i < n_inst; i++) {
-			CK(hipMemcpyAsync(dfs + (size_t) i * n, fseries[i], (size_t) n * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy spectra");
+			memcpy(hin + (size_t) i * n, fseries[i], (size_t) n * sizeof(cplx<double>));
 			fsd[i] = dfs + (size_t) i * n;
 		}
+		CK(hipMemcpyAsync(dfs, hin, in_bytes, hipMemcpyHostToDevice, st), "copy spectra");
 		for(int b = 0; b < nb; b++) {
 			ia[b] = bl[b].index_a;
 			ib[b] = bl[b].index_b;
@@ -425,31 +529,10 @@ extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int
 		}
 	}
 	for(int b = 0; b < nb; b++) {
-		CK(hipMemcpyAsync(ddp + dp_off[b], bl[b].dp, (size_t) n * bl[b].stride * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy delay product");
 		/* power_1d = (1/n) sum_f dp[f] X[f] (correlator.c:690-703) */
-		if(launch_dotc(ddp + dp_off[b], n, bl[b].stride, dfsp + (size_t) b * n, NULL, 1.0 / n, dpart, nsplit, dpw + pw_off[b], st)) {
+		if(launch_dotc(pd[b]->d_dp, n, bl[b].stride, dfsp + (size_t) b * n, NULL, 1.0 / n, dpart, nsplit, dpw + pw_off[b], st)) {
 			sphr_set_error("projection launch failed");
 			goto fail;
-		}
-		const int Lb = (int) bl[b].l_max;
-		if(bl[b].polar) {
-			const size_t nr = (size_t) (Lb + 1) * (Lb + 1);
-			double *r = (double *) realloc(rows, 2 * nr * sizeof(double));
-			if(!r)
-				goto fail;
-			rows = r;
-			if(Lb > 0)
-				sphr_polar_rows(Lb, bl[b].blocks, rows);
-			else {
-				rows[0] = 1.0;
-				rows[1] = 0.0;
-			}
-			CK(hipMemcpyAsync(drot + rot_off[b], rows, nr * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy rotation rows");
-			/* the rows must reach the device before the buffer is
-			 * reused for the next baseline */
-			CK(hipStreamSynchronize(st), "synchronize");
-		} else if(Lb > 0) {
-			CK(hipMemcpyAsync(drot + rot_off[b], bl[b].blocks, block_offset_host(Lb + 1) * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy rotation blocks");
 		}
 	}
 	{
@@ -457,7 +540,7 @@ extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int
 		int Lb[SPHR_MAX_BASELINES], pol[SPHR_MAX_BASELINES];
 		for(int b = 0; b < nb; b++) {
 			pw[b] = dpw + pw_off[b];
-			bk[b] = drot + rot_off[b];
+			bk[b] = pd[b]->d_rot;
 			Lb[b] = (int) bl[b].l_max;
 			pol[b] = bl[b].polar;
 		}
@@ -466,19 +549,19 @@ extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int
 			goto fail;
 		}
 	}
-	CK(hipMemcpyAsync(sky, dsky, nsky * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy sky");
-	for(int b = 0; b < nb; b++) {
-		CK(hipMemcpyAsync(bl[b].power_1d, dpw + pw_off[b], (size_t) bl[b].stride * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy power");
-		if(bl[b].fsp)
-			CK(hipMemcpyAsync(bl[b].fsp, dfsp + (size_t) b * n, (size_t) n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy cross spectrum");
-	}
+	/* one copy back: sky, the baselines' powers and cross spectra */
+	CK(hipMemcpyAsync(hout, dout, out_bytes, hipMemcpyDeviceToHost, st), "copy results");
 	CK(hipStreamSynchronize(st), "synchronize");
+	memcpy(sky, hout, nsky * sizeof(cplx<double>));
+	for(int b = 0; b < nb; b++) {
+		memcpy(bl[b].power_1d, hout + nsky + pw_off[b], (size_t) bl[b].stride * sizeof(cplx<double>));
+		if(bl[b].fsp)
+			memcpy(bl[b].fsp, hout + nsky + pw_total + (size_t) b * n, (size_t) n * sizeof(cplx<double>));
+	}
 	UNLOCK();
-	free(rows);
 	return 0;
 fail:
 	UNLOCK();
-	free(rows);
 	return -1;
 }
 

@@ -12,7 +12,8 @@
  *
  * Tiling: workgroup 256 threads = 4 waves in 2 x 2, block tile 128 x 128,
  * wave tile 64 x 64 = 4 x 4 MFMA tiles of 16 x 16; K chunks of KC staged
- * in LDS.  Workgroup ids are
+ * global -> registers -> LDS, the next chunk's loads in flight during the
+ * current chunk's MFMAs.  Workgroup ids are
  * remapped so the 8 XCDs each work on contiguous column panels of B (each
  * XCD has its own L2).
  *
@@ -50,6 +51,7 @@ __global__ void __launch_bounds__(GM_NT) k_gemm(int M, int N, int K, const TA *_
 {
 	typedef typename mf<T>::v4 v4;
 	const int KC = mf<T>::KC;
+	const int RA = GM_BM * KC / GM_NT, RB = KC * GM_BN / GM_NT;	/* staged elements per thread */
 	/* rows padded to 16 elements past a multiple of 32 (64-bit) or 64
 	 * (32-bit) words: the four 16-lane groups of an operand read land on
 	 * disjoint banks */
@@ -67,31 +69,61 @@ __global__ void __launch_bounds__(GM_NT) k_gemm(int M, int N, int K, const TA *_
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
 	const int li = lane & 15, lk = lane >> 4;
+	/* global -> register staging of the next chunk while this one's
+	 * MFMAs run; A coalesced along k, B along n */
+	T ra[RA], rb[RB];
+	auto load = [&](int k0) {
+#pragma unroll
+		for(int v = 0; v < RA; v++) {
+			const int i = v * GM_NT + threadIdx.x;
+			const int r = i / KC, c = i - r * KC;
+			const int gm = m0 + r, gk = k0 + c;
+			ra[v] = (gm < M && gk < K) ? (T) A[(size_t) gm * lda + gk] : (T) 0;
+		}
+#pragma unroll
+		for(int v = 0; v < RB; v++) {
+			const int i = v * GM_NT + threadIdx.x;
+			const int r = i / GM_BN, c = i - r * GM_BN;
+			const int gk = k0 + r, gn = n0 + c;
+			rb[v] = (gk < K && gn < N) ? B[(size_t) gk * ldb + gn] : (T) 0;
+		}
+	};
+	auto store = [&]() {
+#pragma unroll
+		for(int v = 0; v < RA; v++) {
+			const int i = v * GM_NT + threadIdx.x;
+			const int r = i / KC, c = i - r * KC;
+			sA[c][r] = ra[v];
+		}
+#pragma unroll
+		for(int v = 0; v < RB; v++) {
+			const int i = v * GM_NT + threadIdx.x;
+			const int r = i / GM_BN, c = i - r * GM_BN;
+			sB[r][c] = rb[v];
+		}
+	};
 	v4 acc[4][4];
 	for(int a = 0; a < 4; a++)
 		for(int b = 0; b < 4; b++)
 			acc[a][b] = (v4) 0;
+	load(0);
 	for(int k0 = 0; k0 < K; k0 += KC) {
-		/* A tile: rows m0.., cols k0..k0+KC, coalesced along k */
-		for(int i = threadIdx.x; i < GM_BM * KC; i += GM_NT) {
-			const int r = i / KC, c = i - r * KC;
-			const int gm = m0 + r, gk = k0 + c;
-			sA[c][r] = (gm < M && gk < K) ? (T) A[(size_t) gm * lda + gk] : (T) 0;
-		}
-		/* B tile: rows k0.., cols n0..n0+BN, coalesced along n */
-		for(int i = threadIdx.x; i < KC * GM_BN; i += GM_NT) {
-			const int r = i / GM_BN, c = i - r * GM_BN;
-			const int gk = k0 + r, gn = n0 + c;
-			sB[r][c] = (gk < K && gn < N) ? B[(size_t) gk * ldb + gn] : (T) 0;
-		}
+		store();
 		__syncthreads();
+		if(k0 + KC < K)
+			load(k0 + KC);
+#pragma unroll
 		for(int ks = 0; ks < KC; ks += 4) {
 			T af[4], bf[4];
+#pragma unroll
 			for(int a = 0; a < 4; a++)
 				af[a] = sA[ks + lk][wm + 16 * a + li];
+#pragma unroll
 			for(int b = 0; b < 4; b++)
 				bf[b] = sB[ks + lk][wn + 16 * b + li];
+#pragma unroll
 			for(int a = 0; a < 4; a++)
+#pragma unroll
 				for(int b = 0; b < 4; b++)
 					acc[a][b] = mf<T>::mma(af[a], bf[b], acc[a][b]);
 		}
@@ -176,3 +208,18 @@ int launch_gemm_blas(int M, int N, int K, const T *A, long lda, const T *B, long
 
 template int launch_gemm_blas<float>(int, int, int, const float *, long, const float *, long, float *, long, hipStream_t);
 template int launch_gemm_blas<double>(int, int, int, const double *, long, const double *, long, double *, long, hipStream_t);
+
+/*
+ * Test and timing hook: C [M][N] = A [M][K] B [K][N] (row-major device
+ * arrays, float64 when f64 else float32) on the library kernel (impl 0)
+ * or rocBLAS (impl 1), queued on stream.
+ */
+extern "C" int sphr_gemm_run(int impl, int f64, int M, int N, int K, const void *A, long lda, const void *B, long ldb, void *C, long ldc, void *stream)
+{
+	hipStream_t st = (hipStream_t) stream;
+	if(f64)
+		return impl ? launch_gemm_blas<double>(M, N, K, (const double *) A, lda, (const double *) B, ldb, (double *) C, ldc, st)
+			    : launch_gemm<double, double>(M, N, K, (const double *) A, lda, (const double *) B, ldb, (double *) C, ldc, st);
+	return impl ? launch_gemm_blas<float>(M, N, K, (const float *) A, lda, (const float *) B, ldb, (float *) C, ldc, st)
+		    : launch_gemm<float, float>(M, N, K, (const float *) A, lda, (const float *) B, ldb, (float *) C, ldc, st);
+}

@@ -1,0 +1,247 @@
+/*
+ * Split-FP16 GEMM on gfx950 MFMA (v_mfma_f32_16x16x32_f16), the mixed
+ * precision form of the localization products:
+ *
+ *   C[i][j] = ira[i] icb[j] sum_k (Ah[i][k] Bh[j][k] + Ah[i][k] Bl[j][k] + Al[i][k] Bh[j][k])
+ *
+ * where a row of A scaled by ra[i] = 1/ira[i] (a power of two) is split into
+ * fp16 parts a s = ah + al (ah = fp16(a s), al = fp16(a s - ah)) and a column
+ * of B likewise with its own power of two.  The three products keep ~22
+ * significant bits of each operand (the dropped al bl term is 2^-22 of the
+ * product); the scales put every row's and column's largest element at
+ * 2^13..2^14, far from fp16's overflow, and keep the low parts normal
+ * down to 2^-17 of it.  Accumulation is float32 in the MFMA.
+ *
+ * Operands are K-contiguous: A [M][lda], B [N][ldb] (B transposed), K a
+ * multiple of 64 (producers zero-pad), lda and ldb multiples of 8.
+ *
+ * Tiling: 256 threads = 4 waves in 2 x 2, block tile 128 x 128, wave tile
+ * 64 x 64 = 4 x 4 MFMA tiles; K chunks of 64 staged global -> registers
+ * (the next chunk's loads in flight during this chunk's MFMAs) -> LDS
+ * (rows padded to 72 halves: the 16-byte fragment reads of 16 rows land
+ * on disjoint banks).  Workgroup ids are remapped so each of the 8 XCDs
+ * works on a contiguous range of tiles (its own L2).
+ *
+ * MFMA 16x16x32 f16 operand layout (64 lanes): lane i + 16 q holds
+ * A[i][8q .. 8q+7] and B[8q .. 8q+7][i]; accumulator element e of lane
+ * j + 16 q holds D[4q + e][j].
+ */
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+#define H3_BM 128
+#define H3_BN 128
+#define H3_BK 64
+#define H3_LD 72	/* LDS row stride in halves */
+#define H3_NT 256
+
+template <typename TO>
+__global__ void __launch_bounds__(H3_NT) k_gemm_h3(int M, int N, int K, const _Float16 *__restrict__ Ah, const _Float16 *__restrict__ Al, long lda, const float *__restrict__ ira,
+						  const _Float16 *__restrict__ Bh, const _Float16 *__restrict__ Bl, long ldb, const float *__restrict__ icb, TO *__restrict__ C, long ldc, int tiles_m, int tiles_n)
+{
+	__shared__ __align__(16) _Float16 sA[2][H3_BM * H3_LD];	/* [hi/lo][row][k] */
+	__shared__ __align__(16) _Float16 sB[2][H3_BN * H3_LD];	/* [hi/lo][col][k] */
+	const int nwg = tiles_m * tiles_n;
+	const int id = blockIdx.x;
+	int t = id;
+	if(nwg % 8 == 0)
+		t = (id % 8) * (nwg / 8) + id / 8;
+	const int tm = t % tiles_m, tn = t / tiles_m;
+	const int m0 = tm * H3_BM, n0 = tn * H3_BN;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+	const int li = lane & 15, lq = lane >> 4;
+	/* global -> register staging: each thread moves 4 rows x 16 bytes of
+	 * each of the four operand tiles per chunk (128 rows x 8 vectors) */
+	h8 ra[2][4], rb[2][4];
+	const int vr = threadIdx.x >> 3, vc = (threadIdx.x & 7) * 8;	/* row 0..31 (+32 per step), k offset */
+	auto load = [&](int k0) {
+#pragma unroll
+		for(int s = 0; s < 4; s++) {
+			const int r = vr + 32 * s;
+			const int gm = m0 + r, gn = n0 + r;
+			if(gm < M) {
+				ra[0][s] = *reinterpret_cast<const h8 *>(Ah + (size_t) gm * lda + k0 + vc);
+				ra[1][s] = *reinterpret_cast<const h8 *>(Al + (size_t) gm * lda + k0 + vc);
+			} else {
+				ra[0][s] = (h8) 0;
+				ra[1][s] = (h8) 0;
+			}
+			if(gn < N) {
+				rb[0][s] = *reinterpret_cast<const h8 *>(Bh + (size_t) gn * ldb + k0 + vc);
+				rb[1][s] = *reinterpret_cast<const h8 *>(Bl + (size_t) gn * ldb + k0 + vc);
+			} else {
+				rb[0][s] = (h8) 0;
+				rb[1][s] = (h8) 0;
+			}
+		}
+	};
+	auto store = [&]() {
+#pragma unroll
+		for(int p = 0; p < 2; p++)
+#pragma unroll
+			for(int s = 0; s < 4; s++) {
+				const int r = vr + 32 * s;
+				*reinterpret_cast<h8 *>(&sA[p][r * H3_LD + vc]) = ra[p][s];
+				*reinterpret_cast<h8 *>(&sB[p][r * H3_LD + vc]) = rb[p][s];
+			}
+	};
+	f4 acc[4][4];
+#pragma unroll
+	for(int a = 0; a < 4; a++)
+#pragma unroll
+		for(int b = 0; b < 4; b++)
+			acc[a][b] = (f4) 0;
+	load(0);
+	for(int k0 = 0; k0 < K; k0 += H3_BK) {
+		store();
+		__syncthreads();
+		if(k0 + H3_BK < K)
+			load(k0 + H3_BK);
+#pragma unroll
+		for(int kk = 0; kk < H3_BK; kk += 32) {
+			h8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+			for(int a = 0; a < 4; a++) {
+				const int r = (wm + 16 * a + li) * H3_LD + kk + 8 * lq;
+				ah[a] = *reinterpret_cast<const h8 *>(&sA[0][r]);
+				al[a] = *reinterpret_cast<const h8 *>(&sA[1][r]);
+			}
+#pragma unroll
+			for(int b = 0; b < 4; b++) {
+				const int r = (wn + 16 * b + li) * H3_LD + kk + 8 * lq;
+				bh[b] = *reinterpret_cast<const h8 *>(&sB[0][r]);
+				bl[b] = *reinterpret_cast<const h8 *>(&sB[1][r]);
+			}
+#pragma unroll
+			for(int a = 0; a < 4; a++)
+#pragma unroll
+				for(int b = 0; b < 4; b++) {
+					acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+					acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+					acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+				}
+		}
+		__syncthreads();
+	}
+#pragma unroll
+	for(int b = 0; b < 4; b++) {
+		const int gn = n0 + wn + 16 * b + li;
+		if(gn >= N)
+			continue;
+		const float cs = icb[gn];
+#pragma unroll
+		for(int a = 0; a < 4; a++)
+#pragma unroll
+			for(int e = 0; e < 4; e++) {
+				const int gm = m0 + wm + 16 * a + 4 * lq + e;
+				if(gm < M)
+					C[(size_t) gm * ldc + gn] = (TO) (acc[a][b][e] * (ira[gm] * cs));
+			}
+	}
+}
+
+/*
+ * Row split of a real matrix X [rows][K] (float or double, leading dimension
+ * ldx) into the fp16 pair and the inverse row scale, one workgroup per row;
+ * columns K .. Kp-1 of the outputs are zero.
+ */
+template <typename TI>
+__global__ void __launch_bounds__(256) k_split_rows(int rows, int K, int Kp, const TI *__restrict__ X, long ldx, _Float16 *__restrict__ Xh, _Float16 *__restrict__ Xl, long ldh, float *__restrict__ irs)
+{
+	__shared__ float red[4];
+	const int r = blockIdx.x;
+	if(r >= rows)
+		return;
+	const TI *x = X + (size_t) r * ldx;
+	float mx = 0.f;
+	for(int k = threadIdx.x; k < K; k += 256)
+		mx = fmaxf(mx, fabsf((float) x[k]));
+	for(int o = 32; o > 0; o >>= 1)
+		mx = fmaxf(mx, __shfl_xor(mx, o));
+	if((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = mx;
+	__syncthreads();
+	mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+	int ex = 0;
+	if(mx > 0.f) {
+		frexpf(mx, &ex);	/* mx = f 2^ex, f in [0.5, 1) */
+		ex = 14 - ex;		/* mx 2^ex in [2^13, 2^14) */
+	}
+	const float s = ldexpf(1.0f, ex);
+	for(int k = threadIdx.x; k < Kp; k += 256) {
+		const float v = k < K ? (float) x[k] * s : 0.f;
+		const _Float16 h = (_Float16) v;
+		Xh[(size_t) r * ldh + k] = h;
+		Xl[(size_t) r * ldh + k] = (_Float16) (v - (float) h);
+	}
+	if(threadIdx.x == 0)
+		irs[r] = ldexpf(1.0f, -ex);
+}
+
+}	/* namespace */
+
+template <typename TO>
+int launch_gemm_h3(int M, int N, int K, const _Float16 *Ah, const _Float16 *Al, long lda, const float *ira, const _Float16 *Bh, const _Float16 *Bl, long ldb, const float *icb, TO *C, long ldc, hipStream_t st)
+{
+	if(M <= 0 || N <= 0 || K <= 0)
+		return 0;
+	if(K % H3_BK || lda % 8 || ldb % 8 || lda < K || ldb < K)
+		return -1;
+	const int tm = (M + H3_BM - 1) / H3_BM, tn = (N + H3_BN - 1) / H3_BN;
+	hipLaunchKernelGGL((k_gemm_h3<TO>), dim3(tm * tn), dim3(H3_NT), 0, st, M, N, K, Ah, Al, lda, ira, Bh, Bl, ldb, icb, C, ldc, tm, tn);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename TI>
+int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *Xh, _Float16 *Xl, long ldh, float *irs, hipStream_t st)
+{
+	if(rows <= 0)
+		return 0;
+	if(Kp < K || ldh < Kp)
+		return -1;
+	hipLaunchKernelGGL((k_split_rows<TI>), dim3(rows), dim3(256), 0, st, rows, K, Kp, X, ldx, Xh, Xl, ldh, irs);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template int launch_gemm_h3<float>(int, int, int, const _Float16 *, const _Float16 *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, float *, long, hipStream_t);
+template int launch_gemm_h3<double>(int, int, int, const _Float16 *, const _Float16 *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, double *, long, hipStream_t);
+template int launch_split_rows<float>(int, int, int, const float *, long, _Float16 *, _Float16 *, long, float *, hipStream_t);
+template int launch_split_rows<double>(int, int, int, const double *, long, _Float16 *, _Float16 *, long, float *, hipStream_t);
+
+/*
+ * Test hook: C [M][N] (float32) = A [M][K] B^T with A and B [N][K] given
+ * as float32 device arrays, through the split-FP16 path (rows of A and of
+ * B split with their own scales, K zero-padded to a multiple of 64).
+ * Allocates its split buffers; synchronous.
+ */
+extern "C" int sphr_gemm_h3_check(int M, int N, int K, const float *A, const float *B, float *C)
+{
+	if(M <= 0 || N <= 0 || K <= 0 || !A || !B || !C)
+		return -1;
+	const int Kp = (K + H3_BK - 1) / H3_BK * H3_BK;
+	_Float16 *ah = NULL, *al = NULL, *bh = NULL, *bl = NULL;
+	float *ra = NULL, *cb = NULL;
+	int rc = -1;
+	if(hipMalloc((void **) &ah, (size_t) M * Kp * 2) != hipSuccess || hipMalloc((void **) &al, (size_t) M * Kp * 2) != hipSuccess
+	   || hipMalloc((void **) &bh, (size_t) N * Kp * 2) != hipSuccess || hipMalloc((void **) &bl, (size_t) N * Kp * 2) != hipSuccess
+	   || hipMalloc((void **) &ra, (size_t) M * 4) != hipSuccess || hipMalloc((void **) &cb, (size_t) N * 4) != hipSuccess)
+		goto done;
+	if(launch_split_rows<float>(M, K, Kp, A, K, ah, al, Kp, ra, 0) || launch_split_rows<float>(N, K, Kp, B, K, bh, bl, Kp, cb, 0)
+	   || launch_gemm_h3<float>(M, N, Kp, ah, al, Kp, ra, bh, bl, Kp, cb, C, N, 0) || hipDeviceSynchronize() != hipSuccess)
+		goto done;
+	rc = 0;
+done:
+	hipFree(ah);
+	hipFree(al);
+	hipFree(bh);
+	hipFree(bl);
+	hipFree(ra);
+	hipFree(cb);
+	return rc;
+}

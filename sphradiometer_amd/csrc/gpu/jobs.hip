@@ -122,8 +122,14 @@ __global__ void k_pg_tables(const struct jb_bl *__restrict__ bl, int b0, const d
 			pbar_row(x, B.a, lc, ws + B.off_Pt + (size_t) i * (B.a + 1));
 			const double y = B.r_dt * cos(acos(x));
 			const double ww = nodes[B.off_nt + B.nt + i] * (2. * M_PI / B.nt);
+			/* sin(pi y) = (-1)^n sin(pi (y - n)), n = rint(y): y - n is
+			 * exact and sin of the reduced argument keeps its relative
+			 * accuracy where y is close to an integer, which the taps
+			 * divide by (kk + y) with kk = -n */
+			const double n = rint(y), sr = sin(M_PI * (y - n));
+			const double sy = ((long long) n & 1) ? -sr : sr;
 			ws[B.off_Y + i] = y;
-			ws[B.off_Y + nth + i] = sin(M_PI * y) / M_PI * (double) B.nt * ww;
+			ws[B.off_Y + nth + i] = sy / M_PI * (double) B.nt * ww;
 			ws[B.off_Y + 2 * nth + i] = ww;
 		} else {
 			const int q = i - nth;

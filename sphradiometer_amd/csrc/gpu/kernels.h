@@ -92,7 +92,7 @@ int sky_fused_order(int L);	/* 1 when launch_sky_fused is compiled for order L *
  * compiled for; 0, or -1 with the error set */
 int sky_fused_tables(int L, const struct synth_tables *tb, struct sky_fused_dev *d);
 template <typename T>
-int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st);
+int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st, int occ = 1);
 template <typename T>
 int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *map, hipStream_t st);
 /* GEMM-form synthesis of S packed series (order L, non-polar) through the
@@ -119,6 +119,13 @@ int launch_gemm(int M, int N, int K, const TA *A, long lda, const T *B, long ldb
 template <typename T>
 int launch_gemm_blas(int M, int N, int K, const T *A, long lda, const T *B, long ldb, T *C, long ldc, hipStream_t st);
 
+/* split-FP16 GEMM (gemm_h3.hip): C = diag(ira) (Ah Bh^T + Ah Bl^T + Al Bh^T)
+ * diag(icb), operands K-contiguous fp16, K % 64 == 0 */
+template <typename TO>
+int launch_gemm_h3(int M, int N, int K, const _Float16 *Ah, const _Float16 *Al, long lda, const float *ira, const _Float16 *Bh, const _Float16 *Bl, long ldb, const float *icb, TO *C, long ldc, hipStream_t st);
+/* rows of X [rows][K] -> fp16 pairs scaled per row (columns K..Kp-1 zero) */
+template <typename TI>
+int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *Xh, _Float16 *Xl, long ldh, float *irs, hipStream_t st);
 /* drop-in helpers (FP64) */
 int launch_xspec(const cplx<double> *const *fs_dev, int n, int nb, const int *ia, const int *ib, cplx<double> *fsp, hipStream_t st);
 int launch_dotc(const cplx<double> *coeff, int n, int stride, const cplx<double> *vec, const double *vec_real, double scale, cplx<double> *partial, int nsplit, cplx<double> *out, hipStream_t st);

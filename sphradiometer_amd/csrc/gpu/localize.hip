@@ -34,6 +34,16 @@ struct sphr_loc {
 	void *X;	/* T [max_events][2 nb N] */
 	void *S;	/* T [max_events][4 ncoef] */
 	void *Fr;	/* cplx<T> [2 max_events][2(L+1)][2L+1] ring sums */
+	/* mixed precision: both products as split-FP16 GEMMs (gemm_h3.hip) */
+	int Kw, Kd;		/* K of the sky and DFT products, padded to 64 */
+	_Float16 *Wh, *Wl;	/* [4 ncoef][Kw]: W transposed, per-column scale */
+	float *icw;		/* [4 ncoef] */
+	_Float16 *Dh, *Dl;	/* [2 N][Kd]: DFT matrix transposed, per-column scale */
+	float *icd;		/* [2 N] */
+	_Float16 *Ah, *Al;	/* [D max_events][Kd]: the series, split per row */
+	float *ira;		/* [D max_events] */
+	_Float16 *Xh, *Xl;	/* [max_events][Kw]: cross spectra, split per row */
+	float *irx;		/* [max_events] */
 	struct synth_dev syn;
 };
 
@@ -54,6 +64,50 @@ __global__ void k_loc_xspec(const cplx<T> *__restrict__ F, int E, int N, int nb,
 	const cplx<T> Fa = F[((size_t) p.ia[b] * E + e) * N + f];
 	const cplx<T> Fb = F[((size_t) p.ib[b] * E + e) * N + f];
 	X[i] = Fb * conjg(Fa);
+}
+
+/* cross spectra of one event (one workgroup), split into the fp16 pair of
+ * the sky product with a power-of-two row scale (gemm_h3.hip) */
+__global__ void __launch_bounds__(256) k_loc_xspec_h3(const cplx<float> *__restrict__ F, int E, int N, int nb, struct loc_pairs p, int Kp, _Float16 *__restrict__ Xh, _Float16 *__restrict__ Xl, float *__restrict__ irx)
+{
+	__shared__ float red[4];
+	const int e = blockIdx.x;
+	const int nc = nb * N;
+	float mx = 0.f;
+	for(int i = threadIdx.x; i < nc; i += 256) {
+		const int b = i / N, f = i - b * N;
+		const cplx<float> X = F[((size_t) p.ib[b] * E + e) * N + f] * conjg(F[((size_t) p.ia[b] * E + e) * N + f]);
+		mx = fmaxf(mx, fmaxf(fabsf(X.re), fabsf(X.im)));
+	}
+	for(int o = 32; o > 0; o >>= 1)
+		mx = fmaxf(mx, __shfl_xor(mx, o));
+	if((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = mx;
+	__syncthreads();
+	mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+	int ex = 0;
+	if(mx > 0.f) {
+		frexpf(mx, &ex);
+		ex = 14 - ex;
+	}
+	const float s = ldexpf(1.0f, ex);
+	_Float16 *xh = Xh + (size_t) e * Kp, *xl = Xl + (size_t) e * Kp;
+	for(int i = threadIdx.x; i < nc; i += 256) {
+		const int b = i / N, f = i - b * N;
+		const cplx<float> X = F[((size_t) p.ib[b] * E + e) * N + f] * conjg(F[((size_t) p.ia[b] * E + e) * N + f]);
+		const float vr = X.re * s, vi = X.im * s;
+		const _Float16 hr = (_Float16) vr, hi = (_Float16) vi;
+		xh[2 * i] = hr;
+		xh[2 * i + 1] = hi;
+		xl[2 * i] = (_Float16) (vr - (float) hr);
+		xl[2 * i + 1] = (_Float16) (vi - (float) hi);
+	}
+	for(int i = 2 * nc + threadIdx.x; i < Kp; i += 256) {
+		xh[i] = (_Float16) 0.f;
+		xl[i] = (_Float16) 0.f;
+	}
+	if(threadIdx.x == 0)
+		irx[e] = ldexpf(1.0f, -ex);
 }
 
 template <typename T>
@@ -112,11 +166,63 @@ static int upload_expanded(void **dst, const double *src, size_t K, size_t N)
 	return rc ? -1 : 0;
 }
 
+/* a complex matrix [K][N] real-expanded to [2K][2N] (rows 2k = (re, im),
+ * 2k+1 = (-im, re) per column pair), transposed to [2N][Kp] and split
+ * into fp16 pairs with a power-of-two scale per column (gemm_h3.hip) */
+static int upload_split_cols(const double *src, size_t K, size_t N, int Kp, _Float16 **dh, _Float16 **dl, float **dic)
+{
+	const size_t rows = 2 * N;
+	_Float16 *h = (_Float16 *) calloc(rows * Kp, sizeof(_Float16)), *l = (_Float16 *) calloc(rows * Kp, sizeof(_Float16));
+	float *ic = (float *) malloc(rows * sizeof(float));
+	int rc = -1;
+	if(!h || !l || !ic)
+		goto done;
+	for(size_t j = 0; j < rows; j++) {
+		const size_t n = j / 2;
+		const int im = (int) (j & 1);
+		/* column j of the expanded matrix: row 2k holds re or im of
+		 * src[k][n], row 2k+1 holds -im or re */
+		double mx = 0.0;
+		for(size_t k = 0; k < K; k++) {
+			const double re = src[2 * (k * N + n)], ii = src[2 * (k * N + n) + 1];
+			mx = fmax(mx, fmax(fabs(re), fabs(ii)));
+		}
+		int ex = 0;
+		if(mx > 0.0) {
+			frexp(mx, &ex);
+			ex = 14 - ex;
+		}
+		const double s = ldexp(1.0, ex);
+		for(size_t k = 0; k < K; k++) {
+			const double re = src[2 * (k * N + n)], ii = src[2 * (k * N + n) + 1];
+			const double v0 = (im ? ii : re) * s, v1 = (im ? re : -ii) * s;
+			const _Float16 h0 = (_Float16) v0, h1 = (_Float16) v1;
+			h[j * Kp + 2 * k] = h0;
+			l[j * Kp + 2 * k] = (_Float16) (v0 - (double) h0);
+			h[j * Kp + 2 * k + 1] = h1;
+			l[j * Kp + 2 * k + 1] = (_Float16) (v1 - (double) h1);
+		}
+		ic[j] = (float) ldexp(1.0, -ex);
+	}
+	if(sphr_rt_check(hipMalloc((void **) dh, rows * Kp * sizeof(_Float16)), "hipMalloc") || sphr_rt_check(hipMalloc((void **) dl, rows * Kp * sizeof(_Float16)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc((void **) dic, rows * sizeof(float)), "hipMalloc")
+	   || sphr_rt_check(hipMemcpy(*dh, h, rows * Kp * sizeof(_Float16), hipMemcpyHostToDevice), "hipMemcpy")
+	   || sphr_rt_check(hipMemcpy(*dl, l, rows * Kp * sizeof(_Float16), hipMemcpyHostToDevice), "hipMemcpy")
+	   || sphr_rt_check(hipMemcpy(*dic, ic, rows * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy"))
+		goto done;
+	rc = 0;
+done:
+	free(h);
+	free(l);
+	free(ic);
+	return rc;
+}
+
 extern "C" void sphr_loc_free(struct sphr_loc *lc)
 {
 	if(!lc)
 		return;
-	void *p[] = { lc->Wx, lc->Dx, lc->prior, lc->F, lc->X, lc->S, lc->Fr };
+	void *p[] = { lc->Wx, lc->Dx, lc->prior, lc->F, lc->X, lc->S, lc->Fr, lc->Wh, lc->Wl, lc->icw, lc->Dh, lc->Dl, lc->icd, lc->Ah, lc->Al, lc->ira, lc->Xh, lc->Xl, lc->irx };
 	for(unsigned i = 0; i < sizeof(p) / sizeof(p[0]); i++)
 		if(p[i])
 			hipFree(p[i]);
@@ -136,15 +242,29 @@ static int create_t(struct sphr_loc *lc, const struct sphr_loc_desc *d)
 	for(size_t k = 0; k < K; k++)
 		for(int beta = 0; beta < 2; beta++)
 			memcpy(Wr + 2 * (k * 2 * ncoef + beta * ncoef), d->W + 2 * ((beta * K + k) * ncoef), 2 * ncoef * sizeof(double));
-	int rc = upload_expanded<T>(&lc->Wx, Wr, K, 2 * ncoef);
+	const int split = sizeof(T) == 4;
+	lc->Kw = (int) ((2 * K + 63) / 64 * 64);
+	lc->Kd = (2 * d->len + 63) / 64 * 64;
+	int rc = split ? upload_split_cols(Wr, K, 2 * ncoef, lc->Kw, &lc->Wh, &lc->Wl, &lc->icw) : upload_expanded<T>(&lc->Wx, Wr, K, 2 * ncoef);
 	free(Wr);
-	if(rc || upload_expanded<T>(&lc->Dx, d->dft, d->len, d->N) || upload_as<T>(&lc->prior, d->prior, 2 * ncoef))
+	if(rc)
+		return -1;
+	rc = split ? upload_split_cols(d->dft, d->len, d->N, lc->Kd, &lc->Dh, &lc->Dl, &lc->icd) : upload_expanded<T>(&lc->Dx, d->dft, d->len, d->N);
+	if(rc || upload_as<T>(&lc->prior, d->prior, 2 * ncoef))
 		return -1;
 	const size_t E = d->max_events;
 	if(sphr_rt_check(hipMalloc(&lc->F, (size_t) d->D * E * 2 * d->N * sizeof(T)), "hipMalloc")
-	   || sphr_rt_check(hipMalloc(&lc->X, E * 2 * K * sizeof(T)), "hipMalloc")
 	   || sphr_rt_check(hipMalloc(&lc->S, E * 4 * ncoef * sizeof(T)), "hipMalloc")
 	   || sphr_rt_check(hipMalloc(&lc->Fr, 2 * E * 2 * (d->L + 1) * (2 * d->L + 1) * 2 * sizeof(T)), "hipMalloc"))
+		return -1;
+	if(!split)
+		return sphr_rt_check(hipMalloc(&lc->X, E * 2 * K * sizeof(T)), "hipMalloc") ? -1 : 0;
+	if(sphr_rt_check(hipMalloc((void **) &lc->Ah, (size_t) d->D * E * lc->Kd * sizeof(_Float16)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc((void **) &lc->Al, (size_t) d->D * E * lc->Kd * sizeof(_Float16)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc((void **) &lc->ira, (size_t) d->D * E * sizeof(float)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc((void **) &lc->Xh, E * lc->Kw * sizeof(_Float16)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc((void **) &lc->Xl, E * lc->Kw * sizeof(_Float16)), "hipMalloc")
+	   || sphr_rt_check(hipMalloc((void **) &lc->irx, E * sizeof(float)), "hipMalloc"))
 		return -1;
 	return 0;
 }
@@ -177,8 +297,29 @@ extern "C" struct sphr_loc *sphr_loc_create(const struct sphr_loc_desc *d)
 	return lc;
 }
 
+/* the mixed-precision products: the series and the cross spectra split
+ * per row, the DFT and sky matrices per column (at creation), three fp16
+ * MFMA products each with float32 accumulation */
+static int run_products_h3(struct sphr_loc *lc, const void *series, int series_f64, int E, hipStream_t st)
+{
+	const int N = lc->N, nb = lc->nb, L = lc->L, DE = lc->D * E;
+	const int ncoef = (L + 1) * (L + 1);
+	/* 1: spectra; 2: cross spectra; 3: skies */
+	int r = series_f64 ? launch_split_rows<double>(DE, 2 * lc->len, lc->Kd, (const double *) series, 2L * lc->len, lc->Ah, lc->Al, lc->Kd, lc->ira, st)
+			   : launch_split_rows<float>(DE, 2 * lc->len, lc->Kd, (const float *) series, 2L * lc->len, lc->Ah, lc->Al, lc->Kd, lc->ira, st);
+	if(r || launch_gemm_h3<float>(DE, 2 * N, lc->Kd, lc->Ah, lc->Al, lc->Kd, lc->ira, lc->Dh, lc->Dl, lc->Kd, lc->icd, (float *) lc->F, 2L * N, st))
+		return -1;
+	struct loc_pairs p;
+	memcpy(p.ia, lc->ia, sizeof(p.ia));
+	memcpy(p.ib, lc->ib, sizeof(p.ib));
+	hipLaunchKernelGGL(k_loc_xspec_h3, dim3(E), dim3(256), 0, st, (const cplx<float> *) lc->F, E, N, nb, p, lc->Kw, lc->Xh, lc->Xl, lc->irx);
+	if(hipGetLastError() != hipSuccess)
+		return -1;
+	return launch_gemm_h3<float>(E, 4 * ncoef, lc->Kw, lc->Xh, lc->Xl, lc->Kw, lc->irx, lc->Wh, lc->Wl, lc->Kw, lc->icw, (float *) lc->S, 4L * ncoef, st);
+}
+
 template <typename T>
-static int run_t(struct sphr_loc *lc, const void *series, int series_f64, int E, const double *gmst, void *alm, void *map, hipStream_t st)
+static int run_products_f64(struct sphr_loc *lc, const void *series, int series_f64, int E, hipStream_t st)
 {
 	const int N = lc->N, nb = lc->nb, L = lc->L;
 	const int ncoef = (L + 1) * (L + 1);
@@ -205,6 +346,14 @@ static int run_t(struct sphr_loc *lc, const void *series, int series_f64, int E,
 	/* 3: skies, [E][2 nb N] x [2 nb N][4 ncoef] (rocBLAS) */
 	if(launch_gemm_blas<T>(E, 4 * ncoef, 2 * nb * N, (const T *) lc->X, 2L * nb * N, (const T *) lc->Wx, 4L * ncoef, (T *) lc->S, 4L * ncoef, st))
 		return -1;
+	return 0;
+}
+
+template <typename T>
+static int run_epilogue(struct sphr_loc *lc, int E, const double *gmst, void *alm, void *map, hipStream_t st)
+{
+	const int L = lc->L;
+	const int ncoef = (L + 1) * (L + 1);
 	/* 4: prior, gmst */
 	const long ne = 2L * E * ncoef;
 	hipLaunchKernelGGL((k_loc_epilogue<T>), dim3((unsigned) ((ne + 255) / 256)), dim3(256), 0, st, (const T *) lc->S, E, L, (const cplx<T> *) lc->prior, gmst, (cplx<T> *) alm);
@@ -216,6 +365,17 @@ static int run_t(struct sphr_loc *lc, const void *series, int series_f64, int E,
 	   && launch_synth<T, T, T>((const cplx<T> *) alm, 2 * E, 0, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st))
 		return -1;
 	return 0;
+}
+
+template <typename T>
+static int run_t(struct sphr_loc *lc, const void *series, int series_f64, int E, const double *gmst, void *alm, void *map, hipStream_t st)
+{
+	if(sizeof(T) == 4) {
+		if(run_products_h3(lc, series, series_f64, E, st))
+			return -1;
+	} else if(run_products_f64<T>(lc, series, series_f64, E, st))
+		return -1;
+	return run_epilogue<T>(lc, E, gmst, alm, map, st);
 }
 
 extern "C" int sphr_loc_run(struct sphr_loc *lc, const void *series, int series_f64, int n_events, const double *gmst, void *alm, void *map, void *stream)

@@ -4,19 +4,29 @@
  * (rotation rows, baseline average, rotate_z) -> Legendre ring sums F in
  * LDS -> phi DFT on MFMA -> mesh.  F never leaves the chip.
  *
+ * Real skies.  The batched skies are cross powers of real time series, so
+ * a_{l,-m} = (-1)^m conj(a_lm) and the ring sums obey F[r][-m] =
+ * conj(F[r][m]) (F[r][m] = sum_l a_lm Pbar_l|m|(x_r), times (-1)^m for
+ * m < 0): the map is real,
+ *     map[r][k] = Re F[r][0] + 2 Re sum_{m > 0} F[r][m] exp(i m phi_k),
+ * and only the m >= 0 half of F is formed.  The map is written in the
+ * reference's complex layout with a zero imaginary part (the reference's
+ * is round-off).  The a_lm are written in full.
+ *
  * phi DFT with the half-turn split.  The mesh has n = 2(L+1) points
  * phi_k = 2 pi k / n, so phi_{k + n/2} = phi_k + pi and
- * exp(i m phi_{k+n/2}) = (-1)^m exp(i m phi_k).  With Ev[r][k] the sum over
- * even m and Od[r][k] over odd m of F[r][m] exp(i m phi_k), k < n/2:
+ * exp(i m phi_{k+n/2}) = (-1)^m exp(i m phi_k).  With Ev[r][k] the part of
+ * the sum over even m and Od[r][k] over odd m, k < n/2:
  *     map[r][k] = Ev + Od,      map[r][k + n/2] = Ev - Od,
- * which halves the DFT work.  Ev and Od are real-expanded GEMMs
- * (rows r = rings, K = (re, im) of the modes of one parity, columns
- * (re, im) of the half-turn points) on v_mfma_{f32,f64}_16x16x4.  Each of
- * the 4 waves owns row tiles of 16 rings; the columns that do not fill a
- * 16-wide tile (one point for L = 32) and the rings beyond the last full
- * row tile (two for L = 32) are summed on the VALU.
+ * which halves the DFT work.  Ev and Od are real GEMMs (rows r = rings,
+ * K = (Re, Im) of the m >= 0 modes of one parity, columns the half-turn
+ * points; entries w cos(m phi_k) and -w sin(m phi_k), w = 1 at m = 0 and
+ * 2 otherwise) on v_mfma_{f32,f64}_16x16x4.  Each of the 4 waves owns row
+ * tiles of 16 rings; the points that do not fill a 16-wide tile (one for
+ * L = 32) and the rings beyond the last full row tile (two for L = 32)
+ * are summed on the VALU.
  *
- * B operands (the DFT matrix, entries exp(2 pi i t / n), t = m k mod n)
+ * B operands (the real DFT matrix from exp(2 pi i t / n), t = m k mod n)
  * are laid out once per order in the lane order of the MFMA fragments
  * ([k-step][column tile][lane], coalesced).  In float32 each wave loads
  * its B fragments into registers for the life of the workgroup (the
@@ -25,10 +35,10 @@
  *
  * Legendre stage.  F[r][m] = sum_l a_lm Pbar_l|m|(x_r) for ring pairs
  * (x_j, -x_j) at once through the parity (-1)^(l+|m|).  Item (ring
- * group, a) handles |m| = a and |m| = L - a for 2 or 3 rings (3 where
+ * group, a) handles m = a and m = L - a for 2 or 3 rings (3 where
  * items of 2 would not fit one round of the 256 threads: 187 items instead
  * of 289 for L = 32), so every item runs L + 2 recurrence steps (balanced
- * lanes), each step serving m = +|m| and -|m|.  The
+ * lanes).  The
  * recurrence Pbar_l = alpha_l x Pbar_{l-1} - (alpha_l / alpha_{l-1})
  * Pbar_{l-2} uses a division-free coefficient table built once per
  * workgroup, and runs two steps (one even, one odd l - |m|) per
@@ -78,12 +88,10 @@ template <int L> struct sf_geom {
 		nm = 2 * L + 1,		/* orders m = -L..L */
 		ncoef = (L + 1) * (L + 1),
 		nal = (L + 1) * (L + 2) / 2,
-		NE = (L % 2 == 0) ? L + 1 : L,	/* even orders */
-		NO = nm - NE,			/* odd orders */
-		ME0 = (L % 2 == 0) ? -L : -L + 1,	/* smallest even m */
-		MO0 = (L % 2 == 0) ? -L + 1 : -L,	/* smallest odd m */
-		KSE = (2 * NE + 3) / 4,	/* MFMA k-steps, even / odd */
-		KSO = (2 * NO + 3) / 4,
+		NM0 = L / 2 + 1,	/* even orders m >= 0 */
+		NM1 = (L + 1) / 2,	/* odd orders m > 0 */
+		KSE = (2 * NM0 + 3) / 4,	/* MFMA k-steps, even / odd */
+		KSO = (2 * NM1 + 3) / 4,
 		KMAX = 4 * (KSE > KSO ? KSE : KSO),
 		/* row stride of the F tiles in LDS: >= KMAX and = 4 mod 32
 		 * words, so the 16 rows x 4 columns of an A fragment fall on
@@ -91,28 +99,26 @@ template <int L> struct sf_geom {
 		FS = ((KMAX + 27) / 32) * 32 + 4,
 		NRT = n / 16,		/* full row tiles */
 		XR = n - 16 * NRT,	/* rings summed on the VALU */
-		NCT = (2 * B) / 16,	/* full column tiles of the half turn */
-		XP = B - 8 * NCT,	/* half-turn points summed on the VALU */
+		NCT = B / 16,		/* full column tiles of the half turn */
+		XP = B - 16 * NCT,	/* half-turn points summed on the VALU */
 		NA = L / 2 + 1,		/* Legendre item pairs |m| = a, L - a */
 	};
 };
 
-/* element (kr, c) of the real-expanded DFT matrix of one parity:
- * kr = 2 q + (0: re, 1: im of mode q), c = 2 k + (0: re, 1: im of point k) */
+/* element (kr, c) of the real DFT matrix of parity par: kr = 2 j + (0: Re,
+ * 1: Im of mode m = par + 2 j), c = point k; w cos(m phi_k) for Re and
+ * -w sin(m phi_k) for Im, w = 1 at m = 0 and 2 otherwise */
 template <int L>
-__device__ inline double sf_bval(const double *twp, int m0, int nq, int kr, int c)
+__device__ inline double sf_bval(const double *twp, int par, int kr, int c)
 {
 	typedef sf_geom<L> G;
-	const int q = kr >> 1;
-	if(q >= nq)
+	const int j = kr >> 1;
+	if(j >= (par ? G::NM1 : G::NM0))
 		return 0.0;
-	const int m = m0 + 2 * q, k = c >> 1;
-	int t = (m * k) % G::n;
-	if(t < 0)
-		t += G::n;
-	const double wr = twp[2 * t], wi = twp[2 * t + 1];
-	const int ci = kr & 1, d = c & 1;
-	return ci == d ? wr : (ci == 0 ? wi : -wi);
+	const int m = par + 2 * j;
+	const int t = (m * c) % G::n;
+	const double w = m ? 2.0 : 1.0;
+	return (kr & 1) ? -w * twp[2 * t + 1] : w * twp[2 * t];
 }
 
 /* B-fragment table: entry ((kidx NCT + ct) 64 + lane), kidx < KSE the even
@@ -127,7 +133,7 @@ __global__ void k_sky_bfrag(const double *__restrict__ twp, T *__restrict__ out)
 	const int lane = idx & 63, ct = (idx >> 6) % G::NCT, kidx = (idx >> 6) / G::NCT;
 	const int li = lane & 15, lk = lane >> 4;
 	const int odd = kidx >= G::KSE, ks = odd ? kidx - G::KSE : kidx;
-	out[idx] = (T) sf_bval<L>(twp, odd ? G::MO0 : G::ME0, odd ? G::NO : G::NE, 4 * ks + lk, 16 * ct + li);
+	out[idx] = (T) sf_bval<L>(twp, odd, 4 * ks + lk, 16 * ct + li);
 }
 
 /* rings per Legendre item: 2, or 3 where items of 2 rings would not fit
@@ -382,36 +388,32 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 					if(h && am == a)
 						break;
 					const int cbase = lm_offset(L, am, am);	/* coefficient table slot too */
-					const int nbase = am ? lm_offset(L, am, -am) : cbase;
-					const cplx<T> *sp = sa + cbase, *sn = sa + nbase, *cc = cf + cbase;
+					const cplx<T> *sp = sa + cbase, *cc = cf + cbase;
 					const int cnt = L - am;	/* steps after l = am */
 					T p2[SF_RG], p1[SF_RG];
-					cplx<T> Ep[SF_RG], En[SF_RG], Op[SF_RG], On[SF_RG];
+					cplx<T> Ep[SF_RG], Op[SF_RG];
 					{
-						const cplx<T> s0 = sp[0], n0 = sn[0];
-						const cplx<T> c1 = cc[1], s1 = sp[1], n1 = sn[1];
+						const cplx<T> s0 = sp[0];
+						const cplx<T> c1 = cc[1], s1 = sp[1];
 #pragma unroll
 						for(int r = 0; r < SF_RG; r++) {
 							const int j = SF_RG * tg + r;
 							p2[r] = j < G::B ? smm[j * (L + 1) + am] : (T) 0;
 							Ep[r] = s0 * p2[r];
-							En[r] = n0 * p2[r];
 							/* l = am + 1 (c1.im is 0; unused past the end) */
 							p1[r] = cnt >= 1 ? c1.re * x[r] * p2[r] : (T) 0;
 							Op[r] = s1 * p1[r];
-							On[r] = n1 * p1[r];
 						}
 					}
 					/* SF_US steps (E, O, ...) per iteration with all
 					 * operands loaded first, then two, then one */
 					int i = 2;
 					for(; i + SF_US - 1 <= cnt; i += SF_US) {
-						cplx<T> c[SF_US], a4[SF_US], b4[SF_US];
+						cplx<T> c[SF_US], a4[SF_US];
 #pragma unroll
 						for(int u = 0; u < SF_US; u++) {
 							c[u] = cc[i + u];
 							a4[u] = sp[i + u];
-							b4[u] = sn[i + u];
 						}
 #pragma unroll
 						for(int r = 0; r < SF_RG; r++) {
@@ -420,53 +422,45 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 								const T pe = c[u].re * x[r] * p1[r] - c[u].im * p2[r];
 								const T po = c[u + 1].re * x[r] * pe - c[u + 1].im * p1[r];
 								cfma(Ep[r], a4[u], pe);
-								cfma(En[r], b4[u], pe);
 								cfma(Op[r], a4[u + 1], po);
-								cfma(On[r], b4[u + 1], po);
 								p2[r] = pe;
 								p1[r] = po;
 							}
 						}
 					}
 					if(i + 1 <= cnt) {
-						const cplx<T> c0 = cc[i], c1 = cc[i + 1], a0 = sp[i], b0 = sn[i], a1 = sp[i + 1], b1 = sn[i + 1];
+						const cplx<T> c0 = cc[i], c1 = cc[i + 1], a0 = sp[i], a1 = sp[i + 1];
 #pragma unroll
 						for(int r = 0; r < SF_RG; r++) {
 							const T pe = c0.re * x[r] * p1[r] - c0.im * p2[r];
 							const T po = c1.re * x[r] * pe - c1.im * p1[r];
 							cfma(Ep[r], a0, pe);
-							cfma(En[r], b0, pe);
 							cfma(Op[r], a1, po);
-							cfma(On[r], b1, po);
 							p2[r] = pe;
 							p1[r] = po;
 						}
 						i += 2;
 					}
 					if(i == cnt) {
-						const cplx<T> cl = cc[i], a0 = sp[i], b0 = sn[i];
+						const cplx<T> cl = cc[i], a0 = sp[i];
 #pragma unroll
 						for(int r = 0; r < SF_RG; r++) {
 							const T pe = cl.re * x[r] * p1[r] - cl.im * p2[r];
 							cfma(Ep[r], a0, pe);
-							cfma(En[r], b0, pe);
 						}
 					}
-					/* m = +am, then m = -am (Pbar_l,-m = (-1)^m Pbar_lm) */
-#pragma unroll
-					for(int sg = 0; sg < 2; sg++) {
-						if(sg && !am)
-							break;
-						const int m = sg ? -am : am;
-						const T f = (sg && (am & 1)) ? (T) -1 : (T) 1;
-						T *Fp = (m & 1) ? Fo : Fe;
-						const int col = 2 * ((m - ((m & 1) ? G::MO0 : G::ME0)) >> 1);
+					/* F[r][am] of the northern ring and its mirror in the
+					 * tile of am's parity, columns 2 j (Re), 2 j + 1 (Im) of
+					 * mode am = parity + 2 j */
+					{
+						T *Fp = (am & 1) ? Fo : Fe;
+						const int col = 2 * (am >> 1);
 #pragma unroll
 						for(int r = 0; r < SF_RG; r++) {
 							const int j = SF_RG * tg + r;
 							if(j >= G::B)
 								continue;
-							const cplx<T> N = (sg ? En[r] + On[r] : Ep[r] + Op[r]) * f, Sth = (sg ? En[r] - On[r] : Ep[r] - Op[r]) * f;
+							const cplx<T> N = Ep[r] + Op[r], Sth = Ep[r] - Op[r];
 							Fp[j * G::FS + col] = N.re;
 							Fp[j * G::FS + col + 1] = N.im;
 							Fp[(G::n - 1 - j) * G::FS + col] = Sth.re;
@@ -480,7 +474,7 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 
 		if(act) {
 			/* ---- phi DFT of the full tiles on MFMA ---- */
-			T *mr = reinterpret_cast<T *>(map + (size_t) s * G::n * G::n);	/* [n][2n] real */
+			cplx<T> *mr = map + (size_t) s * G::n * G::n;	/* [n][n] complex */
 			for(int rt = wave; rt < G::NRT; rt += SF_NT / 64) {
 				v4 ae[G::NCT], ao[G::NCT];
 #pragma unroll
@@ -511,9 +505,9 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 				for(int ct = 0; ct < G::NCT; ct++)
 #pragma unroll
 					for(int e = 0; e < 4; e++) {
-						const int r = 16 * rt + sf_mf<T>::row(olk, e), c = 16 * ct + oli;
-						mr[(size_t) r * 2 * G::n + c] = ae[ct][e] + ao[ct][e];
-						mr[(size_t) r * 2 * G::n + G::n + c] = ae[ct][e] - ao[ct][e];
+						const int r = 16 * rt + sf_mf<T>::row(olk, e), k = 16 * ct + oli;
+						mr[(size_t) r * G::n + k] = mk(ae[ct][e] + ao[ct][e], (T) 0);
+						mr[(size_t) r * G::n + G::B + k] = mk(ae[ct][e] - ao[ct][e], (T) 0);
 					}
 			}
 
@@ -531,51 +525,39 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 				} else {
 					const int t2 = task - G::XR * G::B;
 					r = t2 / (G::XP > 0 ? G::XP : 1);
-					k = 8 * G::NCT + t2 % (G::XP > 0 ? G::XP : 1);
+					k = 16 * G::NCT + t2 % (G::XP > 0 ? G::XP : 1);
 				}
+				/* sum_j w (Re F cos(m phi_k) - Im F sin(m phi_k)) over the
+				 * modes m = par + 2 j of this parity */
 				const T *Fr = (par ? Fo : Fe) + r * G::FS;
-				const int nq = par ? G::NO : G::NE, m0 = par ? G::MO0 : G::ME0;
-				cplx<T> acc = mk((T) 0, (T) 0), acc2 = acc;
-				int t = (m0 * k) % G::n;
-				if(t < 0)
-					t += G::n;
+				const int nq = par ? G::NM1 : G::NM0;
+				T a0 = 0, acc = 0, acc2 = 0;
+				int t = par ? k : 0;
 				const int dt = (2 * k) % G::n;
-				/* four terms per iteration, loads first */
 				int q = 0;
-				for(; q + 4 <= nq; q += 4) {
-					int tq[4];
-#pragma unroll
-					for(int u4 = 0; u4 < 4; u4++) {
-						tq[u4] = t;
-						t += dt;
-						if(t >= G::n)
-							t -= G::n;
-					}
-					cplx<T> f[4], w[4];
-#pragma unroll
-					for(int u4 = 0; u4 < 4; u4++) {
-						f[u4] = mk(Fr[2 * (q + u4)], Fr[2 * (q + u4) + 1]);
-						w[u4] = tw[tq[u4]];
-					}
-					cfma(acc, f[0], w[0]);
-					cfma(acc2, f[1], w[1]);
-					cfma(acc, f[2], w[2]);
-					cfma(acc2, f[3], w[3]);
+				if(!par) {
+					/* m = 0: weight 1, the other modes weight 2 */
+					a0 = Fr[0];
+					q = 1;
+					t = dt;
 				}
-				for(; q < nq; q++) {
-					cfma(acc, mk(Fr[2 * q], Fr[2 * q + 1]), tw[t]);
-					t += dt;
-					if(t >= G::n)
-						t -= G::n;
+				for(; q + 2 <= nq; q += 2) {
+					const int t1 = t + dt >= G::n ? t + dt - G::n : t + dt;
+					const cplx<T> w0 = tw[t], w1 = tw[t1];
+					acc += Fr[2 * q] * w0.re - Fr[2 * q + 1] * w0.im;
+					acc2 += Fr[2 * q + 2] * w1.re - Fr[2 * q + 3] * w1.im;
+					t = t1 + dt >= G::n ? t1 + dt - G::n : t1 + dt;
 				}
-				acc = acc + acc2;
+				if(q < nq)
+					acc += Fr[2 * q] * tw[t].re - Fr[2 * q + 1] * tw[t].im;
+				const T v = a0 + (T) 2 * (acc + acc2);
 				/* the pair (Ev, Od) sits in lanes tid, tid ^ 1 of one
 				 * wave (u and u ^ 1: both in range or both out) */
-				const T ore = __shfl_xor(acc.re, 1), oim = __shfl_xor(acc.im, 1);
+				const T ov = __shfl_xor(v, 1);
 				if(!par) {
 					cplx<T> *row = map + (size_t) s * G::n * G::n + (size_t) r * G::n;
-					row[k] = mk(acc.re + ore, acc.im + oim);
-					row[k + G::B] = mk(acc.re - ore, acc.im - oim);
+					row[k] = mk(v + ov, (T) 0);
+					row[k + G::B] = mk(v - ov, (T) 0);
 				}
 			}
 		}
@@ -628,7 +610,7 @@ template <typename T> struct sf_sub {
 };
 
 template <typename T, int L>
-static int sky_fused_at(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int ncu, hipStream_t st)
+static int sky_fused_at(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int ncu, int occ, hipStream_t st)
 {
 	typedef sf_geom<L> G;
 	const int NS = sf_sub<T>::N;
@@ -636,18 +618,22 @@ static int sky_fused_at(const T *partial, int nsplit, int S, const struct xproj_
 	const size_t stat = NS * (2 * (size_t) G::n * G::FS * sizeof(T) + (size_t) (G::ncoef + L + 1) * sizeof(cplx<T>))
 			    + (size_t) (G::nal + 2 + G::n) * sizeof(cplx<T>) + G::ncoef * sizeof(short) + (size_t) G::B * (L + 2) * sizeof(T) + 1024;
 	const size_t rbytes = (size_t) xg->nb * G::ncoef * sizeof(cplx<T>), pbytes = (size_t) NS * xg->CT * sizeof(T);
-	/* the rotation rows go to LDS when they fit beside the rest (160 KiB
-	 * per CU, one workgroup per CU) */
-	const int rlds = stat + rbytes + pbytes <= 160 * 1024;
+	/* persistent workgroups, occ per CU (1 or 2: the second needs the
+	 * LDS of both within the CU's 160 KiB); the rotation rows go to LDS
+	 * when they fit beside the rest at that occupancy */
+	const size_t cu_lds = 160 * 1024;
+	if(occ > 1 && 2 * (stat + pbytes) > cu_lds)
+		occ = 1;
+	const int rlds = occ * (stat + rbytes + pbytes) <= cu_lds;
 	const size_t dyn = (rlds ? rbytes : 0) + pbytes;
 	const int nwg = (S + NS - 1) / NS;
-	const int grid = nwg < ncu ? nwg : ncu;
+	const int grid = nwg < occ * ncu ? nwg : occ * ncu;
 	hipLaunchKernelGGL((k_sky_fused<T, L, sf_sub<T>::N>), dim3(grid), dim3(SF_NT * NS), dyn, st, partial, nsplit, S, *xg, R, rlds, lm, gmst, *tb, alm, map);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <typename T>
-int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st)
+int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st, int occ)
 {
 	static int ncu = 0;
 	if(!ncu) {
@@ -661,13 +647,13 @@ int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geo
 	/* persistent workgroups, one per CU */
 	switch(g->L) {
 	case 25:
-		return sky_fused_at<T, 25>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, st);
+		return sky_fused_at<T, 25>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
 	case 32:
-		return sky_fused_at<T, 32>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, st);
+		return sky_fused_at<T, 32>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
 	default:
 		return -1;
 	}
 }
 
-template int launch_sky_fused<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
-template int launch_sky_fused<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);
+template int launch_sky_fused<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t, int);
+template int launch_sky_fused<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t, int);

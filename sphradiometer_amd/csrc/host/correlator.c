@@ -372,6 +372,7 @@ fail:
 void correlator_plan_fd_free(struct correlator_plan_fd *p)
 {
 	if(p) {
+		sphr_gpu_plan_forget(p);
 		free(p->fseries_product);
 		sh_series_array_free(p->delay_product);
 		sh_series_free(p->power_1d);
@@ -410,6 +411,7 @@ struct correlator_plan_fd *correlator_plan_fd_set_l(struct correlator_plan_fd *p
 		return NULL;
 	if(l_max == p->power_1d->l_max)
 		return p;
+	sphr_gpu_plan_forget(p);
 	if(!sh_series_array_set_l(p->delay_product, l_max) || !sh_series_resize(p->power_1d, l_max) || !sh_series_rotation_plan_set_l(p->rotation_plan, l_max))
 		return NULL;
 	return p;
@@ -484,6 +486,7 @@ struct correlator_network_plan_fd *correlator_network_plan_fd_set_l(struct corre
 
 static void fill_baseline(struct sphr_fd_baseline *b, struct correlator_plan_fd *p, int ia, int ib)
 {
+	b->key = p;
 	b->index_a = ia;
 	b->index_b = ib;
 	b->l_max = p->power_1d->l_max;
@@ -517,6 +520,11 @@ struct sh_series *correlator_baseline_integrate_power_fd(const double complex *f
 		return NULL;
 	}
 	return out;
+}
+
+void correlator_plan_fd_device_invalidate(struct correlator_plan_fd *p)
+{
+	sphr_gpu_plan_forget(p);
 }
 
 /* correlator.c:993-1010: sky = (1/nb) sum_k rotate(power_1d_k) (GPU) */

@@ -286,6 +286,7 @@ baseline_done:
 			sh_series_array_free(res);
 			return -1;
 		}
+		sphr_gpu_plan_forget(p);
 		sh_series_array_free(p->delay_product);
 		p->delay_product = res;
 		/* inspiral_sky_map.c:518-532 */

@@ -91,6 +91,7 @@ int sphr_gpu_rotate_z(unsigned l_max, const double *coeff, double phi, double *r
  * fseries_product scratch of the reference plan, may be NULL) and
  * sky (non-polar, sky_l) = (1/nb) sum_k rotate(power_1d[k]). */
 struct sphr_fd_baseline {
+	const void *key;           /* the plan: its device copy is kept across calls */
 	int index_a, index_b;
 	unsigned l_max;
 	int polar;
@@ -101,6 +102,9 @@ struct sphr_fd_baseline {
 	double *fsp;               /* out: n complex, optional */
 };
 int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int n, int nb, const struct sphr_fd_baseline *bl, unsigned sky_l, double scale, double *sky);
+/* drops the device copy of a plan's delay product and rotation table
+ * (called by every function that changes or frees them) */
+void sphr_gpu_plan_forget(const void *key);
 
 /* a network FD plan flattened for the batched device pipeline */
 #define SPHR_DESC_MAX 16
