@@ -224,7 +224,9 @@ void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map)
 
 /* Autotuner: runs the batch on these buffers with each candidate shape of
  * the projection (tiles per wave, segments per workgroup, K split), keeps
- * the fastest, returns its index (time per run in best_ms) or -1 */
+ * the fastest, then tries the fused sky kernel at two workgroups per CU;
+ * returns the projection shape's index (time per run in best_ms), or -1
+ * with the previous settings kept */
 int sphr_batch_tune(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, double *best_ms);
 /* Select one of the autotuner's shapes by its index (0..27, as
  * sphr_batch_tune returns it); 0, or -1 for an index out of range */
@@ -234,14 +236,19 @@ int sphr_batch_set_shape(struct sphr_batch *, int index);
  * [4] projection columns, [5] K splits, [6] precision, [7] K per split,
  * [8] 1 when the sky stage runs as the one fused kernel (orders it is
  * compiled for), 0 for the two-kernel form, [9] projection tiles per wave,
- * [10] projection segments per workgroup */
-void sphr_batch_info(const struct sphr_batch *, int info[11]);
+ * [10] projection segments per workgroup, [11] fused sky workgroups per
+ * CU (1 or 2, picked by the autotuner) */
+void sphr_batch_info(const struct sphr_batch *, int info[12]);
 
 /* Select the sky stage's form: 1 (default) the fused kernel where it is
  * compiled for the batch's order, 0 the two-kernel form (a_lm and ring
  * sums, then the phi DFT as a GEMM).  Both give the same results to
  * rounding. */
 void sphr_batch_set_fused(struct sphr_batch *, int on);
+/* Fused sky kernel's persistent workgroups per CU: 1 (default) or 2 (the
+ * rotation rows then stay in global memory when both would not fit the
+ * CU's LDS); same results; -1 for other values */
+int sphr_batch_set_sky_occupancy(struct sphr_batch *, int workgroups_per_cu);
 
 #ifdef __cplusplus
 }

@@ -228,6 +228,7 @@ _PROTOS = {
     "sphr_batch_profile_read": (c_int, [c_void_p, P(c_double), P(c_int)]),
     "sphr_batch_info": (None, [c_void_p, P(c_int)]),
     "sphr_batch_set_fused": (None, [c_void_p, c_int]),
+    "sphr_batch_set_sky_occupancy": (c_int, [c_void_p, c_int]),
     "sphr_batch_set_shape": (c_int, [c_void_p, c_int]),
     "sphr_batch_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sphr_batch_tune": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P(c_double)]),
@@ -534,10 +535,10 @@ class Batch:
         self.network = network
 
     def info(self):
-        a = (c_int * 11)()
+        a = (c_int * 12)()
         lib().sphr_batch_info(self.h, a)
         keys = ["detectors", "baselines", "N", "sky_l", "columns", "k_splits", "precision", "k_per_split", "fused",
-                "tiles_per_wave", "segments_per_workgroup"]
+                "tiles_per_wave", "segments_per_workgroup", "sky_workgroups_per_cu"]
         return dict(zip(keys, list(a)))
 
     def set_shape(self, index):
@@ -549,6 +550,10 @@ class Batch:
         """Sky stage as the one fused kernel (default, where compiled for
         the order) or as the two-kernel form."""
         lib().sphr_batch_set_fused(self.h, int(bool(on)))
+
+    def set_sky_occupancy(self, workgroups_per_cu):
+        if lib().sphr_batch_set_sky_occupancy(self.h, int(workgroups_per_cu)):
+            raise RuntimeError("sphr_batch_set_sky_occupancy failed: " + last_error())
 
     def run(self, d_tseries, n_segments, d_alm, d_map=None, d_window=None, d_gmst=None, f32_input=False, stream=None):
         r = lib().sphr_batch_run(self.h, d_tseries, int(f32_input), n_segments, d_window, d_gmst, d_alm, d_map, stream)

@@ -35,6 +35,7 @@ struct sphr_batch {
 	int tpw;			/* projection tiles per wave (autotuned) */
 	int xms;			/* projection segments per workgroup, 16 or 32 (autotuned) */
 	int fused;			/* use the one-kernel sky stage where compiled (default 1) */
+	int sky_occ;			/* fused sky workgroups per CU, 1 or 2 (autotuned) */
 	int ksplit_opt[4], nsplit_opt[4];	/* K-split candidates: 1, 2, 4 x CUs workgroups, 64 per split */
 	struct xproj_geom g;
 	struct synth_dev syn;
@@ -202,6 +203,7 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 		bt->tpw = 0;
 		bt->xms = 64;
 		bt->fused = 1;
+		bt->sky_occ = 1;
 	}
 	int *colbl = (int *) malloc((CT / 16) * sizeof(int));
 	const int ncoef = (bt->L + 1) * (bt->L + 1);
@@ -258,7 +260,7 @@ static int run_t(struct sphr_batch *bt, void *spec, void *partial, void *F, cons
 	/* the fused sky kernel at the orders it is compiled for; else a_lm
 	 * and ring sums, then the phi DFT as a GEMM; the one-kernel form when
 	 * the coefficients do not fit the LDS budget */
-	if(bt->fused && launch_sky_fused<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st) == 0) {
+	if(bt->fused && launch_sky_fused<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st, bt->sky_occ) == 0) {
 		if(ev)
 			hipEventRecord(ev[3], st);
 	} else if((F || !map) && launch_sky_lf<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) F : (cplx<T> *) NULL, st) == 0) {
@@ -409,8 +411,19 @@ extern "C" void sphr_batch_set_fused(struct sphr_batch *bt, int on)
 		bt->fused = on ? 1 : 0;
 }
 
-extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[11])
+extern "C" int sphr_batch_set_sky_occupancy(struct sphr_batch *bt, int workgroups_per_cu)
 {
+	if(!bt || workgroups_per_cu < 1 || workgroups_per_cu > 2) {
+		sphr_set_error("sphr_batch_set_sky_occupancy: 1 or 2 workgroups per CU");
+		return -1;
+	}
+	bt->sky_occ = workgroups_per_cu;
+	return 0;
+}
+
+extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[12])
+{
+	info[11] = bt->sky_occ;
 	info[8] = bt->fused && sky_fused_order(bt->L);
 	info[9] = bt->tpw;
 	info[10] = bt->xms;
@@ -469,47 +482,77 @@ extern "C" int sphr_batch_set_shape(struct sphr_batch *bt, int index)
 
 extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, int S, const double *window, const double *gmst, void *alm, void *map, double *best_ms)
 {
-	if(!bt || S < 1 || S > bt->max_segments || !ts || !alm)
+	if(!bt || S < 1 || S > bt->max_segments || !ts || !alm) {
+		sphr_set_error("sphr_batch_tune: bad arguments");
 		return -1;
+	}
 	hipStream_t st = sphr_rt_stream();
-	hipEvent_t e0, e1;
-	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
-		return -1;
-	const int *tpws = tune_tpw, *mss = tune_ms;
-	float best = 1e30f;
-	int bo = 1, bw = 0;
+	const int kps0 = bt->k_per_split, ns0 = bt->nsplit, tpw0 = bt->tpw, xms0 = bt->xms, occ0 = bt->sky_occ;
 	const int prof = bt->profile;
+	hipEvent_t e0 = NULL, e1 = NULL;
+	float best = 1e30f;
+	int bo = -1, bw = 0, bocc = 1;
 	bt->profile = 0;
+	/* time one candidate: a run, then three timed runs; -1 on failure */
+	auto timed = [&](float *ms) -> int {
+		int r = sphr_batch_run(bt, ts, f32, S, window, gmst, alm, map, st);
+		r = r || sphr_rt_check(hipEventRecord(e0, st), "tune");
+		for(int i = 0; i < 3 && !r; i++)
+			r = sphr_batch_run(bt, ts, f32, S, window, gmst, alm, map, st);
+		r = r || sphr_rt_check(hipEventRecord(e1, st), "tune") || sphr_rt_check(hipEventSynchronize(e1), "tune");
+		return r || sphr_rt_check(hipEventElapsedTime(ms, e0, e1), "tune") ? -1 : 0;
+	};
+	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
+		goto fail;
+	/* the projection's shapes and K splits with the fused sky kernel at
+	 * one workgroup per CU, then two workgroups per CU at the best */
+	bt->sky_occ = 1;
 	for(int o = 0; o < 4; o++)
 		for(int w = 0; w < 7; w++) {
 			bt->k_per_split = bt->ksplit_opt[o];
 			bt->nsplit = bt->nsplit_opt[o];
-			bt->tpw = tpws[w];
-			bt->xms = mss[w];
-			if(sphr_batch_run(bt, ts, f32, S, window, gmst, alm, map, st))
-				return -1;
-			hipEventRecord(e0, st);
-			for(int r = 0; r < 3; r++)
-				sphr_batch_run(bt, ts, f32, S, window, gmst, alm, map, st);
-			hipEventRecord(e1, st);
-			if(sphr_rt_check(hipEventSynchronize(e1), "tune"))
-				return -1;
+			bt->tpw = tune_tpw[w];
+			bt->xms = tune_ms[w];
 			float ms = 0.f;
-			hipEventElapsedTime(&ms, e0, e1);
+			if(timed(&ms))
+				goto fail;
 			if(ms < best) {
 				best = ms;
 				bo = o;
 				bw = w;
 			}
 		}
+	bt->k_per_split = bt->ksplit_opt[bo];
+	bt->nsplit = bt->nsplit_opt[bo];
+	bt->tpw = tune_tpw[bw];
+	bt->xms = tune_ms[bw];
+	if(bt->fused && sky_fused_order(bt->L) && map) {
+		bt->sky_occ = 2;
+		float ms = 0.f;
+		if(timed(&ms))
+			goto fail;
+		if(ms < best) {
+			best = ms;
+			bocc = 2;
+		}
+	}
+	bt->sky_occ = bocc;
 	hipEventDestroy(e0);
 	hipEventDestroy(e1);
 	bt->profile = prof;
-	bt->k_per_split = bt->ksplit_opt[bo];
-	bt->nsplit = bt->nsplit_opt[bo];
-	bt->tpw = tpws[bw];
-	bt->xms = mss[bw];
 	if(best_ms)
 		*best_ms = best / 3.0;
 	return 7 * bo + bw;
+fail:
+	if(e0)
+		hipEventDestroy(e0);
+	if(e1)
+		hipEventDestroy(e1);
+	bt->k_per_split = kps0;
+	bt->nsplit = ns0;
+	bt->tpw = tpw0;
+	bt->xms = xms0;
+	bt->sky_occ = occ0;
+	bt->profile = prof;
+	return -1;
 }

@@ -287,15 +287,20 @@ extern "C" int sphr_gpu_to_mesh(unsigned l_max, int polar, const double *coeff, 
 	struct synth_dev *t = synth_for(L);
 	cplx<double> *da = (cplx<double> *) sphr_rt_scratch(SL_A, nc * sizeof(cplx<double>));
 	cplx<double> *dm = (cplx<double> *) sphr_rt_scratch(SL_B, (size_t) n * n * sizeof(cplx<double>));
-	if(!t || !da || !dm)
+	/* pinned staging both ways: one DMA each, no driver-side bounce */
+	cplx<double> *hin = (cplx<double> *) sphr_rt_pinned(2, nc * sizeof(cplx<double>));
+	cplx<double> *hout = (cplx<double> *) sphr_rt_pinned(3, (size_t) n * n * sizeof(cplx<double>));
+	if(!t || !da || !dm || !hin || !hout)
 		goto fail;
-	CK(hipMemcpyAsync(da, coeff, nc * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy coefficients");
+	memcpy(hin, coeff, nc * sizeof(cplx<double>));
+	CK(hipMemcpyAsync(da, hin, nc * sizeof(cplx<double>), hipMemcpyHostToDevice, st), "copy coefficients");
 	if(launch_synth<double, double, double>(da, 1, polar, &t->g, &t->tb, dm, st)) {
 		sphr_set_error("synthesis launch failed");
 		goto fail;
 	}
-	CK(hipMemcpyAsync(mesh, dm, (size_t) n * n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy mesh");
+	CK(hipMemcpyAsync(hout, dm, (size_t) n * n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy mesh");
 	CK(hipStreamSynchronize(st), "synchronize");
+	memcpy(mesh, hout, (size_t) n * n * sizeof(cplx<double>));
 	UNLOCK();
 	return 0;
 fail:
@@ -602,9 +607,13 @@ extern "C" int sphr_fft_exec(struct sphr_fft *f)
 	const cplx<double> *tw = twiddles_for(n);
 	double *din = (double *) sphr_rt_scratch(SL_A, in_bytes);
 	cplx<double> *dout = (cplx<double> *) sphr_rt_scratch(SL_B, (size_t) n * sizeof(cplx<double>));
-	if(!tw || !din || !dout)
+	/* pinned staging both ways */
+	double *hin = (double *) sphr_rt_pinned(2, in_bytes);
+	cplx<double> *hout = (cplx<double> *) sphr_rt_pinned(3, (size_t) n * sizeof(cplx<double>));
+	if(!tw || !din || !dout || !hin || !hout)
 		goto fail;
-	CK(hipMemcpyAsync(din, f->in, in_bytes, hipMemcpyHostToDevice, st), "copy series");
+	memcpy(hin, f->in, in_bytes);
+	CK(hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, st), "copy series");
 	{
 		const int pow2 = !(n & (n - 1));
 		const int r = (pow2 && (size_t) n * sizeof(cplx<double>) <= sphr_rt_lds_max()) ? launch_fft_single(din, f->real_input, n, tw, dout, st) : launch_dft_direct(din, f->real_input, n, tw, dout, st);
@@ -613,8 +622,9 @@ extern "C" int sphr_fft_exec(struct sphr_fft *f)
 			goto fail;
 		}
 	}
-	CK(hipMemcpyAsync(f->out, dout, (size_t) n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy spectrum");
+	CK(hipMemcpyAsync(hout, dout, (size_t) n * sizeof(cplx<double>), hipMemcpyDeviceToHost, st), "copy spectrum");
 	CK(hipStreamSynchronize(st), "synchronize");
+	memcpy(f->out, hout, (size_t) n * sizeof(cplx<double>));
 	UNLOCK();
 	return 0;
 fail:

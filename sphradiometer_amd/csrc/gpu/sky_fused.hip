@@ -148,8 +148,12 @@ template <int L> struct sf_rg {
 #define SF_NBF 4
 
 /* a_lm of one segment: sum over baselines of rotation row x the baseline's
- * projected (l) column, then rotate_z; RP is an LDS or a global array of
+ * projected (l) column, accumulated in float64 whatever T (the network's
+ * cross-baseline sum), then rotate_z; RP is an LDS or a global array of
  * the rotation rows [nb][ncoef] */
+__device__ inline cplx<double> sf_wide(cplx<float> a) { return mk((double) a.re, (double) a.im); }
+__device__ inline cplx<double> sf_wide(cplx<double> a) { return a; }
+
 template <typename T, int L, typename RP>
 __device__ inline void sf_alm(RP Rr, const T *pw, const short *slm, const cplx<T> *rot, bool rz, const struct xproj_geom &xg, const int *csb, const int *lbb, int tid, cplx<T> *sa, cplx<T> *alm)
 {
@@ -159,7 +163,7 @@ __device__ inline void sf_alm(RP Rr, const T *pw, const short *slm, const cplx<T
 	for(int idx = tid; idx < G::ncoef; idx += SF_NT) {
 		const int q = slm[idx];
 		const int l = q >> 8, m = (signed char) (q & 255);
-		cplx<T> acc = mk((T) 0, (T) 0);
+		cplx<double> acc = mk(0.0, 0.0);
 		if(nb <= SF_NBF) {
 			cplx<T> r[SF_NBF];
 #pragma unroll
@@ -170,19 +174,20 @@ __device__ inline void sf_alm(RP Rr, const T *pw, const short *slm, const cplx<T
 			for(int b = 0; b < SF_NBF; b++)
 				if(b < nb && l <= lbb[b]) {
 					const T *p = pw + csb[b] + 2 * l;
-					cfma(acc, r[b], mk(p[0], p[1]));
+					cfma(acc, sf_wide(r[b]), mk((double) p[0], (double) p[1]));
 				}
 		} else {
 			for(int b = 0; b < nb; b++)
 				if(l <= xg.Lb[b]) {
 					const T *p = pw + xg.col_start[b] + 2 * l;
-					cfma(acc, Rr[b * G::ncoef + idx], mk(p[0], p[1]));
+					cfma(acc, sf_wide(Rr[b * G::ncoef + idx]), mk((double) p[0], (double) p[1]));
 				}
 		}
+		cplx<T> a = mk((T) acc.re, (T) acc.im);
 		if(rz && m)
-			acc = acc * (m > 0 ? rot[m] : conjg(rot[-m]));
-		sa[idx] = acc;
-		alm[idx] = acc;
+			a = a * (m > 0 ? rot[m] : conjg(rot[-m]));
+		sa[idx] = a;
+		alm[idx] = a;
 	}
 }
 

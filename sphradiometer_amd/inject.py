@@ -235,3 +235,55 @@ def uneven_jobs(n_jobs, seed=0, delta_t=1.0 / 4096, l_range=(16, 64), det_range=
         jobs.append({"positions": pos, "N": N, "l_max": L, "burst": burst})
         chunks.append(x.ravel())
     return jobs, np.concatenate(chunks)
+
+
+def proposal_jobs(n_jobs, seed=0, l_max=32, delta_t=1.0 / 4096, det_range=(2, 5),
+                  durations=(0.125, 0.25, 0.5, 1.0, 2.0), snr_range=(8.0, 30.0)):
+    """Skies of the uneven workload at one order, for the learned peak
+    proposal: per job a detector count ~ U{det_range} with sites uniform on
+    the 6371 km sphere, a segment length ~ U(durations) at 1 / delta_t, and
+    one sine-Gaussian plane wave (f0 ~ U(100, 400) Hz, Q = 9, network SNR
+    ~ U(snr_range), isotropic direction) delayed per site by r . s,
+    mid-segment, in white N(0,1) float32 noise.  The jobs come out ordered
+    by class of equal (detectors, length), so each class's packed data
+    [job][detector][sample] is one contiguous block that the device
+    injector (sphr_inject_bursts with one detector per series) fills.
+    Returns (jobs, classes, injections): classes a list of (float offset,
+    series = jobs x detectors, length, params float32 [series][4] = amp,
+    f0, tau, delay); injections the theta, phi, snr per job in job order."""
+    rng = np.random.default_rng(seed)
+    r = EARTH_RADIUS_SPHERE / C_SI
+    D = rng.integers(det_range[0], det_range[1] + 1, n_jobs)
+    Nn = np.array([int(round(d / delta_t)) for d in durations])
+    N = Nn[rng.integers(0, len(durations), n_jobs)]
+    jobs, classes = [], []
+    th_all, ph_all, snr_all = [], [], []
+    off = 0
+    for d in range(det_range[0], det_range[1] + 1):
+        for n in Nn:
+            S = int(np.sum((D == d) & (N == n)))
+            if S == 0:
+                continue
+            v = rng.standard_normal((S, d, 3))
+            pos = r * v / np.linalg.norm(v, axis=2, keepdims=True)
+            f0 = rng.uniform(100.0, 400.0, S)
+            cth = rng.uniform(-1.0, 1.0, S)
+            phi = rng.uniform(0.0, 2 * math.pi, S)
+            snr = rng.uniform(*snr_range, S)
+            st = np.sqrt(1.0 - cth * cth)
+            sdir = np.stack([st * np.cos(phi), st * np.sin(phi), cth], axis=1)
+            tau = 9.0 / (math.sqrt(2.0) * math.pi * f0)
+            amp = snr / np.sqrt(d * tau * math.sqrt(math.pi / 2) / (2 * delta_t))
+            par = np.empty((S, d, 4), dtype=np.float32)
+            par[:, :, 0] = amp[:, None]
+            par[:, :, 1] = f0[:, None]
+            par[:, :, 2] = tau[:, None]
+            par[:, :, 3] = np.einsum("skj,sj->sk", pos, sdir)
+            classes.append((off, S * d, int(n), par.reshape(S * d, 4)))
+            off += S * d * int(n)
+            for s in range(S):
+                jobs.append({"positions": pos[s], "N": int(n), "l_max": l_max, "burst": True})
+            th_all.append(np.arccos(cth))
+            ph_all.append(phi)
+            snr_all.append(snr)
+    return jobs, classes, {"theta": np.concatenate(th_all), "phi": np.concatenate(ph_all), "snr": np.concatenate(snr_all)}

@@ -153,10 +153,11 @@ def test_network_integrate_three_site_and_golden(oracle, three_site):
     assert rel(mesh, gold["three_site_map_0"]) < TOL64
 
 
-def _run_batch(net, L, ts, precision, window=None, gmst=None, fused=True):
+def _run_batch(net, L, ts, precision, window=None, gmst=None, fused=True, occ=1):
     D, Sg, N = ts.shape
     b = S.Batch(net, L, precision, max_segments=Sg)
     b.set_fused(fused)
+    b.set_sky_occupancy(occ)
     cdt = np.complex128 if precision == S.FP64 else np.complex64
     d_ts = S.DeviceBuffer.from_numpy(ts.astype(np.float64))
     d_alm = S.DeviceBuffer(Sg * (L + 1) ** 2 * np.dtype(cdt).itemsize)
@@ -207,11 +208,13 @@ def test_batch_sky_stage_forms_agree(oracle, three_site, precision):
     gm = np.linspace(0.3, 1.7, 5)
     a1, m1, i1 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True)
     a2, m2, i2 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=False)
-    assert i1["fused"] == 1 and i2["fused"] == 0
+    # two persistent workgroups per CU (rotation rows from global memory)
+    a3, m3, i3 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True, occ=2)
+    assert i1["fused"] == 1 and i2["fused"] == 0 and i3["sky_workgroups_per_cu"] == 2
     onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
     for s in range(5):
         sky, mesh = onet.segment(ts[:, s], 32, gmst=float(gm[s]))
-        for alm, mp in ((a1, m1), (a2, m2)):
+        for alm, mp in ((a1, m1), (a2, m2), (a3, m3)):
             assert rel(alm[s], sky) < tol
             assert rel(mp[s], mesh) < (tol if prec == S.FP64 else 5e-6)
 
