@@ -507,7 +507,7 @@ extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int
 	const size_t out_bytes = (nsky + pw_total + (size_t) nb * n) * sizeof(cplx<double>);
 	cplx<double> *dfs = (cplx<double> *) sphr_rt_scratch(SL_A, in_bytes);
 	cplx<double> *dout = (cplx<double> *) sphr_rt_scratch(SL_B, out_bytes);
-	cplx<double> *dpart = (cplx<double> *) sphr_rt_scratch(SL_F, (size_t) nsplit * maxstride * sizeof(cplx<double>));
+	cplx<double> *dpart = (cplx<double> *) sphr_rt_scratch(SL_F, (size_t) nb * nsplit * maxstride * sizeof(cplx<double>));
 	cplx<double> *hin = (cplx<double> *) sphr_rt_pinned(0, in_bytes);
 	cplx<double> *hout = (cplx<double> *) sphr_rt_pinned(1, out_bytes);
 	cplx<double> *dsky = dout, *dpw = dout + nsky, *dfsp = dpw + pw_total;
@@ -533,9 +533,18 @@ extern "C" int sphr_gpu_network_fd(int n_inst, const double *const *fseries, int
 			goto fail;
 		}
 	}
-	for(int b = 0; b < nb; b++) {
-		/* power_1d = (1/n) sum_f dp[f] X[f] (correlator.c:690-703) */
-		if(launch_dotc(pd[b]->d_dp, n, bl[b].stride, dfsp + (size_t) b * n, NULL, 1.0 / n, dpart, nsplit, dpw + pw_off[b], st)) {
+	{
+		/* power_1d = (1/n) sum_f dp[f] X[f] (correlator.c:690-703), all
+		 * baselines in one pair of launches */
+		struct dotc_net_args da;
+		for(int b = 0; b < nb; b++) {
+			da.coeff[b] = pd[b]->d_dp;
+			da.vec[b] = dfsp + (size_t) b * n;
+			da.partial[b] = dpart + (size_t) b * nsplit * maxstride;
+			da.out[b] = dpw + pw_off[b];
+			da.stride[b] = bl[b].stride;
+		}
+		if(launch_dotc_net(&da, nb, n, nsplit, 1.0 / n, st)) {
 			sphr_set_error("projection launch failed");
 			goto fail;
 		}

@@ -129,6 +129,15 @@ int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *
 /* drop-in helpers (FP64) */
 int launch_xspec(const cplx<double> *const *fs_dev, int n, int nb, const int *ia, const int *ib, cplx<double> *fsp, hipStream_t st);
 int launch_dotc(const cplx<double> *coeff, int n, int stride, const cplx<double> *vec, const double *vec_real, double scale, cplx<double> *partial, int nsplit, cplx<double> *out, hipStream_t st);
+/* the projections of every baseline of a network at once: out[b] =
+ * scale sum_i coeff[b][i][.] vec[b][i]; partial[b] holds nsplit x
+ * stride[b] complex */
+struct dotc_net_args {
+	const cplx<double> *coeff[SPHR_MAX_BASELINES], *vec[SPHR_MAX_BASELINES];
+	cplx<double> *partial[SPHR_MAX_BASELINES], *out[SPHR_MAX_BASELINES];
+	int stride[SPHR_MAX_BASELINES];
+};
+int launch_dotc_net(const struct dotc_net_args *a, int nb, int n, int nsplit, double scale, hipStream_t st);
 int launch_rotsum(int nb, const cplx<double> *const *power, const int *Lb, const int *polar, const cplx<double> *const *blocks, int L, double scale, cplx<double> *sky, hipStream_t st);
 /* the rows D^l_{0 m'} of packed rotation blocks (l = 1 first, each
  * (2l+1)^2) in the coefficient layout of order L, as k_rotsum reads them

@@ -1134,6 +1134,13 @@ int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom 
 	const size_t nal = polar ? L + 1 : (size_t) (L + 1) * (L + 2) / 2;
 	int rp = (int) (24576 / (2 * n * sizeof(cplx<T>)));
 	rp = rp < 1 ? 1 : rp > g->B ? g->B : rp;
+	/* few series (the drop-in's one): fewer ring pairs per workgroup, so
+	 * the launch spreads over the CUs instead of a handful of them */
+	const int cus = sphr_rt_cus() > 0 ? sphr_rt_cus() : 256;
+	if((long) S * ((g->B + rp - 1) / rp) < 2L * cus) {
+		const int want = (int) (((long) g->B * S + 2L * cus - 1) / (2L * cus));
+		rp = want < 1 ? 1 : want < rp ? want : rp;
+	}
 	size_t lds = (size_t) (2 * rp + 1) * n * sizeof(cplx<T>);
 	const int a_in_lds = lds + ncoef * sizeof(cplx<T>) <= budget - 4096;
 	if(a_in_lds)
@@ -1230,6 +1237,56 @@ int launch_dotc(const cplx<double> *coeff, int n, int stride, const cplx<double>
 	const int rps = (n + nsplit - 1) / nsplit;
 	hipLaunchKernelGGL(k_dotc_partial, dim3((stride + 63) / 64, nsplit), dim3(NT), 0, st, coeff, n, stride, vec, vec_real, rps, partial);
 	hipLaunchKernelGGL(k_dotc_reduce, dim3((stride + NT - 1) / NT), dim3(NT), 0, st, partial, nsplit, stride, scale, out);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+/* the projections of all baselines of a network in two launches: grid
+ * (column blocks, K splits, baselines), then (column blocks, baselines) */
+__global__ void __launch_bounds__(NT) k_dotc_net_partial(struct dotc_net_args a, int n, int rows_per_split, int nsplit)
+{
+	__shared__ cplx<double> red[4][64];
+	const int b = blockIdx.z;
+	const int stride = a.stride[b];
+	const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+	const int lane = threadIdx.x >> 6;
+	const int i0 = blockIdx.y * rows_per_split;
+	const int i1 = min(n, i0 + rows_per_split);
+	const cplx<double> *coeff = a.coeff[b], *vec = a.vec[b];
+	cplx<double> acc = mk(0.0, 0.0);
+	if(c < stride)
+		for(int i = i0 + lane; i < i1; i += 4)
+			cfma(acc, coeff[(size_t) i * stride + c], vec[i]);
+	red[lane][threadIdx.x & 63] = acc;
+	__syncthreads();
+	if(lane == 0 && c < stride) {
+		cplx<double> t = red[0][threadIdx.x];
+		t = t + red[1][threadIdx.x];
+		t = t + red[2][threadIdx.x];
+		t = t + red[3][threadIdx.x];
+		a.partial[b][(size_t) blockIdx.y * stride + c] = t;
+	}
+}
+
+__global__ void k_dotc_net_reduce(struct dotc_net_args a, int nsplit, double scale)
+{
+	const int b = blockIdx.y;
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if(c >= a.stride[b])
+		return;
+	cplx<double> t = mk(0.0, 0.0);
+	for(int sp = 0; sp < nsplit; sp++)
+		t = t + a.partial[b][(size_t) sp * a.stride[b] + c];
+	a.out[b][c] = t * scale;
+}
+
+int launch_dotc_net(const struct dotc_net_args *a, int nb, int n, int nsplit, double scale, hipStream_t st)
+{
+	int maxs = 1;
+	for(int b = 0; b < nb; b++)
+		maxs = a->stride[b] > maxs ? a->stride[b] : maxs;
+	const int rps = (n + nsplit - 1) / nsplit;
+	hipLaunchKernelGGL(k_dotc_net_partial, dim3((maxs + 63) / 64, nsplit, nb), dim3(NT), 0, st, *a, n, rps, nsplit);
+	hipLaunchKernelGGL(k_dotc_net_reduce, dim3((maxs + NT - 1) / NT, nb), dim3(NT), 0, st, *a, nsplit, scale);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -340,6 +340,14 @@ def summary_jobs(res, tol_deg=0.1, gap=0.01):
         out[k] = {"within_%.2f_deg" % tol_deg: float(np.mean(s <= tol)), "median_deg": float(np.degrees(np.median(s))),
                   "p99_deg": float(np.degrees(np.quantile(s, 0.99)))}
     out["fallback_fraction"] = float(np.mean(res["fallback"]))
+    # the scale of the skies' values, against which the absolute gap acts
+    pv = res["plain_value"]
+    out["plain_value_quantiles"] = {"p10": float(np.quantile(pv, 0.1)), "median": float(np.median(pv)), "p90": float(np.quantile(pv, 0.9))}
+    # the same proposals with a fallback relative to the plain peak's value
+    # (refined value short of it by more than 1e-5 of it)
+    fb_rel = res["best_value"] < pv - 1e-5 * np.abs(pv)
+    sep_rel = np.where(fb_rel, 0.0, res["proposal"])
+    out["final_relative_gap_1e-5"] = {"within_%.2f_deg" % tol_deg: float(np.mean(sep_rel <= tol)), "fallback_fraction": float(np.mean(fb_rel))}
     # a final answer away from the plain peak whose value is within the gap
     # of it: a second maximum of (nearly) the same height
     far = res["final"] > tol
@@ -481,7 +489,7 @@ def main():
                          "three_site: the fixed three-site network, 0.25 s segments")
     ap.add_argument("--train", type=int, default=1 << 20)
     ap.add_argument("--eval", type=int, default=65536)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=16)
     ap.add_argument("--batch", type=int, default=2048)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--chunk", type=int, default=8192)
