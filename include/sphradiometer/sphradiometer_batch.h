@@ -237,7 +237,7 @@ int sphr_batch_set_shape(struct sphr_batch *, int index);
  * [8] 1 when the sky stage runs as the one fused kernel (orders it is
  * compiled for), 0 for the two-kernel form, [9] projection tiles per wave,
  * [10] projection segments per workgroup, [11] fused sky workgroups per
- * CU (1 or 2, picked by the autotuner) */
+ * CU requested (1-3, picked by the autotuner) */
 void sphr_batch_info(const struct sphr_batch *, int info[12]);
 
 /* Select the sky stage's form: 1 (default) the fused kernel where it is
@@ -245,9 +245,10 @@ void sphr_batch_info(const struct sphr_batch *, int info[12]);
  * sums, then the phi DFT as a GEMM).  Both give the same results to
  * rounding. */
 void sphr_batch_set_fused(struct sphr_batch *, int on);
-/* Fused sky kernel's persistent workgroups per CU: 1 (default) or 2 (the
+/* Fused sky kernel's persistent workgroups per CU: 1 (default), 2 (the
  * rotation rows then stay in global memory when both would not fit the
- * CU's LDS); same results; -1 for other values */
+ * CU's LDS) or 3 (one segment per workgroup); fewer where the LDS does
+ * not allow as many; same results; -1 for other values */
 int sphr_batch_set_sky_occupancy(struct sphr_batch *, int workgroups_per_cu);
 
 #ifdef __cplusplus

@@ -413,8 +413,8 @@ extern "C" void sphr_batch_set_fused(struct sphr_batch *bt, int on)
 
 extern "C" int sphr_batch_set_sky_occupancy(struct sphr_batch *bt, int workgroups_per_cu)
 {
-	if(!bt || workgroups_per_cu < 1 || workgroups_per_cu > 2) {
-		sphr_set_error("sphr_batch_set_sky_occupancy: 1 or 2 workgroups per CU");
+	if(!bt || workgroups_per_cu < 1 || workgroups_per_cu > 3) {
+		sphr_set_error("sphr_batch_set_sky_occupancy: 1, 2 or 3 workgroups per CU");
 		return -1;
 	}
 	bt->sky_occ = workgroups_per_cu;
@@ -526,16 +526,17 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	bt->nsplit = bt->nsplit_opt[bo];
 	bt->tpw = tune_tpw[bw];
 	bt->xms = tune_ms[bw];
-	if(bt->fused && sky_fused_order(bt->L) && map) {
-		bt->sky_occ = 2;
-		float ms = 0.f;
-		if(timed(&ms))
-			goto fail;
-		if(ms < best) {
-			best = ms;
-			bocc = 2;
+	if(bt->fused && sky_fused_order(bt->L) && map)
+		for(int occ = 2; occ <= 3; occ++) {
+			bt->sky_occ = occ;
+			float ms = 0.f;
+			if(timed(&ms))
+				goto fail;
+			if(ms < best) {
+				best = ms;
+				bocc = occ;
+			}
 		}
-	}
 	bt->sky_occ = bocc;
 	hipEventDestroy(e0);
 	hipEventDestroy(e1);

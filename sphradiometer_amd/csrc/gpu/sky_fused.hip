@@ -614,27 +614,37 @@ template <typename T> struct sf_sub {
 	enum { N = sizeof(T) == 4 ? 2 : 1 };
 };
 
-template <typename T, int L>
+template <typename T, int L, int NS>
 static int sky_fused_at(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int ncu, int occ, hipStream_t st)
 {
 	typedef sf_geom<L> G;
-	const int NS = sf_sub<T>::N;
 	/* static LDS of the kernel (F tiles, a_lm, tables), with slack */
 	const size_t stat = NS * (2 * (size_t) G::n * G::FS * sizeof(T) + (size_t) (G::ncoef + L + 1) * sizeof(cplx<T>))
 			    + (size_t) (G::nal + 2 + G::n) * sizeof(cplx<T>) + G::ncoef * sizeof(short) + (size_t) G::B * (L + 2) * sizeof(T) + 1024;
 	const size_t rbytes = (size_t) xg->nb * G::ncoef * sizeof(cplx<T>), pbytes = (size_t) NS * xg->CT * sizeof(T);
-	/* persistent workgroups, occ per CU (1 or 2: the second needs the
-	 * LDS of both within the CU's 160 KiB); the rotation rows go to LDS
+	/* persistent workgroups, occ per CU, as many as the LDS of all of
+	 * them allows within the CU's 160 KiB; the rotation rows go to LDS
 	 * when they fit beside the rest at that occupancy */
 	const size_t cu_lds = 160 * 1024;
-	if(occ > 1 && 2 * (stat + pbytes) > cu_lds)
-		occ = 1;
+	while(occ > 1 && occ * (stat + pbytes) > cu_lds)
+		occ--;
 	const int rlds = occ * (stat + rbytes + pbytes) <= cu_lds;
 	const size_t dyn = (rlds ? rbytes : 0) + pbytes;
 	const int nwg = (S + NS - 1) / NS;
 	const int grid = nwg < occ * ncu ? nwg : occ * ncu;
-	hipLaunchKernelGGL((k_sky_fused<T, L, sf_sub<T>::N>), dim3(grid), dim3(SF_NT * NS), dyn, st, partial, nsplit, S, *xg, R, rlds, lm, gmst, *tb, alm, map);
+	hipLaunchKernelGGL((k_sky_fused<T, L, NS>), dim3(grid), dim3(SF_NT * NS), dyn, st, partial, nsplit, S, *xg, R, rlds, lm, gmst, *tb, alm, map);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+/* occupancy 1 or 2: the default sub-groups per workgroup (float32 two
+ * segments sharing the tables, float64 one); 3 or more: one segment per
+ * workgroup and that many workgroups per CU, where the LDS allows */
+template <typename T, int L>
+static int sky_fused_form(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int ncu, int occ, hipStream_t st)
+{
+	if(occ >= 3 || sf_sub<T>::N == 1)
+		return sky_fused_at<T, L, 1>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
+	return sky_fused_at<T, L, sf_sub<T>::N>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
 }
 
 template <typename T>
@@ -652,9 +662,9 @@ int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geo
 	/* persistent workgroups, one per CU */
 	switch(g->L) {
 	case 25:
-		return sky_fused_at<T, 25>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
+		return sky_fused_form<T, 25>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
 	case 32:
-		return sky_fused_at<T, 32>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
+		return sky_fused_form<T, 32>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
 	default:
 		return -1;
 	}

@@ -533,10 +533,15 @@ static int td_power_dev(const double *a, const double *b, double *w, const doubl
 		/* the window itself, for the first row and column */
 		hipLaunchKernelGGL(k_td_wfill, dim3((nt + 255) / 256), dim3(256), 0, st, sidx, p->d_sv + 2 * nnz, nnz, nt, w);
 	}
-	/* K splits of about TD_TSPLIT terms: enough workgroups on a short
-	 * series, partial sums that stay small on a long one */
+	/* K splits of about TD_TSPLIT terms, enough workgroups on a short
+	 * series; on a long one at most eight workgroups per CU's worth of
+	 * splits, each then longer, so the partial sums (nz np^2 doubles)
+	 * stay bounded whatever the stream's length */
 	const int tiles = (np + TD_TILE - 1) / TD_TILE;
 	int nz = (nk + TD_TSPLIT - 1) / TD_TSPLIT;
+	const int nz_max = 8 * (sphr_rt_cus() > 0 ? sphr_rt_cus() : 256) / (tiles * tiles);
+	if(nz > nz_max)
+		nz = nz_max;
 	if(nz < 1)
 		nz = 1;
 	const int tpz = ((nk + nz - 1) / nz + TD_KC - 1) / TD_KC * TD_KC;

@@ -208,13 +208,15 @@ def test_batch_sky_stage_forms_agree(oracle, three_site, precision):
     gm = np.linspace(0.3, 1.7, 5)
     a1, m1, i1 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True)
     a2, m2, i2 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=False)
-    # two persistent workgroups per CU (rotation rows from global memory)
+    # two persistent workgroups per CU (rotation rows from global memory),
+    # and three of one segment each
     a3, m3, i3 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True, occ=2)
-    assert i1["fused"] == 1 and i2["fused"] == 0 and i3["sky_workgroups_per_cu"] == 2
+    a4, m4, i4 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True, occ=3)
+    assert i1["fused"] == 1 and i2["fused"] == 0 and i3["sky_workgroups_per_cu"] == 2 and i4["sky_workgroups_per_cu"] == 3
     onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
     for s in range(5):
         sky, mesh = onet.segment(ts[:, s], 32, gmst=float(gm[s]))
-        for alm, mp in ((a1, m1), (a2, m2), (a3, m3)):
+        for alm, mp in ((a1, m1), (a2, m2), (a3, m3), (a4, m4)):
             assert rel(alm[s], sky) < tol
             assert rel(mp[s], mesh) < (tol if prec == S.FP64 else 5e-6)
 

@@ -216,3 +216,27 @@ def test_gpu_td_stitching(oracle):
     td = oracle.PlanTD(POS[1], POS[0], DT)
     assert rel(p_f, td.integrate(a, b, wf)[0]) < TOL64
     S.lib().correlator_plan_td_free(plan)
+
+
+@pytest.mark.gpu
+def test_gpu_td_long_stream_bounded_splits():
+    """a stream long enough that the correlation's K splits hit their cap
+    (eight workgroups per CU's worth): the capped call over 245760 output
+    samples equals the sum of the two uncapped calls over its halves with
+    the same window (the TD power is linear in the window and local in
+    time, correlator.c:611-646); the different summation orders agree to
+    1e-11"""
+    from sphradiometer_amd import inject
+    print("td long stream", flush=True)
+    net = _Net(inject.positions("AB"))
+    plan = S.lib().correlator_plan_td_new(net.baseline(0), DT)
+    tr = plan.contents.transient
+    nt, s = 240 * 1024, 100000
+    rng = np.random.default_rng(17)
+    a, b = rng.standard_normal(nt + 2 * tr), rng.standard_normal(nt + 2 * tr)
+    w = rng.uniform(0.5, 1.0, nt) / nt
+    p_full = _integrate(plan, a, b, w)
+    p_1 = _integrate(plan, a[:s + 2 * tr], b[:s + 2 * tr], w[:s])
+    p_2 = _integrate(plan, a[s:], b[s:], w[s:])
+    assert rel(p_1 + p_2, p_full) < 1e-11, rel(p_1 + p_2, p_full)
+    S.lib().correlator_plan_td_free(plan)
