@@ -182,8 +182,9 @@ int sphr_inject_bursts(const float *params, int n_det, int n_segments, int n, do
 int sphr_mlp_gemm_check(int bf16, int ta, int tb, int M, int N, int K, const float *A, const float *B, float *C);
 /* test hook of the split-FP16 product: C [M][N] = A [M][K] B[N][K]^T
  * (float32 device arrays) through the fp16 pair split and three MFMA
- * products; synchronous */
-int sphr_gemm_h3_check(int M, int N, int K, const float *A, const float *B, float *C);
+ * products; a_f32: A split while staged (the form the localization meshes
+ * use) instead of split beforehand; synchronous */
+int sphr_gemm_h3_check(int a_f32, int M, int N, int K, const float *A, const float *B, float *C);
 /* test and timing hook of the real GEMM: C [M][N] = A [M][K] B [K][N]
  * (row-major device arrays, float64 when f64 else float32) on the
  * library's kernel (impl 0) or rocBLAS (impl 1), queued on stream */
@@ -224,12 +225,15 @@ void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map)
 
 /* Autotuner: runs the batch on these buffers with each candidate shape of
  * the projection (tiles per wave, segments per workgroup, K split), keeps
- * the fastest, then tries the fused sky kernel at two workgroups per CU;
+ * the fastest, then tries the fused sky kernel's other forms (two
+ * workgroups per CU; one segment per workgroup at three per CU; four
+ * segments per workgroup, mixed precision);
  * returns the projection shape's index (time per run in best_ms), or -1
  * with the previous settings kept */
 int sphr_batch_tune(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, double *best_ms);
 /* Select one of the autotuner's shapes by its index (0..27, as
- * sphr_batch_tune returns it); 0, or -1 for an index out of range */
+ * sphr_batch_tune returns it: 7 x K split + shape); 0, or -1 for an index
+ * out of range */
 int sphr_batch_set_shape(struct sphr_batch *, int index);
 
 /* Shape information: [0] detectors, [1] baselines, [2] N, [3] sky l_max,
@@ -237,8 +241,9 @@ int sphr_batch_set_shape(struct sphr_batch *, int index);
  * [8] 1 when the sky stage runs as the one fused kernel (orders it is
  * compiled for), 0 for the two-kernel form, [9] projection tiles per wave,
  * [10] projection segments per workgroup, [11] fused sky workgroups per
- * CU requested (1-3, picked by the autotuner) */
-void sphr_batch_info(const struct sphr_batch *, int info[12]);
+ * CU requested (1-3, picked by the autotuner), [12] fused sky segments per
+ * workgroup requested (0 for the precision's default, 1, 2 or 4) */
+void sphr_batch_info(const struct sphr_batch *, int info[13]);
 
 /* Select the sky stage's form: 1 (default) the fused kernel where it is
  * compiled for the batch's order, 0 the two-kernel form (a_lm and ring
@@ -250,6 +255,12 @@ void sphr_batch_set_fused(struct sphr_batch *, int on);
  * CU's LDS) or 3 (one segment per workgroup); fewer where the LDS does
  * not allow as many; same results; -1 for other values */
 int sphr_batch_set_sky_occupancy(struct sphr_batch *, int workgroups_per_cu);
+/* Fused sky kernel's segments per workgroup: 0 (default: two in mixed
+ * precision, one in FP64, one at three workgroups per CU), 1, 2, or 4
+ * (mixed precision: the four share one LDS copy of the tables and the
+ * rotation rows; one workgroup per CU); a form whose LDS does not fit runs
+ * the unfused sky stage; same results; -1 for other values */
+int sphr_batch_set_sky_segments(struct sphr_batch *, int segments_per_workgroup);
 
 #ifdef __cplusplus
 }

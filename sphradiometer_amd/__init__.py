@@ -193,7 +193,7 @@ _PROTOS = {
     "correlator_network_integrate_power_td": (P(ShSeries), [P(ShSeries), P(c_void_p), c_int, P(c_void_p), P(NetworkPlanTD)]),
     "correlator_network_integrate_power_fd": (P(ShSeries), [P(ShSeries), P(c_void_p), P(NetworkPlanFD)]),
     "correlator_plan_fd_device_invalidate": (None, [P(PlanFD)]),
-    "sphr_gemm_h3_check": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "sphr_gemm_h3_check": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "sphr_gemm_run": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, ctypes.c_long, c_void_p, ctypes.c_long, c_void_p, ctypes.c_long, c_void_p]),
     "correlator_tseries_to_fseries_plan": (c_void_p, [c_void_p, c_void_p, c_int]),
     "correlator_tseries_to_fseries": (None, [c_void_p, c_void_p, c_int, c_void_p]),
@@ -229,6 +229,7 @@ _PROTOS = {
     "sphr_batch_info": (None, [c_void_p, P(c_int)]),
     "sphr_batch_set_fused": (None, [c_void_p, c_int]),
     "sphr_batch_set_sky_occupancy": (c_int, [c_void_p, c_int]),
+    "sphr_batch_set_sky_segments": (c_int, [c_void_p, c_int]),
     "sphr_batch_set_shape": (c_int, [c_void_p, c_int]),
     "sphr_batch_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sphr_batch_tune": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P(c_double)]),
@@ -535,10 +536,10 @@ class Batch:
         self.network = network
 
     def info(self):
-        a = (c_int * 12)()
+        a = (c_int * 13)()
         lib().sphr_batch_info(self.h, a)
         keys = ["detectors", "baselines", "N", "sky_l", "columns", "k_splits", "precision", "k_per_split", "fused",
-                "tiles_per_wave", "segments_per_workgroup", "sky_workgroups_per_cu"]
+                "tiles_per_wave", "segments_per_workgroup", "sky_workgroups_per_cu", "sky_segments_per_workgroup"]
         return dict(zip(keys, list(a)))
 
     def set_shape(self, index):
@@ -554,6 +555,10 @@ class Batch:
     def set_sky_occupancy(self, workgroups_per_cu):
         if lib().sphr_batch_set_sky_occupancy(self.h, int(workgroups_per_cu)):
             raise RuntimeError("sphr_batch_set_sky_occupancy failed: " + last_error())
+
+    def set_sky_segments(self, segments_per_workgroup):
+        if lib().sphr_batch_set_sky_segments(self.h, int(segments_per_workgroup)):
+            raise RuntimeError("sphr_batch_set_sky_segments failed: " + last_error())
 
     def run(self, d_tseries, n_segments, d_alm, d_map=None, d_window=None, d_gmst=None, f32_input=False, stream=None):
         r = lib().sphr_batch_run(self.h, d_tseries, int(f32_input), n_segments, d_window, d_gmst, d_alm, d_map, stream)

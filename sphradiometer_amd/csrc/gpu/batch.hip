@@ -35,7 +35,8 @@ struct sphr_batch {
 	int tpw;			/* projection tiles per wave (autotuned) */
 	int xms;			/* projection segments per workgroup, 16 or 32 (autotuned) */
 	int fused;			/* use the one-kernel sky stage where compiled (default 1) */
-	int sky_occ;			/* fused sky workgroups per CU, 1 or 2 (autotuned) */
+	int sky_occ;			/* fused sky workgroups per CU, 1 to 3 (autotuned) */
+	int sky_nsub;			/* fused sky segments per workgroup, 0 (default), 1, 2 or 4 (autotuned) */
 	int ksplit_opt[4], nsplit_opt[4];	/* K-split candidates: 1, 2, 4 x CUs workgroups, 64 per split */
 	struct xproj_geom g;
 	struct synth_dev syn;
@@ -204,6 +205,7 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 		bt->xms = 64;
 		bt->fused = 1;
 		bt->sky_occ = 1;
+		bt->sky_nsub = 0;
 	}
 	int *colbl = (int *) malloc((CT / 16) * sizeof(int));
 	const int ncoef = (bt->L + 1) * (bt->L + 1);
@@ -260,7 +262,7 @@ static int run_t(struct sphr_batch *bt, void *spec, void *partial, void *F, cons
 	/* the fused sky kernel at the orders it is compiled for; else a_lm
 	 * and ring sums, then the phi DFT as a GEMM; the one-kernel form when
 	 * the coefficients do not fit the LDS budget */
-	if(bt->fused && launch_sky_fused<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st, bt->sky_occ) == 0) {
+	if(bt->fused && launch_sky_fused<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st, bt->sky_occ, bt->sky_nsub) == 0) {
 		if(ev)
 			hipEventRecord(ev[3], st);
 	} else if((F || !map) && launch_sky_lf<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) F : (cplx<T> *) NULL, st) == 0) {
@@ -421,8 +423,20 @@ extern "C" int sphr_batch_set_sky_occupancy(struct sphr_batch *bt, int workgroup
 	return 0;
 }
 
-extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[12])
+extern "C" int sphr_batch_set_sky_segments(struct sphr_batch *bt, int segments_per_workgroup)
 {
+	const int n = segments_per_workgroup;
+	if(!bt || !(n == 0 || n == 1 || n == 2 || n == 4) || (n == 4 && bt->precision == SPHR_FP64)) {
+		sphr_set_error("sphr_batch_set_sky_segments: 0, 1, 2 or (mixed precision) 4 segments per workgroup");
+		return -1;
+	}
+	bt->sky_nsub = n;
+	return 0;
+}
+
+extern "C" void sphr_batch_info(const struct sphr_batch *bt, int info[13])
+{
+	info[12] = bt->sky_nsub;
 	info[11] = bt->sky_occ;
 	info[8] = bt->fused && sky_fused_order(bt->L);
 	info[9] = bt->tpw;
@@ -464,8 +478,17 @@ extern "C" int sphr_batch_peaks(struct sphr_batch *bt, const void *alm, const vo
 /* the projection shapes the autotuner tries: one baseline per workgroup
  * (tpw 0), then tiles per wave x segments per workgroup; with the 4 K
  * splits, shape index = 7 x split + shape */
-static const int tune_tpw[7] = { 0, 1, 2, 4, 1, 2, 4 }, tune_ms[7] = { 64, 16, 16, 16, 32, 32, 32 };
-#define SPHR_BATCH_SHAPES 28
+#define SPHR_TUNE_W 7
+static const int tune_tpw[SPHR_TUNE_W] = { 0, 1, 2, 4, 1, 2, 4 }, tune_ms[SPHR_TUNE_W] = { 64, 16, 16, 16, 32, 32, 32 };
+#define SPHR_BATCH_SHAPES (4 * SPHR_TUNE_W)
+
+static void set_shape(struct sphr_batch *bt, int o, int w)
+{
+	bt->k_per_split = bt->ksplit_opt[o];
+	bt->nsplit = bt->nsplit_opt[o];
+	bt->tpw = tune_tpw[w];
+	bt->xms = tune_ms[w];
+}
 
 extern "C" int sphr_batch_set_shape(struct sphr_batch *bt, int index)
 {
@@ -473,10 +496,7 @@ extern "C" int sphr_batch_set_shape(struct sphr_batch *bt, int index)
 		sphr_set_error("sphr_batch_set_shape: index %d outside 0..%d", index, SPHR_BATCH_SHAPES - 1);
 		return -1;
 	}
-	bt->k_per_split = bt->ksplit_opt[index / 7];
-	bt->nsplit = bt->nsplit_opt[index / 7];
-	bt->tpw = tune_tpw[index % 7];
-	bt->xms = tune_ms[index % 7];
+	set_shape(bt, index / SPHR_TUNE_W, index % SPHR_TUNE_W);
 	return 0;
 }
 
@@ -487,11 +507,11 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 		return -1;
 	}
 	hipStream_t st = sphr_rt_stream();
-	const int kps0 = bt->k_per_split, ns0 = bt->nsplit, tpw0 = bt->tpw, xms0 = bt->xms, occ0 = bt->sky_occ;
+	const int kps0 = bt->k_per_split, ns0 = bt->nsplit, tpw0 = bt->tpw, xms0 = bt->xms, occ0 = bt->sky_occ, nsub0 = bt->sky_nsub;
 	const int prof = bt->profile;
 	hipEvent_t e0 = NULL, e1 = NULL;
 	float best = 1e30f;
-	int bo = -1, bw = 0, bocc = 1;
+	int bo = -1, bw = 0, bocc = 1, bnsub = 0;
 	bt->profile = 0;
 	/* time one candidate: a run, then three timed runs; -1 on failure */
 	auto timed = [&](float *ms) -> int {
@@ -504,15 +524,13 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	};
 	if(sphr_rt_check(hipEventCreate(&e0), "event") || sphr_rt_check(hipEventCreate(&e1), "event"))
 		goto fail;
-	/* the projection's shapes and K splits with the fused sky kernel at
-	 * one workgroup per CU, then two workgroups per CU at the best */
+	/* the projection's shapes and K splits with the fused sky kernel in
+	 * its default form, then the sky kernel's other forms at the best */
 	bt->sky_occ = 1;
+	bt->sky_nsub = 0;
 	for(int o = 0; o < 4; o++)
-		for(int w = 0; w < 7; w++) {
-			bt->k_per_split = bt->ksplit_opt[o];
-			bt->nsplit = bt->nsplit_opt[o];
-			bt->tpw = tune_tpw[w];
-			bt->xms = tune_ms[w];
+		for(int w = 0; w < SPHR_TUNE_W; w++) {
+			set_shape(bt, o, w);
 			float ms = 0.f;
 			if(timed(&ms))
 				goto fail;
@@ -522,28 +540,33 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 				bw = w;
 			}
 		}
-	bt->k_per_split = bt->ksplit_opt[bo];
-	bt->nsplit = bt->nsplit_opt[bo];
-	bt->tpw = tune_tpw[bw];
-	bt->xms = tune_ms[bw];
-	if(bt->fused && sky_fused_order(bt->L) && map)
-		for(int occ = 2; occ <= 3; occ++) {
-			bt->sky_occ = occ;
+	set_shape(bt, bo, bw);
+	if(bt->fused && sky_fused_order(bt->L) && map) {
+		/* (workgroups per CU, segments per workgroup) */
+		static const int forms[3][2] = { { 2, 0 }, { 3, 1 }, { 1, 4 } };
+		for(int f = 0; f < 3; f++) {
+			if(forms[f][1] == 4 && bt->precision == SPHR_FP64)
+				continue;
+			bt->sky_occ = forms[f][0];
+			bt->sky_nsub = forms[f][1];
 			float ms = 0.f;
 			if(timed(&ms))
 				goto fail;
 			if(ms < best) {
 				best = ms;
-				bocc = occ;
+				bocc = forms[f][0];
+				bnsub = forms[f][1];
 			}
 		}
+	}
 	bt->sky_occ = bocc;
+	bt->sky_nsub = bnsub;
 	hipEventDestroy(e0);
 	hipEventDestroy(e1);
 	bt->profile = prof;
 	if(best_ms)
 		*best_ms = best / 3.0;
-	return 7 * bo + bw;
+	return SPHR_TUNE_W * bo + bw;
 fail:
 	if(e0)
 		hipEventDestroy(e0);
@@ -554,6 +577,7 @@ fail:
 	bt->tpw = tpw0;
 	bt->xms = xms0;
 	bt->sky_occ = occ0;
+	bt->sky_nsub = nsub0;
 	bt->profile = prof;
 	return -1;
 }

@@ -13,7 +13,10 @@
  * down to 2^-17 of it.  Accumulation is float32 in the MFMA.
  *
  * Operands are K-contiguous: A [M][lda], B [N][ldb] (B transposed), K a
- * multiple of 64 (producers zero-pad), lda and ldb multiples of 8.
+ * multiple of 64 (producers zero-pad), lda and ldb multiples of 8.  A
+ * second form reads A as float32 [M][lda] (lda even, K columns, no
+ * padding) with the inverse row scales given, and splits it while staging
+ * it into LDS: no split copy of A goes through HBM.
  *
  * Tiling: 256 threads = 4 waves in 2 x 2, block tile 128 x 128, wave tile
  * 64 x 64 = 4 x 4 MFMA tiles; K chunks of 64 staged global -> registers
@@ -40,8 +43,12 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 #define H3_LD 72	/* LDS row stride in halves */
 #define H3_NT 256
 
-template <typename TO>
-__global__ void __launch_bounds__(H3_NT) k_gemm_h3(int M, int N, int K, const _Float16 *__restrict__ Ah, const _Float16 *__restrict__ Al, long lda, const float *__restrict__ ira,
+/* AF32: A is float32 (Af, K columns, split while staged, inverse row
+ * scales ira); else the fp16 pair (Ah, Al).  NFAST: the tiles of one row
+ * panel run next to each other on one XCD (A read once from HBM when N
+ * spans few tiles); else those of one column panel (B reused) */
+template <typename TO, bool AF32, bool NFAST>
+__global__ void __launch_bounds__(H3_NT) k_gemm_h3(int M, int N, int K, const _Float16 *__restrict__ Ah, const _Float16 *__restrict__ Al, const float *__restrict__ Af, long lda, const float *__restrict__ ira,
 						  const _Float16 *__restrict__ Bh, const _Float16 *__restrict__ Bl, long ldb, const float *__restrict__ icb, TO *__restrict__ C, long ldc, int tiles_m, int tiles_n)
 {
 	__shared__ __align__(16) _Float16 sA[2][H3_BM * H3_LD];	/* [hi/lo][row][k] */
@@ -51,21 +58,35 @@ __global__ void __launch_bounds__(H3_NT) k_gemm_h3(int M, int N, int K, const _F
 	int t = id;
 	if(nwg % 8 == 0)
 		t = (id % 8) * (nwg / 8) + id / 8;
-	const int tm = t % tiles_m, tn = t / tiles_m;
+	const int tm = NFAST ? t / tiles_n : t % tiles_m, tn = NFAST ? t % tiles_n : t / tiles_m;
 	const int m0 = tm * H3_BM, n0 = tn * H3_BN;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
 	const int li = lane & 15, lq = lane >> 4;
 	/* global -> register staging: each thread moves 4 rows x 16 bytes of
-	 * each of the four operand tiles per chunk (128 rows x 8 vectors) */
+	 * each of the four operand tiles per chunk (128 rows x 8 vectors); a
+	 * float32 A as 4 rows x 8 values, split when stored */
 	h8 ra[2][4], rb[2][4];
+	float2 rf[AF32 ? 4 : 1][4];
+	float rs[AF32 ? 4 : 1];
 	const int vr = threadIdx.x >> 3, vc = (threadIdx.x & 7) * 8;	/* row 0..31 (+32 per step), k offset */
+	if constexpr(AF32)
+#pragma unroll
+		for(int s = 0; s < 4; s++) {
+			const int gm = m0 + vr + 32 * s;
+			rs[s] = gm < M ? 1.0f / ira[gm] : 0.f;
+		}
 	auto load = [&](int k0) {
 #pragma unroll
 		for(int s = 0; s < 4; s++) {
 			const int r = vr + 32 * s;
 			const int gm = m0 + r, gn = n0 + r;
-			if(gm < M) {
+			if constexpr(AF32) {
+				const float *a = Af + (size_t) gm * lda + k0 + vc;
+#pragma unroll
+				for(int u = 0; u < 4; u++)
+					rf[s][u] = gm < M && k0 + vc + 2 * u < K ? *reinterpret_cast<const float2 *>(a + 2 * u) : make_float2(0.f, 0.f);
+			} else if(gm < M) {
 				ra[0][s] = *reinterpret_cast<const h8 *>(Ah + (size_t) gm * lda + k0 + vc);
 				ra[1][s] = *reinterpret_cast<const h8 *>(Al + (size_t) gm * lda + k0 + vc);
 			} else {
@@ -82,6 +103,18 @@ __global__ void __launch_bounds__(H3_NT) k_gemm_h3(int M, int N, int K, const _F
 		}
 	};
 	auto store = [&]() {
+		if constexpr(AF32)
+#pragma unroll
+			for(int s = 0; s < 4; s++)
+#pragma unroll
+				for(int u = 0; u < 4; u++) {
+					const float v0 = rf[s][u].x * rs[s], v1 = rf[s][u].y * rs[s];
+					const _Float16 h0 = (_Float16) v0, h1 = (_Float16) v1;
+					ra[0][s][2 * u] = h0;
+					ra[0][s][2 * u + 1] = h1;
+					ra[1][s][2 * u] = (_Float16) (v0 - (float) h0);
+					ra[1][s][2 * u + 1] = (_Float16) (v1 - (float) h1);
+				}
 #pragma unroll
 		for(int p = 0; p < 2; p++)
 #pragma unroll
@@ -194,7 +227,22 @@ int launch_gemm_h3(int M, int N, int K, const _Float16 *Ah, const _Float16 *Al, 
 	if(K % H3_BK || lda % 8 || ldb % 8 || lda < K || ldb < K)
 		return -1;
 	const int tm = (M + H3_BM - 1) / H3_BM, tn = (N + H3_BN - 1) / H3_BN;
-	hipLaunchKernelGGL((k_gemm_h3<TO>), dim3(tm * tn), dim3(H3_NT), 0, st, M, N, K, Ah, Al, lda, ira, Bh, Bl, ldb, icb, C, ldc, tm, tn);
+	hipLaunchKernelGGL((k_gemm_h3<TO, false, false>), dim3(tm * tn), dim3(H3_NT), 0, st, M, N, K, Ah, Al, (const float *) NULL, lda, ira, Bh, Bl, ldb, icb, C, ldc, tm, tn);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename TO>
+int launch_gemm_h3a(int M, int N, int K, const float *A, long lda, const float *ira, const _Float16 *Bh, const _Float16 *Bl, long ldb, const float *icb, TO *C, long ldc, hipStream_t st)
+{
+	if(M <= 0 || N <= 0 || K <= 0)
+		return 0;
+	/* the chunks run to K rounded up to 64, where B's padding is zero; a
+	 * float2 of A never straddles K */
+	const int Kp = (K + H3_BK - 1) / H3_BK * H3_BK;
+	if(K % 2 || lda % 2 || ldb % 8 || lda < K || ldb < Kp)
+		return -1;
+	const int tm = (M + H3_BM - 1) / H3_BM, tn = (N + H3_BN - 1) / H3_BN;
+	hipLaunchKernelGGL((k_gemm_h3<TO, true, true>), dim3(tm * tn), dim3(H3_NT), 0, st, M, N, K, (const _Float16 *) NULL, (const _Float16 *) NULL, A, lda, ira, Bh, Bl, ldb, icb, C, ldc, tm, tn);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -211,29 +259,41 @@ int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *
 
 template int launch_gemm_h3<float>(int, int, int, const _Float16 *, const _Float16 *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, float *, long, hipStream_t);
 template int launch_gemm_h3<double>(int, int, int, const _Float16 *, const _Float16 *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, double *, long, hipStream_t);
+template int launch_gemm_h3a<float>(int, int, int, const float *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, float *, long, hipStream_t);
 template int launch_split_rows<float>(int, int, int, const float *, long, _Float16 *, _Float16 *, long, float *, hipStream_t);
 template int launch_split_rows<double>(int, int, int, const double *, long, _Float16 *, _Float16 *, long, float *, hipStream_t);
 
 /*
  * Test hook: C [M][N] (float32) = A [M][K] B^T with A and B [N][K] given
  * as float32 device arrays, through the split-FP16 path (rows of A and of
- * B split with their own scales, K zero-padded to a multiple of 64).
- * Allocates its split buffers; synchronous.
+ * B split with their own scales, K zero-padded to a multiple of 64); with
+ * a_f32, A goes in as float32 (copied to an even row length) and is split
+ * while staged, its row scales from the row split.  Allocates its
+ * buffers; synchronous.
  */
-extern "C" int sphr_gemm_h3_check(int M, int N, int K, const float *A, const float *B, float *C)
+extern "C" int sphr_gemm_h3_check(int a_f32, int M, int N, int K, const float *A, const float *B, float *C)
 {
 	if(M <= 0 || N <= 0 || K <= 0 || !A || !B || !C)
 		return -1;
-	const int Kp = (K + H3_BK - 1) / H3_BK * H3_BK;
+	const int Kp = (K + H3_BK - 1) / H3_BK * H3_BK, Ke = (K + 1) / 2 * 2;
 	_Float16 *ah = NULL, *al = NULL, *bh = NULL, *bl = NULL;
-	float *ra = NULL, *cb = NULL;
+	float *ra = NULL, *cb = NULL, *ae = NULL;
 	int rc = -1;
 	if(hipMalloc((void **) &ah, (size_t) M * Kp * 2) != hipSuccess || hipMalloc((void **) &al, (size_t) M * Kp * 2) != hipSuccess
 	   || hipMalloc((void **) &bh, (size_t) N * Kp * 2) != hipSuccess || hipMalloc((void **) &bl, (size_t) N * Kp * 2) != hipSuccess
-	   || hipMalloc((void **) &ra, (size_t) M * 4) != hipSuccess || hipMalloc((void **) &cb, (size_t) N * 4) != hipSuccess)
+	   || hipMalloc((void **) &ra, (size_t) M * 4) != hipSuccess || hipMalloc((void **) &cb, (size_t) N * 4) != hipSuccess
+	   || hipMalloc((void **) &ae, (size_t) M * Ke * 4) != hipSuccess)
 		goto done;
-	if(launch_split_rows<float>(M, K, Kp, A, K, ah, al, Kp, ra, 0) || launch_split_rows<float>(N, K, Kp, B, K, bh, bl, Kp, cb, 0)
-	   || launch_gemm_h3<float>(M, N, Kp, ah, al, Kp, ra, bh, bl, Kp, cb, C, N, 0) || hipDeviceSynchronize() != hipSuccess)
+	if(launch_split_rows<float>(M, K, Kp, A, K, ah, al, Kp, ra, 0) || launch_split_rows<float>(N, K, Kp, B, K, bh, bl, Kp, cb, 0))
+		goto done;
+	if(a_f32) {
+		if(hipMemset(ae, 0, (size_t) M * Ke * 4) != hipSuccess
+		   || hipMemcpy2D(ae, (size_t) Ke * 4, A, (size_t) K * 4, (size_t) K * 4, M, hipMemcpyDeviceToDevice) != hipSuccess
+		   || launch_gemm_h3a<float>(M, N, Ke, ae, Ke, ra, bh, bl, Kp, cb, C, N, 0))
+			goto done;
+	} else if(launch_gemm_h3<float>(M, N, Kp, ah, al, Kp, ra, bh, bl, Kp, cb, C, N, 0))
+		goto done;
+	if(hipDeviceSynchronize() != hipSuccess)
 		goto done;
 	rc = 0;
 done:
@@ -243,5 +303,6 @@ done:
 	hipFree(bl);
 	hipFree(ra);
 	hipFree(cb);
+	hipFree(ae);
 	return rc;
 }

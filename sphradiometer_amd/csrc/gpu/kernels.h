@@ -9,6 +9,8 @@
 
 #define SPHR_MAX_BASELINES 16
 #define SPHR_MAX_FACTORS 16
+/* rings per hemisphere of the ring-sum kernel's row scales (l_max < 128) */
+#define SPHR_MAX_MESH_B 128
 
 /* per-plan constants of the cross-power projection (real time series);
  * CT counts the projection columns, each baseline padded to whole
@@ -91,10 +93,17 @@ int sky_fused_order(int L);	/* 1 when launch_sky_fused is compiled for order L *
  * tables tb (x, smm, alpha, twp set); all NULL at orders it is not
  * compiled for; 0, or -1 with the error set */
 int sky_fused_tables(int L, const struct synth_tables *tb, struct sky_fused_dev *d);
+/* occ persistent workgroups per CU, nsub segments per workgroup (0: the
+ * precision's default); -1 for a form whose LDS does not fit */
 template <typename T>
-int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st, int occ = 1);
+int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st, int occ = 1, int nsub = 0);
 template <typename T>
 int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *map, hipStream_t st);
+/* the Legendre ring sums F [S][2(L+1)][2L+1] of S series (the first half
+ * of launch_synth_gemm); with irs [S][2(L+1)], each ring's inverse split
+ * scale for launch_gemm_h3a */
+template <typename T>
+int launch_ring_sums(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, hipStream_t st, float *irs);
 /* GEMM-form synthesis of S packed series (order L, non-polar) through the
  * ring-sum scratch F; -1 when a series exceeds the LDS budget */
 template <typename T>
@@ -123,6 +132,12 @@ int launch_gemm_blas(int M, int N, int K, const T *A, long lda, const T *B, long
  * diag(icb), operands K-contiguous fp16, K % 64 == 0 */
 template <typename TO>
 int launch_gemm_h3(int M, int N, int K, const _Float16 *Ah, const _Float16 *Al, long lda, const float *ira, const _Float16 *Bh, const _Float16 *Bl, long ldb, const float *icb, TO *C, long ldc, hipStream_t st);
+/* the same product with A float32 [M][lda] (K columns, K and lda even),
+ * split while staged, inverse row scales ira given (powers of two that
+ * put each row's largest element at 2^13..2^14); B [N][ldb] split with K
+ * zero-padded to a multiple of 64 */
+template <typename TO>
+int launch_gemm_h3a(int M, int N, int K, const float *A, long lda, const float *ira, const _Float16 *Bh, const _Float16 *Bl, long ldb, const float *icb, TO *C, long ldc, hipStream_t st);
 /* rows of X [rows][K] -> fp16 pairs scaled per row (columns K..Kp-1 zero) */
 template <typename TI>
 int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *Xh, _Float16 *Xl, long ldh, float *irs, hipStream_t st);

@@ -920,6 +920,124 @@ __global__ void __launch_bounds__(NT) k_sky(const T *__restrict__ partial, int n
 }
 
 /*
+ * Legendre ring sums of one series into F [n rings][2L+1] (ring rows in
+ * map order, columns m = 0..L, -L..-1), a workgroup of NT threads:
+ *   F[r][m] = sum_l a_lm Pbar_l|m|(x_r), times (-1)^m for odd m < 0,
+ * for the ring pairs (x_j, -x_j) at once through the parity (-1)^(l+|m|).
+ * Item (group of RG rings, a) runs the recurrence over l for |m| = a and
+ * for |m| = L - a (L + 2 steps for every item), each step serving both
+ * signs of m and all RG rings.  The recurrence
+ *   Pbar_l = alpha_l x Pbar_{l-1} - (alpha_l / alpha_{l-1}) Pbar_{l-2}
+ * reads the pairs (alpha_l, alpha_l / alpha_{l-1}) from cf (LDS, the
+ * (l, m >= 0) slots of the coefficient layout, two zero pads).  With
+ * rmax, each ring's largest real or imaginary part (as float bits) is
+ * folded in by atomicMax.
+ */
+#define RS_RG 3
+template <typename T>
+__device__ inline void ring_sums(const cplx<T> *sa, const cplx<T> *cf, int L, int B, const double *__restrict__ xg, const double *__restrict__ smm, cplx<T> *__restrict__ F, unsigned *rmax)
+{
+	const int nm = 2 * L + 1, NA = L / 2 + 1, NG = (B + RS_RG - 1) / RS_RG;
+	for(int it = threadIdx.x; it < NG * NA; it += NT) {
+		const int tg = it % NG, a = it / NG;
+		T x[RS_RG];
+#pragma unroll
+		for(int r = 0; r < RS_RG; r++) {
+			const int j = RS_RG * tg + r;
+			x[r] = j < B ? (T) xg[j] : (T) 0;
+		}
+		for(int h = 0; h < 2; h++) {
+			const int am = h ? L - a : a;
+			if(h && am == a)
+				break;
+			const int bp = lm_offset(L, am, am), bn = am ? lm_offset(L, am, -am) : bp;
+			const cplx<T> *sp = sa + bp, *sn = sa + bn, *cc = cf + bp;
+			const int cnt = L - am;
+			T p2[RS_RG], p1[RS_RG];
+			cplx<T> Ep[RS_RG], Op[RS_RG], En[RS_RG], On[RS_RG];
+			{
+				const cplx<T> ap0 = sp[0], an0 = sn[0], ap1 = sp[1], an1 = sn[1];
+				const T c1 = cc[1].re;
+#pragma unroll
+				for(int r = 0; r < RS_RG; r++) {
+					const int j = RS_RG * tg + r;
+					p2[r] = j < B ? (T) smm[(size_t) j * (L + 1) + am] : (T) 0;
+					Ep[r] = ap0 * p2[r];
+					En[r] = an0 * p2[r];
+					/* l = am + 1 (unused past the end: cnt = 0) */
+					p1[r] = cnt >= 1 ? c1 * x[r] * p2[r] : (T) 0;
+					Op[r] = cnt >= 1 ? ap1 * p1[r] : mk((T) 0, (T) 0);
+					On[r] = cnt >= 1 ? an1 * p1[r] : mk((T) 0, (T) 0);
+				}
+			}
+			/* two steps (even, odd l - am) per iteration, then one */
+			int i = 2;
+			for(; i + 1 <= cnt; i += 2) {
+				const cplx<T> c0 = cc[i], c1 = cc[i + 1];
+				const cplx<T> ap0 = sp[i], ap1 = sp[i + 1], an0 = sn[i], an1 = sn[i + 1];
+#pragma unroll
+				for(int r = 0; r < RS_RG; r++) {
+					const T pe = c0.re * x[r] * p1[r] - c0.im * p2[r];
+					const T po = c1.re * x[r] * pe - c1.im * p1[r];
+					cfma(Ep[r], ap0, pe);
+					cfma(En[r], an0, pe);
+					cfma(Op[r], ap1, po);
+					cfma(On[r], an1, po);
+					p2[r] = pe;
+					p1[r] = po;
+				}
+			}
+			if(i == cnt) {
+				const cplx<T> c0 = cc[i], ap0 = sp[i], an0 = sn[i];
+#pragma unroll
+				for(int r = 0; r < RS_RG; r++) {
+					const T pe = c0.re * x[r] * p1[r] - c0.im * p2[r];
+					cfma(Ep[r], ap0, pe);
+					cfma(En[r], an0, pe);
+				}
+			}
+			const T sg = (am & 1) ? (T) -1 : (T) 1;
+			/* ring row, column, value */
+			auto put = [&](int row, int col, cplx<T> v) {
+				F[row * nm + col] = v;
+				if(rmax)
+					atomicMax(&rmax[row], __float_as_uint((float) fmax(fabs(v.re), fabs(v.im))));
+			};
+#pragma unroll
+			for(int r = 0; r < RS_RG; r++) {
+				const int j = RS_RG * tg + r;
+				if(j >= B)
+					continue;
+				put(j, am, Ep[r] + Op[r]);
+				put(2 * B - 1 - j, am, Ep[r] - Op[r]);
+				if(am) {
+					put(j, nm - am, (En[r] + On[r]) * sg);
+					put(2 * B - 1 - j, nm - am, (En[r] - On[r]) * sg);
+				}
+			}
+		}
+	}
+}
+
+/* the recurrence pairs (alpha_l, alpha_l / alpha_{l-1}) of ring_sums at
+ * the (l, m >= 0) slots, two zero pads */
+template <typename T>
+__device__ inline void ring_coefs(int L, const double *__restrict__ alpha, cplx<T> *cf)
+{
+	const int nal = (L + 1) * (L + 2) / 2;
+	for(int t = threadIdx.x; t < nal + 2; t += NT) {
+		cplx<T> c = mk((T) 0, (T) 0);
+		if(t < nal) {
+			int l, m;
+			lm_decode(L, t, &l, &m);
+			const double al = l > m ? alpha[t] : 0.0;
+			c = mk((T) al, (T) (l > m + 1 ? al / alpha[t - 1] : 0.0));
+		}
+		cf[t] = c;
+	}
+}
+
+/*
  * Sky stage in two kernels (batched path).  k_sky_lf: one workgroup per
  * segment sums the K-split partials, forms a_lm (rotation rows, baseline
  * average, rotate_z) in LDS, writes a_lm, and runs the Legendre stage for
@@ -934,8 +1052,8 @@ __global__ void __launch_bounds__(NT) k_sky_lf(const T *__restrict__ partial, in
 	const int L = g.L, B = g.B, n = g.n, nm = 2 * L + 1;
 	const int ncoef = (L + 1) * (L + 1), nal = (L + 1) * (L + 2) / 2;
 	cplx<T> *sa = reinterpret_cast<cplx<T> *>(smem_raw);	/* [ncoef] */
-	T *al = reinterpret_cast<T *>(sa + ncoef);		/* [nal] */
-	T *pw = al + nal;					/* [CT] */
+	cplx<T> *cf = sa + ncoef;				/* [nal + 2] */
+	T *pw = reinterpret_cast<T *>(cf + nal + 2);		/* [CT] */
 	const int s = blockIdx.x;
 	for(int c = threadIdx.x; c < xg.CT; c += NT) {
 		T v = 0;
@@ -943,8 +1061,7 @@ __global__ void __launch_bounds__(NT) k_sky_lf(const T *__restrict__ partial, in
 			v += partial[((size_t) sp * S + s) * xg.CT + c];
 		pw[c] = v;
 	}
-	for(int t = threadIdx.x; t < nal; t += NT)
-		al[t] = (T) tb.alpha[t];
+	ring_coefs<T>(L, tb.alpha, cf);
 	__syncthreads();
 	double gs = 0.0;
 	if(gmst)
@@ -970,79 +1087,43 @@ __global__ void __launch_bounds__(NT) k_sky_lf(const T *__restrict__ partial, in
 	if(!Fg)
 		return;
 	__syncthreads();
-	cplx<T> *F = Fg + (size_t) s * n * nm;
-	for(int it = threadIdx.x; it < B * nm; it += NT) {
-		const int j = it / nm, mi = it - j * nm;
-		const int m = mi <= L ? mi : mi - nm;
-		const int am = m < 0 ? -m : m;
-		const T x = (T) tb.x[j];
-		const int abase = lm_offset(L, am, m);
-		const int albase = lm_offset(L, am, am);
-		T pprev = 0, pcur = (T) tb.smm[(size_t) j * (L + 1) + am], aprev = 1;
-		cplx<T> E = mk((T) 0, (T) 0), O = mk((T) 0, (T) 0);
-		for(int l = am; l <= L; l++) {
-			if(l > am) {
-				const T a_l = al[albase + l - am];
-				const T pn = (l == am + 1) ? a_l * x * pcur : a_l * (x * pcur - pprev / aprev);
-				pprev = pcur;
-				pcur = pn;
-				aprev = a_l;
-			}
-			const cplx<T> c = sa[abase + l - am];
-			if((l + am) & 1)
-				cfma(O, c, pcur);
-			else
-				cfma(E, c, pcur);
-		}
-		const T sg = (m < 0 && (am & 1)) ? (T) -1 : (T) 1;
-		F[(size_t) j * nm + mi] = (E + O) * sg;
-		F[(size_t) (2 * B - 1 - j) * nm + mi] = (E - O) * sg;
-	}
+	ring_sums<T>(sa, cf, L, B, tb.x, tb.smm, Fg + (size_t) s * n * nm, NULL);
 }
 
 /* Legendre ring sums of given series (one workgroup per series), the
- * first half of the GEMM-form synthesis */
+ * first half of the GEMM-form synthesis; with irs, also each ring's
+ * inverse split scale (the power of two 2^-e that puts the ring's largest
+ * real or imaginary part at 2^13..2^14 when multiplied by 2^e), the row
+ * scales of the split-FP16 phi DFT */
 template <typename T>
-__global__ void __launch_bounds__(NT) k_alm_lf(const cplx<T> *__restrict__ alm, struct synth_geom g, struct synth_tables tb, cplx<T> *__restrict__ Fg)
+__global__ void __launch_bounds__(NT) k_alm_lf(const cplx<T> *__restrict__ alm, struct synth_geom g, struct synth_tables tb, cplx<T> *__restrict__ Fg, float *__restrict__ irs)
 {
 	extern __shared__ unsigned char smem_raw[];
+	__shared__ unsigned rmax[2 * SPHR_MAX_MESH_B];
 	const int L = g.L, B = g.B, n = g.n, nm = 2 * L + 1;
 	const int ncoef = (L + 1) * (L + 1), nal = (L + 1) * (L + 2) / 2;
 	cplx<T> *sa = reinterpret_cast<cplx<T> *>(smem_raw);
-	T *al = reinterpret_cast<T *>(sa + ncoef);
+	cplx<T> *cf = sa + ncoef;	/* [nal + 2] */
 	const int s = blockIdx.x;
 	for(int t = threadIdx.x; t < ncoef; t += NT)
 		sa[t] = alm[(size_t) s * ncoef + t];
-	for(int t = threadIdx.x; t < nal; t += NT)
-		al[t] = (T) tb.alpha[t];
+	ring_coefs<T>(L, tb.alpha, cf);
+	if(irs)
+		for(int t = threadIdx.x; t < n; t += NT)
+			rmax[t] = 0;
 	__syncthreads();
-	cplx<T> *F = Fg + (size_t) s * n * nm;
-	for(int it = threadIdx.x; it < B * nm; it += NT) {
-		const int j = it / nm, mi = it - j * nm;
-		const int m = mi <= L ? mi : mi - nm;
-		const int am = m < 0 ? -m : m;
-		const T x = (T) tb.x[j];
-		const int abase = lm_offset(L, am, m);
-		const int albase = lm_offset(L, am, am);
-		T pprev = 0, pcur = (T) tb.smm[(size_t) j * (L + 1) + am], aprev = 1;
-		cplx<T> E = mk((T) 0, (T) 0), O = mk((T) 0, (T) 0);
-		for(int l = am; l <= L; l++) {
-			if(l > am) {
-				const T a_l = al[albase + l - am];
-				const T pn = (l == am + 1) ? a_l * x * pcur : a_l * (x * pcur - pprev / aprev);
-				pprev = pcur;
-				pcur = pn;
-				aprev = a_l;
-			}
-			const cplx<T> c = sa[abase + l - am];
-			if((l + am) & 1)
-				cfma(O, c, pcur);
-			else
-				cfma(E, c, pcur);
+	ring_sums<T>(sa, cf, L, B, tb.x, tb.smm, Fg + (size_t) s * n * nm, irs ? rmax : NULL);
+	if(!irs)
+		return;
+	__syncthreads();
+	for(int r = threadIdx.x; r < n; r += NT) {
+		const float mx = __uint_as_float(rmax[r]);
+		int ex = 0;
+		if(mx > 0.f) {
+			frexpf(mx, &ex);
+			ex = 14 - ex;
 		}
-		const T sg = (m < 0 && (am & 1)) ? (T) -1 : (T) 1;
-		F[(size_t) j * nm + mi] = (E + O) * sg;
-		F[(size_t) (2 * B - 1 - j) * nm + mi] = (E - O) * sg;
+		irs[(size_t) s * n + r] = ldexpf(1.0f, -ex);
 	}
 }
 
@@ -1058,7 +1139,7 @@ template <typename T>
 int launch_sky_lf(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *F, hipStream_t st)
 {
 	const int L = g->L;
-	const size_t lds = (size_t) (L + 1) * (L + 1) * sizeof(cplx<T>) + ((size_t) (L + 1) * (L + 2) / 2 + xg->CT) * sizeof(T);
+	const size_t lds = ((size_t) (L + 1) * (L + 1) + (size_t) (L + 1) * (L + 2) / 2 + 2) * sizeof(cplx<T>) + (size_t) xg->CT * sizeof(T);
 	if(lds > 65536)
 		return -1;
 	hipLaunchKernelGGL((k_sky_lf<T>), dim3(S), dim3(NT), lds, st, partial, nsplit, S, *xg, R, lm, gmst, *g, *tb, alm, F);
@@ -1075,17 +1156,26 @@ int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct
 template int launch_sky_lf<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
 template int launch_sky_lf<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);
 template <typename T>
-int launch_synth_gemm(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, cplx<T> *map, hipStream_t st)
+int launch_ring_sums(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, hipStream_t st, float *irs)
 {
 	const int L = g->L;
-	const size_t lds = (size_t) (L + 1) * (L + 1) * sizeof(cplx<T>) + (size_t) (L + 1) * (L + 2) / 2 * sizeof(T);
-	if(lds > 65536)
+	const size_t lds = ((size_t) (L + 1) * (L + 1) + (size_t) (L + 1) * (L + 2) / 2 + 2) * sizeof(cplx<T>);
+	if(lds > 65536 || g->B > SPHR_MAX_MESH_B)
 		return -1;
-	hipLaunchKernelGGL((k_alm_lf<T>), dim3(S), dim3(NT), lds, st, alm, *g, *tb, F);
-	if(hipGetLastError() != hipSuccess)
+	hipLaunchKernelGGL((k_alm_lf<T>), dim3(S), dim3(NT), lds, st, alm, *g, *tb, F, irs);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T>
+int launch_synth_gemm(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, cplx<T> *map, hipStream_t st)
+{
+	if(launch_ring_sums<T>(alm, S, g, tb, F, st, NULL))
 		return -1;
 	return launch_phi<T>(F, S, g, tb, map, st);
 }
+
+template int launch_ring_sums<double>(const cplx<double> *, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, hipStream_t, float *);
+template int launch_ring_sums<float>(const cplx<float> *, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, hipStream_t, float *);
 
 template int launch_synth_gemm<double>(const cplx<double> *, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
 template int launch_synth_gemm<float>(const cplx<float> *, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);

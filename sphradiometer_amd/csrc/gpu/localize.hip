@@ -44,6 +44,10 @@ struct sphr_loc {
 	float *ira;		/* [D max_events] */
 	_Float16 *Xh, *Xl;	/* [max_events][Kw]: cross spectra, split per row */
 	float *irx;		/* [max_events] */
+	int Kp;			/* K of the phi DFT, 2 (2L+1) padded to 64 */
+	_Float16 *Ph, *Pl;	/* [2 n][Kp]: the phi DFT matrix transposed, per-column scale */
+	float *icp;		/* [2 n] */
+	float *irr;		/* [2 max_events n]: inverse split scales of the ring sums */
 	struct synth_dev syn;
 };
 
@@ -222,7 +226,8 @@ extern "C" void sphr_loc_free(struct sphr_loc *lc)
 {
 	if(!lc)
 		return;
-	void *p[] = { lc->Wx, lc->Dx, lc->prior, lc->F, lc->X, lc->S, lc->Fr, lc->Wh, lc->Wl, lc->icw, lc->Dh, lc->Dl, lc->icd, lc->Ah, lc->Al, lc->ira, lc->Xh, lc->Xl, lc->irx };
+	void *p[] = { lc->Wx, lc->Dx, lc->prior, lc->F, lc->X, lc->S, lc->Fr, lc->Wh, lc->Wl, lc->icw, lc->Dh, lc->Dl, lc->icd, lc->Ah, lc->Al, lc->ira, lc->Xh, lc->Xl, lc->irx,
+		      lc->Ph, lc->Pl, lc->icp, lc->irr };
 	for(unsigned i = 0; i < sizeof(p) / sizeof(p[0]); i++)
 		if(p[i])
 			hipFree(p[i]);
@@ -266,6 +271,32 @@ static int create_t(struct sphr_loc *lc, const struct sphr_loc_desc *d)
 	   || sphr_rt_check(hipMalloc((void **) &lc->Xl, E * lc->Kw * sizeof(_Float16)), "hipMalloc")
 	   || sphr_rt_check(hipMalloc((void **) &lc->irx, E * sizeof(float)), "hipMalloc"))
 		return -1;
+	/* the phi DFT of the meshes: exp(i m phi_k), m = 0..L then -L..-1
+	 * (the ring sums' column order), k < n */
+	{
+		const int L = d->L, n = 2 * (L + 1), nm = 2 * L + 1;
+		double *ph = (double *) malloc((size_t) nm * n * 2 * sizeof(double)), *tw = (double *) malloc(2 * (size_t) n * sizeof(double));
+		if(!ph || !tw) {
+			free(ph);
+			free(tw);
+			return -1;
+		}
+		sphr_twiddles(n, +1, tw);
+		for(int i = 0; i < nm; i++) {
+			const int m = i <= L ? i : i - nm;
+			for(int k = 0; k < n; k++) {
+				const long r = (((long) k * m) % n + n) % n;
+				ph[2 * ((size_t) i * n + k)] = tw[2 * r];
+				ph[2 * ((size_t) i * n + k) + 1] = tw[2 * r + 1];
+			}
+		}
+		lc->Kp = (2 * nm + 63) / 64 * 64;
+		rc = upload_split_cols(ph, nm, n, lc->Kp, &lc->Ph, &lc->Pl, &lc->icp);
+		free(ph);
+		free(tw);
+		if(rc || sphr_rt_check(hipMalloc((void **) &lc->irr, 2 * E * (size_t) n * sizeof(float)), "hipMalloc"))
+			return -1;
+	}
 	return 0;
 }
 
@@ -360,11 +391,18 @@ static int run_epilogue(struct sphr_loc *lc, int E, const double *gmst, void *al
 	if(hipGetLastError() != hipSuccess)
 		return -1;
 	/* 5: meshes of both skies: Legendre ring sums, then the phi DFT as a
-	 * GEMM (the one-kernel synthesis when the order is too large for it) */
-	if(map && launch_synth_gemm<T>((const cplx<T> *) alm, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) lc->Fr, (cplx<T> *) map, st)
-	   && launch_synth<T, T, T>((const cplx<T> *) alm, 2 * E, 0, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st))
-		return -1;
-	return 0;
+	 * GEMM (the one-kernel synthesis when the order is too large for it);
+	 * in mixed precision the GEMM is the split-FP16 product, each ring's
+	 * split scale from the ring-sum kernel, the split done while staging */
+	if(!map)
+		return 0;
+	if(sizeof(T) == 4 && lc->irr) {
+		const int n = 2 * (L + 1), nm = 2 * L + 1;
+		if(launch_ring_sums<T>((const cplx<T> *) alm, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) lc->Fr, st, lc->irr) == 0)
+			return launch_gemm_h3a<float>(2 * E * n, 2 * n, 2 * nm, (const float *) lc->Fr, 2L * nm, lc->irr, lc->Ph, lc->Pl, lc->Kp, lc->icp, (float *) map, 2L * n, st);
+	} else if(launch_synth_gemm<T>((const cplx<T> *) alm, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) lc->Fr, (cplx<T> *) map, st) == 0)
+		return 0;
+	return launch_synth<T, T, T>((const cplx<T> *) alm, 2 * E, 0, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st);
 }
 
 template <typename T>

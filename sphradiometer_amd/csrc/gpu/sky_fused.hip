@@ -142,6 +142,8 @@ template <int L> struct sf_rg {
 	enum { V = ((L + 2) / 2) * (L / 2 + 1) > 256 ? 3 : 2 };
 };
 #define SF_US 4
+/* K splits the first segment's partial sums are prefetched for */
+#define SF_PF 8
 
 /* baselines handled with all rotation-row loads issued first; networks
  * with more take the general loop */
@@ -240,13 +242,15 @@ __global__ void k_sky_blob(const double *__restrict__ alpha, const double *__res
 }
 
 template <typename T, int L, int NSUB>
-__global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict__ partial, int nsplit, int S, struct xproj_geom xg, const cplx<T> *__restrict__ R, int rlds, const int2 *__restrict__ lm, const double *__restrict__ gmst, struct synth_tables tb, cplx<T> *__restrict__ alm, cplx<T> *__restrict__ map)
+__global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict__ partial, int nsplit, int S, struct xproj_geom xg, const cplx<T> *__restrict__ R, int rmode, const int2 *__restrict__ lm, const double *__restrict__ gmst, struct synth_tables tb, cplx<T> *__restrict__ alm, cplx<T> *__restrict__ map)
 {
 	typedef sf_geom<L> G;
 	typedef typename sf_mf<T>::v4 v4;
 	/* per sub-group (one segment each) */
-	__shared__ T Fe_[NSUB][G::n * G::FS];	/* even orders [ring][2 q + re/im] */
-	__shared__ T Fo_[NSUB][G::n * G::FS];	/* odd orders */
+	/* F tiles: [0] even orders, [1] odd orders, [sub][ring][2 q + re/im];
+	 * with rmode 2 they hold the rotation rows from the partial-sum phase
+	 * to the a_lm phase */
+	__shared__ __align__(16) T Ft_[2][NSUB][G::n * G::FS];
 	__shared__ cplx<T> sa_[NSUB][G::ncoef];	/* a_lm of the segment */
 	__shared__ cplx<T> rot_[NSUB][L + 1];	/* exp(-i m gmst) */
 	/* shared by the sub-groups: the table blob */
@@ -258,15 +262,45 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 	const T *smm = reinterpret_cast<const T *>(tab + Bl::SM);
 	const short *slm = reinterpret_cast<const short *>(tab + Bl::LM);
 	extern __shared__ __align__(16) unsigned char sf_dyn[];
-	/* dynamic: the rotation rows [nb][ncoef] when rlds, then [NSUB][CT]
-	 * partial sums */
-	cplx<T> *sR = reinterpret_cast<cplx<T> *>(sf_dyn);
+	/* rotation rows [nb][ncoef]: rmode 0 global memory, 1 the dynamic LDS
+	 * (then the [NSUB][CT] partial sums), 2 the F tiles */
+	cplx<T> *sR = reinterpret_cast<cplx<T> *>(sf_dyn), *sRf = reinterpret_cast<cplx<T> *>(&Ft_[0][0][0]);
 
 	const int gtid = threadIdx.x, sub = gtid / SF_NT, tid = gtid % SF_NT;
 	const int wave = tid >> 6;
-	T *Fe = Fe_[sub], *Fo = Fo_[sub];
+	T *Fe = Ft_[0][sub], *Fo = Ft_[1][sub];
 	cplx<T> *sa = sa_[sub], *rot = rot_[sub];
-	T *pw = reinterpret_cast<T *>(sf_dyn + (rlds ? (size_t) xg.nb * G::ncoef * sizeof(cplx<T>) : 0)) + (size_t) sub * xg.CT;
+	T *pw = reinterpret_cast<T *>(sf_dyn + (rmode == 1 ? (size_t) xg.nb * G::ncoef * sizeof(cplx<T>) : 0)) + (size_t) sub * xg.CT;
+	/* the rotation rows to LDS at dst, all threads of the workgroup, eight
+	 * loads in flight per thread */
+	auto load_rows = [&](cplx<T> *dst) {
+		constexpr int NT = SF_NT * NSUB;
+		const int nr = xg.nb * G::ncoef;
+		for(int t = gtid; t < nr; t += 8 * NT) {
+			cplx<T> w[8];
+#pragma unroll
+			for(int u = 0; u < 8; u++)
+				if(t + u * NT < nr)
+					w[u] = R[t + u * NT];
+#pragma unroll
+			for(int u = 0; u < 8; u++)
+				if(t + u * NT < nr)
+					dst[t + u * NT] = w[u];
+		}
+	};
+
+	/* the first segment's K-split partial sums, when each thread sums one
+	 * column of at most SF_PF splits: loads issued before the table
+	 * prologue's, so the two global round trips overlap */
+	const bool pf = xg.CT <= SF_NT && nsplit <= SF_PF;
+	T pv[SF_PF];
+	{
+		const int s = blockIdx.x * NSUB + sub;
+		const size_t st = (size_t) S * xg.CT;
+#pragma unroll
+		for(int u = 0; u < SF_PF; u++)
+			pv[u] = (pf && s < S && tid < xg.CT && u < nsplit) ? partial[(size_t) s * xg.CT + tid + u * st] : (T) 0;
+	}
 
 	/* once per workgroup: the table blob and the rotation rows to LDS
 	 * (all loads of a thread issued before its stores), the zero padding
@@ -279,28 +313,17 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 		for(int r = 0; r < RB; r++)
 			if(gtid + r * NT < N16)
 				v[r] = src[gtid + r * NT];
-		if(rlds) {
-			const int nr = xg.nb * G::ncoef;
-			for(int t = gtid; t < nr; t += 8 * NT) {
-				cplx<T> w[8];
-#pragma unroll
-				for(int u = 0; u < 8; u++)
-					if(t + u * NT < nr)
-						w[u] = R[t + u * NT];
-#pragma unroll
-				for(int u = 0; u < 8; u++)
-					if(t + u * NT < nr)
-						sR[t + u * NT] = w[u];
-			}
-		}
+		if(rmode)
+			load_rows(rmode == 1 ? sR : sRf);
 #pragma unroll
 		for(int r = 0; r < RB; r++)
 			if(gtid + r * NT < N16)
 				reinterpret_cast<uint4 *>(tab)[gtid + r * NT] = v[r];
-		for(int t = gtid; t < NSUB * G::n * G::FS; t += NT) {
-			Fe_[0][t] = (T) 0;
-			Fo_[0][t] = (T) 0;
-		}
+		/* the F tiles' zero padding (with rmode 2 refreshed in each
+		 * segment's Legendre phase instead) */
+		if(rmode != 2)
+			for(int t = gtid; t < 2 * NSUB * G::n * G::FS; t += NT)
+				(&Ft_[0][0][0])[t] = (T) 0;
 	}
 	/* per-baseline columns and orders of the first baselines, uniform */
 	int csb[SF_NBF], lbb[SF_NBF];
@@ -335,7 +358,11 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 		const int s = base + sub;
 		const bool act = s < S;
 		/* ---- partial sums over the K splits, rotate_z phases ---- */
-		if(act) {
+		if(act && pf && base == (int) blockIdx.x * NSUB) {
+			/* the first segment's, loaded before the prologue */
+			if(otid < xg.CT)
+				pw[otid] = ((pv[0] + pv[1]) + (pv[2] + pv[3])) + ((pv[4] + pv[5]) + (pv[6] + pv[7]));
+		} else if(act) {
 			for(int c = otid; c < xg.CT; c += SF_NT) {
 				/* eight independent loads in flight */
 				const T *p = partial + (size_t) s * xg.CT + c;
@@ -353,19 +380,23 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 					v[sp & 7] += p[sp * st];
 				pw[c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
 			}
-			if(otid <= L) {
-				double sn = 0.0, cs = 1.0;
-				if(gmst && otid)
-					sincos(-(double) otid * gmst[s], &sn, &cs);
-				rot[otid] = mk((T) cs, (T) sn);
-			}
 		}
+		if(act && otid <= L) {
+			double sn = 0.0, cs = 1.0;
+			if(gmst && otid)
+				sincos(-(double) otid * gmst[s], &sn, &cs);
+			rot[otid] = mk((T) cs, (T) sn);
+		}
+		/* rotation rows in the F tiles: reloaded after the previous
+		 * segment's phi DFT has read them */
+		if(rmode == 2 && base != (int) blockIdx.x * NSUB)
+			load_rows(sRf);
 		__syncthreads();
 
 		/* ---- a_lm: rotation rows over the baselines, rotate_z ---- */
 		if(act) {
-			if(rlds)
-				sf_alm<T, L>(sR, pw, slm, rot, gmst != NULL, xg, csb, lbb, otid, sa, alm + (size_t) s * G::ncoef);
+			if(rmode)
+				sf_alm<T, L>(rmode == 1 ? sR : sRf, pw, slm, rot, gmst != NULL, xg, csb, lbb, otid, sa, alm + (size_t) s * G::ncoef);
 			else
 				sf_alm<T, L>(R, pw, slm, rot, gmst != NULL, xg, csb, lbb, otid, sa, alm + (size_t) s * G::ncoef);
 		}
@@ -376,6 +407,19 @@ __global__ void __launch_bounds__(SF_NT * NSUB) k_sky_fused(const T *__restrict_
 		/* ---- Legendre ring sums into the F tiles: item (a, ring group)
 		 * runs the recurrences of SF_RG rings at once, so each
 		 * coefficient and a_lm read from LDS serves SF_RG rings ---- */
+		if(rmode == 2) {
+			/* the zero padding the rotation rows overwrote: columns
+			 * past the modes up to the last MFMA k-step, disjoint from
+			 * the ring sums written below */
+			constexpr int PE = 4 * G::KSE - 2 * G::NM0, PO = 4 * G::KSO - 2 * G::NM1;
+			for(int t = otid; t < G::n * (PE + PO); t += SF_NT) {
+				const int r = t / (PE + PO), c = t - r * (PE + PO);
+				if(c < PE)
+					Fe[r * G::FS + 2 * G::NM0 + c] = (T) 0;
+				else
+					Fo[r * G::FS + 2 * G::NM1 + c - PE] = (T) 0;
+			}
+		}
 		if(act) {
 			constexpr int SF_RG = sf_rg<L>::V;
 			constexpr int NG = (G::B + SF_RG - 1) / SF_RG;
@@ -607,48 +651,69 @@ int sky_fused_tables(int L, const struct synth_tables *tb, struct sky_fused_dev 
 	}
 }
 
-/* the orders with a compiled fused kernel */
-/* sub-groups (segments in flight) per workgroup: float32 two, sharing the
- * tables; float64 one (its F tiles take most of the LDS) */
+/* sub-groups (segments in flight) per workgroup by default: float32 two,
+ * sharing the tables; float64 one (its F tiles take most of the LDS) */
 template <typename T> struct sf_sub {
 	enum { N = sizeof(T) == 4 ? 2 : 1 };
 };
+
+/* static LDS of the kernel at NS sub-groups (F tiles, a_lm, tables), with
+ * slack */
+template <typename T, int L>
+static size_t sky_fused_lds(int NS)
+{
+	typedef sf_geom<L> G;
+	return NS * (2 * (size_t) G::n * G::FS * sizeof(T) + (size_t) (G::ncoef + L + 1) * sizeof(cplx<T>))
+	       + (size_t) (G::nal + 2 + G::n) * sizeof(cplx<T>) + G::ncoef * sizeof(short) + (size_t) G::B * (L + 2) * sizeof(T) + 1024;
+}
 
 template <typename T, int L, int NS>
 static int sky_fused_at(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int ncu, int occ, hipStream_t st)
 {
 	typedef sf_geom<L> G;
-	/* static LDS of the kernel (F tiles, a_lm, tables), with slack */
-	const size_t stat = NS * (2 * (size_t) G::n * G::FS * sizeof(T) + (size_t) (G::ncoef + L + 1) * sizeof(cplx<T>))
-			    + (size_t) (G::nal + 2 + G::n) * sizeof(cplx<T>) + G::ncoef * sizeof(short) + (size_t) G::B * (L + 2) * sizeof(T) + 1024;
+	const size_t stat = sky_fused_lds<T, L>(NS);
 	const size_t rbytes = (size_t) xg->nb * G::ncoef * sizeof(cplx<T>), pbytes = (size_t) NS * xg->CT * sizeof(T);
 	/* persistent workgroups, occ per CU, as many as the LDS of all of
 	 * them allows within the CU's 160 KiB; the rotation rows go to LDS
 	 * when they fit beside the rest at that occupancy */
 	const size_t cu_lds = 160 * 1024;
+	if(stat + pbytes > cu_lds)
+		return -1;
 	while(occ > 1 && occ * (stat + pbytes) > cu_lds)
 		occ--;
-	const int rlds = occ * (stat + rbytes + pbytes) <= cu_lds;
-	const size_t dyn = (rlds ? rbytes : 0) + pbytes;
+	/* rotation rows in the dynamic LDS when they fit beside the rest at
+	 * this occupancy, else in the F tiles when those hold them, else read
+	 * from global memory */
+	const size_t ftiles = 2 * (size_t) NS * G::n * G::FS * sizeof(T);
+	const int rmode = occ * (stat + rbytes + pbytes) <= cu_lds ? 1 : rbytes <= ftiles ? 2 : 0;
+	const size_t dyn = (rmode == 1 ? rbytes : 0) + pbytes;
 	const int nwg = (S + NS - 1) / NS;
 	const int grid = nwg < occ * ncu ? nwg : occ * ncu;
-	hipLaunchKernelGGL((k_sky_fused<T, L, NS>), dim3(grid), dim3(SF_NT * NS), dyn, st, partial, nsplit, S, *xg, R, rlds, lm, gmst, *tb, alm, map);
+	hipLaunchKernelGGL((k_sky_fused<T, L, NS>), dim3(grid), dim3(SF_NT * NS), dyn, st, partial, nsplit, S, *xg, R, rmode, lm, gmst, *tb, alm, map);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-/* occupancy 1 or 2: the default sub-groups per workgroup (float32 two
- * segments sharing the tables, float64 one); 3 or more: one segment per
- * workgroup and that many workgroups per CU, where the LDS allows */
+/* nsub segments per workgroup (0: the default, float32 two sharing the
+ * tables, float64 one; float32 four at one workgroup per CU share the
+ * tables and the rotation rows), occ workgroups per CU where the LDS
+ * allows */
 template <typename T, int L>
-static int sky_fused_form(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int ncu, int occ, hipStream_t st)
+static int sky_fused_form(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, int ncu, int occ, int nsub, hipStream_t st)
 {
-	if(occ >= 3 || sf_sub<T>::N == 1)
+	if(!nsub)
+		nsub = occ >= 3 ? 1 : sf_sub<T>::N;
+	if(nsub == 1)
 		return sky_fused_at<T, L, 1>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
-	return sky_fused_at<T, L, sf_sub<T>::N>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
+	if(nsub == 2)
+		return sky_fused_at<T, L, 2>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
+	if constexpr(sizeof(T) == 4)
+		if(nsub == 4)
+			return sky_fused_at<T, L, 4>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
+	return -1;
 }
 
 template <typename T>
-int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st, int occ)
+int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st, int occ, int nsub)
 {
 	static int ncu = 0;
 	if(!ncu) {
@@ -662,13 +727,13 @@ int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geo
 	/* persistent workgroups, one per CU */
 	switch(g->L) {
 	case 25:
-		return sky_fused_form<T, 25>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
+		return sky_fused_form<T, 25>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, nsub, st);
 	case 32:
-		return sky_fused_form<T, 32>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, st);
+		return sky_fused_form<T, 32>(partial, nsplit, S, xg, R, lm, gmst, tb, alm, map, ncu, occ, nsub, st);
 	default:
 		return -1;
 	}
 }
 
-template int launch_sky_fused<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t, int);
-template int launch_sky_fused<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t, int);
+template int launch_sky_fused<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t, int, int);
+template int launch_sky_fused<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t, int, int);

@@ -20,8 +20,11 @@ def gpu():
         pytest.fail("no GPU: " + S.last_error())
 
 
+@pytest.mark.parametrize("a_f32", [0, 1])
 @pytest.mark.parametrize("M,N,K", [(200, 150, 300), (128, 128, 64), (1, 17, 5), (333, 260, 1000)])
-def test_gemm_h3_matches_float64(M, N, K):
+def test_gemm_h3_matches_float64(M, N, K, a_f32):
+    # a_f32: A staged as float32 and split in the loader (the localization
+    # meshes' form), with the row scales of the separate split
     rng = np.random.default_rng(M + N + K)
     A = rng.standard_normal((M, K)) * 10.0 ** rng.uniform(-6, 6, (M, 1))
     B = rng.standard_normal((N, K)) * 10.0 ** rng.uniform(-6, 6, (N, 1))
@@ -29,8 +32,8 @@ def test_gemm_h3_matches_float64(M, N, K):
     B[N // 3] = 0.0
     A32, B32 = A.astype(np.float32), B.astype(np.float32)
     dA, dB, dC = S.DeviceBuffer.from_numpy(A32), S.DeviceBuffer.from_numpy(B32), S.DeviceBuffer(M * N * 4)
-    print("gemm_h3", M, N, K, flush=True)
-    assert S.lib().sphr_gemm_h3_check(M, N, K, dA.ptr, dB.ptr, dC.ptr) == 0, S.last_error()
+    print("gemm_h3", a_f32, M, N, K, flush=True)
+    assert S.lib().sphr_gemm_h3_check(a_f32, M, N, K, dA.ptr, dB.ptr, dC.ptr) == 0, S.last_error()
     C = dC.to_numpy(np.float32, (M, N)).astype(np.float64)
     ref = A32.astype(np.float64) @ B32.astype(np.float64).T
     bound = np.linalg.norm(A32.astype(np.float64), axis=1)[:, None] * np.linalg.norm(B32.astype(np.float64), axis=1)[None, :]

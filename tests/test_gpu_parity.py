@@ -153,11 +153,12 @@ def test_network_integrate_three_site_and_golden(oracle, three_site):
     assert rel(mesh, gold["three_site_map_0"]) < TOL64
 
 
-def _run_batch(net, L, ts, precision, window=None, gmst=None, fused=True, occ=1):
+def _run_batch(net, L, ts, precision, window=None, gmst=None, fused=True, occ=1, nsub=0):
     D, Sg, N = ts.shape
     b = S.Batch(net, L, precision, max_segments=Sg)
     b.set_fused(fused)
     b.set_sky_occupancy(occ)
+    b.set_sky_segments(nsub)
     cdt = np.complex128 if precision == S.FP64 else np.complex64
     d_ts = S.DeviceBuffer.from_numpy(ts.astype(np.float64))
     d_alm = S.DeviceBuffer(Sg * (L + 1) ** 2 * np.dtype(cdt).itemsize)
@@ -200,6 +201,44 @@ def test_batch_fp64_three_site_ragged_and_gmst(oracle, three_site, nseg):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_gmst_with_few_k_splits(oracle, three_site, precision):
+    # the projection's fewest K splits (shape 0: about one workgroup per CU)
+    # keep the partial sums at most 8 deep, where the fused sky kernel
+    # prefetches each workgroup's first segment before its table prologue;
+    # with a GMST per segment, in each sky-kernel form (the two-workgroup
+    # form keeps the rotation rows in its F tiles); 1100 segments, so every
+    # form's persistent workgroups also run a second segment (their first
+    # is the prefetched one)
+    prec, tol = (S.FP64, TOL64) if precision == "fp64" else (S.MIXED, 1e-5)
+    nseg = 1100
+    ts, _ = inject.three_site(nseg, 1024, seed=5)
+    gm = np.linspace(0.1, 6.1, nseg)
+    onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
+    check = list(range(0, nseg, 37)) + [1023, 1024, 1050, nseg - 1]
+    want = {s: onet.segment(ts[:, s], 32, gmst=float(gm[s])) for s in check}
+    cdt = np.complex128 if prec == S.FP64 else np.complex64
+    b = S.Batch(three_site, 32, prec, max_segments=nseg)
+    d_ts = S.DeviceBuffer.from_numpy(ts.astype(np.float64))
+    d_g = S.DeviceBuffer.from_numpy(gm)
+    d_alm = S.DeviceBuffer(nseg * 33 ** 2 * np.dtype(cdt).itemsize)
+    d_map = S.DeviceBuffer(nseg * 66 ** 2 * np.dtype(cdt).itemsize)
+    b.set_shape(0)
+    assert b.info()["k_splits"] <= 8
+    forms = [(1, 0), (2, 0), (3, 1)] + ([(1, 4)] if prec == S.MIXED else [])
+    for occ, nsub in forms:
+        b.set_sky_occupancy(occ)
+        b.set_sky_segments(nsub)
+        print("gmst, few splits", precision, occ, nsub, flush=True)
+        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, d_gmst=d_g.ptr)
+        S.synchronize()
+        alm = d_alm.to_numpy(cdt, (nseg, 33 ** 2))
+        mp = d_map.to_numpy(cdt, (nseg, 66, 66))
+        for s in check:
+            assert rel(alm[s], want[s][0]) < tol, (occ, nsub, s)
+            assert rel(mp[s], want[s][1]) < (tol if prec == S.FP64 else 5e-6), (occ, nsub, s)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_sky_stage_forms_agree(oracle, three_site, precision):
     # the fused sky kernel (default at l_max 25 and 32) and the two-kernel
     # form (ring sums, then the phi DFT as a GEMM) against the oracle
@@ -213,10 +252,22 @@ def test_batch_sky_stage_forms_agree(oracle, three_site, precision):
     a3, m3, i3 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True, occ=2)
     a4, m4, i4 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True, occ=3)
     assert i1["fused"] == 1 and i2["fused"] == 0 and i3["sky_workgroups_per_cu"] == 2 and i4["sky_workgroups_per_cu"] == 3
+    forms = [(a1, m1), (a2, m2), (a3, m3), (a4, m4)]
+    if prec == S.MIXED:
+        # four segments per workgroup sharing the tables and rotation rows
+        # (the second workgroup has one active segment of four)
+        a5, m5, i5 = _run_batch(three_site, 32, ts, prec, gmst=gm, fused=True, occ=1, nsub=4)
+        assert i5["sky_segments_per_workgroup"] == 4
+        forms.append((a5, m5))
+        assert np.array_equal(a5, a1) and np.array_equal(m5, m1)
+    else:
+        b = S.Batch(three_site, 32, prec, max_segments=5)
+        with pytest.raises(RuntimeError):
+            b.set_sky_segments(4)
     onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
     for s in range(5):
         sky, mesh = onet.segment(ts[:, s], 32, gmst=float(gm[s]))
-        for alm, mp in ((a1, m1), (a2, m2), (a3, m3), (a4, m4)):
+        for alm, mp in forms:
             assert rel(alm[s], sky) < tol
             assert rel(mp[s], mesh) < (tol if prec == S.FP64 else 5e-6)
 
