@@ -592,8 +592,8 @@ def bench_copy(S, args):
 # dispatches on this workload, scaled by the counters' reading of a 256 MiB
 # copy (FETCH_SIZE reads half the copied bytes, WRITE_SIZE all of them);
 # the summary with the commands is profiles/three_site_l32_traffic.md.
-SKY_FUSED_F32_WRITE = 48900 * 1024
-SKY_FUSED_F32_READ = 4630 * 1024 * 2
+SKY_FUSED_F32_WRITE = 52168 * 1024
+SKY_FUSED_F32_READ = 6602 * 1024 * 2
 
 
 def sky_fused_traffic():

@@ -47,6 +47,35 @@ __host__ __device__ inline int lm_offset(int L, int l, int m)
 	return base + l - (m < 0 ? -m : m);
 }
 
+/* the order m of coefficient idx in a series of order L (closed form of
+ * lm_decode's m: the block of idx from the quadratic of the block starts,
+ * then corrected by whole blocks) */
+__host__ __device__ inline int lm_order(int L, int idx)
+{
+	const int L1 = L + 1;
+	const int npos = L1 * (L1 + 1) / 2;
+	if(idx < npos) {
+		/* m >= 0: block mm starts at L1 mm - mm (mm - 1) / 2 */
+		const float b = 2.0f * L1 + 1.0f;
+		int mm = (int) ((b - sqrtf(b * b - 8.0f * idx)) * 0.5f);
+		mm = mm < 0 ? 0 : mm > L ? L : mm;
+		while(mm < L && L1 * (mm + 1) - (mm + 1) * mm / 2 <= idx)
+			mm++;
+		while(mm > 0 && L1 * mm - mm * (mm - 1) / 2 > idx)
+			mm--;
+		return mm;
+	}
+	/* m = -L + k, k < L: block k starts k (k + 1) / 2 past npos */
+	const int rem = idx - npos;
+	int k = (int) ((sqrtf(8.0f * rem + 1.0f) - 1.0f) * 0.5f);
+	k = k < 0 ? 0 : k > L - 1 ? L - 1 : k;
+	while(k < L - 1 && (k + 1) * (k + 2) / 2 <= rem)
+		k++;
+	while(k > 0 && k * (k + 1) / 2 > rem)
+		k--;
+	return -L + k;
+}
+
 /* inverse of lm_offset: decode index -> (l, m) */
 __host__ __device__ inline void lm_decode(int L, int idx, int *l, int *m)
 {

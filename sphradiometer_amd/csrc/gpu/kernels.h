@@ -99,11 +99,23 @@ template <typename T>
 int launch_sky_fused(const T *partial, int nsplit, int S, const struct xproj_geom *xg, const cplx<T> *R, const int2 *lm, const double *gmst, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *alm, cplx<T> *map, hipStream_t st, int occ = 1, int nsub = 0);
 template <typename T>
 int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *map, hipStream_t st);
+/* the a_lm of the localization skies formed where they are read: series
+ * s = beta E + e is row (e, beta) of the sky product S [E][2][ncoef]
+ * (complex) plus prior [ncoef], rotated by exp(-i m gmst[e]) (gmst may be
+ * NULL), and written to alm [2E][ncoef] */
+template <typename T> struct sky_epi {
+	const T *S;
+	const cplx<T> *prior;
+	const double *gmst;
+	int E;
+	cplx<T> *alm;
+};
 /* the Legendre ring sums F [S][2(L+1)][2L+1] of S series (the first half
  * of launch_synth_gemm); with irs [S][2(L+1)], each ring's inverse split
- * scale for launch_gemm_h3a */
+ * scale for launch_gemm_h3a; with epi, the series are formed from the sky
+ * product (alm unused) */
 template <typename T>
-int launch_ring_sums(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, hipStream_t st, float *irs);
+int launch_ring_sums(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, hipStream_t st, float *irs, const struct sky_epi<T> *epi);
 /* GEMM-form synthesis of S packed series (order L, non-polar) through the
  * ring-sum scratch F; -1 when a series exceeds the LDS budget */
 template <typename T>

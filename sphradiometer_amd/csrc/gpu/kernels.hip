@@ -217,25 +217,34 @@ __global__ void __launch_bounds__(256) k_rfft4(const TI *__restrict__ in, const 
 	}
 }
 
+/* persistent: up to 4 workgroups per CU, each looping over pairs with the
+ * next pair's samples loaded during this pair's stages.  Measured at 1536
+ * pairs (three sites, 1024 segments, float32): 1024 workgroups 10.4 us,
+ * 768 with two pairs each 10.8 us, 1536 with one pair each (the kernel's
+ * full occupancy) 11.2 us */
+template <typename T, typename TI, int NQ>
+static void launch_rfft4_at(const TI *in, const double *window, int nseries, const cplx<T> *tw, cplx<T> *spec, hipStream_t st)
+{
+	const int npairs = (nseries + 1) / 2, cap = 4 * sphr_rt_cus();
+	hipLaunchKernelGGL((k_rfft4<T, TI, NQ>), dim3(npairs < cap ? npairs : cap), dim3(256), 0, st, in, window, nseries, tw, spec);
+}
+
 template <typename T>
 int launch_rfft_pairs(const void *in, int in_f32, const double *window, int nseries, int N, const cplx<T> *tw, cplx<T> *spec, hipStream_t st)
 {
 	const size_t lds = (size_t) N * sizeof(cplx<T>);
 	dim3 grid((nseries + 1) / 2);
 	if(N == 1024 || (N == 4096 && 2 * N * sizeof(cplx<T>) <= 160 * 1024)) {
-		/* persistent: a few pairs per workgroup, 4 workgroups per CU */
-		const int npairs = (nseries + 1) / 2, cap = 4 * sphr_rt_cus();
-		grid = dim3(npairs < cap ? npairs : cap);
 		if(N == 1024) {
 			if(in_f32)
-				hipLaunchKernelGGL((k_rfft4<T, float, 1>), grid, dim3(256), 0, st, (const float *) in, window, nseries, tw, spec);
+				launch_rfft4_at<T, float, 1>((const float *) in, window, nseries, tw, spec, st);
 			else
-				hipLaunchKernelGGL((k_rfft4<T, double, 1>), grid, dim3(256), 0, st, (const double *) in, window, nseries, tw, spec);
+				launch_rfft4_at<T, double, 1>((const double *) in, window, nseries, tw, spec, st);
 		} else {
 			if(in_f32)
-				hipLaunchKernelGGL((k_rfft4<T, float, 4>), grid, dim3(256), 0, st, (const float *) in, window, nseries, tw, spec);
+				launch_rfft4_at<T, float, 4>((const float *) in, window, nseries, tw, spec, st);
 			else
-				hipLaunchKernelGGL((k_rfft4<T, double, 4>), grid, dim3(256), 0, st, (const double *) in, window, nseries, tw, spec);
+				launch_rfft4_at<T, double, 4>((const double *) in, window, nseries, tw, spec, st);
 		}
 		return hipGetLastError() == hipSuccess ? 0 : -1;
 	}
@@ -1096,17 +1105,38 @@ __global__ void __launch_bounds__(NT) k_sky_lf(const T *__restrict__ partial, in
  * real or imaginary part at 2^13..2^14 when multiplied by 2^e), the row
  * scales of the split-FP16 phi DFT */
 template <typename T>
-__global__ void __launch_bounds__(NT) k_alm_lf(const cplx<T> *__restrict__ alm, struct synth_geom g, struct synth_tables tb, cplx<T> *__restrict__ Fg, float *__restrict__ irs)
+__global__ void __launch_bounds__(NT) k_alm_lf(const cplx<T> *__restrict__ alm, struct synth_geom g, struct synth_tables tb, cplx<T> *__restrict__ Fg, float *__restrict__ irs, struct sky_epi<T> epi)
 {
 	extern __shared__ unsigned char smem_raw[];
 	__shared__ unsigned rmax[2 * SPHR_MAX_MESH_B];
+	__shared__ cplx<T> rot[SPHR_MAX_MESH_B];
 	const int L = g.L, B = g.B, n = g.n, nm = 2 * L + 1;
 	const int ncoef = (L + 1) * (L + 1), nal = (L + 1) * (L + 2) / 2;
 	cplx<T> *sa = reinterpret_cast<cplx<T> *>(smem_raw);
 	cplx<T> *cf = sa + ncoef;	/* [nal + 2] */
 	const int s = blockIdx.x;
-	for(int t = threadIdx.x; t < ncoef; t += NT)
-		sa[t] = alm[(size_t) s * ncoef + t];
+	if(epi.S) {
+		/* series s = beta E + e: the product's (beta, e) row plus the
+		 * prior, rotated by exp(-i m gmst_e); written out, kept in LDS */
+		const int beta = s / epi.E, e = s - beta * epi.E;
+		const T *src = epi.S + (size_t) e * 4 * ncoef + (size_t) beta * 2 * ncoef;
+		const double ge = epi.gmst ? epi.gmst[e] : 0.0;
+		for(int m = threadIdx.x; m <= L; m += NT) {
+			double sn = 0.0, cs = 1.0;
+			if(m)
+				sincos(-(double) m * ge, &sn, &cs);
+			rot[m] = mk((T) cs, (T) sn);
+		}
+		__syncthreads();
+		for(int t = threadIdx.x; t < ncoef; t += NT) {
+			const int m = lm_order(L, t);
+			const cplx<T> v = (mk(src[2 * t], src[2 * t + 1]) + epi.prior[t]) * (m >= 0 ? rot[m] : conjg(rot[-m]));
+			sa[t] = v;
+			epi.alm[(size_t) s * ncoef + t] = v;
+		}
+	} else
+		for(int t = threadIdx.x; t < ncoef; t += NT)
+			sa[t] = alm[(size_t) s * ncoef + t];
 	ring_coefs<T>(L, tb.alpha, cf);
 	if(irs)
 		for(int t = threadIdx.x; t < n; t += NT)
@@ -1156,26 +1186,27 @@ int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct
 template int launch_sky_lf<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
 template int launch_sky_lf<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);
 template <typename T>
-int launch_ring_sums(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, hipStream_t st, float *irs)
+int launch_ring_sums(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, hipStream_t st, float *irs, const struct sky_epi<T> *epi)
 {
 	const int L = g->L;
 	const size_t lds = ((size_t) (L + 1) * (L + 1) + (size_t) (L + 1) * (L + 2) / 2 + 2) * sizeof(cplx<T>);
 	if(lds > 65536 || g->B > SPHR_MAX_MESH_B)
 		return -1;
-	hipLaunchKernelGGL((k_alm_lf<T>), dim3(S), dim3(NT), lds, st, alm, *g, *tb, F, irs);
+	struct sky_epi<T> none = {};
+	hipLaunchKernelGGL((k_alm_lf<T>), dim3(S), dim3(NT), lds, st, alm, *g, *tb, F, irs, epi ? *epi : none);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <typename T>
 int launch_synth_gemm(const cplx<T> *alm, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *F, cplx<T> *map, hipStream_t st)
 {
-	if(launch_ring_sums<T>(alm, S, g, tb, F, st, NULL))
+	if(launch_ring_sums<T>(alm, S, g, tb, F, st, NULL, NULL))
 		return -1;
 	return launch_phi<T>(F, S, g, tb, map, st);
 }
 
-template int launch_ring_sums<double>(const cplx<double> *, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, hipStream_t, float *);
-template int launch_ring_sums<float>(const cplx<float> *, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, hipStream_t, float *);
+template int launch_ring_sums<double>(const cplx<double> *, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, hipStream_t, float *, const struct sky_epi<double> *);
+template int launch_ring_sums<float>(const cplx<float> *, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, hipStream_t, float *, const struct sky_epi<float> *);
 
 template int launch_synth_gemm<double>(const cplx<double> *, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
 template int launch_synth_gemm<float>(const cplx<float> *, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);

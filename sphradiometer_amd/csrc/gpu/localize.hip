@@ -115,25 +115,27 @@ __global__ void __launch_bounds__(256) k_loc_xspec_h3(const cplx<float> *__restr
 }
 
 template <typename T>
-__global__ void k_loc_epilogue(const T *__restrict__ S, int E, int L, const cplx<T> *__restrict__ prior, const double *__restrict__ gmst, cplx<T> *__restrict__ alm)
+__global__ void __launch_bounds__(256) k_loc_epilogue(const T *__restrict__ S, int E, int L, const cplx<T> *__restrict__ prior, const double *__restrict__ gmst, cplx<T> *__restrict__ alm)
 {
+	/* one workgroup per series s = beta E + e: the phases exp(-i m gmst_e)
+	 * once per series, then the coefficients */
+	extern __shared__ unsigned char epi_lds[];
+	cplx<T> *rot = reinterpret_cast<cplx<T> *>(epi_lds);	/* [L + 1] */
 	const int ncoef = (L + 1) * (L + 1);
-	const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
-	if(i >= 2L * E * ncoef)
-		return;
-	const int beta = (int) (i / ((long) E * ncoef));
-	const long r = i - (long) beta * E * ncoef;
-	const int e = (int) (r / ncoef), c = (int) (r - (long) e * ncoef);
-	const T *s = S + (size_t) e * 4 * ncoef + (size_t) beta * 2 * ncoef + 2 * c;
-	cplx<T> v = mk(s[0], s[1]) + prior[c];
-	if(gmst) {
-		int l, m;
-		lm_decode(L, c, &l, &m);
-		double sn, cs;
-		sincos(-(double) m * gmst[e], &sn, &cs);
-		v = v * mk((T) cs, (T) sn);
+	const int s = blockIdx.x, beta = s / E, e = s - beta * E;
+	const double ge = gmst ? gmst[e] : 0.0;
+	for(int m = threadIdx.x; m <= L; m += blockDim.x) {
+		double sn = 0.0, cs = 1.0;
+		if(m)
+			sincos(-(double) m * ge, &sn, &cs);
+		rot[m] = mk((T) cs, (T) sn);
 	}
-	alm[i] = v;
+	__syncthreads();
+	const T *src = S + (size_t) e * 4 * ncoef + (size_t) beta * 2 * ncoef;
+	for(int c = threadIdx.x; c < ncoef; c += blockDim.x) {
+		const int m = lm_order(L, c);
+		alm[(size_t) s * ncoef + c] = (mk(src[2 * c], src[2 * c + 1]) + prior[c]) * (m >= 0 ? rot[m] : conjg(rot[-m]));
+	}
 }
 
 template <typename T>
@@ -384,24 +386,27 @@ template <typename T>
 static int run_epilogue(struct sphr_loc *lc, int E, const double *gmst, void *alm, void *map, hipStream_t st)
 {
 	const int L = lc->L;
-	const int ncoef = (L + 1) * (L + 1);
-	/* 4: prior, gmst */
-	const long ne = 2L * E * ncoef;
-	hipLaunchKernelGGL((k_loc_epilogue<T>), dim3((unsigned) ((ne + 255) / 256)), dim3(256), 0, st, (const T *) lc->S, E, L, (const cplx<T> *) lc->prior, gmst, (cplx<T> *) alm);
-	if(hipGetLastError() != hipSuccess)
-		return -1;
+	/* 4: prior, gmst: with the meshes, formed by the ring-sum kernel as it
+	 * reads them; without, by the epilogue kernel */
+	struct sky_epi<T> epi = { (const T *) lc->S, (const cplx<T> *) lc->prior, gmst, E, (cplx<T> *) alm };
+	auto epilogue = [&]() -> int {
+		hipLaunchKernelGGL((k_loc_epilogue<T>), dim3(2 * E), dim3(256), (L + 1) * sizeof(cplx<T>), st, (const T *) lc->S, E, L, (const cplx<T> *) lc->prior, gmst, (cplx<T> *) alm);
+		return hipGetLastError() == hipSuccess ? 0 : -1;
+	};
+	if(!map)
+		return epilogue();
 	/* 5: meshes of both skies: Legendre ring sums, then the phi DFT as a
 	 * GEMM (the one-kernel synthesis when the order is too large for it);
 	 * in mixed precision the GEMM is the split-FP16 product, each ring's
 	 * split scale from the ring-sum kernel, the split done while staging */
-	if(!map)
-		return 0;
-	if(sizeof(T) == 4 && lc->irr) {
-		const int n = 2 * (L + 1), nm = 2 * L + 1;
-		if(launch_ring_sums<T>((const cplx<T> *) alm, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) lc->Fr, st, lc->irr) == 0)
+	const int n = 2 * (L + 1), nm = 2 * L + 1;
+	if(launch_ring_sums<T>(NULL, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) lc->Fr, st, sizeof(T) == 4 ? lc->irr : NULL, &epi) == 0) {
+		if(sizeof(T) == 4)
 			return launch_gemm_h3a<float>(2 * E * n, 2 * n, 2 * nm, (const float *) lc->Fr, 2L * nm, lc->irr, lc->Ph, lc->Pl, lc->Kp, lc->icp, (float *) map, 2L * n, st);
-	} else if(launch_synth_gemm<T>((const cplx<T> *) alm, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) lc->Fr, (cplx<T> *) map, st) == 0)
-		return 0;
+		return launch_phi<T>((const cplx<T> *) lc->Fr, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st);
+	}
+	if(epilogue())
+		return -1;
 	return launch_synth<T, T, T>((const cplx<T> *) alm, 2 * E, 0, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st);
 }
 

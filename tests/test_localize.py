@@ -323,6 +323,14 @@ def test_gpu_batched_localization_matches_oracle(oracle, local_minimum):
         loc = S.Localizer(netp, netn, lp, L, 21, DT_SMALL, precision=prec, max_events=6)
         print("localizer", loc.info(), flush=True)
         alm, mp, peaks = _run_loc(loc, x, meta["gmst"], L, cdt)
+        # without meshes the a_lm come from the epilogue kernel, with them
+        # from the ring-sum kernel as it reads the product: the same values
+        d_x = S.DeviceBuffer.from_numpy(np.ascontiguousarray(x))
+        d_g = S.DeviceBuffer.from_numpy(np.asarray(meta["gmst"], dtype=np.float64))
+        d_a = S.DeviceBuffer(2 * 6 * (L + 1) ** 2 * np.dtype(cdt).itemsize)
+        loc.run(d_x.ptr, 6, d_a.ptr, d_map=None, d_gmst=d_g.ptr, series_f64=x.dtype == np.complex128)
+        S.synchronize()
+        assert np.array_equal(d_a.to_numpy(cdt, (2, 6, (L + 1) ** 2)), alm)
         for e in range(6):
             want = [o.localize(x[:, e].astype(np.complex128), DT_SMALL, meta["gmst"][e], lp, L) for o in (onp, onn)]
             for bi in range(2):
