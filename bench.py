@@ -30,15 +30,20 @@ def stage_bytes(info, S, esz, in_bytes_per_sample=8):
     """Algorithmic HBM bytes per launch of each stage: FFT, projection,
     a_lm + Legendre ring sums, phi DFT (GEMM); with the fused sky kernel
     the third stage is partials + rotation rows in, a_lm + maps out, and
-    the fourth is empty."""
-    D, nb, N, L, CT, ns = info["detectors"], info["baselines"], info["N"], info["sky_l"], info["columns"], info["k_splits"]
+    the fourth is empty.  On the lag-domain route the first stage is the
+    one kernel that reads the series and the lag table and writes one row
+    of partial sums per segment, and the second is empty."""
+    D, nb, N, L, CT = info["detectors"], info["baselines"], info["N"], info["sky_l"], info["columns"]
+    ns = 1 if info.get("lag") else info["k_splits"]
     Nh = N // 2 + 1
     ncoef = (L + 1) ** 2
     n = 2 * (L + 1)
     nm = 2 * L + 1
     spec = D * S * Nh * 2 * esz
     fft = D * S * N * in_bytes_per_sample + spec
-    xproj = spec + (N + 2) * CT * esz + ns * S * CT * esz
+    xproj = spec + info["table_bytes"] + ns * S * CT * esz
+    if info.get("lag"):
+        fft, xproj = D * S * N * in_bytes_per_sample + info["table_bytes"] + S * CT * esz, 0
     ring = S * n * nm * 2 * esz
     lf = ns * S * CT * esz + nb * ncoef * 2 * esz + S * ncoef * 2 * esz + ring
     phi = ring + 2 * nm * 2 * n * esz + S * n * n * 2 * esz
@@ -549,7 +554,7 @@ def bench_two_site(S, args):
     map_err = float(np.linalg.norm(rmx["map"] - r64["map"]) / np.linalg.norm(r64["map"]))
     value = nseg / (rmx["ms_per_step"] * 1e-3)
     ncoef, nmesh = (L + 1) ** 2, (2 * L + 2) ** 2
-    b_map = 2 * N * 8 + (ncoef + nmesh) * 8 + (N + 2) * rmx["info"]["columns"] * 4 / nseg
+    b_map = 2 * N * 8 + (ncoef + nmesh) * 8 + rmx["info"]["table_bytes"] / nseg
     gbs = b_map * nseg / (rmx["ms_per_step"] * 1e-3) / 1e9
     out = {
         "metric": "sky maps/s, 2-det l_max=25, 16 x 4096-sample Hann segments (batched mixed precision)",
@@ -671,7 +676,8 @@ def main():
     value = nseg / (rmx["ms_per_step"] * 1e-3)
     fp64_value = nseg / (r64["ms_per_step"] * 1e-3)
 
-    names = ["fft", "xproj", "sky_fused", "none"] if rmx["info"].get("fused") else ["fft", "xproj", "alm_legendre", "phi_gemm"]
+    names = ["xcorr", "none"] if rmx["info"].get("lag") else ["fft", "xproj"]
+    names += ["sky_fused", "none"] if rmx["info"].get("fused") else ["alm_legendre", "phi_gemm"]
     sb = stage_bytes(rmx["info"], nseg, rmx["esz"])
     k = int(np.argmax(rmx["stage_ms"]))
     sb64 = stage_bytes(r64["info"], nseg, r64["esz"])
@@ -682,12 +688,12 @@ def main():
     info, esz = rmx["info"], rmx["esz"]
     ncoef, nmesh = (L + 1) ** 2, (2 * L + 2) ** 2
     sky_out = nseg * (ncoef + nmesh) * 2 * esz + info["baselines"] * ncoef * 2 * esz
-    partial_rd = info["k_splits"] * nseg * info["columns"] * esz
+    partial_rd = (1 if info.get("lag") else info["k_splits"]) * nseg * info["columns"] * esz
     t_dom = rmx["stage_ms"][k] * 1e-3
     achieved = (sky_out if names[k] == "sky_fused" else sb[k]) / t_dom / 1e9
     # the whole step against the per-map bytes of the path (series in,
     # a_lm + map out, the delay-product table shared by the batch)
-    b_map = 3 * N * 8 + ncoef * 2 * esz + nmesh * 2 * esz + (N + 2) * info["columns"] * esz / nseg
+    b_map = 3 * N * 8 + ncoef * 2 * esz + nmesh * 2 * esz + info["table_bytes"] / nseg
     step_gbs = b_map * nseg / (rmx["ms_per_step"] * 1e-3) / 1e9
     # peak direction: mixed against float64 (both map-seeded + extremize)
     sep = np.degrees(np.arccos(np.clip(np.cos(rmx["peaks"][0]) * np.cos(r64["peaks"][0]) + np.sin(rmx["peaks"][0]) * np.sin(r64["peaks"][0])

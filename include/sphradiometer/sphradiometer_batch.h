@@ -26,6 +26,7 @@
 #ifndef SPHRADIOMETER_BATCH_H
 #define SPHRADIOMETER_BATCH_H
 
+#include <stddef.h>
 #include <sphradiometer/correlator.h>
 
 #ifdef __cplusplus
@@ -224,7 +225,8 @@ void sphr_jobs_info(const struct sphr_jobs *, long info[8]);
 void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map);
 
 /* Autotuner: runs the batch on these buffers with each candidate shape of
- * the projection (tiles per wave, segments per workgroup, K split), keeps
+ * the projection (tiles per wave, segments per workgroup, K split; on the
+ * lag-domain route its kernel forms instead), keeps
  * the fastest, then tries the fused sky kernel's other forms (two
  * workgroups per CU; one segment per workgroup at three per CU; four
  * segments per workgroup, mixed precision);
@@ -261,6 +263,35 @@ int sphr_batch_set_sky_occupancy(struct sphr_batch *, int workgroups_per_cu);
  * rotation rows; one workgroup per CU); a form whose LDS does not fit runs
  * the unfused sky stage; same results; -1 for other values */
 int sphr_batch_set_sky_segments(struct sphr_batch *, int segments_per_workgroup);
+
+/* The projection's route.  For real series the frequency-domain product
+ * (1/N) sum_f dp[f] X[f] equals a sum over lags of the segments' circular
+ * cross-correlation with the inverse DFT of the delay product, which
+ * vanishes beyond 2 transient samples.  Where the batch's shape allows it
+ * (two or three detectors at 1024 samples, two at 4096 in mixed
+ * precision) and its plan passes the checks of that identity (the table
+ * vanishes past 2 transient, is real, and has the baseline's mirror
+ * symmetry, each to 1e-12 of its largest value), sphr_batch_new selects
+ * this lag-domain route: one kernel forms the FFTs, the cross spectra, the
+ * cross-correlations and the projection of every segment pair, and the
+ * spectra and K splits are not used.  sphr_batch_set_lag(0) selects the
+ * frequency-domain route; 1 the lag-domain route, or -1 where it does not
+ * apply.  sphr_batch_get_lag: 1 while the lag-domain route is in use.
+ * Results agree to rounding. */
+int sphr_batch_set_lag(struct sphr_batch *, int on);
+int sphr_batch_get_lag(const struct sphr_batch *);
+/* Bytes of the projection table the route in use reads per run (the
+ * folded delay products, or the lag-domain table) */
+size_t sphr_batch_table_bytes(const struct sphr_batch *);
+/* The lag-domain kernel's form (picked by the autotuner): 1 (default), a
+ * radix-4 Stockham FFT over 256 threads per transform with the projection
+ * table held in registers (one workgroup per CU); 2, the same with the
+ * table read from L2 (two per CU); 3 (mixed precision at 1024 samples),
+ * each transform in one wavefront, two segment pairs per workgroup.  Same results to rounding; -1 for a form not compiled for the
+ * batch.  sphr_batch_get_xcorr_form: the form in use, 0 on the
+ * frequency-domain route. */
+int sphr_batch_set_xcorr_form(struct sphr_batch *, int form);
+int sphr_batch_get_xcorr_form(const struct sphr_batch *);
 
 #ifdef __cplusplus
 }

@@ -231,6 +231,11 @@ _PROTOS = {
     "sphr_batch_set_sky_occupancy": (c_int, [c_void_p, c_int]),
     "sphr_batch_set_sky_segments": (c_int, [c_void_p, c_int]),
     "sphr_batch_set_shape": (c_int, [c_void_p, c_int]),
+    "sphr_batch_set_lag": (c_int, [c_void_p, c_int]),
+    "sphr_batch_get_lag": (c_int, [c_void_p]),
+    "sphr_batch_table_bytes": (c_size_t, [c_void_p]),
+    "sphr_batch_set_xcorr_form": (c_int, [c_void_p, c_int]),
+    "sphr_batch_get_xcorr_form": (c_int, [c_void_p]),
     "sphr_batch_peaks": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sphr_batch_tune": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P(c_double)]),
     "sphr_jobs_new": (c_void_p, [c_int, P(c_int), P(c_double), P(c_int), P(c_int), c_double, c_int]),
@@ -540,12 +545,34 @@ class Batch:
         lib().sphr_batch_info(self.h, a)
         keys = ["detectors", "baselines", "N", "sky_l", "columns", "k_splits", "precision", "k_per_split", "fused",
                 "tiles_per_wave", "segments_per_workgroup", "sky_workgroups_per_cu", "sky_segments_per_workgroup"]
-        return dict(zip(keys, list(a)))
+        d = dict(zip(keys, list(a)))
+        d["lag"] = int(lib().sphr_batch_get_lag(self.h))
+        d["xcorr_form"] = int(lib().sphr_batch_get_xcorr_form(self.h))
+        d["table_bytes"] = int(lib().sphr_batch_table_bytes(self.h))
+        return d
 
     def set_shape(self, index):
         """One of the autotuner's projection shapes (0..27)."""
         if lib().sphr_batch_set_shape(self.h, int(index)):
             raise ValueError(last_error())
+
+    def set_lag(self, on):
+        """Projection route: the lag-domain kernel (the default where it
+        applies) or the frequency-domain FFT + GEMM pair."""
+        if lib().sphr_batch_set_lag(self.h, int(bool(on))):
+            raise ValueError(last_error())
+
+    @property
+    def lag(self):
+        """True while the lag-domain route is in use."""
+        return bool(lib().sphr_batch_get_lag(self.h))
+
+    def set_xcorr_form(self, form):
+        """The lag-domain kernel's form: 1 (Stockham FFT, table rows in
+        registers), 2 (rows from L2), 3 (one wavefront per transform;
+        mixed precision at 1024 samples)."""
+        if lib().sphr_batch_set_xcorr_form(self.h, int(form)):
+            raise RuntimeError("sphr_batch_set_xcorr_form failed: " + last_error())
 
     def set_fused(self, on):
         """Sky stage as the one fused kernel (default, where compiled for

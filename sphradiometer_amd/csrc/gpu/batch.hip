@@ -14,6 +14,10 @@
  *   R     [nb][(L+1)^2] (1/nb) D^l_{0 m'} of each baseline's rotation, in
  *         the sky coefficient layout (zero above the baseline's l_max)
  *   synthesis tables of order L and the FFT twiddles
+ *   M     the lag-domain projection tables of xcorr.hip, where that route
+ *         applies (two or three detectors, 1024 samples, or two detectors
+ *         at 4096 in float32): the FFT, cross spectra and projection then
+ *         run as one kernel and the spectra and K splits are not used
  */
 #include <hip/hip_runtime.h>
 #include <pthread.h>
@@ -38,15 +42,27 @@ struct sphr_batch {
 	int sky_occ;			/* fused sky workgroups per CU, 1 to 3 (autotuned) */
 	int sky_nsub;			/* fused sky segments per workgroup, 0 (default), 1, 2 or 4 (autotuned) */
 	int ksplit_opt[4], nsplit_opt[4];	/* K-split candidates: 1, 2, 4 x CUs workgroups, 64 per split */
+	int shape;			/* the projection shape index in use (sphr_batch_set_shape) */
+	int lag_ok, lag;		/* lag-domain route built / in use */
+	size_t m_bytes;			/* its table's bytes */
+	int xc_forms, xc_form;		/* its kernel forms (bit f - 1 for form f), the one in use (autotuned) */
 	struct xproj_geom g;
+	struct xcorr_geom xg;
 	struct synth_dev syn;
-	void *W, *R, *tw, *spec, *partial, *F;
+	void *W, *R, *tw, *spec, *partial, *F, *M, *twW;
 	int *col_bl;
 	int2 *lm;
 	int profile;
 	struct ev_set *ev;
 	int ev_used, ev_cap;
 };
+
+/* the projection shapes the autotuner tries: one baseline per workgroup
+ * (tpw 0), then tiles per wave x segments per workgroup; with the 4 K
+ * splits, shape index = 7 x split + shape */
+#define SPHR_TUNE_W 7
+static const int tune_tpw[SPHR_TUNE_W] = { 0, 1, 2, 4, 1, 2, 4 }, tune_ms[SPHR_TUNE_W] = { 64, 16, 16, 16, 32, 32, 32 };
+#define SPHR_BATCH_SHAPES (4 * SPHR_TUNE_W)
 
 static size_t block_offset_h(int l)
 {
@@ -118,11 +134,141 @@ done:
 	return rc;
 }
 
+/*
+ * The lag-domain tables (xcorr.hip): for each baseline,
+ *   M[u][l] = (1/N) sum_f dp[f][l] exp(-2 pi i f u / N)
+ * from its own delay product.  The route is taken only where the kernel is
+ * compiled for the shape and the plan has the properties the kernel relies
+ * on, each checked against the largest |M| of the baseline:
+ *   - M vanishes past |u| = 2 transient (largest dropped value <= 1e-13),
+ *   - M is real (imaginary parts <= 1e-12),
+ *   - M[-u][l] = (-1)^l M[u][l] (to 1e-12).
+ * The table holds Ms[u][l] / (4 N), Ms the symmetric part (the mean of
+ * M[u][l] and (-1)^l M[-u][l]), as [chunk][row][l] with lag
+ * u = chunk SPHR_XC_MR + row (zero past the last lag); 4 N undoes the
+ * scale of the kernel's packed transforms.  0 with
+ * the route off when it does not apply; -1 on an allocation failure.
+ */
+template <typename T>
+static int build_lag(struct sphr_batch *bt, const struct sphr_net_desc *d)
+{
+	const int N = bt->N, nb = bt->nb;
+	struct xcorr_geom *g = &bt->xg;
+	bt->lag_ok = bt->lag = 0;
+	bt->xc_forms = xcorr_forms<T>(N, bt->D, nb);
+	bt->xc_form = 1;
+	if(!bt->xc_forms)
+		return 0;
+	memset(g, 0, sizeof(*g));
+	g->D = bt->D;
+	g->nb = nb;
+	g->CT = bt->g.CT;
+	size_t total = 0;
+	for(int b = 0; b < nb; b++) {
+		const int nl = d->Lb[b] + 1, U = 2 * d->transient[b];
+		if(nl > 256 || U < 0 || 2 * U + 1 > N)
+			return 0;
+		/* chunks of SPHR_XC_MR lags, one thread per (chunk, l) */
+		const int rows = SPHR_XC_MR, nch = (U + 1 + rows - 1) / rows;
+		if(nch * nl > 256 || nch * rows > N / 2)
+			return 0;
+		g->ia[b] = bt->g.ia[b];
+		g->ib[b] = bt->g.ib[b];
+		g->nl[b] = nl;
+		g->nch[b] = nch;
+		g->rows[b] = rows;
+		g->U1[b] = U + 1;
+		g->col_start[b] = bt->g.col_start[b];
+		g->width[b] = (b + 1 < nb ? bt->g.col_start[b + 1] : bt->g.CT) - bt->g.col_start[b];
+		g->moff[b] = (int) total;
+		total += (size_t) nch * rows * nl;
+	}
+	double *col = (double *) malloc(2 * (size_t) N * sizeof(double));
+	double *Mr = (double *) malloc((size_t) N * sizeof(double));
+	T *tab = (T *) calloc(total, sizeof(T));
+	int rc = -1, ok = 1;
+	if(!col || !Mr || !tab)
+		goto done;
+	for(int b = 0; b < nb && ok; b++) {
+		const int nl = g->nl[b], U = g->U1[b] - 1;
+		const double *dp = d->dp[b];
+		double mmax = 0.0, tail = 0.0, imag = 0.0, asym = 0.0;
+		T *tb = tab + g->moff[b];
+		/* the largest |M| of the baseline first, then the checks and
+		 * the table, one coefficient at a time */
+		for(int pass = 0; pass < 2; pass++)
+			for(int l = 0; l < nl; l++) {
+				for(int f = 0; f < N; f++) {
+					col[2 * f] = dp[2 * ((size_t) f * nl + l)];
+					col[2 * f + 1] = dp[2 * ((size_t) f * nl + l) + 1];
+				}
+				sphr_host_dft(col, N, -1);
+				for(int u = 0; u < N; u++) {
+					const double re = col[2 * u] / N, im = col[2 * u + 1] / N;
+					const int uu = u < N - u ? u : N - u;	/* |u| */
+					if(pass == 0) {
+						const double a = sqrt(re * re + im * im);
+						mmax = a > mmax ? a : mmax;
+						continue;
+					}
+					Mr[u] = re;
+					if(uu > U)
+						tail = fmax(tail, sqrt(re * re + im * im));
+					else
+						imag = fmax(imag, fabs(im));
+				}
+				if(pass == 0)
+					continue;
+				const double sg = (l & 1) ? -1.0 : 1.0;
+				for(int u = 0; u <= U; u++) {
+					const double ms = u == 0 ? Mr[0] : 0.5 * (Mr[u] + sg * Mr[N - u]);
+					if(u > 0)
+						asym = fmax(asym, fabs(Mr[u] - sg * Mr[N - u]));
+					const int ch = u / g->rows[b], j = u - ch * g->rows[b];
+					tb[((size_t) ch * g->rows[b] + j) * nl + l] = (T) (ms / (4.0 * N));
+				}
+			}
+		if(!(mmax > 0.0) || tail > 1e-13 * mmax || imag > 1e-12 * mmax || asym > 1e-12 * mmax)
+			ok = 0;
+	}
+	if(ok) {
+		if(upload(&bt->M, tab, total))
+			goto done;
+		bt->m_bytes = total * sizeof(T);
+		if(bt->xc_forms & 4) {
+			/* the wave-level form's twiddles (xcorr_wave_twiddles) */
+			T twl[2 * SPHR_XC_TWW];
+			double *tw64 = (double *) malloc(2 * (size_t) N * sizeof(double));
+			if(!tw64)
+				goto done;
+			sphr_twiddles(N, -1, tw64);
+			for(int k = 0; k < 16; k++)
+				for(int i = 0; i < 64; i++) {
+					const int e1 = (i * k) % N, e2 = (16 * (i & 3) * k) % N;
+					twl[2 * (k * 64 + i)] = (T) tw64[2 * e1];
+					twl[2 * (k * 64 + i) + 1] = (T) tw64[2 * e1 + 1];
+					twl[2 * (1024 + k * 64 + i)] = (T) tw64[2 * e2];
+					twl[2 * (1024 + k * 64 + i) + 1] = (T) tw64[2 * e2 + 1];
+				}
+			free(tw64);
+			if(upload(&bt->twW, twl, 2 * SPHR_XC_TWW))
+				goto done;
+		}
+		bt->lag_ok = bt->lag = 1;
+	}
+	rc = 0;
+done:
+	free(col);
+	free(Mr);
+	free(tab);
+	return rc;
+}
+
 extern "C" void sphr_batch_free(struct sphr_batch *bt)
 {
 	if(!bt)
 		return;
-	void *ptrs[] = { bt->W, bt->R, bt->tw, bt->spec, bt->partial, bt->col_bl, bt->lm, bt->F };
+	void *ptrs[] = { bt->W, bt->R, bt->tw, bt->spec, bt->partial, bt->col_bl, bt->lm, bt->F, bt->M, bt->twW };
 	for(unsigned i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
 		if(ptrs[i])
 			hipFree(ptrs[i]);
@@ -203,6 +349,7 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 		bt->nsplit = bt->nsplit_opt[1];
 		bt->tpw = 0;
 		bt->xms = 64;
+		bt->shape = SPHR_TUNE_W * 1 + 0;
 		bt->fused = 1;
 		bt->sky_occ = 1;
 		bt->sky_nsub = 0;
@@ -229,6 +376,8 @@ extern "C" struct sphr_batch *sphr_batch_create(const struct sphr_net_desc *d, u
 	lm = NULL;
 	if((precision == SPHR_MIXED ? build_tables<float>(bt, d) : build_tables<double>(bt, d)))
 		goto fail;
+	if((precision == SPHR_MIXED ? build_lag<float>(bt, d) : build_lag<double>(bt, d)))
+		goto fail;
 	if(synth_tables_build(bt->L, &bt->syn))
 		goto fail;
 	if(sphr_rt_check(hipMalloc(&bt->spec, (size_t) bt->D * max_segments * bt->Nh * 2 * esz), "hipMalloc spectra")
@@ -251,21 +400,32 @@ static int run_t(struct sphr_batch *bt, void *spec, void *partial, void *F, cons
 {
 	if(ev)
 		hipEventRecord(ev[0], st);
-	if(launch_rfft_pairs<T>(ts, f32, window, bt->D * S, bt->N, (const cplx<T> *) bt->tw, (cplx<T> *) spec, st))
-		return -1;
-	if(ev)
-		hipEventRecord(ev[1], st);
-	if(launch_xproj<T>((const cplx<T> *) spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) partial, st, bt->tpw, bt->xms))
-		return -1;
-	if(ev)
-		hipEventRecord(ev[2], st);
+	if(bt->lag) {
+		/* FFT, cross spectra and projection in one kernel: stage 0 */
+		if(launch_xcorr<T>(ts, f32, window, S, bt->N, &bt->xg, (const cplx<T> *) bt->tw, (const cplx<T> *) bt->twW, (const T *) bt->M, (T *) partial, bt->xc_form, st))
+			return -1;
+		if(ev) {
+			hipEventRecord(ev[1], st);
+			hipEventRecord(ev[2], st);
+		}
+	} else {
+		if(launch_rfft_pairs<T>(ts, f32, window, bt->D * S, bt->N, (const cplx<T> *) bt->tw, (cplx<T> *) spec, st))
+			return -1;
+		if(ev)
+			hipEventRecord(ev[1], st);
+		if(launch_xproj<T>((const cplx<T> *) spec, S, &bt->g, (const T *) bt->W, bt->col_bl, bt->k_per_split, bt->nsplit, (T *) partial, st, bt->tpw, bt->xms))
+			return -1;
+		if(ev)
+			hipEventRecord(ev[2], st);
+	}
+	const int nsplit = bt->lag ? 1 : bt->nsplit;
 	/* the fused sky kernel at the orders it is compiled for; else a_lm
 	 * and ring sums, then the phi DFT as a GEMM; the one-kernel form when
 	 * the coefficients do not fit the LDS budget */
-	if(bt->fused && launch_sky_fused<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st, bt->sky_occ, bt->sky_nsub) == 0) {
+	if(bt->fused && launch_sky_fused<T>((const T *) partial, nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st, bt->sky_occ, bt->sky_nsub) == 0) {
 		if(ev)
 			hipEventRecord(ev[3], st);
-	} else if((F || !map) && launch_sky_lf<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) F : (cplx<T> *) NULL, st) == 0) {
+	} else if((F || !map) && launch_sky_lf<T>((const T *) partial, nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, map ? (cplx<T> *) F : (cplx<T> *) NULL, st) == 0) {
 		if(ev)
 			hipEventRecord(ev[3], st);
 		if(map && launch_phi<T>((const cplx<T> *) F, S, &bt->syn.g, &bt->syn.tb, (cplx<T> *) map, st))
@@ -273,7 +433,7 @@ static int run_t(struct sphr_batch *bt, void *spec, void *partial, void *F, cons
 	} else {
 		if(ev)
 			hipEventRecord(ev[3], st);
-		if(launch_sky<T>((const T *) partial, bt->nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st))
+		if(launch_sky<T>((const T *) partial, nsplit, S, &bt->g, (const cplx<T> *) bt->R, bt->lm, gmst, &bt->syn.g, &bt->syn.tb, (cplx<T> *) alm, (cplx<T> *) map, st))
 			return -1;
 	}
 	if(ev)
@@ -363,8 +523,8 @@ extern "C" int sphr_batch_run_static(struct sphr_batch *bt, const void *ts, int 
 		return -1;
 	const size_t esz = bt->precision == SPHR_MIXED ? sizeof(float) : sizeof(double);
 	const int need_F = map && !(bt->fused && sky_fused_order(bt->L));
-	const size_t b_spec = align256((size_t) bt->D * S * bt->Nh * 2 * esz);
-	const size_t b_part = align256((size_t) bt->nsplit * S * bt->g.CT * esz);
+	const size_t b_spec = bt->lag ? 0 : align256((size_t) bt->D * S * bt->Nh * 2 * esz);
+	const size_t b_part = align256((size_t) (bt->lag ? 1 : bt->nsplit) * S * bt->g.CT * esz);
 	const size_t b_F = need_F ? align256((size_t) S * 2 * (bt->L + 1) * (2 * bt->L + 1) * 2 * esz) : 0;
 	if(b_spec + b_part + b_F > SPHR_SCRATCH_BYTES) {
 		sphr_set_error("sphr_batch_run_static: %d segments need %zu bytes of scratch, more than the static %zu; run fewer segments per call", S, b_spec + b_part + b_F, SPHR_SCRATCH_BYTES);
@@ -475,15 +635,10 @@ extern "C" int sphr_batch_peaks(struct sphr_batch *bt, const void *alm, const vo
 /* autotuner: time the projection's tile width (1, 2, 4 tiles per wave)
  * and K split (about 1, 2, 4 workgroups per CU) on the caller's data and
  * keep the fastest whole run; returns the chosen candidate */
-/* the projection shapes the autotuner tries: one baseline per workgroup
- * (tpw 0), then tiles per wave x segments per workgroup; with the 4 K
- * splits, shape index = 7 x split + shape */
-#define SPHR_TUNE_W 7
-static const int tune_tpw[SPHR_TUNE_W] = { 0, 1, 2, 4, 1, 2, 4 }, tune_ms[SPHR_TUNE_W] = { 64, 16, 16, 16, 32, 32, 32 };
-#define SPHR_BATCH_SHAPES (4 * SPHR_TUNE_W)
 
 static void set_shape(struct sphr_batch *bt, int o, int w)
 {
+	bt->shape = SPHR_TUNE_W * o + w;
 	bt->k_per_split = bt->ksplit_opt[o];
 	bt->nsplit = bt->nsplit_opt[o];
 	bt->tpw = tune_tpw[w];
@@ -508,6 +663,7 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	}
 	hipStream_t st = sphr_rt_stream();
 	const int kps0 = bt->k_per_split, ns0 = bt->nsplit, tpw0 = bt->tpw, xms0 = bt->xms, occ0 = bt->sky_occ, nsub0 = bt->sky_nsub;
+	const int shape0 = bt->shape, xform0 = bt->xc_form;
 	const int prof = bt->profile;
 	hipEvent_t e0 = NULL, e1 = NULL;
 	float best = 1e30f;
@@ -528,19 +684,39 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	 * its default form, then the sky kernel's other forms at the best */
 	bt->sky_occ = 1;
 	bt->sky_nsub = 0;
-	for(int o = 0; o < 4; o++)
-		for(int w = 0; w < SPHR_TUNE_W; w++) {
-			set_shape(bt, o, w);
+	if(bt->lag) {
+		/* the lag-domain route: its kernel forms */
+		int bx = xform0;
+		for(int x = 1; x <= 3; x++) {
+			if(!(bt->xc_forms & (1 << (x - 1))))
+				continue;
+			bt->xc_form = x;
 			float ms = 0.f;
 			if(timed(&ms))
 				goto fail;
 			if(ms < best) {
 				best = ms;
-				bo = o;
-				bw = w;
+				bx = x;
 			}
 		}
-	set_shape(bt, bo, bw);
+		bt->xc_form = bx;
+		bo = shape0 / SPHR_TUNE_W;
+		bw = shape0 % SPHR_TUNE_W;
+	} else {
+		for(int o = 0; o < 4; o++)
+			for(int w = 0; w < SPHR_TUNE_W; w++) {
+				set_shape(bt, o, w);
+				float ms = 0.f;
+				if(timed(&ms))
+					goto fail;
+				if(ms < best) {
+					best = ms;
+					bo = o;
+					bw = w;
+				}
+			}
+		set_shape(bt, bo, bw);
+	}
 	if(bt->fused && sky_fused_order(bt->L) && map) {
 		/* (workgroups per CU, segments per workgroup) */
 		static const int forms[3][2] = { { 2, 0 }, { 3, 1 }, { 1, 4 } };
@@ -578,6 +754,45 @@ fail:
 	bt->xms = xms0;
 	bt->sky_occ = occ0;
 	bt->sky_nsub = nsub0;
+	bt->shape = shape0;
+	bt->xc_form = xform0;
 	bt->profile = prof;
 	return -1;
+}
+
+extern "C" int sphr_batch_set_lag(struct sphr_batch *bt, int on)
+{
+	if(!bt || (on && !bt->lag_ok)) {
+		sphr_set_error("sphr_batch_set_lag: the lag-domain route does not apply to this batch");
+		return -1;
+	}
+	bt->lag = on ? 1 : 0;
+	return 0;
+}
+
+extern "C" int sphr_batch_get_lag(const struct sphr_batch *bt)
+{
+	return bt && bt->lag ? 1 : 0;
+}
+
+extern "C" size_t sphr_batch_table_bytes(const struct sphr_batch *bt)
+{
+	if(!bt)
+		return 0;
+	return bt->lag ? bt->m_bytes : (size_t) bt->g.K2 * bt->g.CT * (bt->precision == SPHR_MIXED ? sizeof(float) : sizeof(double));
+}
+
+extern "C" int sphr_batch_set_xcorr_form(struct sphr_batch *bt, int form)
+{
+	if(!bt || form < 1 || form > 3 || !(bt->xc_forms & (1 << (form - 1)))) {
+		sphr_set_error("sphr_batch_set_xcorr_form: form %d is not compiled for this batch", form);
+		return -1;
+	}
+	bt->xc_form = form;
+	return 0;
+}
+
+extern "C" int sphr_batch_get_xcorr_form(const struct sphr_batch *bt)
+{
+	return bt && bt->lag ? bt->xc_form : 0;
 }

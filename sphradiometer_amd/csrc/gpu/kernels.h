@@ -57,6 +57,37 @@ struct sky_fused_dev {
 template <typename T>
 int launch_rfft_pairs(const void *in, int in_f32, const double *window, int nseries, int N, const cplx<T> *tw, cplx<T> *spec, hipStream_t st);
 
+/* the lag-domain route of the projection (xcorr.hip): per baseline b the
+ * folded lag table [nch][SPHR_XC_MR][nl] at moff[b] of the table array,
+ * row (ch, j) holding lag u = ch SPHR_XC_MR + j for u < U1 (zero beyond) */
+#define SPHR_XC_MAXB 3		/* baselines (and detectors) of the route */
+#define SPHR_XC_MR 40		/* lags per chunk: the table rows a thread holds */
+struct xcorr_geom {
+	int D, nb, CT;
+	int ia[SPHR_XC_MAXB], ib[SPHR_XC_MAXB];
+	int nl[SPHR_XC_MAXB];		/* coefficients: l_b + 1 */
+	int nch[SPHR_XC_MAXB];		/* lag chunks per coefficient */
+	int rows[SPHR_XC_MAXB];		/* lags per chunk: SPHR_XC_MR */
+	int U1[SPHR_XC_MAXB];		/* folded lags: largest |u| + 1 */
+	int col_start[SPHR_XC_MAXB], width[SPHR_XC_MAXB];
+	int moff[SPHR_XC_MAXB];
+};
+
+/* FFT + cross-correlation + lag-domain projection of every segment pair
+ * into partial[S][CT] (one split: the power_1d of each baseline in its
+ * columns, real parts at even columns, zero elsewhere); N is 1024 or 4096;
+ * tw the N twiddles exp(-2 pi i k / N); twW the wave-level form's tables
+ * (xcorr_wave_twiddles); form 1, 2 or 3 (xcorr.hip); -1 for a form or
+ * shape that is not compiled */
+template <typename T>
+int launch_xcorr(const void *in, int in_f32, const double *window, int S, int N, const struct xcorr_geom *g, const cplx<T> *tw, const cplx<T> *twW, const T *M, T *partial, int form, hipStream_t st);
+/* the forms compiled for this shape (precision by T): bit f - 1 for form f */
+template <typename T>
+int xcorr_forms(int N, int D, int nb);
+/* the wave-level form's twiddle tables, 2 x 16 x 64 complex values:
+ * [k][n1] = exp(-2 pi i n1 k / 1024), then [c][lane] = exp(-2 pi i 16 (lane % 4) c / 1024) */
+#define SPHR_XC_TWW (2 * 16 * 64)
+
 /* projection GEMM: partial[split][S][CT]; tile_bl[t] is the baseline of
  * column tile t; k_per_split a multiple of the K chunk (32); tpw the
  * 16-column tiles per wave (1, 2 or 4) */

@@ -31,6 +31,7 @@ struct sphr_batch *sphr_batch_new(const struct correlator_network_plan_fd *plan,
 			d.ib[k] = j;
 			d.Lb[k] = (int) p->power_1d->l_max;
 			d.polar[k] = p->power_1d->polar;
+			d.transient[k] = p->transient;
 			d.dp[k] = (const double *) p->delay_product->coeff;
 			d.blocks[k] = sphr_rotation_blocks(p->rotation_plan);
 		}

@@ -111,6 +111,7 @@ void sphr_gpu_plan_forget(const void *key);
 struct sphr_net_desc {
 	int n_det, nb, N;
 	int ia[SPHR_DESC_MAX], ib[SPHR_DESC_MAX], Lb[SPHR_DESC_MAX], polar[SPHR_DESC_MAX];
+	int transient[SPHR_DESC_MAX];           /* the plan's transient: taps 2 t + 1 */
 	const double *dp[SPHR_DESC_MAX];       /* N x stride complex */
 	const double *blocks[SPHR_DESC_MAX];   /* packed D blocks */
 };

@@ -222,6 +222,7 @@ def test_batch_gmst_with_few_k_splits(oracle, three_site, precision):
     d_g = S.DeviceBuffer.from_numpy(gm)
     d_alm = S.DeviceBuffer(nseg * 33 ** 2 * np.dtype(cdt).itemsize)
     d_map = S.DeviceBuffer(nseg * 66 ** 2 * np.dtype(cdt).itemsize)
+    b.set_lag(False)
     b.set_shape(0)
     assert b.info()["k_splits"] <= 8
     forms = [(1, 0), (2, 0), (3, 1)] + ([(1, 4)] if prec == S.MIXED else [])
@@ -283,6 +284,8 @@ def test_batch_every_tuner_shape_matches_oracle(oracle, three_site):
         b = S.Batch(three_site, 32, prec, max_segments=3)
         d_alm = S.DeviceBuffer(3 * 33 ** 2 * np.dtype(cdt).itemsize)
         d_map = S.DeviceBuffer(3 * 66 ** 2 * np.dtype(cdt).itemsize)
+        # the shapes belong to the frequency-domain route
+        b.set_lag(False)
         for shape in range(28):
             b.set_shape(shape)
             b.run(d_ts.ptr, 3, d_alm.ptr, d_map.ptr)
@@ -294,6 +297,77 @@ def test_batch_every_tuner_shape_matches_oracle(oracle, three_site):
                 assert rel(mp[s], want[s][1]) < (tol if prec == S.FP64 else 5e-6), (prec, shape, s)
         with pytest.raises(ValueError):
             b.set_shape(28)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nseg", [1, 2, 5, 37])
+def test_batch_lag_route_matches_oracle(oracle, three_site, precision, nseg):
+    # the lag-domain route (one kernel: FFTs, cross spectra, inverse
+    # transforms, folded lags x the table) is the default at three sites
+    # and 1024 samples; odd counts leave the last pair with one segment,
+    # counts that are not multiples of four leave the wave-level form's
+    # last pass with a missing pair; with a Hann window and a GMST per
+    # segment, in every kernel form, against the oracle and against the
+    # frequency-domain route
+    prec, tol = (S.FP64, TOL64) if precision == "fp64" else (S.MIXED, 1e-5)
+    ts, _ = inject.three_site(nseg, 1024, seed=40 + nseg)
+    w = inject.hann(1024)
+    gm = np.linspace(0.2, 5.9, nseg)
+    cdt = np.complex128 if prec == S.FP64 else np.complex64
+    b = S.Batch(three_site, 32, prec, max_segments=nseg)
+    assert b.lag
+    # form 1 is the default; the wave-level form (3) is compiled for mixed
+    # precision
+    forms = (1, 2, 3) if prec == S.MIXED else (1, 2)
+    assert b.info()["xcorr_form"] == forms[0]
+    d_ts = S.DeviceBuffer.from_numpy(ts.astype(np.float64))
+    d_w = S.DeviceBuffer.from_numpy(w)
+    d_g = S.DeviceBuffer.from_numpy(gm)
+    d_alm = S.DeviceBuffer(nseg * 33 ** 2 * np.dtype(cdt).itemsize)
+    d_map = S.DeviceBuffer(nseg * 66 ** 2 * np.dtype(cdt).itemsize)
+
+    def run():
+        b.run(d_ts.ptr, nseg, d_alm.ptr, d_map.ptr, d_window=d_w.ptr, d_gmst=d_g.ptr)
+        S.synchronize()
+        return d_alm.to_numpy(cdt, (nseg, 33 ** 2)), d_map.to_numpy(cdt, (nseg, 66, 66))
+
+    onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
+    for s in range(nseg):
+        if nseg > 5 and s not in (0, 1, 17, nseg - 2, nseg - 1):
+            continue
+        sky, mesh = onet.segment(ts[:, s], 32, window=w, gmst=float(gm[s]))
+        for form in forms:
+            b.set_xcorr_form(form)
+            alm, mp = run()
+            print("lag route", precision, nseg, s, form, rel(alm[s], sky), flush=True)
+            assert rel(alm[s], sky) < tol, (form, s)
+            assert rel(mp[s], mesh) < (tol if prec == S.FP64 else 5e-6), (form, s)
+    b.set_lag(False)
+    alm_fd, mp_fd = run()
+    b.set_lag(True)
+    alm, mp = run()
+    assert rel(alm, alm_fd) < (1e-13 if prec == S.FP64 else 2e-6)
+    with pytest.raises(RuntimeError):
+        b.set_xcorr_form(4 if prec == S.MIXED else 3)
+
+
+def test_batch_lag_route_two_site_mixed(oracle, two_site):
+    # two detectors at 4096 samples: the lag-domain route in mixed
+    # precision (float64 keeps the frequency-domain route there)
+    ts = inject.two_site(16, 4096, seed=0)
+    w = inject.hann(4096)
+    b64 = S.Batch(two_site, 25, S.FP64, max_segments=16)
+    assert not b64.lag
+    with pytest.raises(ValueError):
+        b64.set_lag(True)
+    alm, mp, _ = _run_batch(two_site, 25, ts, S.MIXED, window=w)
+    onet = oracle.Network(inject.positions("AB"), 4096, DT, 25)
+    for s in (0, 7, 15):
+        sky, mesh = onet.segment(ts[:, s], 25, window=w)
+        assert rel(alm[s], sky) < 1e-5
+        assert rel(mp[s], mesh) < 1e-5
+    bm = S.Batch(two_site, 25, S.MIXED, max_segments=16)
+    assert bm.lag
 
 
 def test_batch_full_size_three_site_properties(three_site):

@@ -42,14 +42,18 @@ def _run(entry, b, d_ts, nseg, d_gmst, L, cdt):
     return d_alm.to_numpy(cdt, (nseg, (L + 1) ** 2)), d_map.to_numpy(cdt, (nseg, n, n))
 
 
+@pytest.mark.parametrize("lag", [True, False])
 @pytest.mark.parametrize("L", [32, 20])
 @pytest.mark.parametrize("prec", [S.FP64, S.MIXED])
-def test_static_run_matches_regular_run(oracle, L, prec):
+def test_static_run_matches_regular_run(oracle, L, prec, lag):
+    # both projection routes: the lag-domain kernel's one partial-sum row
+    # per segment, and the spectra + K splits of the frequency-domain one
     lib = S.lib()
     cdt = np.complex128 if prec == S.FP64 else np.complex64
     pos = inject.positions("ABC")
     net = S.Network(pos, 1024, DT, L)
     b = S.Batch(net, L, prec, max_segments=64)
+    b.set_lag(lag)
     ts, _ = inject.three_site(80, 1024, seed=5)
     rng = np.random.default_rng(L)
     gmst = rng.uniform(0, 2 * np.pi, 80)
@@ -72,10 +76,15 @@ def test_static_run_refuses_more_than_its_array():
     lib = S.lib()
     net = S.Network(inject.positions("ABC"), 1024, DT, 32)
     b = S.Batch(net, 32, S.FP64, max_segments=16)
-    nseg = 2048
-    d_ts = S.DeviceBuffer(3 * nseg * 1024 * 8)
-    d_alm = S.DeviceBuffer(nseg * 33 * 33 * 16)
-    r = lib.sphr_batch_run_static(ctypes.c_void_p(b.h), ctypes.c_void_p(d_ts.ptr), 0, nseg, None, None,
-                                  ctypes.c_void_p(d_alm.ptr), None, None)
-    assert r == -1
-    assert "scratch" in S.last_error()
+    # the frequency-domain route keeps spectra and K-split partial sums
+    # (about 37 KB per segment here); the lag-domain route only its one
+    # partial-sum row per segment (1.8 KB): each refused past 64 MiB
+    for lag, nseg in ((False, 2048), (True, 40000)):
+        b.set_lag(lag)
+        d_ts = S.DeviceBuffer(3 * nseg * 1024 * 8)
+        d_alm = S.DeviceBuffer(nseg * 33 * 33 * 16)
+        r = lib.sphr_batch_run_static(ctypes.c_void_p(b.h), ctypes.c_void_p(d_ts.ptr), 0, nseg, None, None,
+                                      ctypes.c_void_p(d_alm.ptr), None, None)
+        assert r == -1
+        assert "scratch" in S.last_error()
+        del d_ts, d_alm
