@@ -287,7 +287,9 @@ size_t sphr_batch_table_bytes(const struct sphr_batch *);
  * radix-4 Stockham FFT over 256 threads per transform with the projection
  * table held in registers (one workgroup per CU); 2, the same with the
  * table read from L2 (two per CU); 3 (mixed precision at 1024 samples),
- * each transform in one wavefront, two segment pairs per workgroup.  Same results to rounding; -1 for a form not compiled for the
+ * each transform in one wavefront, two segment pairs per workgroup; 4
+ * (mixed precision at 1024 samples), two segment pairs at once, 128
+ * threads per transform, the table in registers.  Same results to rounding; -1 for a form not compiled for the
  * batch.  sphr_batch_get_xcorr_form: the form in use, 0 on the
  * frequency-domain route. */
 int sphr_batch_set_xcorr_form(struct sphr_batch *, int form);

@@ -570,7 +570,8 @@ class Batch:
     def set_xcorr_form(self, form):
         """The lag-domain kernel's form: 1 (Stockham FFT, table rows in
         registers), 2 (rows from L2), 3 (one wavefront per transform;
-        mixed precision at 1024 samples)."""
+        mixed precision at 1024 samples), 4 (two pairs at once, 128
+        threads per transform; mixed precision at 1024 samples)."""
         if lib().sphr_batch_set_xcorr_form(self.h, int(form)):
             raise RuntimeError("sphr_batch_set_xcorr_form failed: " + last_error())
 

@@ -687,7 +687,7 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	if(bt->lag) {
 		/* the lag-domain route: its kernel forms */
 		int bx = xform0;
-		for(int x = 1; x <= 3; x++) {
+		for(int x = 1; x <= 4; x++) {
 			if(!(bt->xc_forms & (1 << (x - 1))))
 				continue;
 			bt->xc_form = x;
@@ -784,7 +784,7 @@ extern "C" size_t sphr_batch_table_bytes(const struct sphr_batch *bt)
 
 extern "C" int sphr_batch_set_xcorr_form(struct sphr_batch *bt, int form)
 {
-	if(!bt || form < 1 || form > 3 || !(bt->xc_forms & (1 << (form - 1)))) {
+	if(!bt || form < 1 || form > 4 || !(bt->xc_forms & (1 << (form - 1)))) {
 		sphr_set_error("sphr_batch_set_xcorr_form: form %d is not compiled for this batch", form);
 		return -1;
 	}

@@ -41,68 +41,78 @@
 #include "kernels.h"
 #include "runtime.h"
 
-template <typename T, typename TI, int NQ, int G, bool MREG>
+template <typename T, typename TI, int NQ, int G, bool MREG, int PP>
 __global__ void __launch_bounds__(256 * G) k_xcorr(const TI *__restrict__ in, const double *__restrict__ window, int S, struct xcorr_geom g, const cplx<T> *__restrict__ tw, const T *__restrict__ M, T *__restrict__ partial)
 {
 	constexpr int N = 1024 * NQ, N4 = N / 4, NH = N / 2;
 	constexpr int NS = NQ == 1 ? 5 : 6;	/* radix-4 stages */
-	__shared__ cplx<T> buf[2][G][N];
-	const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8);
-	int tid = threadIdx.x & 255;
-	const int npairs = (S + 1) / 2;
+	/* PP segment pairs per pass, each transform over TPT threads with
+	 * NB butterflies per thread per stage */
+	constexpr int TPT = 256 / PP, NB = NQ * PP;
+	__shared__ cplx<T> buf[2][PP * G][N];
+	const int fg = __builtin_amdgcn_readfirstlane(threadIdx.x / TPT);
+	const int ps = fg / G, fd = fg - G * ps;	/* pair slot, detector / baseline */
+	int ft = threadIdx.x & (TPT - 1);
+	const int npg = ((S + 1) / 2 + PP - 1) / PP;	/* passes of PP pairs */
 	/* twiddles of every stage: w_r = tw[r k N / 4p], k = q mod p (the
 	 * inverse transform is a forward one too); where registers are short
-	 * (float64, 4096 samples) only w_1 is held and w_2, w_3 are formed
-	 * from it at each use */
-	constexpr bool TWF = sizeof(T) == sizeof(float) && NQ == 1;
+	 * (float64, 4096 samples, two pairs per pass) only w_1 is held and
+	 * w_2, w_3 are formed from it at each use */
+	constexpr bool TWF = sizeof(T) == sizeof(float) && NB == 1;
 	constexpr int NTW = TWF ? 3 : 1;
-	cplx<T> wt[NS][NQ][NTW];
+	cplx<T> wt[NS][NB][NTW];
 #pragma unroll
 	for(int st = 1; st < NS; st++) {
 		const int p = 1 << (2 * st);
 #pragma unroll
-		for(int u = 0; u < NQ; u++) {
-			const int k = (tid + 256 * u) & (p - 1);
+		for(int u = 0; u < NB; u++) {
+			const int k = (ft + TPT * u) & (p - 1);
 #pragma unroll
 			for(int r = 1; r <= NTW; r++)
 				wt[st][u][r - 1] = tw[r * k * (N / (4 * p))];
 		}
 	}
-	T wv[NQ][4];
+	T wv[NB][4];
 #pragma unroll
-	for(int u = 0; u < NQ; u++)
+	for(int u = 0; u < NB; u++)
 #pragma unroll
 		for(int r = 0; r < 4; r++)
-			wv[u][r] = window ? (T) window[tid + 256 * u + r * N4] : (T) 1;
-	/* this thread's projection: baseline grp, coefficient l, lag chunk ch */
-	const bool fw = grp < g.D, iv = grp < g.nb;
-	const int nl = iv ? g.nl[grp] : 1;
+			wv[u][r] = window ? (T) window[ft + TPT * u + r * N4] : (T) 1;
+	const bool fw = fd < g.D, iv = fd < g.nb;
+	/* this thread's projection: baseline pb, coefficient l, lag chunk ch
+	 * (256 threads per baseline) */
+	const int pb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
+	const bool pv = pb < g.nb;
+	const int nl = pv ? g.nl[pb] : 1;
 	constexpr int R = SPHR_XC_MR;	/* lags per chunk */
-	const int ch = tid / nl, l = tid - ch * nl;
-	const bool act = iv && ch < g.nch[grp];
+	const int ch = t / nl, l = t - ch * nl;
+	const bool act = pv && ch < g.nch[pb];
 	/* the table rows: with MREG (float32 at 1024 samples), held in
-	 * registers, loaded once per workgroup (about 115 VGPRs: one 768-thread
-	 * workgroup per CU); otherwise read from L2 in the projection loop
-	 * itself (84 VGPRs in float32: two workgroups per CU) */
+	 * registers, loaded once per workgroup (one 768-thread workgroup per
+	 * CU); otherwise read from L2 in the projection loop itself (two
+	 * workgroups per CU in float32) */
 	constexpr bool MPRE = MREG && sizeof(T) == sizeof(float) && NQ == 1;
 	constexpr int MRR = MPRE ? SPHR_XC_MR : 1;
 	T mr[MRR];
-	const T *Mb = M + (iv ? g.moff[grp] : 0);
+	const T *Mb = M + (pv ? g.moff[pb] : 0);
 	if(MPRE) {
 #pragma unroll
 		for(int j = 0; j < MRR; j++)
 			mr[j] = act ? Mb[(ch * R + j) * nl + l] : (T) 0;
 	}
-	/* samples q + r N/4 of detector grp, segments 2 pp and 2 pp + 1 */
-	TI xv[NQ][4], yv[NQ][4];
-	auto load = [&](int pp) {
-		const int s0 = 2 * pp, s1 = s0 + 1 < S ? s0 + 1 : s0;
-		const TI *x = in + ((size_t) grp * S + s0) * N, *y = in + ((size_t) grp * S + s1) * N;
+	/* samples q + r N/4 of detector fd, segments 2 pr and 2 pr + 1 of
+	 * pair pr = PP pg + ps (a missing pair reads the last one) */
+	TI xv[NB][4], yv[NB][4];
+	auto load = [&](int pg) {
+		int pr = PP * pg + ps;
+		pr = 2 * pr < S ? pr : (S - 1) / 2;
+		const int s0 = 2 * pr, s1 = s0 + 1 < S ? s0 + 1 : s0;
+		const TI *x = in + ((size_t) fd * S + s0) * N, *y = in + ((size_t) fd * S + s1) * N;
 #pragma unroll
-		for(int u = 0; u < NQ; u++)
+		for(int u = 0; u < NB; u++)
 #pragma unroll
 			for(int r = 0; r < 4; r++) {
-				const int i = tid + 256 * u + r * N4;
+				const int i = ft + TPT * u + r * N4;
 				xv[u][r] = x[i];
 				yv[u][r] = y[i];
 			}
@@ -110,11 +120,11 @@ __global__ void __launch_bounds__(256 * G) k_xcorr(const TI *__restrict__ in, co
 	/* one radix-4 Stockham stage of the transform in a[] into o (the
 	 * stage with sub-transform size p = 4^st: butterfly q, k = q mod p,
 	 * outputs to (q / p) 4p + k + r p) */
-	auto stage = [&](cplx<T> (&a)[NQ][4], int st, cplx<T> *o) {
+	auto stage = [&](cplx<T> (&a)[NB][4], int st, cplx<T> *o) {
 		const int p = 1 << (2 * st);
 #pragma unroll
-		for(int u = 0; u < NQ; u++) {
-			const int q = tid + 256 * u, k = q & (p - 1);
+		for(int u = 0; u < NB; u++) {
+			const int q = ft + TPT * u, k = q & (p - 1);
 			cplx<T> u0 = a[u][0], u1 = a[u][1], u2 = a[u][2], u3 = a[u][3];
 			if(st > 0) {
 				const cplx<T> w1 = wt[st][u][0];
@@ -133,62 +143,62 @@ __global__ void __launch_bounds__(256 * G) k_xcorr(const TI *__restrict__ in, co
 			o[base + 3 * p] = d02 - jd13;
 		}
 	};
-	if(fw && (int) blockIdx.x < npairs)
+	if(fw && (int) blockIdx.x < npg)
 		load(blockIdx.x);
-	for(int pp = blockIdx.x; pp < npairs; pp += gridDim.x) {
-		/* the thread index and the twiddles enter each pair through an
+	for(int pg = blockIdx.x; pg < npg; pg += gridDim.x) {
+		/* the thread index and the twiddles enter each pass through an
 		 * empty asm, so per-thread index arithmetic and derived twiddles
 		 * are formed where they are used instead of being lifted out of
 		 * the loop and held across every phase */
-		asm volatile("" : "+v"(tid));
+		asm volatile("" : "+v"(ft));
 		if(!TWF) {
 #pragma unroll
 			for(int st = 1; st < NS; st++)
 #pragma unroll
-				for(int u = 0; u < NQ; u++)
+				for(int u = 0; u < NB; u++)
 					asm volatile("" : "+v"(wt[st][u][0].re), "+v"(wt[st][u][0].im));
 		}
-		const int s0 = 2 * pp;
-		const bool has1 = s0 + 1 < S;
-		cplx<T> a[NQ][4];
+		const int s0 = 2 * (PP * pg + ps);
+		const bool has0 = s0 < S, has1 = s0 + 1 < S;
+		cplx<T> a[NB][4];
 		/* 1. forward transforms, one detector per group */
 		if(fw) {
 #pragma unroll
-			for(int u = 0; u < NQ; u++)
+			for(int u = 0; u < NB; u++)
 #pragma unroll
 				for(int r = 0; r < 4; r++) {
 					const T w = wv[u][r];
 					a[u][r] = mk((T) xv[u][r] * w, has1 ? (T) yv[u][r] * w : (T) 0);
 				}
-			if(pp + (int) gridDim.x < npairs)
-				load(pp + gridDim.x);
+			if(pg + (int) gridDim.x < npg)
+				load(pg + gridDim.x);
 		}
 #pragma unroll
 		for(int st = 0; st < NS; st++) {
 			if(fw) {
 				if(st > 0) {
-					const cplx<T> *iv_ = buf[(st - 1) & 1][grp];
+					const cplx<T> *iv_ = buf[(st - 1) & 1][fg];
 #pragma unroll
-					for(int u = 0; u < NQ; u++)
+					for(int u = 0; u < NB; u++)
 #pragma unroll
 						for(int r = 0; r < 4; r++)
-							a[u][r] = iv_[tid + 256 * u + r * N4];
+							a[u][r] = iv_[ft + TPT * u + r * N4];
 				}
-				stage(a, st, buf[st & 1][grp]);
+				stage(a, st, buf[st & 1][fg]);
 			}
 			__syncthreads();
 		}
-		/* 2. packed cross spectra of baseline grp: with Z_d the packed
+		/* 2. packed cross spectra of baseline fd: with Z_d the packed
 		 * spectrum of detector d, 2 F_s = Z[f] + conj Z[-f] and
 		 * 2 i F_s+1 = Z[f] - conj Z[-f], so 4 X_s = sb conj(sa) and
 		 * 4 X_s+1 = db conj(da); the transform input is conj(4 Y) */
 		if(iv) {
-			const cplx<T> *Za = buf[(NS - 1) & 1][g.ia[grp]], *Zb = buf[(NS - 1) & 1][g.ib[grp]];
+			const cplx<T> *Za = buf[(NS - 1) & 1][G * ps + g.ia[fd]], *Zb = buf[(NS - 1) & 1][G * ps + g.ib[fd]];
 #pragma unroll
-			for(int u = 0; u < NQ; u++)
+			for(int u = 0; u < NB; u++)
 #pragma unroll
 				for(int r = 0; r < 4; r++) {
-					const int f = tid + 256 * u + r * N4, fn = (N - f) & (N - 1);
+					const int f = ft + TPT * u + r * N4, fn = (N - f) & (N - 1);
 					const cplx<T> za = Za[f], zac = conjg(Za[fn]), zb = Zb[f], zbc = conjg(Zb[fn]);
 					const cplx<T> sa = za + zac, da = za - zac, sb = zb + zbc, db = zb - zbc;
 					const cplx<T> x0 = sb * conjg(sa), x1 = db * conjg(da);
@@ -199,85 +209,106 @@ __global__ void __launch_bounds__(256 * G) k_xcorr(const TI *__restrict__ in, co
 		for(int st = 0; st < NS; st++) {
 			if(iv) {
 				if(st > 0) {
-					const cplx<T> *iv_ = buf[(NS + st - 1) & 1][grp];
+					const cplx<T> *iv_ = buf[(NS + st - 1) & 1][fg];
 #pragma unroll
-					for(int u = 0; u < NQ; u++)
+					for(int u = 0; u < NB; u++)
 #pragma unroll
 						for(int r = 0; r < 4; r++)
-							a[u][r] = iv_[tid + 256 * u + r * N4];
+							a[u][r] = iv_[ft + TPT * u + r * N4];
 				}
-				stage(a, st, buf[(NS + st) & 1][grp]);
+				stage(a, st, buf[(NS + st) & 1][fg]);
 			}
 			__syncthreads();
 		}
-		/* 3. 4 N (c_s - i c_s+1) is in buf[1][grp] ((2 NS - 1) is odd);
-		 * fold the lags into buf[0][grp]: even l at [0, N/2), odd l at
+		/* 3. 4 N (c_s - i c_s+1) is in buf[1][G ps + b] ((2 NS - 1) is
+		 * odd); the 256 threads of baseline pb fold its lags into
+		 * buf[0][G ps + pb] for each pair: even l at [0, N/2), odd l at
 		 * [N/2, N), zero past the last lag */
-		if(iv) {
-			const cplx<T> *y = buf[1][grp];
-			cplx<T> *f0 = buf[0][grp], *f1 = f0 + NH;
-			const int up = g.nch[grp] * R, U1 = g.U1[grp];
-			for(int u = tid; u < up; u += 256) {
-				cplx<T> P = mk((T) 0, (T) 0), Q = P;
-				if(u < U1) {
-					const cplx<T> yu = y[u];
-					if(u == 0) {
-						P = yu;
-					} else {
-						const cplx<T> yn = y[N - u];
-						P = yu + yn;
-						Q = yu - yn;
+		if(pv) {
+			const int up = g.nch[pb] * R, U1 = g.U1[pb];
+#pragma unroll
+			for(int pq = 0; pq < PP; pq++) {
+				const cplx<T> *y = buf[1][G * pq + pb];
+				cplx<T> *f0 = buf[0][G * pq + pb], *f1 = f0 + NH;
+				for(int u = t; u < up; u += 256) {
+					cplx<T> P = mk((T) 0, (T) 0), Q = P;
+					if(u < U1) {
+						const cplx<T> yu = y[u];
+						if(u == 0) {
+							P = yu;
+						} else {
+							const cplx<T> yn = y[N - u];
+							P = yu + yn;
+							Q = yu - yn;
+						}
 					}
+					f0[u] = P;
+					f1[u] = Q;
 				}
-				f0[u] = P;
-				f1[u] = Q;
 			}
 		}
 		__syncthreads();
-		/* 4. this thread's chunk of lags against its table rows */
-		T acc0 = (T) 0, acc1 = (T) 0;
+		/* 4. this thread's chunk of lags against its table rows, for each
+		 * pair of the pass; the chunk sums into buf[1] (y is read) */
+		T acc0[PP], acc1[PP];
+#pragma unroll
+		for(int pq = 0; pq < PP; pq++)
+			acc0[pq] = acc1[pq] = (T) 0;
 		if(act) {
-			const cplx<T> *F = buf[0][grp] + ((l & 1) ? NH : 0) + ch * R;
-			/* opaque per pair: otherwise the loads are lifted out of the
-			 * pair loop and their values held across the transforms */
+			/* opaque per pass: otherwise the loads are lifted out of the
+			 * pass loop and their values held across the transforms */
 			int mo = ch * R * nl + l;
 			asm volatile("" : "+v"(mo));
 			const T *Mt = Mb + mo;
 #pragma unroll
 			for(int j = 0; j < R; j++) {
-				const cplx<T> v = F[j];
 				const T m = MPRE ? mr[MPRE ? j : 0] : Mt[j * nl];
-				acc0 = fma(m, v.re, acc0);
-				acc1 = fma(m, v.im, acc1);
+#pragma unroll
+				for(int pq = 0; pq < PP; pq++) {
+					const cplx<T> v = buf[0][G * pq + pb][((l & 1) ? NH : 0) + ch * R + j];
+					acc0[pq] = fma(m, v.re, acc0[pq]);
+					acc1[pq] = fma(m, v.im, acc1[pq]);
+				}
 			}
 		}
-		if(iv)
-			buf[1][grp][tid] = mk(acc0, acc1);
+		if(pv) {
+#pragma unroll
+			for(int pq = 0; pq < PP; pq++)
+				buf[1][G * pq + pb][t] = mk(acc0[pq], acc1[pq]);
+		}
 		__syncthreads();
-		if(iv) {
-			const int c0 = g.col_start[grp];
-			T *o0 = partial + (size_t) s0 * g.CT + c0, *o1 = o0 + g.CT;
-			if(tid < nl) {
-				T v0 = (T) 0, v1 = (T) 0;
-				for(int c = 0; c < g.nch[grp]; c++) {
-					const cplx<T> v = buf[1][grp][c * nl + tid];
-					v0 += v.re;
-					v1 += v.im;
+		if(pv) {
+			const int c0 = g.col_start[pb];
+#pragma unroll
+			for(int pq = 0; pq < PP; pq++) {
+				const int s = 2 * (PP * pg + pq);
+				if(s >= S)
+					break;
+				const bool h1 = s + 1 < S;
+				T *o0 = partial + (size_t) s * g.CT + c0, *o1 = o0 + g.CT;
+				if(t < nl) {
+					T v0 = (T) 0, v1 = (T) 0;
+					for(int c = 0; c < g.nch[pb]; c++) {
+						const cplx<T> v = buf[1][G * pq + pb][c * nl + t];
+						v0 += v.re;
+						v1 += v.im;
+					}
+					o0[2 * t] = v0;
+					o0[2 * t + 1] = (T) 0;
+					if(h1) {
+						o1[2 * t] = -v1;
+						o1[2 * t + 1] = (T) 0;
+					}
+				} else if(t < nl + g.width[pb] - 2 * nl) {
+					/* the baseline's padding columns */
+					const int c = nl + t;
+					o0[c] = (T) 0;
+					if(h1)
+						o1[c] = (T) 0;
 				}
-				o0[2 * tid] = v0;
-				o0[2 * tid + 1] = (T) 0;
-				if(has1) {
-					o1[2 * tid] = -v1;
-					o1[2 * tid + 1] = (T) 0;
-				}
-			} else if(tid < nl + g.width[grp] - 2 * nl) {
-				/* the baseline's padding columns */
-				const int c = nl + tid;
-				o0[c] = (T) 0;
-				if(has1)
-					o1[c] = (T) 0;
 			}
 		}
+		(void) has0;
 	}
 }
 
@@ -545,7 +576,9 @@ __global__ void __launch_bounds__(768) k_xcorr_w(const TI *__restrict__ in, cons
 
 /* form 1: the Stockham kernel with the table rows in registers, one
  * workgroup per CU; form 2: the same with the rows read from L2, two per
- * CU; form 3: the wave-level kernel (float32, 1024 samples) */
+ * CU; form 3: the wave-level kernel (float32, 1024 samples); form 4: the
+ * Stockham kernel with two pairs per pass, 128 threads per transform
+ * (float32, 1024 samples) */
 template <typename T, typename TI, int NQ, int G>
 static int xc_at(const void *in, const double *window, int S, const struct xcorr_geom *g, const cplx<T> *tw, const cplx<T> *twW, const T *M, T *partial, int form, hipStream_t st)
 {
@@ -558,12 +591,22 @@ static int xc_at(const void *in, const double *window, int S, const struct xcorr
 		}
 		return -1;
 	}
-	const int npairs = (S + 1) / 2, cap = (form == 1 ? 1 : 2) * ncu;
-	const dim3 grid(npairs < cap ? npairs : cap), block(256 * G);
+	const int npairs = (S + 1) / 2;
+	const dim3 block(256 * G);
+	if(form == 4) {
+		if constexpr(sizeof(T) == sizeof(float) && NQ == 1) {
+			const int npg = (npairs + 1) / 2;
+			hipLaunchKernelGGL((k_xcorr<T, TI, NQ, G, true, 2>), dim3(npg < ncu ? npg : ncu), block, 0, st, (const TI *) in, window, S, *g, tw, M, partial);
+			return hipGetLastError() == hipSuccess ? 0 : -1;
+		}
+		return -1;
+	}
+	const int cap = (form == 1 ? 1 : 2) * ncu;
+	const dim3 grid(npairs < cap ? npairs : cap);
 	if(form == 1)
-		hipLaunchKernelGGL((k_xcorr<T, TI, NQ, G, true>), grid, block, 0, st, (const TI *) in, window, S, *g, tw, M, partial);
+		hipLaunchKernelGGL((k_xcorr<T, TI, NQ, G, true, 1>), grid, block, 0, st, (const TI *) in, window, S, *g, tw, M, partial);
 	else
-		hipLaunchKernelGGL((k_xcorr<T, TI, NQ, G, false>), grid, block, 0, st, (const TI *) in, window, S, *g, tw, M, partial);
+		hipLaunchKernelGGL((k_xcorr<T, TI, NQ, G, false, 1>), grid, block, 0, st, (const TI *) in, window, S, *g, tw, M, partial);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -583,14 +626,14 @@ int xcorr_forms(int N, int D, int nb)
 	if(nb < 1 || nb > D || D < 2 || D > SPHR_XC_MAXB)
 		return 0;
 	if(N == 1024)
-		return sizeof(T) == sizeof(float) ? 7 : 3;
+		return sizeof(T) == sizeof(float) ? 15 : 3;
 	return N == 4096 && D == 2 && sizeof(T) == sizeof(float) ? 3 : 0;
 }
 
 template <typename T>
 int launch_xcorr(const void *in, int in_f32, const double *window, int S, int N, const struct xcorr_geom *g, const cplx<T> *tw, const cplx<T> *twW, const T *M, T *partial, int form, hipStream_t st)
 {
-	if(form < 1 || form > 3 || !(xcorr_forms<T>(N, g->D, g->nb) & (1 << (form - 1))))
+	if(form < 1 || form > 4 || !(xcorr_forms<T>(N, g->D, g->nb) & (1 << (form - 1))))
 		return -1;
 	if(N == 1024)
 		return g->D == 2 ? xc_in<T, 1, 2>(in, in_f32, window, S, g, tw, twW, M, partial, form, st) : xc_in<T, 1, 3>(in, in_f32, window, S, g, tw, twW, M, partial, form, st);

@@ -316,9 +316,9 @@ def test_batch_lag_route_matches_oracle(oracle, three_site, precision, nseg):
     cdt = np.complex128 if prec == S.FP64 else np.complex64
     b = S.Batch(three_site, 32, prec, max_segments=nseg)
     assert b.lag
-    # form 1 is the default; the wave-level form (3) is compiled for mixed
-    # precision
-    forms = (1, 2, 3) if prec == S.MIXED else (1, 2)
+    # form 1 is the default; the wave-level form (3) and the form with two
+    # pairs per pass (4) are compiled for mixed precision
+    forms = (1, 2, 3, 4) if prec == S.MIXED else (1, 2)
     assert b.info()["xcorr_form"] == forms[0]
     d_ts = S.DeviceBuffer.from_numpy(ts.astype(np.float64))
     d_w = S.DeviceBuffer.from_numpy(w)
@@ -348,7 +348,7 @@ def test_batch_lag_route_matches_oracle(oracle, three_site, precision, nseg):
     alm, mp = run()
     assert rel(alm, alm_fd) < (1e-13 if prec == S.FP64 else 2e-6)
     with pytest.raises(RuntimeError):
-        b.set_xcorr_form(4 if prec == S.MIXED else 3)
+        b.set_xcorr_form(5 if prec == S.MIXED else 3)
 
 
 def test_batch_lag_route_two_site_mixed(oracle, two_site):
