@@ -597,8 +597,13 @@ def bench_copy(S, args):
 # dispatches on this workload, scaled by the counters' reading of a 256 MiB
 # copy (FETCH_SIZE reads half the copied bytes, WRITE_SIZE all of them);
 # the summary with the commands is profiles/three_site_l32_traffic.md.
-SKY_FUSED_F32_WRITE = 52168 * 1024
-SKY_FUSED_F32_READ = 6602 * 1024 * 2
+SKY_FUSED_F32_WRITE = 49048 * 1024
+SKY_FUSED_F32_READ = 1781 * 1024 * 2
+# the lag-domain kernel (float32, two pairs per pass) on the same workload:
+# the series, the lag table and its register spills in, one row of partial
+# sums per segment and the spills out
+XCORR_F32_WRITE = 3264 * 1024
+XCORR_F32_READ = 13063 * 1024 * 2
 
 
 def sky_fused_traffic():
@@ -695,6 +700,16 @@ def main():
     # a_lm + map out, the delay-product table shared by the batch)
     b_map = 3 * N * 8 + ncoef * 2 * esz + nmesh * 2 * esz + info["table_bytes"] / nseg
     step_gbs = b_map * nseg / (rmx["ms_per_step"] * 1e-3) / 1e9
+    # the lag-domain kernel (first stage): series and table in, partial
+    # sums out, against HBM
+    xcorr_line = None
+    if info.get("lag"):
+        xb = sb[0]
+        xt = rmx["stage_ms"][0] * 1e-3
+        xcorr_line = {"kernel": "xcorr (form %d)" % info.get("xcorr_form", 0), "algorithmic_bytes": xb,
+                      "achieved": round(xb / xt / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(xb / xt / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": XCORR_F32_WRITE + XCORR_F32_READ if info.get("xcorr_form") == 4 and nseg == 1024 and L == 32 else None}
     # peak direction: mixed against float64 (both map-seeded + extremize)
     sep = np.degrees(np.arccos(np.clip(np.cos(rmx["peaks"][0]) * np.cos(r64["peaks"][0]) + np.sin(rmx["peaks"][0]) * np.sin(r64["peaks"][0])
                                        * np.cos(rmx["peaks"][1] - r64["peaks"][1]), -1.0, 1.0)))
@@ -738,6 +753,7 @@ def main():
                      "algorithmic_basis": "a_lm + maps written, rotation rows read",
                      "intermediate_read_bytes": partial_rd if names[k] == "sky_fused" else None,
                      "achieved_with_intermediates": round((sky_out + partial_rd) / t_dom / 1e9, 1) if names[k] == "sky_fused" else None},
+        "xcorr_roofline": xcorr_line,
         "step_roofline": {"bytes_per_map": round(b_map, 1), "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4), "roofline_maps_per_s": round(HBM_PEAK_GBS * 1e9 / b_map, 1)},
         "plan_build_s": round(plan_s, 2),
