@@ -11,9 +11,16 @@
  * complex multiply-add, as in the complex product itself.
  *
  * Tiling: workgroup 256 threads = 4 waves in 2 x 2, block tile 128 x 128,
- * wave tile 64 x 64 = 4 x 4 MFMA tiles of 16 x 16; K chunks of KC staged
- * global -> registers -> LDS, the next chunk's loads in flight during the
- * current chunk's MFMAs.  Workgroup ids are
+ * wave tile 64 x 64 = 4 x 4 MFMA tiles of 16 x 16; K chunks of KC (8 in
+ * float64, 32 in float32) staged global -> registers -> LDS through two
+ * LDS buffers: the next chunk's global loads are in flight during the
+ * current chunk's MFMAs and are stored to the other buffer after them, one
+ * barrier per chunk.  Both tiles stay in their global orientation (A as
+ * [row][k], B as [k][col]) with padded rows: the A rows are KC + 2 (float64)
+ * or KC + 4 (float32) elements long, the B rows 128 + 8 or 128 + 16, so
+ * the stores of a wave and the MFMA fragment reads (16 rows x 4 k of A,
+ * 4 k x 16 columns of B) each hit every LDS bank equally (a transposed A
+ * tile put the 16 k of a row on one bank).  Workgroup ids are
  * remapped so the 8 XCDs each work on contiguous column panels of B (each
  * XCD has its own L2).
  *
@@ -39,24 +46,32 @@ template <> struct mf<double> {
 	typedef double v4 __attribute__((ext_vector_type(4)));
 	static __device__ inline v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 	static __device__ inline int row(int q, int e) { return q + 4 * e; }
-	enum { KC = 16 };
+	/* 8: the staging registers of a 16-deep chunk would push the kernel
+	 * past 256 registers per lane (one workgroup per CU) */
+	enum { KC = 8 };
 };
 
 #define GM_BM 128
 #define GM_BN 128
 #define GM_NT 256
 
+template <typename T> struct gm_pad;
+template <> struct gm_pad<float> {
+	enum { A = 4, B = 16 };
+};
+template <> struct gm_pad<double> {
+	enum { A = 2, B = 8 };
+};
+
 template <typename T, typename TA>
 __global__ void __launch_bounds__(GM_NT) k_gemm(int M, int N, int K, const TA *__restrict__ A, long lda, const T *__restrict__ B, long ldb, T *__restrict__ C, long ldc, int tiles_m, int tiles_n)
 {
 	typedef typename mf<T>::v4 v4;
-	const int KC = mf<T>::KC;
-	const int RA = GM_BM * KC / GM_NT, RB = KC * GM_BN / GM_NT;	/* staged elements per thread */
-	/* rows padded to 16 elements past a multiple of 32 (64-bit) or 64
-	 * (32-bit) words: the four 16-lane groups of an operand read land on
-	 * disjoint banks */
-	__shared__ T sA[KC][GM_BM + 16];	/* transposed: [k][row] */
-	__shared__ T sB[KC][GM_BN + 16];
+	constexpr int KC = mf<T>::KC;
+	constexpr int RA = GM_BM * KC / GM_NT, RB = KC * GM_BN / GM_NT;	/* staged elements per thread */
+	constexpr int SA = KC + gm_pad<T>::A, SB = GM_BN + gm_pad<T>::B;
+	__shared__ T sA[2][GM_BM * SA];	/* [row][k] */
+	__shared__ T sB[2][KC * SB];	/* [k][col] */
 	/* XCD-aware remap: hardware places workgroup id on XCD id % 8; give
 	 * each XCD a contiguous range of tiles (column panel major) */
 	const int nwg = tiles_m * tiles_n;
@@ -69,8 +84,7 @@ __global__ void __launch_bounds__(GM_NT) k_gemm(int M, int N, int K, const TA *_
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
 	const int li = lane & 15, lk = lane >> 4;
-	/* global -> register staging of the next chunk while this one's
-	 * MFMAs run; A coalesced along k, B along n */
+	/* global -> register staging; A coalesced along k, B along n */
 	T ra[RA], rb[RB];
 	auto load = [&](int k0) {
 #pragma unroll
@@ -88,46 +102,55 @@ __global__ void __launch_bounds__(GM_NT) k_gemm(int M, int N, int K, const TA *_
 			rb[v] = (gk < K && gn < N) ? B[(size_t) gk * ldb + gn] : (T) 0;
 		}
 	};
-	auto store = [&]() {
+	auto store = [&](int buf) {
 #pragma unroll
 		for(int v = 0; v < RA; v++) {
 			const int i = v * GM_NT + threadIdx.x;
 			const int r = i / KC, c = i - r * KC;
-			sA[c][r] = ra[v];
+			sA[buf][r * SA + c] = ra[v];
 		}
 #pragma unroll
 		for(int v = 0; v < RB; v++) {
 			const int i = v * GM_NT + threadIdx.x;
 			const int r = i / GM_BN, c = i - r * GM_BN;
-			sB[r][c] = rb[v];
+			sB[buf][r * SB + c] = rb[v];
 		}
 	};
 	v4 acc[4][4];
+#pragma unroll
 	for(int a = 0; a < 4; a++)
+#pragma unroll
 		for(int b = 0; b < 4; b++)
 			acc[a][b] = (v4) 0;
 	load(0);
+	store(0);
+	__syncthreads();
+	int cur = 0;
 	for(int k0 = 0; k0 < K; k0 += KC) {
-		store();
-		__syncthreads();
-		if(k0 + KC < K)
+		const bool next = k0 + KC < K;
+		if(next)
 			load(k0 + KC);
+		const T *pa = sA[cur] + (wm + li) * SA + lk;
+		const T *pb = sB[cur] + lk * SB + wn + li;
 #pragma unroll
 		for(int ks = 0; ks < KC; ks += 4) {
 			T af[4], bf[4];
 #pragma unroll
 			for(int a = 0; a < 4; a++)
-				af[a] = sA[ks + lk][wm + 16 * a + li];
+				af[a] = pa[16 * a * SA + ks];
 #pragma unroll
 			for(int b = 0; b < 4; b++)
-				bf[b] = sB[ks + lk][wn + 16 * b + li];
+				bf[b] = pb[ks * SB + 16 * b];
 #pragma unroll
 			for(int a = 0; a < 4; a++)
 #pragma unroll
 				for(int b = 0; b < 4; b++)
 					acc[a][b] = mf<T>::mma(af[a], bf[b], acc[a][b]);
 		}
+		if(next)
+			store(cur ^ 1);
 		__syncthreads();
+		cur ^= 1;
 	}
 	for(int a = 0; a < 4; a++)
 		for(int b = 0; b < 4; b++) {

@@ -1180,7 +1180,7 @@ template <typename T>
 int launch_phi(const cplx<T> *F, int S, const struct synth_geom *g, const struct synth_tables *tb, cplx<T> *map, hipStream_t st)
 {
 	const int n = g->n, nm = 2 * g->L + 1;
-	return launch_gemm_blas<T>(S * n, 2 * n, 2 * nm, (const T *) F, 2L * nm, phi_table<T>::get(tb), 2L * n, (T *) map, 2L * n, st);
+	return launch_gemm<T, T>(S * n, 2 * n, 2 * nm, (const T *) F, 2L * nm, phi_table<T>::get(tb), 2L * n, (T *) map, 2L * n, st);
 }
 
 template int launch_sky_lf<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);

@@ -356,13 +356,10 @@ static int run_products_f64(struct sphr_loc *lc, const void *series, int series_
 {
 	const int N = lc->N, nb = lc->nb, L = lc->L;
 	const int ncoef = (L + 1) * (L + 1);
-	/* 1: spectra, [D E][2 len] x [2 len][2 N] (rocBLAS when the series
-	 * are in the working precision, the library kernel when they are
-	 * widened or narrowed on the way in) */
+	/* 1: spectra, [D E][2 len] x [2 len][2 N] (the series widened or
+	 * narrowed on the way in when they are not in the working precision) */
 	int r;
-	if(series_f64 == (sizeof(T) == 8))
-		r = launch_gemm_blas<T>(lc->D * E, 2 * N, 2 * lc->len, (const T *) series, 2L * lc->len, (const T *) lc->Dx, 2L * N, (T *) lc->F, 2L * N, st);
-	else if(series_f64)
+	if(series_f64)
 		r = launch_gemm<T, double>(lc->D * E, 2 * N, 2 * lc->len, (const double *) series, 2L * lc->len, (const T *) lc->Dx, 2L * N, (T *) lc->F, 2L * N, st);
 	else
 		r = launch_gemm<T, float>(lc->D * E, 2 * N, 2 * lc->len, (const float *) series, 2L * lc->len, (const T *) lc->Dx, 2L * N, (T *) lc->F, 2L * N, st);
@@ -376,8 +373,8 @@ static int run_products_f64(struct sphr_loc *lc, const void *series, int series_
 	hipLaunchKernelGGL((k_loc_xspec<T>), dim3((unsigned) ((nx + 255) / 256)), dim3(256), 0, st, (const cplx<T> *) lc->F, E, N, nb, p, (cplx<T> *) lc->X);
 	if(hipGetLastError() != hipSuccess)
 		return -1;
-	/* 3: skies, [E][2 nb N] x [2 nb N][4 ncoef] (rocBLAS) */
-	if(launch_gemm_blas<T>(E, 4 * ncoef, 2 * nb * N, (const T *) lc->X, 2L * nb * N, (const T *) lc->Wx, 4L * ncoef, (T *) lc->S, 4L * ncoef, st))
+	/* 3: skies, [E][2 nb N] x [2 nb N][4 ncoef] */
+	if(launch_gemm<T, T>(E, 4 * ncoef, 2 * nb * N, (const T *) lc->X, 2L * nb * N, (const T *) lc->Wx, 4L * ncoef, (T *) lc->S, 4L * ncoef, st))
 		return -1;
 	return 0;
 }
