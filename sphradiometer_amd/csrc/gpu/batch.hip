@@ -236,22 +236,25 @@ static int build_lag(struct sphr_batch *bt, const struct sphr_net_desc *d)
 			goto done;
 		bt->m_bytes = total * sizeof(T);
 		if(bt->xc_forms & 4) {
-			/* the wave-level form's twiddles (xcorr_wave_twiddles) */
-			T twl[2 * SPHR_XC_TWW];
+			/* the wave-level form's twiddle tables (SPHR_XC_TWW) */
+			T *twl = (T *) malloc(2 * (size_t) SPHR_XC_TWW * sizeof(T));
 			double *tw64 = (double *) malloc(2 * (size_t) N * sizeof(double));
-			if(!tw64)
-				goto done;
-			sphr_twiddles(N, -1, tw64);
-			for(int k = 0; k < 16; k++)
-				for(int i = 0; i < 64; i++) {
-					const int e1 = (i * k) % N, e2 = (16 * (i & 3) * k) % N;
-					twl[2 * (k * 64 + i)] = (T) tw64[2 * e1];
-					twl[2 * (k * 64 + i) + 1] = (T) tw64[2 * e1 + 1];
-					twl[2 * (1024 + k * 64 + i)] = (T) tw64[2 * e2];
-					twl[2 * (1024 + k * 64 + i) + 1] = (T) tw64[2 * e2 + 1];
-				}
+			int bad = !twl || !tw64;
+			if(!bad) {
+				sphr_twiddles(N, -1, tw64);
+				for(int k = 0; k < 16; k++)
+					for(int i = 0; i < 64; i++) {
+						const int e1 = (i * k) % N, e2 = (16 * (i & 3) * k) % N;
+						twl[2 * (k * 64 + i)] = (T) tw64[2 * e1];
+						twl[2 * (k * 64 + i) + 1] = (T) tw64[2 * e1 + 1];
+						twl[2 * (1024 + k * 64 + i)] = (T) tw64[2 * e2];
+						twl[2 * (1024 + k * 64 + i) + 1] = (T) tw64[2 * e2 + 1];
+					}
+				bad = upload(&bt->twW, twl, 2 * SPHR_XC_TWW);
+			}
+			free(twl);
 			free(tw64);
-			if(upload(&bt->twW, twl, 2 * SPHR_XC_TWW))
+			if(bad)
 				goto done;
 		}
 		bt->lag_ok = bt->lag = 1;

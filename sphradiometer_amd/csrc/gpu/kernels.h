@@ -77,7 +77,7 @@ struct xcorr_geom {
  * into partial[S][CT] (one split: the power_1d of each baseline in its
  * columns, real parts at even columns, zero elsewhere); N is 1024 or 4096;
  * tw the N twiddles exp(-2 pi i k / N); twW the wave-level form's tables
- * (xcorr_wave_twiddles); form 1, 2 or 3 (xcorr.hip); -1 for a form or
+ * (SPHR_XC_TWW complex values); form 1 to 4 (xcorr.hip); -1 for a form or
  * shape that is not compiled */
 template <typename T>
 int launch_xcorr(const void *in, int in_f32, const double *window, int S, int N, const struct xcorr_geom *g, const cplx<T> *tw, const cplx<T> *twW, const T *M, T *partial, int form, hipStream_t st);

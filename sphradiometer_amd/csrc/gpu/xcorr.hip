@@ -21,8 +21,10 @@
  * of the N + 2 of the frequency-domain product, and the spectra never
  * leave the chip.
  *
- * One workgroup of 256 G threads per pair of segments (s, s + 1), G = the
- * detector count (2 or 3), persistent over pairs:
+ * One workgroup of 256 G threads (G = the detector count, 2 or 3),
+ * persistent over passes of PP segment pairs (s, s + 1) each: with PP = 1
+ * each transform runs over 256 threads, with PP = 2 (two pairs at once)
+ * over 128 threads, two butterflies per thread per stage:
  *  1. group d: radix-4 Stockham FFT of z = x_d,s + i x_d,s+1 (windowed),
  *     the series read straight from HBM into registers, the next pair's
  *     samples loaded while this pair is transformed;
@@ -35,7 +37,7 @@
  *     table rows it holds in registers (loaded once per workgroup), the
  *     chunks summed through LDS, power_1d written to the partial-sum
  *     layout of the sky kernel (real part, zero imaginary part).
- * Fourteen barriers per pair; LDS 2 G N complex values.
+ * Fourteen barriers per pass; LDS 2 PP G N complex values.
  */
 #include <hip/hip_runtime.h>
 #include "kernels.h"
@@ -159,7 +161,7 @@ __global__ void __launch_bounds__(256 * G) k_xcorr(const TI *__restrict__ in, co
 					asm volatile("" : "+v"(wt[st][u][0].re), "+v"(wt[st][u][0].im));
 		}
 		const int s0 = 2 * (PP * pg + ps);
-		const bool has0 = s0 < S, has1 = s0 + 1 < S;
+		const bool has1 = s0 + 1 < S;
 		cplx<T> a[NB][4];
 		/* 1. forward transforms, one detector per group */
 		if(fw) {
@@ -308,7 +310,6 @@ __global__ void __launch_bounds__(256 * G) k_xcorr(const TI *__restrict__ in, co
 				}
 			}
 		}
-		(void) has0;
 	}
 }
 

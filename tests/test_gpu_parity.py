@@ -351,6 +351,56 @@ def test_batch_lag_route_matches_oracle(oracle, three_site, precision, nseg):
         b.set_xcorr_form(5 if prec == S.MIXED else 3)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_lag_route_two_detectors_1024(oracle, precision):
+    # two detectors at 1024 samples: the kernel's two-group shape (one
+    # baseline; the second group idle in the inverse transforms), every
+    # form, against the oracle
+    prec, tol = (S.FP64, TOL64) if precision == "fp64" else (S.MIXED, 1e-5)
+    pos = inject.positions("AB")
+    net = S.Network(pos, 1024, DT, 20)
+    ts = inject.two_site(5, 1024, seed=9)
+    cdt = np.complex128 if prec == S.FP64 else np.complex64
+    b = S.Batch(net, 20, prec, max_segments=5)
+    assert b.lag
+    d_ts = S.DeviceBuffer.from_numpy(ts)
+    d_alm = S.DeviceBuffer(5 * 21 ** 2 * np.dtype(cdt).itemsize)
+    d_map = S.DeviceBuffer(5 * 42 ** 2 * np.dtype(cdt).itemsize)
+    onet = oracle.Network(pos, 1024, DT, 20)
+    want = [onet.segment(ts[:, s], 20) for s in range(5)]
+    for form in ((1, 2, 3, 4) if prec == S.MIXED else (1, 2)):
+        b.set_xcorr_form(form)
+        b.run(d_ts.ptr, 5, d_alm.ptr, d_map.ptr)
+        S.synchronize()
+        alm = d_alm.to_numpy(cdt, (5, 21 ** 2))
+        mp = d_map.to_numpy(cdt, (5, 42, 42))
+        for s in range(5):
+            assert rel(alm[s], want[s][0]) < tol, (form, s)
+            assert rel(mp[s], want[s][1]) < (tol if prec == S.FP64 else 5e-6), (form, s)
+
+
+def test_batch_lag_route_refused_when_lags_wrap(oracle):
+    # 256 samples: the delay products' lags (2 transient + 1 of them) do
+    # not fit the segment, so the batch keeps the frequency-domain route,
+    # with the same results as the oracle
+    pos = inject.positions("AB")
+    net = S.Network(pos, 256, DT, 12)
+    onet = oracle.Network(pos, 256, DT, 12)
+    info = onet.info(0)
+    print("transient", info["transient"], flush=True)
+    assert 4 * info["transient"] + 1 > 256
+    ts = inject.two_site(3, 256, seed=4)
+    alm, mp, binfo = _run_batch(net, 12, ts, S.FP64)
+    assert binfo["lag"] == 0 and binfo["xcorr_form"] == 0
+    b = S.Batch(net, 12, S.FP64, max_segments=3)
+    with pytest.raises(ValueError):
+        b.set_lag(True)
+    for s in range(3):
+        sky, mesh = onet.segment(ts[:, s], 12)
+        assert rel(alm[s], sky) < TOL64
+        assert rel(mp[s], mesh) < TOL64
+
+
 def test_batch_lag_route_two_site_mixed(oracle, two_site):
     # two detectors at 4096 samples: the lag-domain route in mixed
     # precision (float64 keeps the frequency-domain route there)
