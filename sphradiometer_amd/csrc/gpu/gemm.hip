@@ -21,8 +21,10 @@
  * the stores of a wave and the MFMA fragment reads (16 rows x 4 k of A,
  * 4 k x 16 columns of B) each hit every LDS bank equally (a transposed A
  * tile put the 16 k of a row on one bank).  Workgroup ids are
- * remapped so the 8 XCDs each work on contiguous column panels of B (each
- * XCD has its own L2).
+ * remapped so the 8 XCDs each work on a contiguous range of tiles (each
+ * XCD has its own L2), ordered in groups of 8 row panels: the workgroups
+ * in flight on an XCD cover 8 row panels x 8 column panels and share them
+ * in its L2.
  *
  * MFMA 16x16x4 operand layout (64 lanes): A[i][k] in lane i + 16 k, B[k][j]
  * in lane j + 16 k; accumulator element e of lane j + 16 q holds D[i][j]
@@ -54,6 +56,7 @@ template <> struct mf<double> {
 #define GM_BM 128
 #define GM_BN 128
 #define GM_NT 256
+#define GM_GM 8		/* row panels per tile group */
 
 template <typename T> struct gm_pad;
 template <> struct gm_pad<float> {
@@ -79,7 +82,13 @@ __global__ void __launch_bounds__(GM_NT) k_gemm(int M, int N, int K, const TA *_
 	int t = id;
 	if(nwg % 8 == 0)
 		t = (id % 8) * (nwg / 8) + id / 8;
-	const int tm = t % tiles_m, tn = t / tiles_m;
+	/* groups of GM_GM row panels side by side, all column tiles of a
+	 * group before the next: the workgroups in flight on an XCD share
+	 * both operands' panels in its L2 */
+	const int gid = t / (GM_GM * tiles_n), first = gid * GM_GM;
+	const int gs = tiles_m - first < GM_GM ? tiles_m - first : GM_GM;
+	const int w = t - gid * GM_GM * tiles_n;
+	const int tm = first + w % gs, tn = w / gs;
 	const int m0 = tm * GM_BM, n0 = tn * GM_BN;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;

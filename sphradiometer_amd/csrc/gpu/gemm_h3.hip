@@ -23,7 +23,11 @@
  * (the next chunk's loads in flight during this chunk's MFMAs) -> LDS
  * (rows padded to 72 halves: the 16-byte fragment reads of 16 rows land
  * on disjoint banks).  Workgroup ids are remapped so each of the 8 XCDs
- * works on a contiguous range of tiles (its own L2).
+ * works on a contiguous range of tiles (its own L2), in groups of 8 row
+ * panels: the 64 workgroups in flight on an XCD cover 8 x 8 tiles, so
+ * each operand panel is read from HBM once per 8 tiles of the other
+ * operand (column-panel order re-read the split A, 115 MB for the
+ * localization sky product, once per column panel: 82 times).
  *
  * MFMA 16x16x32 f16 operand layout (64 lanes): lane i + 16 q holds
  * A[i][8q .. 8q+7] and B[8q .. 8q+7][i]; accumulator element e of lane
@@ -42,6 +46,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 #define H3_BK 64
 #define H3_LD 72	/* LDS row stride in halves */
 #define H3_NT 256
+#define H3_GM 8		/* row panels per tile group */
 
 /* AF32: A is float32 (Af, K columns, split while staged, inverse row
  * scales ira); else the fp16 pair (Ah, Al).  NFAST: the tiles of one row
@@ -58,7 +63,22 @@ __global__ void __launch_bounds__(H3_NT) k_gemm_h3(int M, int N, int K, const _F
 	int t = id;
 	if(nwg % 8 == 0)
 		t = (id % 8) * (nwg / 8) + id / 8;
-	const int tm = NFAST ? t / tiles_n : t % tiles_m, tn = NFAST ? t % tiles_n : t / tiles_m;
+	/* NFAST: the tiles of a row panel side by side; else groups of
+	 * H3_GM row panels side by side, all column tiles of a group before
+	 * the next group, so the workgroups in flight on an XCD share both
+	 * operands' panels in its L2 (A is re-read once per H3_GM x 8
+	 * tiles, not once per column panel) */
+	int tm, tn;
+	if(NFAST) {
+		tm = t / tiles_n;
+		tn = t % tiles_n;
+	} else {
+		const int gid = t / (H3_GM * tiles_n), first = gid * H3_GM;
+		const int gs = tiles_m - first < H3_GM ? tiles_m - first : H3_GM;
+		const int w = t - gid * H3_GM * tiles_n;
+		tm = first + w % gs;
+		tn = w / gs;
+	}
 	const int m0 = tm * H3_BM, n0 = tn * H3_BN;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;

@@ -379,6 +379,32 @@ def test_batch_lag_route_two_detectors_1024(oracle, precision):
             assert rel(mp[s], want[s][1]) < (tol if prec == S.FP64 else 5e-6), (form, s)
 
 
+@pytest.mark.parametrize("lag", [True, False])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_float32_series(oracle, three_site, precision, lag):
+    # float32 input series (the layout the learned proposal's training
+    # batches use), on both projection routes, against the oracle on the
+    # same float32 values widened
+    prec, tol = (S.FP64, TOL64) if precision == "fp64" else (S.MIXED, 1e-5)
+    ts, _ = inject.three_site(6, 1024, seed=77)
+    ts32 = ts.astype(np.float32)
+    cdt = np.complex128 if prec == S.FP64 else np.complex64
+    b = S.Batch(three_site, 32, prec, max_segments=6)
+    b.set_lag(lag)
+    d_ts = S.DeviceBuffer.from_numpy(ts32)
+    d_alm = S.DeviceBuffer(6 * 33 ** 2 * np.dtype(cdt).itemsize)
+    d_map = S.DeviceBuffer(6 * 66 ** 2 * np.dtype(cdt).itemsize)
+    b.run(d_ts.ptr, 6, d_alm.ptr, d_map.ptr, f32_input=True)
+    S.synchronize()
+    alm = d_alm.to_numpy(cdt, (6, 33 ** 2))
+    mp = d_map.to_numpy(cdt, (6, 66, 66))
+    onet = oracle.Network(inject.positions("ABC"), 1024, DT, 32)
+    for s in (0, 3, 5):
+        sky, mesh = onet.segment(ts32[:, s].astype(np.float64), 32)
+        assert rel(alm[s], sky) < tol, s
+        assert rel(mp[s], mesh) < (tol if prec == S.FP64 else 5e-6), s
+
+
 def test_batch_lag_route_refused_when_lags_wrap(oracle):
     # 256 samples: the delay products' lags (2 transient + 1 of them) do
     # not fit the segment, so the batch keeps the frequency-domain route,
