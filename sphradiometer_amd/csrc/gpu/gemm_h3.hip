@@ -200,6 +200,172 @@ __global__ void __launch_bounds__(H3_NT) k_gemm_h3(int M, int N, int K, const _F
 }
 
 /*
+ * The phi DFT of the meshes with the half-turn split (float32 A, split
+ * while staged, as the AF32 form above).  A mesh has n points
+ * phi_k = 2 pi k / n, so exp(i m phi_{k + n/2}) = (-1)^m exp(i m phi_k):
+ * with Ev and Od the sums over the even and the odd orders,
+ *     map[r][k] = Ev[r][k] + Od[r][k],  map[r][k + n/2] = Ev[r][k] - Od[r][k],
+ * k < n/2.  A row of A holds the ring sums of orders m = 0..L, -L..-1 as
+ * (re, im) pairs; the loader gathers one parity class at a time: class
+ * index i < n0 at column c0 + 2 i, the rest at c1 + 2 (i - n0) (the
+ * orders m >= 0 of the class ascending, then m < 0).  Be and Bo [NH][ldb]
+ * are the two half-turn DFT matrices (NH = n real columns: n/2 points x
+ * re/im) over the classes in that order, transposed and split per column.
+ * One workgroup per 128-row panel
+ * computes both sums over all NH (<= 128) columns and writes both halves
+ * of each map row: half the MFMA work of the full [2 nm] x [2 n] product,
+ * one workgroup per panel instead of two.
+ */
+struct phi_cls {
+	int K, c0, n0, c1;	/* real K of the class; its column map */
+};
+
+template <typename TO>
+__global__ void __launch_bounds__(H3_NT) k_phi_h3(int M, int NH, struct phi_cls ce, struct phi_cls co, const float *__restrict__ Af, long lda, const float *__restrict__ ira,
+						  const _Float16 *__restrict__ Beh, const _Float16 *__restrict__ Bel, const float *__restrict__ icbe,
+						  const _Float16 *__restrict__ Boh, const _Float16 *__restrict__ Bol, const float *__restrict__ icbo, long ldb,
+						  TO *__restrict__ C, long ldc, int tiles_m)
+{
+	__shared__ __align__(16) _Float16 sA[2][H3_BM * H3_LD];	/* [hi/lo][row][k] */
+	__shared__ __align__(16) _Float16 sB[2][H3_BN * H3_LD];	/* [hi/lo][col][k] */
+	const int id = blockIdx.x;
+	int t = id;
+	if(tiles_m % 8 == 0)
+		t = (id % 8) * (tiles_m / 8) + id / 8;
+	const int m0 = t * H3_BM;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+	const int li = lane & 15, lq = lane >> 4;
+	h8 ra[2][4], rb[2][4];
+	float2 rf[4][4];
+	float rs[4];
+	const int vr = threadIdx.x >> 3, vc = (threadIdx.x & 7) * 8;	/* row 0..31 (+32 per step), k offset */
+#pragma unroll
+	for(int q = 0; q < 4; q++) {
+		const int gm = m0 + vr + 32 * q;
+		rs[q] = gm < M ? 1.0f / ira[gm] : 0.f;
+	}
+	/* the chunk at k0 of a parity class: its complex index i = (k0 + vc)
+	 * / 2 + u from column c0 + 2 i or c1 + 2 (i - n0) */
+	auto load = [&](const struct phi_cls &c, int k0, const _Float16 *__restrict__ Bh, const _Float16 *__restrict__ Bl) {
+#pragma unroll
+		for(int q = 0; q < 4; q++) {
+			const int r = vr + 32 * q;
+			const int gm = m0 + r;
+			const float *a = Af + (size_t) gm * lda;
+#pragma unroll
+			for(int u = 0; u < 4; u++) {
+				const int i = (k0 + vc) / 2 + u;
+				const int col = i < c.n0 ? c.c0 + 2 * i : c.c1 + 2 * (i - c.n0);
+				rf[q][u] = gm < M && 2 * i < c.K ? *reinterpret_cast<const float2 *>(a + 2 * col) : make_float2(0.f, 0.f);
+			}
+			if(r < NH) {
+				rb[0][q] = *reinterpret_cast<const h8 *>(Bh + (size_t) r * ldb + k0 + vc);
+				rb[1][q] = *reinterpret_cast<const h8 *>(Bl + (size_t) r * ldb + k0 + vc);
+			} else {
+				rb[0][q] = (h8) 0;
+				rb[1][q] = (h8) 0;
+			}
+		}
+	};
+	auto store = [&]() {
+#pragma unroll
+		for(int q = 0; q < 4; q++)
+#pragma unroll
+			for(int u = 0; u < 4; u++) {
+				const float v0 = rf[q][u].x * rs[q], v1 = rf[q][u].y * rs[q];
+				const _Float16 h0 = (_Float16) v0, h1 = (_Float16) v1;
+				ra[0][q][2 * u] = h0;
+				ra[0][q][2 * u + 1] = h1;
+				ra[1][q][2 * u] = (_Float16) (v0 - (float) h0);
+				ra[1][q][2 * u + 1] = (_Float16) (v1 - (float) h1);
+			}
+#pragma unroll
+		for(int p = 0; p < 2; p++)
+#pragma unroll
+			for(int q = 0; q < 4; q++) {
+				const int r = vr + 32 * q;
+				*reinterpret_cast<h8 *>(&sA[p][r * H3_LD + vc]) = ra[p][q];
+				*reinterpret_cast<h8 *>(&sB[p][r * H3_LD + vc]) = rb[p][q];
+			}
+	};
+	auto mma = [&](f4 (&acc)[4][4]) {
+#pragma unroll
+		for(int kk = 0; kk < H3_BK; kk += 32) {
+			h8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+			for(int a = 0; a < 4; a++) {
+				const int r = (wm + 16 * a + li) * H3_LD + kk + 8 * lq;
+				ah[a] = *reinterpret_cast<const h8 *>(&sA[0][r]);
+				al[a] = *reinterpret_cast<const h8 *>(&sA[1][r]);
+			}
+#pragma unroll
+			for(int b = 0; b < 4; b++) {
+				const int r = (wn + 16 * b + li) * H3_LD + kk + 8 * lq;
+				bh[b] = *reinterpret_cast<const h8 *>(&sB[0][r]);
+				bl[b] = *reinterpret_cast<const h8 *>(&sB[1][r]);
+			}
+#pragma unroll
+			for(int a = 0; a < 4; a++)
+#pragma unroll
+				for(int b = 0; b < 4; b++) {
+					acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+					acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+					acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+				}
+		}
+	};
+	f4 ae[4][4], ao[4][4];
+#pragma unroll
+	for(int a = 0; a < 4; a++)
+#pragma unroll
+		for(int b = 0; b < 4; b++) {
+			ae[a][b] = (f4) 0;
+			ao[a][b] = (f4) 0;
+		}
+	/* the even orders, then the odd ones, each into its own sums (two
+	 * loops, so each MFMA chain stays on fixed registers) */
+	load(ce, 0, Beh, Bel);
+	for(int k0 = 0; k0 < ce.K; k0 += H3_BK) {
+		store();
+		__syncthreads();
+		if(k0 + H3_BK < ce.K)
+			load(ce, k0 + H3_BK, Beh, Bel);
+		else
+			load(co, 0, Boh, Bol);
+		mma(ae);
+		__syncthreads();
+	}
+	for(int k0 = 0; k0 < co.K; k0 += H3_BK) {
+		store();
+		__syncthreads();
+		if(k0 + H3_BK < co.K)
+			load(co, k0 + H3_BK, Boh, Bol);
+		mma(ao);
+		__syncthreads();
+	}
+#pragma unroll
+	for(int b = 0; b < 4; b++) {
+		const int gn = wn + 16 * b + li;
+		if(gn >= NH)
+			continue;
+		const float ce = icbe[gn], co = icbo[gn];
+#pragma unroll
+		for(int a = 0; a < 4; a++)
+#pragma unroll
+			for(int e = 0; e < 4; e++) {
+				const int gm = m0 + wm + 16 * a + 4 * lq + e;
+				if(gm < M) {
+					const float ir = ira[gm];
+					const float ev = ae[a][b][e] * (ir * ce), od = ao[a][b][e] * (ir * co);
+					C[(size_t) gm * ldc + gn] = (TO) (ev + od);
+					C[(size_t) gm * ldc + NH + gn] = (TO) (ev - od);
+				}
+			}
+	}
+}
+
+/*
  * Row split of a real matrix X [rows][K] (float or double, leading dimension
  * ldx) into the fp16 pair and the inverse row scale, one workgroup per row;
  * columns K .. Kp-1 of the outputs are zero.
@@ -266,6 +432,24 @@ int launch_gemm_h3a(int M, int N, int K, const float *A, long lda, const float *
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <typename TO>
+int launch_phi_h3(int M, int L, const float *A, long lda, const float *ira, const _Float16 *Beh, const _Float16 *Bel, const float *icbe, const _Float16 *Boh, const _Float16 *Bol, const float *icbo, long ldb, TO *C, long ldc, hipStream_t st)
+{
+	if(M <= 0)
+		return 0;
+	const int nm = 2 * L + 1, NH = 2 * (L + 1), ev = L / 2 * 2, om = (L & 1) ? L : L - 1;
+	/* even orders: 0, 2, .., ev at columns 0, 2, ..; -ev, .., -2 at
+	 * nm - ev, ..; odd orders: 1, 3, .., om at 1, 3, ..; -om, .., -1 at
+	 * nm - om, .. */
+	const struct phi_cls ce = { 2 * (ev + 1), 0, ev / 2 + 1, nm - ev };
+	const struct phi_cls co = { 2 * (nm - ev - 1), 1, (om + 1) / 2, nm - om };
+	if(L < 1 || NH > H3_BN || lda < 2 * nm || ldb % 8 || ldb < (ce.K + H3_BK - 1) / H3_BK * H3_BK || ldb < (co.K + H3_BK - 1) / H3_BK * H3_BK)
+		return -1;
+	const int tm = (M + H3_BM - 1) / H3_BM;
+	hipLaunchKernelGGL((k_phi_h3<TO>), dim3(tm), dim3(H3_NT), 0, st, M, NH, ce, co, A, lda, ira, Beh, Bel, icbe, Boh, Bol, icbo, ldb, C, ldc, tm);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <typename TI>
 int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *Xh, _Float16 *Xl, long ldh, float *irs, hipStream_t st)
 {
@@ -280,6 +464,7 @@ int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *
 template int launch_gemm_h3<float>(int, int, int, const _Float16 *, const _Float16 *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, float *, long, hipStream_t);
 template int launch_gemm_h3<double>(int, int, int, const _Float16 *, const _Float16 *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, double *, long, hipStream_t);
 template int launch_gemm_h3a<float>(int, int, int, const float *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, float *, long, hipStream_t);
+template int launch_phi_h3<float>(int, int, const float *, long, const float *, const _Float16 *, const _Float16 *, const float *, const _Float16 *, const _Float16 *, const float *, long, float *, long, hipStream_t);
 template int launch_split_rows<float>(int, int, int, const float *, long, _Float16 *, _Float16 *, long, float *, hipStream_t);
 template int launch_split_rows<double>(int, int, int, const double *, long, _Float16 *, _Float16 *, long, float *, hipStream_t);
 

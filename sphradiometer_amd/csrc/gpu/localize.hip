@@ -48,6 +48,11 @@ struct sphr_loc {
 	_Float16 *Ph, *Pl;	/* [2 n][Kp]: the phi DFT matrix transposed, per-column scale */
 	float *icp;		/* [2 n] */
 	float *irr;		/* [2 max_events n]: inverse split scales of the ring sums */
+	/* the half-turn form of the phi DFT (L + 1 <= 64): the even and odd
+	 * orders' DFT matrices to the first n/2 points, [n][Kph] each */
+	int ht, Kph;
+	_Float16 *Peh, *Pel, *Poh, *Pol;
+	float *icpe, *icpo;
 	struct synth_dev syn;
 };
 
@@ -229,7 +234,7 @@ extern "C" void sphr_loc_free(struct sphr_loc *lc)
 	if(!lc)
 		return;
 	void *p[] = { lc->Wx, lc->Dx, lc->prior, lc->F, lc->X, lc->S, lc->Fr, lc->Wh, lc->Wl, lc->icw, lc->Dh, lc->Dl, lc->icd, lc->Ah, lc->Al, lc->ira, lc->Xh, lc->Xl, lc->irx,
-		      lc->Ph, lc->Pl, lc->icp, lc->irr };
+		      lc->Ph, lc->Pl, lc->icp, lc->irr, lc->Peh, lc->Pel, lc->Poh, lc->Pol, lc->icpe, lc->icpo };
 	for(unsigned i = 0; i < sizeof(p) / sizeof(p[0]); i++)
 		if(p[i])
 			hipFree(p[i]);
@@ -294,6 +299,28 @@ static int create_t(struct sphr_loc *lc, const struct sphr_loc_desc *d)
 		}
 		lc->Kp = (2 * nm + 63) / 64 * 64;
 		rc = upload_split_cols(ph, nm, n, lc->Kp, &lc->Ph, &lc->Pl, &lc->icp);
+		/* the half-turn matrices: exp(i m phi_k) for k < n/2 over the even
+		 * orders and over the odd ones, each class as m >= 0 ascending
+		 * then m < 0 ascending (the order launch_phi_h3 gathers) */
+		if(!rc && n <= 128) {
+			const int ev = L / 2 * 2, om = (L & 1) ? L : L - 1, NE = ev + 1, NO = nm - NE;
+			lc->Kph = ((2 * (NE > NO ? NE : NO)) + 63) / 64 * 64;
+			for(int par = 0; par < 2 && !rc; par++) {
+				const int cnt = par ? NO : NE;
+				for(int i = 0; i < cnt; i++) {
+					const int np = par ? (om + 1) / 2 : ev / 2 + 1;	/* orders >= 0 of the class */
+					const int m = i < np ? (par ? 1 : 0) + 2 * i : (par ? -om : -ev) + 2 * (i - np);
+					for(int k = 0; k < n / 2; k++) {
+						const long r = (((long) k * m) % n + n) % n;
+						ph[2 * ((size_t) i * (n / 2) + k)] = tw[2 * r];
+						ph[2 * ((size_t) i * (n / 2) + k) + 1] = tw[2 * r + 1];
+					}
+				}
+				rc = par ? upload_split_cols(ph, NO, n / 2, lc->Kph, &lc->Poh, &lc->Pol, &lc->icpo)
+					 : upload_split_cols(ph, NE, n / 2, lc->Kph, &lc->Peh, &lc->Pel, &lc->icpe);
+			}
+			lc->ht = !rc;
+		}
 		free(ph);
 		free(tw);
 		if(rc || sphr_rt_check(hipMalloc((void **) &lc->irr, 2 * E * (size_t) n * sizeof(float)), "hipMalloc"))
@@ -398,6 +425,10 @@ static int run_epilogue(struct sphr_loc *lc, int E, const double *gmst, void *al
 	 * split scale from the ring-sum kernel, the split done while staging */
 	const int n = 2 * (L + 1), nm = 2 * L + 1;
 	if(launch_ring_sums<T>(NULL, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) lc->Fr, st, sizeof(T) == 4 ? lc->irr : NULL, &epi) == 0) {
+		/* mixed precision at orders whose half turn fits one tile: the
+		 * half-turn phi DFT */
+		if(sizeof(T) == 4 && lc->ht)
+			return launch_phi_h3<float>(2 * E * n, L, (const float *) lc->Fr, 2L * nm, lc->irr, lc->Peh, lc->Pel, lc->icpe, lc->Poh, lc->Pol, lc->icpo, lc->Kph, (float *) map, 2L * n, st);
 		if(sizeof(T) == 4)
 			return launch_gemm_h3a<float>(2 * E * n, 2 * n, 2 * nm, (const float *) lc->Fr, 2L * nm, lc->irr, lc->Ph, lc->Pl, lc->Kp, lc->icp, (float *) map, 2L * n, st);
 		return launch_phi<T>((const cplx<T> *) lc->Fr, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st);
