@@ -211,90 +211,120 @@ __global__ void __launch_bounds__(H3_NT) k_gemm_h3(int M, int N, int K, const _F
  * orders m >= 0 of the class ascending, then m < 0).  Be and Bo [NH][ldb]
  * are the two half-turn DFT matrices (NH = n real columns: n/2 points x
  * re/im) over the classes in that order, transposed and split per column.
- * One workgroup per 128-row panel
- * computes both sums over all NH (<= 128) columns and writes both halves
- * of each map row: half the MFMA work of the full [2 nm] x [2 n] product,
- * one workgroup per panel instead of two.
+ *
+ * One workgroup per 64-row panel computes both sums over all NH (<= 128)
+ * columns and writes both halves of each map row: half the MFMA work of
+ * the full [2 nm] x [2 n] product.  4 waves in 2 x 2, wave tile 32 x 64,
+ * so the two classes' accumulators take 64 registers and two workgroups
+ * share a CU (one's map stores overlap the other's loads and MFMAs).  The
+ * K chunks alternate between the classes (even 0, odd 0, even 1, ..):
+ * chunk c of either class reads the same stretch of each A row, so the
+ * second read finds it in cache.
  */
+#define PH_BM 64
+#define PH_NT 256
+/* LDS: the operand tiles, or the map rows of a panel at NH = 128 */
+#define PH_SMEM (PH_BM * 2 * 128 * 4 > 2 * (PH_BM + H3_BN) * H3_LD * 2 ? PH_BM * 2 * 128 * 4 : 2 * (PH_BM + H3_BN) * H3_LD * 2)
+
 struct phi_cls {
 	int K, c0, n0, c1;	/* real K of the class; its column map */
 };
 
-template <typename TO>
-__global__ void __launch_bounds__(H3_NT) k_phi_h3(int M, int NH, struct phi_cls ce, struct phi_cls co, const float *__restrict__ Af, long lda, const float *__restrict__ ira,
-						  const _Float16 *__restrict__ Beh, const _Float16 *__restrict__ Bel, const float *__restrict__ icbe,
-						  const _Float16 *__restrict__ Boh, const _Float16 *__restrict__ Bol, const float *__restrict__ icbo, long ldb,
-						  TO *__restrict__ C, long ldc, int tiles_m)
+__global__ void __launch_bounds__(PH_NT, 2) k_phi_h3(int M, int NH, struct phi_cls ce, struct phi_cls co, const float *__restrict__ Af, long lda, const float *__restrict__ ira,
+						     const _Float16 *__restrict__ Beh, const _Float16 *__restrict__ Bel, const float *__restrict__ icbe,
+						     const _Float16 *__restrict__ Boh, const _Float16 *__restrict__ Bol, const float *__restrict__ icbo, long ldb,
+						     float *__restrict__ C, int tiles_m)
 {
-	__shared__ __align__(16) _Float16 sA[2][H3_BM * H3_LD];	/* [hi/lo][row][k] */
-	__shared__ __align__(16) _Float16 sB[2][H3_BN * H3_LD];	/* [hi/lo][col][k] */
+	/* the operand tiles, then the panel's map rows [PH_BM][2 NH] */
+	__shared__ __align__(16) char smem[PH_SMEM];
+	_Float16 (*sA)[PH_BM * H3_LD] = reinterpret_cast<_Float16 (*)[PH_BM * H3_LD]>(smem);	/* [hi/lo][row][k] */
+	_Float16 (*sB)[H3_BN * H3_LD] = reinterpret_cast<_Float16 (*)[H3_BN * H3_LD]>(smem + 2 * PH_BM * H3_LD * sizeof(_Float16));	/* [hi/lo][col][k] */
+	float *sC = reinterpret_cast<float *>(smem);
 	const int id = blockIdx.x;
 	int t = id;
 	if(tiles_m % 8 == 0)
 		t = (id % 8) * (tiles_m / 8) + id / 8;
-	const int m0 = t * H3_BM;
+	const int m0 = t * PH_BM;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+	const int wm = (wave & 1) * 32, wn = (wave >> 1) * 64;
 	const int li = lane & 15, lq = lane >> 4;
-	h8 ra[2][4], rb[2][4];
-	float2 rf[4][4];
-	float rs[4];
+	h8 rb[2][4];
+	float2 rfe[2][4], rfo[2][4];	/* the next chunks of A of either class */
+	float rs[2];
 	const int vr = threadIdx.x >> 3, vc = (threadIdx.x & 7) * 8;	/* row 0..31 (+32 per step), k offset */
 #pragma unroll
-	for(int q = 0; q < 4; q++) {
+	for(int q = 0; q < 2; q++) {
 		const int gm = m0 + vr + 32 * q;
 		rs[q] = gm < M ? 1.0f / ira[gm] : 0.f;
 	}
-	/* the chunk at k0 of a parity class: its complex index i = (k0 + vc)
-	 * / 2 + u from column c0 + 2 i or c1 + 2 (i - n0) */
-	auto load = [&](const struct phi_cls &c, int k0, const _Float16 *__restrict__ Bh, const _Float16 *__restrict__ Bl) {
+	/* A at k0 of a parity class: its complex index i = (k0 + vc) / 2 + u
+	 * from column c0 + 2 i or c1 + 2 (i - n0).  Every load is issued,
+	 * from a clamped address, and the values out of range are dropped
+	 * when the chunk is stored (bit u of km: column u in range; rs: zero
+	 * for rows past M): with no branch around a load, the wait for one
+	 * chunk leaves the loads issued after it in flight */
+	auto load_a = [&](float2 (&rf)[2][4], int &km, const struct phi_cls &c, int k0) {
+		int col[4];
+		km = 0;
+#pragma unroll
+		for(int u = 0; u < 4; u++) {
+			const int i = (k0 + vc) / 2 + u;
+			const bool ok = 2 * i < c.K;
+			km |= (int) ok << u;
+			const int cp = c.c0 + 2 * i, cn = c.c1 + 2 * (i - c.n0);
+			col[u] = ok ? (i < c.n0 ? cp : cn) : 0;
+		}
+#pragma unroll
+		for(int q = 0; q < 2; q++) {
+			const int gm = m0 + vr + 32 * q;
+			const float *a = Af + (size_t) (gm < M ? gm : M - 1) * lda;
+#pragma unroll
+			for(int u = 0; u < 4; u++)
+				rf[q][u] = *reinterpret_cast<const float2 *>(a + 2 * col[u]);
+		}
+	};
+	/* B rows past NH (clamped to row 0 here) are stored as zeros */
+	auto load_b = [&](int k0, const _Float16 *__restrict__ Bh, const _Float16 *__restrict__ Bl) {
 #pragma unroll
 		for(int q = 0; q < 4; q++) {
 			const int r = vr + 32 * q;
-			const int gm = m0 + r;
-			const float *a = Af + (size_t) gm * lda;
-#pragma unroll
-			for(int u = 0; u < 4; u++) {
-				const int i = (k0 + vc) / 2 + u;
-				const int col = i < c.n0 ? c.c0 + 2 * i : c.c1 + 2 * (i - c.n0);
-				rf[q][u] = gm < M && 2 * i < c.K ? *reinterpret_cast<const float2 *>(a + 2 * col) : make_float2(0.f, 0.f);
-			}
-			if(r < NH) {
-				rb[0][q] = *reinterpret_cast<const h8 *>(Bh + (size_t) r * ldb + k0 + vc);
-				rb[1][q] = *reinterpret_cast<const h8 *>(Bl + (size_t) r * ldb + k0 + vc);
-			} else {
-				rb[0][q] = (h8) 0;
-				rb[1][q] = (h8) 0;
-			}
+			const size_t o = (size_t) (r < NH ? r : 0) * ldb + k0 + vc;
+			rb[0][q] = *reinterpret_cast<const h8 *>(Bh + o);
+			rb[1][q] = *reinterpret_cast<const h8 *>(Bl + o);
 		}
 	};
-	auto store = [&]() {
+	auto store = [&](const float2 (&rf)[2][4], int km) {
 #pragma unroll
-		for(int q = 0; q < 4; q++)
+		for(int q = 0; q < 2; q++) {
+			h8 hi, lo;
 #pragma unroll
 			for(int u = 0; u < 4; u++) {
-				const float v0 = rf[q][u].x * rs[q], v1 = rf[q][u].y * rs[q];
+				const float sc = (km >> u) & 1 ? rs[q] : 0.f;
+				const float v0 = rf[q][u].x * sc, v1 = rf[q][u].y * sc;
 				const _Float16 h0 = (_Float16) v0, h1 = (_Float16) v1;
-				ra[0][q][2 * u] = h0;
-				ra[0][q][2 * u + 1] = h1;
-				ra[1][q][2 * u] = (_Float16) (v0 - (float) h0);
-				ra[1][q][2 * u + 1] = (_Float16) (v1 - (float) h1);
+				hi[2 * u] = h0;
+				hi[2 * u + 1] = h1;
+				lo[2 * u] = (_Float16) (v0 - (float) h0);
+				lo[2 * u + 1] = (_Float16) (v1 - (float) h1);
 			}
+			const int r = vr + 32 * q;
+			*reinterpret_cast<h8 *>(&sA[0][r * H3_LD + vc]) = hi;
+			*reinterpret_cast<h8 *>(&sA[1][r * H3_LD + vc]) = lo;
+		}
 #pragma unroll
 		for(int p = 0; p < 2; p++)
 #pragma unroll
 			for(int q = 0; q < 4; q++) {
 				const int r = vr + 32 * q;
-				*reinterpret_cast<h8 *>(&sA[p][r * H3_LD + vc]) = ra[p][q];
-				*reinterpret_cast<h8 *>(&sB[p][r * H3_LD + vc]) = rb[p][q];
+				*reinterpret_cast<h8 *>(&sB[p][r * H3_LD + vc]) = r < NH ? rb[p][q] : (h8) 0;
 			}
 	};
-	auto mma = [&](f4 (&acc)[4][4]) {
+	auto mma = [&](f4 (&acc)[2][4]) {
 #pragma unroll
 		for(int kk = 0; kk < H3_BK; kk += 32) {
-			h8 ah[4], al[4], bh[4], bl[4];
+			h8 ah[2], al[2], bh[4], bl[4];
 #pragma unroll
-			for(int a = 0; a < 4; a++) {
+			for(int a = 0; a < 2; a++) {
 				const int r = (wm + 16 * a + li) * H3_LD + kk + 8 * lq;
 				ah[a] = *reinterpret_cast<const h8 *>(&sA[0][r]);
 				al[a] = *reinterpret_cast<const h8 *>(&sA[1][r]);
@@ -306,7 +336,7 @@ __global__ void __launch_bounds__(H3_NT) k_phi_h3(int M, int NH, struct phi_cls 
 				bl[b] = *reinterpret_cast<const h8 *>(&sB[1][r]);
 			}
 #pragma unroll
-			for(int a = 0; a < 4; a++)
+			for(int a = 0; a < 2; a++)
 #pragma unroll
 				for(int b = 0; b < 4; b++) {
 					acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
@@ -315,54 +345,71 @@ __global__ void __launch_bounds__(H3_NT) k_phi_h3(int M, int NH, struct phi_cls 
 				}
 		}
 	};
-	f4 ae[4][4], ao[4][4];
+	f4 ae[2][4], ao[2][4];
 #pragma unroll
-	for(int a = 0; a < 4; a++)
+	for(int a = 0; a < 2; a++)
 #pragma unroll
 		for(int b = 0; b < 4; b++) {
 			ae[a][b] = (f4) 0;
 			ao[a][b] = (f4) 0;
 		}
-	/* the even orders, then the odd ones, each into its own sums (two
-	 * loops, so each MFMA chain stays on fixed registers) */
-	load(ce, 0, Beh, Bel);
-	for(int k0 = 0; k0 < ce.K; k0 += H3_BK) {
-		store();
+	/* chunks even 0, odd 0, even 1, ..; nc chunks of each class (the
+	 * odd class's last one may be all padding).  A is loaded two chunks
+	 * ahead (one chunk of each class in flight), B one chunk ahead from
+	 * L2; each chunk's B loads go out before the next A chunk's, so the
+	 * wait for a chunk leaves the later A loads in flight.  The last pass
+	 * loads its own chunk again instead of branching around the loads */
+	const int ne = (ce.K + H3_BK - 1) / H3_BK, no = (co.K + H3_BK - 1) / H3_BK;
+	const int nc = ne > no ? ne : no;
+	int kme, kmo;
+	load_b(0, Beh, Bel);
+	load_a(rfe, kme, ce, 0);
+	load_a(rfo, kmo, co, 0);
+	for(int c = 0; c < nc; c++) {
+		const int kn = (c + 1 < nc ? c + 1 : c) * H3_BK;
+		store(rfe, kme);
 		__syncthreads();
-		if(k0 + H3_BK < ce.K)
-			load(ce, k0 + H3_BK, Beh, Bel);
-		else
-			load(co, 0, Boh, Bol);
+		load_b(c * H3_BK, Boh, Bol);
+		load_a(rfe, kme, ce, kn);
 		mma(ae);
 		__syncthreads();
-	}
-	for(int k0 = 0; k0 < co.K; k0 += H3_BK) {
-		store();
+		store(rfo, kmo);
 		__syncthreads();
-		if(k0 + H3_BK < co.K)
-			load(co, k0 + H3_BK, Boh, Bol);
+		load_b(kn, Beh, Bel);
+		load_a(rfo, kmo, co, kn);
 		mma(ao);
 		__syncthreads();
 	}
+	/* the map rows into LDS (the loop's last barrier has freed the
+	 * operand tiles), then out as one contiguous block: C rows are 2 NH
+	 * floats with no gap, so the panel's rows m0 .. m0 + 63 are
+	 * 64 x 8 NH bytes in a row, written whole 16 bytes per lane (every
+	 * cache line of the block written once, by one workgroup) */
+	const int W = 2 * NH;
 #pragma unroll
 	for(int b = 0; b < 4; b++) {
 		const int gn = wn + 16 * b + li;
 		if(gn >= NH)
 			continue;
-		const float ce = icbe[gn], co = icbo[gn];
+		const float se = icbe[gn], so = icbo[gn];
 #pragma unroll
-		for(int a = 0; a < 4; a++)
+		for(int a = 0; a < 2; a++)
 #pragma unroll
 			for(int e = 0; e < 4; e++) {
-				const int gm = m0 + wm + 16 * a + 4 * lq + e;
-				if(gm < M) {
-					const float ir = ira[gm];
-					const float ev = ae[a][b][e] * (ir * ce), od = ao[a][b][e] * (ir * co);
-					C[(size_t) gm * ldc + gn] = (TO) (ev + od);
-					C[(size_t) gm * ldc + NH + gn] = (TO) (ev - od);
-				}
+				const int r = wm + 16 * a + 4 * lq + e;
+				const float ir = ira[m0 + r < M ? m0 + r : M - 1];
+				const float ev = ae[a][b][e] * (ir * se), od = ao[a][b][e] * (ir * so);
+				sC[r * W + gn] = ev + od;
+				sC[r * W + NH + gn] = ev - od;
 			}
 	}
+	__syncthreads();
+	const int rv = M - m0 < PH_BM ? M - m0 : PH_BM;
+	const int n4 = rv * W / 4;
+	float4 *dst = reinterpret_cast<float4 *>(C + (size_t) m0 * W);
+	const float4 *src = reinterpret_cast<const float4 *>(sC);
+	for(int j = threadIdx.x; j < n4; j += PH_NT)
+		dst[j] = src[j];
 }
 
 /*
@@ -432,8 +479,7 @@ int launch_gemm_h3a(int M, int N, int K, const float *A, long lda, const float *
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <typename TO>
-int launch_phi_h3(int M, int L, const float *A, long lda, const float *ira, const _Float16 *Beh, const _Float16 *Bel, const float *icbe, const _Float16 *Boh, const _Float16 *Bol, const float *icbo, long ldb, TO *C, long ldc, hipStream_t st)
+int launch_phi_h3(int M, int L, const float *A, long lda, const float *ira, const _Float16 *Beh, const _Float16 *Bel, const float *icbe, const _Float16 *Boh, const _Float16 *Bol, const float *icbo, long ldb, float *C, hipStream_t st)
 {
 	if(M <= 0)
 		return 0;
@@ -443,10 +489,12 @@ int launch_phi_h3(int M, int L, const float *A, long lda, const float *ira, cons
 	 * nm - om, .. */
 	const struct phi_cls ce = { 2 * (ev + 1), 0, ev / 2 + 1, nm - ev };
 	const struct phi_cls co = { 2 * (nm - ev - 1), 1, (om + 1) / 2, nm - om };
-	if(L < 1 || NH > H3_BN || lda < 2 * nm || ldb % 8 || ldb < (ce.K + H3_BK - 1) / H3_BK * H3_BK || ldb < (co.K + H3_BK - 1) / H3_BK * H3_BK)
+	/* both classes run max(ne, no) chunks: B rows hold that many */
+	const int ne = (ce.K + H3_BK - 1) / H3_BK, no = (co.K + H3_BK - 1) / H3_BK;
+	if(L < 1 || NH > H3_BN || lda < 2 * nm || ldb % 8 || ldb < (ne > no ? ne : no) * H3_BK)
 		return -1;
-	const int tm = (M + H3_BM - 1) / H3_BM;
-	hipLaunchKernelGGL((k_phi_h3<TO>), dim3(tm), dim3(H3_NT), 0, st, M, NH, ce, co, A, lda, ira, Beh, Bel, icbe, Boh, Bol, icbo, ldb, C, ldc, tm);
+	const int tm = (M + PH_BM - 1) / PH_BM;
+	hipLaunchKernelGGL(k_phi_h3, dim3(tm), dim3(PH_NT), 0, st, M, NH, ce, co, A, lda, ira, Beh, Bel, icbe, Boh, Bol, icbo, ldb, C, tm);
 	return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -464,7 +512,6 @@ int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *
 template int launch_gemm_h3<float>(int, int, int, const _Float16 *, const _Float16 *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, float *, long, hipStream_t);
 template int launch_gemm_h3<double>(int, int, int, const _Float16 *, const _Float16 *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, double *, long, hipStream_t);
 template int launch_gemm_h3a<float>(int, int, int, const float *, long, const float *, const _Float16 *, const _Float16 *, long, const float *, float *, long, hipStream_t);
-template int launch_phi_h3<float>(int, int, const float *, long, const float *, const _Float16 *, const _Float16 *, const float *, const _Float16 *, const _Float16 *, const float *, long, float *, long, hipStream_t);
 template int launch_split_rows<float>(int, int, int, const float *, long, _Float16 *, _Float16 *, long, float *, hipStream_t);
 template int launch_split_rows<double>(int, int, int, const double *, long, _Float16 *, _Float16 *, long, float *, hipStream_t);
 

@@ -186,10 +186,9 @@ int launch_gemm_h3a(int M, int N, int K, const float *A, long lda, const float *
  * inverse row scales ira; Be, Bo [NH = 2 (L + 1)][ldb] the split DFT
  * matrices of the even and the odd orders (each class as m >= 0
  * ascending, then m < 0 ascending) to the first L + 1 points, with column
- * scales; C [M][ldc] gets Ev + Od in columns 0..NH-1 and Ev - Od in
- * NH..2NH-1; -1 for L + 1 > 64 */
-template <typename TO>
-int launch_phi_h3(int M, int L, const float *A, long lda, const float *ira, const _Float16 *Beh, const _Float16 *Bel, const float *icbe, const _Float16 *Boh, const _Float16 *Bol, const float *icbo, long ldb, TO *C, long ldc, hipStream_t st);
+ * scales; C [M][2 NH] (rows with no gap) gets Ev + Od in columns 0..NH-1
+ * and Ev - Od in NH..2NH-1; -1 for L + 1 > 64 */
+int launch_phi_h3(int M, int L, const float *A, long lda, const float *ira, const _Float16 *Beh, const _Float16 *Bel, const float *icbe, const _Float16 *Boh, const _Float16 *Bol, const float *icbo, long ldb, float *C, hipStream_t st);
 /* rows of X [rows][K] -> fp16 pairs scaled per row (columns K..Kp-1 zero) */
 template <typename TI>
 int launch_split_rows(int rows, int K, int Kp, const TI *X, long ldx, _Float16 *Xh, _Float16 *Xl, long ldh, float *irs, hipStream_t st);

@@ -428,7 +428,7 @@ static int run_epilogue(struct sphr_loc *lc, int E, const double *gmst, void *al
 		/* mixed precision at orders whose half turn fits one tile: the
 		 * half-turn phi DFT */
 		if(sizeof(T) == 4 && lc->ht)
-			return launch_phi_h3<float>(2 * E * n, L, (const float *) lc->Fr, 2L * nm, lc->irr, lc->Peh, lc->Pel, lc->icpe, lc->Poh, lc->Pol, lc->icpo, lc->Kph, (float *) map, 2L * n, st);
+			return launch_phi_h3(2 * E * n, L, (const float *) lc->Fr, 2L * nm, lc->irr, lc->Peh, lc->Pel, lc->icpe, lc->Poh, lc->Pol, lc->icpo, lc->Kph, (float *) map, st);
 		if(sizeof(T) == 4)
 			return launch_gemm_h3a<float>(2 * E * n, 2 * n, 2 * nm, (const float *) lc->Fr, 2L * nm, lc->irr, lc->Ph, lc->Pl, lc->Kp, lc->icp, (float *) map, 2L * n, st);
 		return launch_phi<T>((const cplx<T> *) lc->Fr, 2 * E, &lc->syn.g, &lc->syn.tb, (cplx<T> *) map, st);
