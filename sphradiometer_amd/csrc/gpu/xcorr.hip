@@ -97,11 +97,6 @@ __global__ void __launch_bounds__(256 * G) k_xcorr(const TI *__restrict__ in, co
 	constexpr int MRR = MPRE ? SPHR_XC_MR : 1;
 	T mr[MRR];
 	const T *Mb = M + (pv ? g.moff[pb] : 0);
-	if(MPRE) {
-#pragma unroll
-		for(int j = 0; j < MRR; j++)
-			mr[j] = act ? Mb[(ch * R + j) * nl + l] : (T) 0;
-	}
 	/* samples q + r N/4 of detector fd, segments 2 pr and 2 pr + 1 of
 	 * pair pr = PP pg + ps (a missing pair reads the last one) */
 	TI xv[NB][4], yv[NB][4];
@@ -147,6 +142,15 @@ __global__ void __launch_bounds__(256 * G) k_xcorr(const TI *__restrict__ in, co
 	};
 	if(fw && (int) blockIdx.x < npg)
 		load(blockIdx.x);
+	/* the table rows after the first pair's samples, with no branch
+	 * around the loads (threads without a chunk read a valid row and never
+	 * use it): the first transform waits for the samples only, the rows
+	 * arrive during the transforms */
+	if(MPRE) {
+#pragma unroll
+		for(int j = 0; j < MRR; j++)
+			mr[j] = Mb[act ? (ch * R + j) * nl + l : 0];
+	}
 	for(int pg = blockIdx.x; pg < npg; pg += gridDim.x) {
 		/* the thread index and the twiddles enter each pass through an
 		 * empty asm, so per-thread index arithmetic and derived twiddles
