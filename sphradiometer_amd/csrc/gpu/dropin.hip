@@ -63,7 +63,8 @@ int synth_tables_build(int L, struct synth_dev *t)
 	sphr_mesh_rule(L, &x, NULL);
 	/* every table in one device allocation, filled by one copy:
 	 * x [B], Pbar_mm [B][L+1], alpha [nal], twiddles [2n], cm [L+1],
-	 * the expanded phi-DFT matrix [2 (2L+1)][2 n] in float64 then float32 */
+	 * the expanded phi-DFT matrix [2 (2L+1)][2 n] in float64 then float32,
+	 * the ring sums' recurrence pairs [nal + 2] in float64 then float32 */
 	const size_t nal = (size_t) (L + 1) * (L + 2) / 2;
 	const int nm = 2 * L + 1;
 	const size_t ne = (size_t) 2 * nm * 2 * n;
@@ -71,7 +72,8 @@ int synth_tables_build(int L, struct synth_dev *t)
 	const size_t o_x = 0, o_s = up(o_x + B * sizeof(double)), o_a = up(o_s + (size_t) B * (L + 1) * sizeof(double));
 	const size_t o_t = up(o_a + nal * sizeof(double)), o_c = up(o_t + 2 * (size_t) n * sizeof(double));
 	const size_t o_e = up(o_c + (L + 1) * sizeof(double)), o_f = up(o_e + ne * sizeof(double));
-	const size_t total = o_f + ne * sizeof(float);
+	const size_t o_r = up(o_f + ne * sizeof(float)), o_q = up(o_r + (nal + 2) * 2 * sizeof(double));
+	const size_t total = o_q + (nal + 2) * 2 * sizeof(float);
 	char *hb = (char *) calloc(total, 1);
 	double *tmp = (double *) malloc((L + 1) * sizeof(double));
 	int rc = -1;
@@ -118,6 +120,18 @@ int synth_tables_build(int L, struct synth_dev *t)
 		}
 		for(size_t q = 0; q < ne; q++)
 			e32[q] = (float) e64[q];
+		/* (alpha_l, alpha_l / alpha_{l-1}) per (l, m >= 0) slot, zero
+		 * where l = m (and the second where l = m + 1) */
+		double *hr = (double *) (hb + o_r);
+		float *hq = (float *) (hb + o_q);
+		for(int m = 0; m <= L; m++)
+			for(int l = m + 1; l <= L; l++) {
+				const size_t t = lm_offset(L, l, m);
+				hr[2 * t] = ha[t];
+				hr[2 * t + 1] = l > m + 1 ? ha[t] / ha[t - 1] : 0.0;
+			}
+		for(size_t q = 0; q < 2 * (nal + 2); q++)
+			hq[q] = (float) hr[q];
 	}
 	if(sphr_rt_check(hipMalloc(&t->blob, total), "hipMalloc") || sphr_rt_check(hipMemcpy(t->blob, hb, total, hipMemcpyHostToDevice), "hipMemcpy"))
 		goto done;
@@ -130,6 +144,8 @@ int synth_tables_build(int L, struct synth_dev *t)
 		t->cm = (double *) (d + o_c);
 		t->E64 = (double *) (d + o_e);
 		t->E32 = (float *) (d + o_f);
+		t->tb.rc64 = (const cplx<double> *) (d + o_r);
+		t->tb.rc32 = (const cplx<float> *) (d + o_q);
 	}
 	t->tb.x = t->x;
 	t->tb.smm = t->smm;

@@ -34,6 +34,10 @@ struct synth_tables {
 	const double *alpha;	/* alpha_lm of the l recurrence at the (l, m >= 0)
 				 * slot of the coefficient layout */
 	const double *twp;	/* 2n: exp(+2 pi i t / n) */
+	/* the ring sums' recurrence pairs (alpha_l, alpha_l / alpha_{l-1}) at
+	 * the (l, m >= 0) slots, two zero pads: (nal + 2) complex values */
+	const cplx<double> *rc64;
+	const cplx<float> *rc32;
 	const double *E64;	/* [2 (2L+1)][2 n]: the phi DFT, real-expanded */
 	const float *E32;
 	/* the fused sky kernel's tables at this order, NULL where it is not

@@ -1029,21 +1029,21 @@ __device__ inline void ring_sums(const cplx<T> *sa, const cplx<T> *cf, int L, in
 }
 
 /* the recurrence pairs (alpha_l, alpha_l / alpha_{l-1}) of ring_sums at
- * the (l, m >= 0) slots, two zero pads */
+ * the (l, m >= 0) slots, two zero pads, from the order's table into LDS */
+template <typename T> struct ring_table;
+template <> struct ring_table<double> {
+	static __device__ inline const cplx<double> *get(const struct synth_tables &tb) { return tb.rc64; }
+};
+template <> struct ring_table<float> {
+	static __device__ inline const cplx<float> *get(const struct synth_tables &tb) { return tb.rc32; }
+};
 template <typename T>
-__device__ inline void ring_coefs(int L, const double *__restrict__ alpha, cplx<T> *cf)
+__device__ inline void ring_coefs(int L, const struct synth_tables &tb, cplx<T> *cf)
 {
-	const int nal = (L + 1) * (L + 2) / 2;
-	for(int t = threadIdx.x; t < nal + 2; t += NT) {
-		cplx<T> c = mk((T) 0, (T) 0);
-		if(t < nal) {
-			int l, m;
-			lm_decode(L, t, &l, &m);
-			const double al = l > m ? alpha[t] : 0.0;
-			c = mk((T) al, (T) (l > m + 1 ? al / alpha[t - 1] : 0.0));
-		}
-		cf[t] = c;
-	}
+	const int n = (L + 1) * (L + 2) / 2 + 2;
+	const cplx<T> *__restrict__ rc = ring_table<T>::get(tb);
+	for(int t = threadIdx.x; t < n; t += NT)
+		cf[t] = rc[t];
 }
 
 /*
@@ -1070,7 +1070,7 @@ __global__ void __launch_bounds__(NT) k_sky_lf(const T *__restrict__ partial, in
 			v += partial[((size_t) sp * S + s) * xg.CT + c];
 		pw[c] = v;
 	}
-	ring_coefs<T>(L, tb.alpha, cf);
+	ring_coefs<T>(L, tb, cf);
 	__syncthreads();
 	double gs = 0.0;
 	if(gmst)
@@ -1137,7 +1137,7 @@ __global__ void __launch_bounds__(NT) k_alm_lf(const cplx<T> *__restrict__ alm, 
 	} else
 		for(int t = threadIdx.x; t < ncoef; t += NT)
 			sa[t] = alm[(size_t) s * ncoef + t];
-	ring_coefs<T>(L, tb.alpha, cf);
+	ring_coefs<T>(L, tb, cf);
 	if(irs)
 		for(int t = threadIdx.x; t < n; t += NT)
 			rmax[t] = 0;
