@@ -85,6 +85,16 @@ __device__ inline void hp_ring(int nside, int i, double *z, double *st, int *nph
 #define HP_RC 16		/* rings per chunk */
 #define HP_SPW 4		/* series per workgroup of the per-ring form */
 
+/* sin and cos of pi a / b in the map's precision */
+__device__ inline void hp_sincospi(int a, int b, double *sn, double *cs)
+{
+	sincospi((double) a / b, sn, cs);
+}
+__device__ inline void hp_sincospi(int a, int b, float *sn, float *cs)
+{
+	sincospif((float) a / (float) b, sn, cs);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(HP_NT) k_healpix(const cplx<T> *__restrict__ alm, int n, int L, int nside, const double *__restrict__ cf, const double *__restrict__ cm, T *__restrict__ map)
 {
@@ -137,10 +147,13 @@ __global__ void __launch_bounds__(HP_NT) k_healpix(const cplx<T> *__restrict__ a
 				int nphi;
 				long start;
 				hp_ring(nside, r0 + rr, &z, &st, &nphi, &start, &phi0);
-				/* Pbar_mm = cm[m] sin^m theta by repeated products */
-				double p2 = cm[m];
-				for(int k = 0; k < m; k++)
-					p2 *= st;
+				/* Pbar_mm = cm[m] sin^m theta, the power by squaring */
+				double p2 = cm[m], b = st;
+				for(int e = m; e; e >>= 1) {
+					if(e & 1)
+						p2 *= b;
+					b *= b;
+				}
 				const cplx<T> *ap = a + lm_offset(L, m, m);
 				const cplx<T> *an = m ? a + lm_offset(L, m, -m) : ap;
 				/* even (e) and odd (o) l - m sums of +m (p) and -m (n) */
@@ -217,15 +230,17 @@ __global__ void __launch_bounds__(HP_NT) k_healpix(const cplx<T> *__restrict__ a
 		 *   q = 0: S0 + S1 + S2 + S3,   q = 1: S0 - T1 - S2 + T3,
 		 *   q = 2: S0 - S1 + S2 - S3,   q = 3: S0 + T1 - S2 - T3 */
 		const int ne = ne_s;
+		/* u only grows, so each thread's ring search resumes where its
+		 * last one stopped */
+		int k = 0;
 		for(int u = threadIdx.x; u < eq[ne]; u += HP_NT) {
-			int k = 0;
 			while(k + 1 < ne && u >= eq[k + 1])
 				k++;
 			const int j = u - eq[k], n4 = en[k] / 4;
-			double sn, cs;
-			sincospi(2. * j / en[k], &sn, &cs);
+			T sn, cs;
+			hp_sincospi(2 * j, en[k], &sn, &cs);
 			const T *ab = AB + 2 * ea[k] * L1;
-			const T er = (T) cs, ei = (T) sn;
+			const T er = cs, ei = sn;
 			T cr = er, ci = ei;	/* exp(i m phi') for m = 1 */
 			T s0 = ab[0], s1 = 0, s2 = 0, s3 = 0, t1 = 0, t3 = 0;
 			for(int m = 1; m <= L; m++) {
