@@ -380,6 +380,9 @@ __global__ void __launch_bounds__(HP_NT) k_healpix_ring(const cplx<T> *__restric
 template <typename T>
 __global__ void __launch_bounds__(HP_NT) k_healpix_posterior(const T *__restrict__ maps, int E, long npix, T *__restrict__ prob)
 {
+	/* exp(x - max) in the maps' precision (float32 maps: float32
+	 * exponentials, float64 sums) */
+	auto ex = [](T x, double m) -> double { return sizeof(T) == sizeof(float) ? (double) expf((float) ((double) x - m)) : exp((double) x - m); };
 	__shared__ double red[2][HP_NT / 64];
 	const int e = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const T *mp = maps + (size_t) e * npix, *mn = maps + ((size_t) E + e) * npix;
@@ -406,8 +409,8 @@ __global__ void __launch_bounds__(HP_NT) k_healpix_posterior(const T *__restrict
 	__syncthreads();
 	double sp = 0., sn = 0.;
 	for(long p = threadIdx.x; p < npix; p += HP_NT) {
-		sp += exp((double) mp[p] - xp);
-		sn += exp((double) mn[p] - xn);
+		sp += ex(mp[p], xp);
+		sn += ex(mn[p], xn);
 	}
 	for(int o = 32; o; o >>= 1) {
 		sp += __shfl_xor(sp, o);
@@ -427,7 +430,7 @@ __global__ void __launch_bounds__(HP_NT) k_healpix_posterior(const T *__restrict
 	const double ip = 0.5 / sp, in = 0.5 / sn;
 	T *pr = prob + (size_t) e * npix;
 	for(long p = threadIdx.x; p < npix; p += HP_NT)
-		pr[p] = (T) (exp((double) mp[p] - xp) * ip + exp((double) mn[p] - xn) * in);
+		pr[p] = (T) (ex(mp[p], xp) * ip + ex(mn[p], xn) * in);
 }
 
 /* ---------------------------------------------------------------- C ABI */
