@@ -225,11 +225,13 @@ void sphr_jobs_info(const struct sphr_jobs *, long info[8]);
 void sphr_jobs_offsets(const struct sphr_jobs *, long *in, long *alm, long *map);
 
 /* Autotuner: runs the batch on these buffers with each candidate shape of
- * the projection (tiles per wave, segments per workgroup, K split; on the
- * lag-domain route its kernel forms instead), keeps
- * the fastest, then tries the fused sky kernel's other forms (two
- * workgroups per CU; one segment per workgroup at three per CU; four
- * segments per workgroup, mixed precision);
+ * the projection (tiles per wave, segments per workgroup, K split) and,
+ * while the lag-domain route is in use, with each of that route's kernel
+ * forms too, and keeps the fastest (so the route may change to the
+ * frequency-domain one; a batch set to the frequency-domain route keeps
+ * it), then tries the fused sky kernel's other forms (two workgroups per
+ * CU; one segment per workgroup at three per CU; four segments per
+ * workgroup, mixed precision);
  * returns the projection shape's index (time per run in best_ms), or -1
  * with the previous settings kept */
 int sphr_batch_tune(struct sphr_batch *, const void *tseries, int tseries_is_f32, int n_segments, const double *window, const double *gmst, void *alm, void *map, double *best_ms);

@@ -666,7 +666,7 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	}
 	hipStream_t st = sphr_rt_stream();
 	const int kps0 = bt->k_per_split, ns0 = bt->nsplit, tpw0 = bt->tpw, xms0 = bt->xms, occ0 = bt->sky_occ, nsub0 = bt->sky_nsub;
-	const int shape0 = bt->shape, xform0 = bt->xc_form;
+	const int shape0 = bt->shape, xform0 = bt->xc_form, lag0 = bt->lag;
 	const int prof = bt->profile;
 	hipEvent_t e0 = NULL, e1 = NULL;
 	float best = 1e30f;
@@ -696,25 +696,31 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 	 * its default form, then the sky kernel's other forms at the best */
 	bt->sky_occ = 1;
 	bt->sky_nsub = 0;
-	if(bt->lag) {
-		/* the lag-domain route: its kernel forms */
-		int bx = xform0;
-		for(int x = 1; x <= 4; x++) {
-			if(!(bt->xc_forms & (1 << (x - 1))))
-				continue;
-			bt->xc_form = x;
-			float ms = 0.f;
-			if(timed(&ms))
-				goto fail;
-			if(ms < best) {
-				best = ms;
-				bx = x;
+	{
+		/* with the lag-domain route in use: its kernel forms, then the
+		 * frequency-domain route's projection shapes, and the faster
+		 * route stays (few segments leave the lag kernel, one segment
+		 * pair per workgroup, with too few workgroups); with the
+		 * frequency-domain route selected, its shapes only */
+		int bx = xform0, lag_best = 0;
+		if(lag0) {
+			bt->lag = 1;
+			for(int x = 1; x <= 4; x++) {
+				if(!(bt->xc_forms & (1 << (x - 1))))
+					continue;
+				bt->xc_form = x;
+				float ms = 0.f;
+				if(timed(&ms))
+					goto fail;
+				if(ms < best) {
+					best = ms;
+					bx = x;
+					lag_best = 1;
+				}
 			}
 		}
 		bt->xc_form = bx;
-		bo = shape0 / SPHR_TUNE_W;
-		bw = shape0 % SPHR_TUNE_W;
-	} else {
+		bt->lag = 0;
 		for(int o = 0; o < 4; o++)
 			for(int w = 0; w < SPHR_TUNE_W; w++) {
 				set_shape(bt, o, w);
@@ -725,9 +731,15 @@ extern "C" int sphr_batch_tune(struct sphr_batch *bt, const void *ts, int f32, i
 					best = ms;
 					bo = o;
 					bw = w;
+					lag_best = 0;
 				}
 			}
+		if(bo < 0) {
+			bo = shape0 / SPHR_TUNE_W;
+			bw = shape0 % SPHR_TUNE_W;
+		}
 		set_shape(bt, bo, bw);
+		bt->lag = lag_best;
 	}
 	if(bt->fused && sky_fused_order(bt->L) && map) {
 		/* (workgroups per CU, segments per workgroup) */
@@ -768,6 +780,7 @@ fail:
 	bt->sky_nsub = nsub0;
 	bt->shape = shape0;
 	bt->xc_form = xform0;
+	bt->lag = lag0;
 	bt->profile = prof;
 	return -1;
 }

@@ -446,6 +446,38 @@ def test_batch_lag_route_two_site_mixed(oracle, two_site):
     assert bm.lag
 
 
+def test_batch_tuner_routes_match_oracle(oracle, two_site):
+    # the autotuner times both projection routes while the lag-domain
+    # route is in use and keeps the faster (16 segments of 4096 samples:
+    # eight segment pairs, eight workgroups of the lag kernel); whichever
+    # it keeps matches the oracle, and a batch set to the
+    # frequency-domain route stays there
+    ts = inject.two_site(16, 4096, seed=3)
+    w = inject.hann(4096)
+    d_ts = S.DeviceBuffer.from_numpy(ts)
+    d_w = S.DeviceBuffer.from_numpy(w)
+    d_alm = S.DeviceBuffer(16 * 26 ** 2 * 8)
+    d_map = S.DeviceBuffer(16 * 52 ** 2 * 8)
+    onet = oracle.Network(inject.positions("AB"), 4096, DT, 25)
+    want = [onet.segment(ts[:, s], 25, window=w) for s in (0, 9, 15)]
+    for lag in (True, False):
+        b = S.Batch(two_site, 25, S.MIXED, max_segments=16)
+        b.set_lag(lag)
+        k, ms = b.tune(d_ts.ptr, 16, d_alm.ptr, d_map.ptr, d_window=d_w.ptr)
+        info = b.info()
+        print("tuned two-site: lag %s -> %d, shape %d, %.4f ms, xcorr form %d" % (lag, info["lag"], k, ms, info["xcorr_form"]), flush=True)
+        assert 0 <= k < 28 and ms > 0
+        if not lag:
+            assert info["lag"] == 0
+        b.run(d_ts.ptr, 16, d_alm.ptr, d_map.ptr, d_window=d_w.ptr)
+        S.synchronize()
+        alm = d_alm.to_numpy(np.complex64, (16, 26 ** 2))
+        mp = d_map.to_numpy(np.complex64, (16, 52, 52))
+        for (sky, mesh), s in zip(want, (0, 9, 15)):
+            assert rel(alm[s], sky) < 1e-5, (lag, s)
+            assert rel(mp[s], mesh) < 1e-5, (lag, s)
+
+
 def test_batch_full_size_three_site_properties(three_site):
     # the 1024-segment workload: golden segments, linearity (x -> 2x gives
     # 4x power), and a real sky for real data
