@@ -311,12 +311,14 @@ def _run_loc(loc, series, gmst, L, cdt):
 
 
 @pytest.mark.gpu
-def test_gpu_batched_localization_matches_oracle(oracle, local_minimum):
-    L = 22
+@pytest.mark.parametrize("L", [22, 21])
+def test_gpu_batched_localization_matches_oracle(oracle, local_minimum, L):
+    # both parities of l_max: the mixed meshes' half-turn phi DFT gathers
+    # the even and the odd orders from the ring sums by different maps
     netp, netn = lib_network(1.0), lib_network(-1.0)
     netp.set_l(L)
     netn.set_l(L)
-    onp, onn = small_network(oracle, 1.0), small_network(oracle, -1.0)
+    onp, onn = small_network(oracle, 1.0, L), small_network(oracle, -1.0, L)
     lp = oracle.log_uniformsky_prior(L)
     x, meta = inject.snr_events(6, length=21, delta_t=DT_SMALL, seed=1)
     for prec, cdt, tol in ((S.FP64, np.complex128, 1e-12), (S.MIXED, np.complex64, 1e-5)):
