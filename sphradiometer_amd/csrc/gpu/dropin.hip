@@ -47,7 +47,7 @@ int sphr_factor(int n, int *fac, int max)
 
 /* ---- synthesis tables (host build, device copy) ---- */
 
-int synth_tables_build(int L, struct synth_dev *t)
+int synth_tables_build(int L, struct synth_dev *t, int gemm_tables)
 {
 	memset(t, 0, sizeof(*t));
 	const int B = L + 1, n = 2 * B;
@@ -67,7 +67,7 @@ int synth_tables_build(int L, struct synth_dev *t)
 	 * the ring sums' recurrence pairs [nal + 2] in float64 then float32 */
 	const size_t nal = (size_t) (L + 1) * (L + 2) / 2;
 	const int nm = 2 * L + 1;
-	const size_t ne = (size_t) 2 * nm * 2 * n;
+	const size_t ne = gemm_tables ? (size_t) 2 * nm * 2 * n : 0;
 	auto up = [](size_t b) { return (b + 255) & ~(size_t) 255; };
 	const size_t o_x = 0, o_s = up(o_x + B * sizeof(double)), o_a = up(o_s + (size_t) B * (L + 1) * sizeof(double));
 	const size_t o_t = up(o_a + nal * sizeof(double)), o_c = up(o_t + 2 * (size_t) n * sizeof(double));
@@ -106,7 +106,7 @@ int synth_tables_build(int L, struct synth_dev *t)
 		}
 		/* rows 2 i, 2 i + 1 of the phi DFT for column i (m = i for i <= L,
 		 * i - (2L+1) above) */
-		for(int i = 0; i < nm; i++) {
+		for(int i = 0; i < (gemm_tables ? nm : 0); i++) {
 			const int m = i <= L ? i : i - nm;
 			for(int k = 0; k < n; k++) {
 				const long r = (((long) k * m) % n + n) % n;
@@ -142,8 +142,8 @@ int synth_tables_build(int L, struct synth_dev *t)
 		t->alpha = (double *) (d + o_a);
 		t->twp = (double *) (d + o_t);
 		t->cm = (double *) (d + o_c);
-		t->E64 = (double *) (d + o_e);
-		t->E32 = (float *) (d + o_f);
+		t->E64 = gemm_tables ? (double *) (d + o_e) : NULL;
+		t->E32 = gemm_tables ? (float *) (d + o_f) : NULL;
 		t->tb.rc64 = (const cplx<double> *) (d + o_r);
 		t->tb.rc32 = (const cplx<float> *) (d + o_q);
 	}
@@ -153,7 +153,7 @@ int synth_tables_build(int L, struct synth_dev *t)
 	t->tb.twp = t->twp;
 	t->tb.E64 = t->E64;
 	t->tb.E32 = t->E32;
-	if(sky_fused_order(L)) {
+	if(gemm_tables && sky_fused_order(L)) {
 		if(sky_fused_tables(L, &t->tb, &t->fz))
 			goto done;
 		t->tb.bf64 = t->fz.bf64;

@@ -973,7 +973,7 @@ extern "C" struct sphr_jobs *sphr_jobs_new(int n_jobs, const int *n_det, const d
 		const int bad = !ha || !hm || sphr_rt_check(hipMalloc(&C.d_aoff, C.count * sizeof(long)), "hipMalloc") || sphr_rt_check(hipMalloc(&C.d_moff, C.count * sizeof(long)), "hipMalloc")
 				|| sphr_rt_check(hipMemcpy(C.d_aoff, ha, C.count * sizeof(long), hipMemcpyHostToDevice), "hipMemcpy")
 				|| sphr_rt_check(hipMemcpy(C.d_moff, hm, C.count * sizeof(long), hipMemcpyHostToDevice), "hipMemcpy")
-				|| synth_tables_build(C.L, &C.syn);
+				|| synth_tables_build(C.L, &C.syn, 0);
 		free(ha);
 		free(hm);
 		if(bad) {

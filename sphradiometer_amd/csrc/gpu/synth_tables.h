@@ -26,7 +26,10 @@ struct synth_dev {
 /* the l_seed low-pass mesh of the extremizer's seed search */
 #define SPHR_L_SEED 6
 
-int synth_tables_build(int L, struct synth_dev *t);
+/* gemm_tables = 0: without the phi-DFT matrices of the GEMM-form
+ * synthesis and the fused sky kernel's tables (callers that synthesise
+ * with launch_synth only) */
+int synth_tables_build(int L, struct synth_dev *t, int gemm_tables = 1);
 void synth_tables_free(struct synth_dev *t);
 
 #endif
