@@ -631,7 +631,7 @@ extern "C" void sphr_jobs_free(struct sphr_jobs *J)
  * end.
  */
 template <typename T>
-static int generate_plans(struct sphr_jobs *J, hipStream_t st)
+static int generate_plans(struct sphr_jobs *J, hipStream_t st, int (*host_work)(struct sphr_jobs *))
 {
 	const size_t budget = (size_t) 1 << 30;
 	const int nbl = J->n_bl;
@@ -811,6 +811,10 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st)
 		if(sphr_rt_check(hipGetLastError(), "plan kernels"))
 			goto done;
 	}
+	/* host work that does not need the plans runs while the GPU makes
+	 * them */
+	if(host_work && host_work(J))
+		goto done;
 	if(sphr_rt_check(hipStreamSynchronize(st), "plan"))
 		goto done;
 	rc = 0;
@@ -829,6 +833,47 @@ done:
 	free(rule_q);
 	free(ch);
 	return rc;
+}
+
+/* the l_max classes of the synthesis: their job offsets and synthesis
+ * tables (needs the jobs only, not their plans) */
+static int build_classes(struct sphr_jobs *J)
+{
+	const int n_jobs = J->n_jobs;
+	J->cls = (struct jb_class *) calloc(n_jobs, sizeof(struct jb_class));
+	if(!J->cls)
+		return -1;
+	for(int j = 0; j < n_jobs; j++) {
+		const int L = J->jobs[j].L;
+		int k = 0;
+		while(k < J->n_classes && J->cls[k].L != L)
+			k++;
+		if(k == J->n_classes) {
+			J->cls[k].L = L;
+			J->n_classes++;
+		}
+		J->cls[k].count++;
+	}
+	for(int k = 0; k < J->n_classes; k++) {
+		struct jb_class &C = J->cls[k];
+		long *ha = (long *) malloc(C.count * sizeof(long)), *hm = (long *) malloc(C.count * sizeof(long));
+		int n = 0;
+		for(int j = 0; j < n_jobs && ha && hm; j++)
+			if(J->jobs[j].L == C.L) {
+				ha[n] = J->jobs[j].off_alm;
+				hm[n] = J->jobs[j].off_map;
+				n++;
+			}
+		const int bad = !ha || !hm || sphr_rt_check(hipMalloc(&C.d_aoff, C.count * sizeof(long)), "hipMalloc") || sphr_rt_check(hipMalloc(&C.d_moff, C.count * sizeof(long)), "hipMalloc")
+				|| sphr_rt_check(hipMemcpy(C.d_aoff, ha, C.count * sizeof(long), hipMemcpyHostToDevice), "hipMemcpy")
+				|| sphr_rt_check(hipMemcpy(C.d_moff, hm, C.count * sizeof(long), hipMemcpyHostToDevice), "hipMemcpy")
+				|| synth_tables_build(C.L, &C.syn, 0);
+		free(ha);
+		free(hm);
+		if(bad)
+			return -1;
+	}
+	return 0;
 }
 
 extern "C" struct sphr_jobs *sphr_jobs_new(int n_jobs, const int *n_det, const double *positions, const int *N, const int *l_max, double delta_t, int precision)
@@ -939,47 +984,9 @@ extern "C" struct sphr_jobs *sphr_jobs_new(int n_jobs, const int *n_det, const d
 		sphr_jobs_free(J);
 		return NULL;
 	}
-	if((precision == SPHR_MIXED ? generate_plans<float>(J, st) : generate_plans<double>(J, st))) {
+	if((precision == SPHR_MIXED ? generate_plans<float>(J, st, build_classes) : generate_plans<double>(J, st, build_classes))) {
 		sphr_jobs_free(J);
 		return NULL;
-	}
-	/* l_max classes for the synthesis */
-	J->cls = (struct jb_class *) calloc(n_jobs, sizeof(struct jb_class));
-	if(!J->cls) {
-		sphr_jobs_free(J);
-		return NULL;
-	}
-	for(int j = 0; j < n_jobs; j++) {
-		const int L = J->jobs[j].L;
-		int k = 0;
-		while(k < J->n_classes && J->cls[k].L != L)
-			k++;
-		if(k == J->n_classes) {
-			J->cls[k].L = L;
-			J->n_classes++;
-		}
-		J->cls[k].count++;
-	}
-	for(int k = 0; k < J->n_classes; k++) {
-		struct jb_class &C = J->cls[k];
-		long *ha = (long *) malloc(C.count * sizeof(long)), *hm = (long *) malloc(C.count * sizeof(long));
-		int n = 0;
-		for(int j = 0; j < n_jobs && ha && hm; j++)
-			if(J->jobs[j].L == C.L) {
-				ha[n] = J->jobs[j].off_alm;
-				hm[n] = J->jobs[j].off_map;
-				n++;
-			}
-		const int bad = !ha || !hm || sphr_rt_check(hipMalloc(&C.d_aoff, C.count * sizeof(long)), "hipMalloc") || sphr_rt_check(hipMalloc(&C.d_moff, C.count * sizeof(long)), "hipMalloc")
-				|| sphr_rt_check(hipMemcpy(C.d_aoff, ha, C.count * sizeof(long), hipMemcpyHostToDevice), "hipMemcpy")
-				|| sphr_rt_check(hipMemcpy(C.d_moff, hm, C.count * sizeof(long), hipMemcpyHostToDevice), "hipMemcpy")
-				|| synth_tables_build(C.L, &C.syn, 0);
-		free(ha);
-		free(hm);
-		if(bad) {
-			sphr_jobs_free(J);
-			return NULL;
-		}
 	}
 	return J;
 }
