@@ -85,23 +85,6 @@ __device__ inline double sinc_pi(double x)
 	return px == 0.0 ? 1.0 : sin(px) / px;
 }
 
-/* normalised Legendre Pbar_l0(x), l = 0..lmax, into row[0..lmax]:
- * Pbar_l0 = a_l x Pbar_{l-1,0} - (a_l / a_{l-1}) Pbar_{l-2,0},
- * a_l = sqrt((4 l^2 - 1) / l^2), with lc[2l] = a_l, lc[2l+1] = a_l / a_{l-1} */
-__device__ inline void pbar_row(double x, int lmax, const double *__restrict__ lc, double *row)
-{
-	double p0 = 0.28209479177387814347, p1 = lc[2] * x * p0;
-	row[0] = p0;
-	if(lmax >= 1)
-		row[1] = p1;
-	for(int l = 2; l <= lmax; l++) {
-		const double pn = lc[2 * l] * x * p1 - lc[2 * l + 1] * p0;
-		p0 = p1;
-		p1 = pn;
-		row[l] = pn;
-	}
-}
-
 /*
  * Per baseline, over the first half of each (mirror-symmetric) rule:
  * Pbar_l0 at the tap nodes (Pt[j][l], j < nt/2, l <= a) and at the
@@ -111,15 +94,26 @@ __device__ inline void pbar_row(double x, int lmax, const double *__restrict__ l
  * / pi and the weight 2 pi w_j / nt; the product-rule weights.
  * Y = [y | g | weight] (nt/2 each) then (nq+1)/2 product weights.
  */
-__global__ void k_pg_tables(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, const double *__restrict__ lc, double *__restrict__ ws)
+#define PG_TL 8		/* orders per staged block of the table rows */
+__global__ void __launch_bounds__(256) k_pg_tables(const struct jb_bl *__restrict__ bl, int b0, const double *__restrict__ nodes, const double *__restrict__ lc, double *__restrict__ ws)
 {
+	/* each wave's 64 rows, PG_TL orders at a time: a lane runs its node's
+	 * normalised Legendre recurrence
+	 *   Pbar_l0 = a_l x Pbar_{l-1,0} - (a_l / a_{l-1}) Pbar_{l-2,0},
+	 *   a_l = sqrt((4 l^2 - 1) / l^2), lc[2l] = a_l, lc[2l+1] = a_l / a_{l-1},
+	 * and the block goes out through LDS as runs of PG_TL consecutive
+	 * orders per row (a row per lane written straight out put 64 rows'
+	 * cache lines in every store) */
+	__shared__ double tile[4][64][PG_TL + 1];
 	const struct jb_bl &B = bl[b0 + blockIdx.x];
 	const int lq = B.a > B.Lt ? B.a : B.Lt;
-	const int nth = B.nt / 2, nqh = (B.nq + 1) / 2;
-	for(int i = threadIdx.x; i < nth + nqh; i += blockDim.x) {
+	const int nth = B.nt / 2, nqh = (B.nq + 1) / 2, nn = nth + nqh;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	for(int base = 0; base < nn; base += 256) {
+		const int i = base + threadIdx.x;
+		double x = 0.0;
 		if(i < nth) {
-			const double x = nodes[B.off_nt + i];
-			pbar_row(x, B.a, lc, ws + B.off_Pt + (size_t) i * (B.a + 1));
+			x = nodes[B.off_nt + i];
 			const double y = B.r_dt * cos(acos(x));
 			const double ww = nodes[B.off_nt + B.nt + i] * (2. * M_PI / B.nt);
 			/* sin(pi y) = (-1)^n sin(pi (y - n)), n = rint(y): y - n is
@@ -131,10 +125,40 @@ __global__ void k_pg_tables(const struct jb_bl *__restrict__ bl, int b0, const d
 			ws[B.off_Y + i] = y;
 			ws[B.off_Y + nth + i] = sy / M_PI * (double) B.nt * ww;
 			ws[B.off_Y + 2 * nth + i] = ww;
-		} else {
+		} else if(i < nn) {
 			const int q = i - nth;
-			pbar_row(nodes[B.off_nq + q], lq, lc, ws + B.off_Pq + (size_t) q * (lq + 1));
+			x = nodes[B.off_nq + q];
 			ws[B.off_Y + 3 * nth + q] = nodes[B.off_nq + B.nq + q];
+		}
+		/* Pbar_l0(x) for l = 0..lq (every lane, the same bound) */
+		double p0 = 0.28209479177387814347, p1 = lc[2] * x * p0;
+		for(int l0 = 0; l0 <= lq; l0 += PG_TL) {
+#pragma unroll
+			for(int c = 0; c < PG_TL; c++) {
+				const int l = l0 + c;
+				double v = 0.0;
+				if(l == 0)
+					v = p0;
+				else if(l == 1)
+					v = p1;
+				else if(l <= lq) {
+					v = lc[2 * l] * x * p1 - lc[2 * l + 1] * p0;
+					p0 = p1;
+					p1 = v;
+				}
+				tile[wave][lane][c] = v;
+			}
+			__syncthreads();
+#pragma unroll
+			for(int k = 0; k < 8; k++) {
+				const int r = (lane >> 3) + 8 * k, c = lane & 7, nd = base + 64 * wave + r, l = l0 + c;
+				if(nd < nth) {
+					if(l <= B.a)
+						ws[B.off_Pt + (size_t) nd * (B.a + 1) + l] = tile[wave][r][c];
+				} else if(nd < nn && l <= lq)
+					ws[B.off_Pq + (size_t) (nd - nth) * (lq + 1) + l] = tile[wave][r][c];
+			}
+			__syncthreads();
 		}
 	}
 }
