@@ -44,7 +44,7 @@ struct jb_bl {
 	int a, np, nt, nq, Lt, lags, L;
 	double r_dt, theta, phi, inv_nb, scale;	/* scale = 1 / N^2 */
 	long off_nt, off_nq;		/* node arrays (x then w) in the chunk's node buffer */
-	long off_Pt, off_Pq, off_C, off_T, off_K, off_Y;	/* chunk workspace (doubles) */
+	long off_Pt, off_Pq, off_C, off_T, off_K, off_Y, off_F;	/* chunk workspace (doubles) */
 	long off_M, off_R, off_c, off_pw;	/* persistent: M (T), R (cplx T), c (T), power slot */
 };
 
@@ -163,6 +163,50 @@ __global__ void __launch_bounds__(256) k_pg_tables(const struct jb_bl *__restric
 	}
 }
 
+/*
+ * The pairs (F[i][j] + F[i][j'], F[i][j] - F[i][j']) of weighted delay
+ * elements of tap kk = i - (np-1)/2 at tap nodes j and j' = nt-1-j
+ * (y_j' = -y_j), [np][nt/2] pairs per baseline: the A operand of the taps
+ * product, made once here rather than in each of its column tiles.
+ * F = (nt w 2 pi / nt) sinc(-kk - y), sinc(-kk - y) = (-1)^kk sin(pi y)
+ * / (pi (kk + y)), so with g = sin(pi y) nt (w 2 pi / nt) / pi,
+ *   F + F' = (-1)^kk g (-2 y) / ((kk + y)(kk - y)),
+ *   F - F' = (-1)^kk g (2 kk) / ((kk + y)(kk - y)): one division.
+ * Grid (blocks of PG_FB pairs, baselines of the chunk); blocks past a
+ * baseline's pairs return.
+ */
+#define PG_FB 1024
+__global__ void __launch_bounds__(256) k_pg_fpairs(const struct jb_bl *__restrict__ bl, int b0, double *__restrict__ ws)
+{
+	const struct jb_bl &B = bl[b0 + blockIdx.y];
+	const int nth = B.nt / 2;
+	const long n = (long) B.np * nth, e0 = (long) blockIdx.x * PG_FB;
+	if(e0 >= n)
+		return;
+	const double *Y = ws + B.off_Y;
+	for(long e = e0 + threadIdx.x; e < n && e < e0 + PG_FB; e += blockDim.x) {
+		const int i = (int) (e / nth), j = (int) (e - (long) i * nth);
+		const int kk = -(B.np - 1) / 2 + i;
+		const double y = Y[j], g = Y[nth + j];
+		const double dp = (double) kk + y, dm = (double) kk - y;
+		const double pr = dp * dm;
+		double sum, dif;
+		if(pr != 0.0) {
+			const double r = ((kk & 1) ? -g : g) / pr;
+			sum = r * (-2.0 * y);
+			dif = r * (2.0 * (double) kk);
+		} else {
+			/* a delay on the sample grid: sinc(0) = 1 there */
+			const double one = (double) B.nt * Y[2 * nth + j];
+			const double f1 = dp == 0.0 ? one : ((kk & 1) ? -g : g) / dp;
+			const double f2 = dm == 0.0 ? one : ((kk & 1) ? g : -g) / dm;
+			sum = f1 + f2;
+			dif = f1 - f2;
+		}
+		*reinterpret_cast<double2 *>(ws + B.off_F + 2 * e) = make_double2(sum, dif);
+	}
+}
+
 /* K[u][q] = sum_i T[i][q] T[i+u][nq-1-q], u = uu - (np-1), q < (nq+1)/2,
  * one thread per entry: for taps too many for the tiled kernel's LDS */
 __global__ void k_pg_lag(const struct jb_bl *__restrict__ bl, int b0, double *__restrict__ ws)
@@ -245,30 +289,6 @@ __global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict_
 	auto kl = [&](int k0, int k) -> int { return MODE == PG_SYNTH ? k0 + (k < 8 ? 2 * k : 2 * (k - 8) + 1) : k0 + k; };
 	const int nth = B.nt / 2;
 	const double *Y = ws + B.off_Y;
-	/* the pair (F[i][j], F[i][j']) of weighted delay elements of tap
-	 * kk = i - (np-1)/2 at tap nodes j and j' = nt-1-j (y_j' = -y_j):
-	 * F = (nt w 2 pi / nt) sinc(-kk - y), sinc(-kk - y) = (-1)^kk sin(pi y)
-	 * / (pi (kk + y)), so with g = sin(pi y) nt (w 2 pi / nt) / pi,
-	 * F + F' = (-1)^kk g (-2 y) / ((kk + y)(kk - y)) and
-	 * F - F' = (-1)^kk g (2 kk) / ((kk + y)(kk - y)): one division */
-	auto fpair = [&](int i, int j, double *sum, double *dif) {
-		const int kk = -(B.np - 1) / 2 + i;
-		const double y = Y[j], g = Y[nth + j];
-		const double dp = (double) kk + y, dm = (double) kk - y;
-		const double pr = dp * dm;
-		if(pr != 0.0) {
-			const double r = ((kk & 1) ? -g : g) / pr;
-			*sum = r * (-2.0 * y);
-			*dif = r * (2.0 * (double) kk);
-		} else {
-			/* a delay on the sample grid: sinc(0) = 1 there */
-			const double one = (double) B.nt * Y[2 * nth + j];
-			const double f1 = dp == 0.0 ? one : ((kk & 1) ? -g : g) / dp;
-			const double f2 = dm == 0.0 ? one : ((kk & 1) ? g : -g) / dm;
-			*sum = f1 + f2;
-			*dif = f1 - f2;
-		}
-	};
 	double ra[NA][4], rb[4];
 	auto load = [&](int k0) {
 #pragma unroll
@@ -277,8 +297,11 @@ __global__ void __launch_bounds__(256) k_pg_gemm(const struct jb_bl *__restrict_
 			const int i = m0 + e / PG_KC, k = kl(k0, e % PG_KC);
 			double v0 = 0.0, v1 = 0.0;
 			if(i < M && k < K) {
-				if(MODE == PG_TAPS)
-					fpair(i, k, &v0, &v1);
+				if(MODE == PG_TAPS) {
+					const double2 f = *reinterpret_cast<const double2 *>(ws + B.off_F + 2 * ((size_t) i * nth + k));
+					v0 = f.x;
+					v1 = f.y;
+				}
 				else if(MODE == PG_SYNTH)
 					v0 = ws[B.off_C + (size_t) i * (B.a + 1) + k];
 				else {
@@ -664,6 +687,7 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st, int (*host_work)(
 	long *rule_t = NULL, *rule_q = NULL;
 	struct pg_chunk {
 		int b0, b1, lq, lag_tiled, np_max, lags_nq_max;
+		long f_max;	/* most taps pairs of a baseline */
 		long tot[4];
 	} *ch = NULL;
 	int nch = 0, rc = -1;
@@ -761,11 +785,13 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st, int (*host_work)(
 		size_t wsz = 0;
 		int b1 = b0;
 		C.np_max = C.lags_nq_max = 0;
+		C.f_max = 0;
 		while(b1 < nbl) {
 			struct jb_bl &B = J->bl[b1];
 			const int lq = B.a > B.Lt ? B.a : B.Lt;
 			const int nth = B.nt / 2, nqh = (B.nq + 1) / 2;
-			const size_t need = (size_t) nth * (B.a + 1) + (size_t) nqh * (lq + 1) + (size_t) B.np * (B.a + 1) + (size_t) B.np * B.nq + (size_t) B.lags * nqh + 3 * (size_t) nth + nqh;
+			const size_t need = (size_t) nth * (B.a + 1) + (size_t) nqh * (lq + 1) + (size_t) B.np * (B.a + 1) + (size_t) B.np * B.nq + (size_t) B.lags * nqh + 3 * (size_t) nth + nqh
+					    + 2 * (size_t) B.np * nth + 1;
 			if(b1 > b0 && (wsz + need) * sizeof(double) > budget)
 				break;
 			B.off_Pt = wsz;
@@ -774,8 +800,13 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st, int (*host_work)(
 			B.off_T = B.off_C + (long) B.np * (B.a + 1);
 			B.off_K = B.off_T + (long) B.np * B.nq;
 			B.off_Y = B.off_K + (long) B.lags * nqh;
+			/* the taps' pairs, 16-byte aligned (the workspace and every
+			 * offset before them count doubles) */
+			B.off_F = B.off_Y + 3L * nth + nqh;
+			B.off_F += B.off_F & 1;
 			wsz += need;
 			C.np_max = B.np > C.np_max ? B.np : C.np_max;
+			C.f_max = (long) B.np * nth > C.f_max ? (long) B.np * nth : C.f_max;
 			C.lags_nq_max = B.lags * B.nq > C.lags_nq_max ? B.lags * B.nq : C.lags_nq_max;	/* bounds lags * nqh */
 			b1++;
 		}
@@ -824,6 +855,7 @@ static int generate_plans(struct sphr_jobs *J, hipStream_t st, int (*host_work)(
 		const int b0 = C.b0, nb = C.b1 - C.b0;
 		const int *ts = d_ts + 4 * (size_t) b0;
 		hipLaunchKernelGGL(k_pg_tables, dim3(nb), dim3(256), 0, st, J->d_bl, b0, nodes, nodes + lc_off, ws);
+		hipLaunchKernelGGL(k_pg_fpairs, dim3((unsigned) ((C.f_max + PG_FB - 1) / PG_FB), nb), dim3(256), 0, st, J->d_bl, b0, ws);
 		hipLaunchKernelGGL((k_pg_gemm<PG_TAPS, double>), dim3(C.tot[0]), dim3(256), 0, st, J->d_bl, b0, nb, ts, ws, (double *) NULL);
 		hipLaunchKernelGGL((k_pg_gemm<PG_SYNTH, double>), dim3(C.tot[1]), dim3(256), 0, st, J->d_bl, b0, nb, ts + nb, ws, (double *) NULL);
 		if(C.lag_tiled)
