@@ -670,7 +670,9 @@ extern "C" void sphr_jobs_free(struct sphr_jobs *J)
 
 /*
  * All plans, in chunks of baselines that share one float64 workspace of at
- * most 1 GiB.  Everything the kernels read besides the workspace is laid
+ * most 4 GiB (fewer, larger launches than at 1 GiB: 12 chunks instead of
+ * 46 for 20,430 baselines, 72 -> 68 ms; 16 GiB gains nothing more).
+ * Everything the kernels read besides the workspace is laid
  * out and uploaded once: the baseline table with each chunk's workspace
  * offsets, every distinct quadrature rule once, and the per-chunk tile
  * prefix sums; the chunks then run back to back on the stream (each
@@ -680,7 +682,7 @@ extern "C" void sphr_jobs_free(struct sphr_jobs *J)
 template <typename T>
 static int generate_plans(struct sphr_jobs *J, hipStream_t st, int (*host_work)(struct sphr_jobs *))
 {
-	const size_t budget = (size_t) 1 << 30;
+	const size_t budget = (size_t) 4 << 30;
 	const int nbl = J->n_bl;
 	double *ws = NULL, *nodes = NULL, *hnodes = NULL;
 	int *d_ts = NULL, *h_ts = NULL;	/* per chunk: tile prefix sums of its four plan launches */
