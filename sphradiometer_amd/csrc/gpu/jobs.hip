@@ -614,10 +614,19 @@ __global__ void k_job_alm(const struct jb_job *__restrict__ jobs, const struct j
 		}
 		const struct jb_bl &B = bl[J.first_bl + b];
 		const T *cb = c + B.off_c, *Mb = M + B.off_M + d;
-		T acc = 0;
-		for(int uu = 0; uu < B.lags; uu++)
-			acc += cb[uu] * Mb[(size_t) uu * (B.Lt + 1)];
-		pw[e] = acc;
+		const size_t ld = B.Lt + 1;
+		/* four partial sums: four loads of the table column in flight */
+		T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+		int uu = 0;
+		for(; uu + 4 <= B.lags; uu += 4) {
+			a0 += cb[uu] * Mb[uu * ld];
+			a1 += cb[uu + 1] * Mb[(uu + 1) * ld];
+			a2 += cb[uu + 2] * Mb[(uu + 2) * ld];
+			a3 += cb[uu + 3] * Mb[(uu + 3) * ld];
+		}
+		for(; uu < B.lags; uu++)
+			a0 += cb[uu] * Mb[uu * ld];
+		pw[e] = (a0 + a1) + (a2 + a3);
 	}
 	__syncthreads();
 	const int L = J.L, ncoef = (L + 1) * (L + 1);
