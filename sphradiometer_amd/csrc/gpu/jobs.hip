@@ -596,6 +596,67 @@ __global__ void k_lagcorr(const struct jb_job *__restrict__ jobs, const struct j
 	}
 }
 
+/*
+ * float32 lag correlation with four consecutive lags per thread: thread
+ * tid of the workgroup sums lags uu0 + 4 tid + r, r < 4, over a register
+ * window of the shifted series.  Per four samples it makes one 16-byte
+ * LDS read of x_a (the same address in every lane) and two of x_b, and
+ * sixteen multiply-adds: a tenth of the LDS instructions of one lag per
+ * thread (two 4-byte reads per multiply-add), which bounded that form.
+ * The x_b chunk starts at sample s0 + u0 (mod N): one modulo per chunk,
+ * the wrap of each sample by subtraction.
+ */
+#define LC_R 4
+__global__ void k_lagcorr4(const struct jb_job *__restrict__ jobs, const struct jb_bl *__restrict__ bl, const float *__restrict__ in, int sc, float *__restrict__ c)
+{
+	extern __shared__ __align__(16) unsigned char lag4_sm_raw[];
+	float *sa = reinterpret_cast<float *>(lag4_sm_raw);	/* [sc] */
+	float *sb = sa + sc;					/* [sc + LC_R nt + LC_R] */
+	const struct jb_bl &B = bl[blockIdx.y];
+	const struct jb_job &J = jobs[B.job];
+	const int nt = blockDim.x;
+	const int uu0 = blockIdx.x * nt * LC_R;
+	if(uu0 >= B.lags)
+		return;
+	const int N = J.N;
+	const long u0 = uu0 - (B.np - 1);
+	const float *xa = in + J.off_in + (size_t) B.ia * N, *xb = in + J.off_in + (size_t) B.ib * N;
+	const int wlen = sc + LC_R * nt + LC_R;
+	const float *bw = sb + LC_R * threadIdx.x;
+	float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+	for(int s0 = 0; s0 < N; s0 += sc) {
+		const int len = N - s0 < sc ? N - s0 : sc;
+		for(int t = threadIdx.x; t < sc; t += nt)
+			sa[t] = t < len ? xa[s0 + t] : 0.f;
+		int base = (int) (((long) s0 + u0) % N);
+		if(base < 0)
+			base += N;
+		for(int t = threadIdx.x; t < wlen; t += nt) {
+			int idx = base + t;
+			while(idx >= N)
+				idx -= N;
+			sb[t] = xb[idx];
+		}
+		__syncthreads();
+		/* sa is zero past len, so the last block of four needs no test */
+		for(int t = 0; t < len; t += 4) {
+			const float4 a = *reinterpret_cast<const float4 *>(sa + t);
+			const float4 w = *reinterpret_cast<const float4 *>(bw + t), v = *reinterpret_cast<const float4 *>(bw + t + 4);
+			a0 += a.x * w.x + a.y * w.y + a.z * w.z + a.w * w.w;
+			a1 += a.x * w.y + a.y * w.z + a.z * w.w + a.w * v.x;
+			a2 += a.x * w.z + a.y * w.w + a.z * v.x + a.w * v.y;
+			a3 += a.x * w.w + a.y * v.x + a.z * v.y + a.w * v.z;
+		}
+		__syncthreads();
+	}
+	const int uu = uu0 + LC_R * threadIdx.x;
+	const float acc[LC_R] = { a0, a1, a2, a3 };
+#pragma unroll
+	for(int r = 0; r < LC_R; r++)
+		if(uu + r < B.lags)
+			c[B.off_c + uu + r] = acc[r] * (float) N;
+}
+
 /* per job: power_1d_b = c_b M_b, a_lm = sum_b R_b power_1d_b[l] */
 template <typename T>
 __global__ void k_job_alm(const struct jb_job *__restrict__ jobs, const struct jb_bl *__restrict__ bl, const T *__restrict__ c, const T *__restrict__ M, const cplx<T> *__restrict__ R, cplx<T> *__restrict__ alm)
@@ -1064,7 +1125,10 @@ static int launch_run(struct sphr_jobs *J, const float *in, void *alm, void *map
 	int lags_max = 0;
 	for(int b = 0; b < J->n_bl; b++)
 		lags_max = J->bl[b].lags > lags_max ? J->bl[b].lags : lags_max;
-	hipLaunchKernelGGL((k_lagcorr<T>), dim3((lags_max + nt - 1) / nt, J->n_bl), dim3(nt), (size_t) (2 * sc + nt) * sizeof(T), st, J->d_jobs, J->d_bl, in, sc, (T *) J->c);
+	if(sizeof(T) == sizeof(float))
+		hipLaunchKernelGGL(k_lagcorr4, dim3((lags_max + LC_R * nt - 1) / (LC_R * nt), J->n_bl), dim3(nt), (size_t) (2 * sc + LC_R * nt + LC_R) * sizeof(float), st, J->d_jobs, J->d_bl, in, sc, (float *) J->c);
+	else
+		hipLaunchKernelGGL((k_lagcorr<T>), dim3((lags_max + nt - 1) / nt, J->n_bl), dim3(nt), (size_t) (2 * sc + nt) * sizeof(T), st, J->d_jobs, J->d_bl, in, sc, (T *) J->c);
 	size_t pw = 0;
 	for(int j = 0; j < J->n_jobs; j++) {
 		size_t t = 0;
