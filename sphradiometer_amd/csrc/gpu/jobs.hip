@@ -69,6 +69,9 @@ struct sphr_jobs {
 	void *M, *R, *c;
 	long n_in, n_alm, n_map, n_pw_max;
 	struct jb_class *cls;
+	struct synth_cls *d_scls;	/* the classes as one synthesis launch */
+	long scls_blocks;
+	size_t scls_lds;
 	int lag_sc, lag_nt;		/* tuned lag-kernel tile */
 	hipGraph_t graph;
 	hipGraphExec_t exec;
@@ -732,6 +735,8 @@ extern "C" void sphr_jobs_free(struct sphr_jobs *J)
 			hipFree(J->cls[k].d_moff);
 		synth_tables_free(&J->cls[k].syn);
 	}
+	if(J->d_scls)
+		hipFree(J->d_scls);
 	free(J->cls);
 	free(J->jobs);
 	free(J->bl);
@@ -1001,7 +1006,41 @@ static int build_classes(struct sphr_jobs *J)
 		if(bad)
 			return -1;
 	}
-	return 0;
+	/* one synthesis launch over all classes, largest order first so the
+	 * longest workgroups start first */
+	struct synth_cls *h = (struct synth_cls *) calloc(J->n_classes, sizeof(struct synth_cls));
+	int *ord = (int *) malloc(J->n_classes * sizeof(int));
+	if(!h || !ord) {
+		free(h);
+		free(ord);
+		return -1;
+	}
+	for(int k = 0; k < J->n_classes; k++) {
+		int i = k;
+		for(; i > 0 && J->cls[ord[i - 1]].L < J->cls[k].L; i--)
+			ord[i] = ord[i - 1];
+		ord[i] = k;
+	}
+	long b0 = 0;
+	size_t lds = 0;
+	for(int i = 0; i < J->n_classes; i++) {
+		const struct jb_class &C = J->cls[ord[i]];
+		struct synth_cls &d = h[i];
+		d.g = C.syn.g;
+		d.tb = C.syn.tb;
+		d.aoff = C.d_aoff;
+		d.moff = C.d_moff;
+		d.b0 = b0;
+		const size_t l = J->precision == SPHR_MIXED ? synth_cls_shape<float>(&d) : synth_cls_shape<double>(&d);
+		lds = l > lds ? l : lds;
+		b0 += (long) d.gx * C.count;
+	}
+	J->scls_blocks = b0;
+	J->scls_lds = lds;
+	const int bad = sphr_rt_check(hipMalloc(&J->d_scls, J->n_classes * sizeof(struct synth_cls)), "hipMalloc") || sphr_rt_check(hipMemcpy(J->d_scls, h, J->n_classes * sizeof(struct synth_cls), hipMemcpyHostToDevice), "hipMemcpy");
+	free(h);
+	free(ord);
+	return bad ? -1 : 0;
 }
 
 extern "C" struct sphr_jobs *sphr_jobs_new(int n_jobs, const int *n_det, const double *positions, const int *N, const int *l_max, double delta_t, int precision)
@@ -1139,12 +1178,8 @@ static int launch_run(struct sphr_jobs *J, const float *in, void *alm, void *map
 	hipLaunchKernelGGL((k_job_alm<T>), dim3(J->n_jobs), dim3(256), pw * sizeof(T), st, J->d_jobs, J->d_bl, (const T *) J->c, (const T *) J->M, (const cplx<T> *) J->R, (cplx<T> *) alm);
 	if(hipGetLastError() != hipSuccess)
 		return -1;
-	if(map)
-		for(int k = 0; k < J->n_classes; k++) {
-			const struct jb_class &C = J->cls[k];
-			if(launch_synth<T, T, T>((const cplx<T> *) alm, C.count, 0, &C.syn.g, &C.syn.tb, (cplx<T> *) map, st, C.d_aoff, C.d_moff))
-				return -1;
-		}
+	if(map && launch_synth_classes<T>((const cplx<T> *) alm, J->d_scls, J->n_classes, J->scls_blocks, J->scls_lds, (cplx<T> *) map, st))
+		return -1;
 	return 0;
 }
 

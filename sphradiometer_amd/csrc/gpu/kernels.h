@@ -108,6 +108,27 @@ int launch_alm(const T *partial, int nsplit, int S, const struct xproj_geom *g, 
 template <typename TI, typename T, typename TO>
 int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom *g, const struct synth_tables *tb, cplx<TO> *map, hipStream_t st, const long *aoff = NULL, const long *moff = NULL);
 
+/* one order class of a ragged synthesis over several orders: full-sphere
+ * series (polar 0) of order g.L at alm + aoff[i], map + moff[i], i < count */
+struct synth_cls {
+	struct synth_geom g;
+	struct synth_tables tb;
+	const long *aoff, *moff;
+	long b0;		/* the class's first workgroup */
+	int rp, gx, a_in_lds, al_in_lds;
+};
+
+/* fills rp, gx and the LDS flags of d from d->g; returns the class's LDS
+ * bytes per workgroup */
+template <typename T>
+size_t synth_cls_shape(struct synth_cls *d);
+
+/* all classes in one launch of nblocks workgroups (sum of gx * count),
+ * each given lds bytes (the largest class's); d_cls in device memory,
+ * sorted by b0 */
+template <typename T>
+int launch_synth_classes(const cplx<T> *alm, const struct synth_cls *d_cls, int ncls, long nblocks, size_t lds, cplx<T> *map, hipStream_t st);
+
 /* fused: partials -> a_lm -> mesh (map may be NULL), one workgroup per
  * segment; falls back to launch_alm + launch_synth when the coefficients
  * exceed the LDS budget */

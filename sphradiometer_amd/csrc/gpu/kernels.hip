@@ -829,7 +829,7 @@ __device__ void phi_stage(const cplx<T> *F, int polar, int rp, int j0, const str
 /* synthesis of a batch of series: one workgroup per (series, group of
  * rp ring pairs); coefficients and recurrence table in LDS when they fit */
 template <typename TI, typename T, typename TO>
-__global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, int polar, int rp, int a_in_lds, int al_in_lds, struct synth_geom g, struct synth_tables tb, cplx<TO> *__restrict__ map, const long *__restrict__ aoff, const long *__restrict__ moff)
+__device__ __forceinline__ void synth_block(int s, int bx, const cplx<TI> *__restrict__ alm, int polar, int rp, int a_in_lds, int al_in_lds, const struct synth_geom &g, const struct synth_tables &tb, cplx<TO> *__restrict__ map, const long *__restrict__ aoff, const long *__restrict__ moff)
 {
 	extern __shared__ unsigned char smem_raw[];
 	const int n = g.n, L = g.L;
@@ -839,7 +839,7 @@ __global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, 
 	cplx<T> *tw = F + 2 * rp * n;				/* [n] */
 	cplx<T> *sa = tw + n;					/* [ncoef] when staged */
 	T *al = reinterpret_cast<T *>(sa + (a_in_lds ? ncoef : 0));	/* [nal] when staged */
-	const int s = blockIdx.y, j0 = blockIdx.x * rp;
+	const int j0 = bx * rp;
 	/* series s at aoff[s] / moff[s] when given (ragged batches), else
 	 * packed at uniform strides */
 	const cplx<TI> *ag = alm + (aoff ? (size_t) aoff[s] : (size_t) s * ncoef);
@@ -861,6 +861,33 @@ __global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, 
 	legendre_stage<TI, T>(ag, polar, rp, j0, g, tb, ls, F);
 	__syncthreads();
 	phi_stage<T, TO>(F, polar, rp, j0, g, ls, map + (moff ? (size_t) moff[s] : (size_t) s * n * n));
+}
+
+template <typename TI, typename T, typename TO>
+__global__ void __launch_bounds__(NT) k_synth(const cplx<TI> *__restrict__ alm, int polar, int rp, int a_in_lds, int al_in_lds, struct synth_geom g, struct synth_tables tb, cplx<TO> *__restrict__ map, const long *__restrict__ aoff, const long *__restrict__ moff)
+{
+	synth_block<TI, T, TO>(blockIdx.y, blockIdx.x, alm, polar, rp, a_in_lds, al_in_lds, g, tb, map, aoff, moff);
+}
+
+/* synthesis of ragged series of several orders in one launch: workgroup b
+ * belongs to the class c with cls[c].b0 <= b < cls[c + 1].b0 (classes
+ * sorted by b0), inside which it is series (b - b0) / gx, ring-pair
+ * group (b - b0) % gx */
+template <typename T>
+__global__ void __launch_bounds__(NT) k_synth_cls(const cplx<T> *__restrict__ alm, const struct synth_cls *__restrict__ cls, int ncls, cplx<T> *__restrict__ map)
+{
+	const long b = blockIdx.x;
+	int lo = 0, hi = ncls - 1;
+	while(lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if(cls[mid].b0 <= b)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	const struct synth_cls &d = cls[lo];
+	const long loc = b - d.b0;
+	synth_block<T, T, T>((int) (loc / d.gx), (int) (loc % d.gx), alm, 0, d.rp, d.a_in_lds, d.al_in_lds, d.g, d.tb, map, d.aoff, d.moff);
 }
 
 /*
@@ -1276,6 +1303,40 @@ int launch_synth(const cplx<TI> *alm, int S, int polar, const struct synth_geom 
 
 template int launch_synth<double, double, double>(const cplx<double> *, int, int, const struct synth_geom *, const struct synth_tables *, cplx<double> *, hipStream_t, const long *, const long *);
 template int launch_synth<float, float, float>(const cplx<float> *, int, int, const struct synth_geom *, const struct synth_tables *, cplx<float> *, hipStream_t, const long *, const long *);
+
+template <typename T>
+size_t synth_cls_shape(struct synth_cls *d)
+{
+	const size_t budget = 65536;
+	const int L = d->g.L, n = d->g.n;
+	const size_t ncoef = (size_t) (L + 1) * (L + 1), nal = (size_t) (L + 1) * (L + 2) / 2;
+	int rp = (int) (24576 / (2 * n * sizeof(cplx<T>)));
+	rp = rp < 1 ? 1 : rp > d->g.B ? d->g.B : rp;
+	size_t lds = (size_t) (2 * rp + 1) * n * sizeof(cplx<T>);
+	d->a_in_lds = lds + ncoef * sizeof(cplx<T>) <= budget - 4096;
+	if(d->a_in_lds)
+		lds += ncoef * sizeof(cplx<T>);
+	d->al_in_lds = lds + nal * sizeof(T) <= budget;
+	if(d->al_in_lds)
+		lds += nal * sizeof(T);
+	d->rp = rp;
+	d->gx = (d->g.B + rp - 1) / rp;
+	return lds;
+}
+
+template <typename T>
+int launch_synth_classes(const cplx<T> *alm, const struct synth_cls *d_cls, int ncls, long nblocks, size_t lds, cplx<T> *map, hipStream_t st)
+{
+	if(ncls < 1 || nblocks < 1 || nblocks > 0x7fffffffL || lds > 65536)
+		return -1;
+	hipLaunchKernelGGL((k_synth_cls<T>), dim3((unsigned) nblocks), dim3(NT), lds, st, alm, d_cls, ncls, map);
+	return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template size_t synth_cls_shape<double>(struct synth_cls *);
+template size_t synth_cls_shape<float>(struct synth_cls *);
+template int launch_synth_classes<double>(const cplx<double> *, const struct synth_cls *, int, long, size_t, cplx<double> *, hipStream_t);
+template int launch_synth_classes<float>(const cplx<float> *, const struct synth_cls *, int, long, size_t, cplx<float> *, hipStream_t);
 template int launch_sky<double>(const double *, int, int, const struct xproj_geom *, const cplx<double> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<double> *, cplx<double> *, hipStream_t);
 template int launch_sky<float>(const float *, int, int, const struct xproj_geom *, const cplx<float> *, const int2 *, const double *, const struct synth_geom *, const struct synth_tables *, cplx<float> *, cplx<float> *, hipStream_t);
 
