@@ -532,89 +532,42 @@ __global__ void k_pg_rot(const struct jb_bl *__restrict__ bl, int b0, cplx<T> *_
 
 /* ---------------------------------------------------------- data path */
 
-/* c[u] = N sum_s x_a[s] x_b[(s+u) mod N] for the baseline's lags; LDS
- * tiles of sc samples, up to nt lags per workgroup (nt a power of two, at
- * least 32, and at most sc) */
-template <typename T>
-__global__ void k_lagcorr(const struct jb_job *__restrict__ jobs, const struct jb_bl *__restrict__ bl, const float *__restrict__ in, int sc, T *__restrict__ c)
-{
-	extern __shared__ unsigned char lag_sm_raw[];
-	T *sa = reinterpret_cast<T *>(lag_sm_raw);	/* [sc] */
-	T *sb = sa + sc;				/* [sc + nt - 1] */
-	const struct jb_bl &B = bl[blockIdx.y];
-	const struct jb_job &J = jobs[B.job];
-	const int nt = blockDim.x;
-	const int uu0 = blockIdx.x * nt;
-	if(uu0 >= B.lags)
-		return;
-	const int N = J.N;
-	const int u0 = uu0 - (B.np - 1);
-	const float *xa = in + J.off_in + (size_t) B.ia * N, *xb = in + J.off_in + (size_t) B.ib * N;
-	/* float64: the nl lags of this workgroup take LR lanes each (a power
-	 * of two, at least 32), and the other lanes of the workgroup split the
-	 * samples, part q of P summing t = q, q + P, ... (float64 run 13.0 ->
-	 * 10.0 ms at 4096 jobs).  float32 keeps one lane per lag: the split
-	 * measured slower there (4.33 -> 4.57 ms). */
-	const int nl = B.lags - uu0 < nt ? B.lags - uu0 : nt;
-	int LR = sizeof(T) == 8 ? 32 : nt;
-	while(LR < nl)
-		LR <<= 1;
-	const int P = nt / LR, li = threadIdx.x % LR, part = threadIdx.x / LR;
-	const T *sbl = sb + li;
-	T acc = 0;
-	for(int s0 = 0; s0 < N; s0 += sc) {
-		const int len = N - s0 < sc ? N - s0 : sc;
-		for(int t = threadIdx.x; t < sc; t += nt)
-			sa[t] = t < len ? (T) xa[s0 + t] : (T) 0;
-		for(int t = threadIdx.x; t < sc + nt - 1; t += nt) {
-			/* x_b at s0 + t + u0, circular */
-			long idx = ((long) s0 + t + u0) % N;
-			if(idx < 0)
-				idx += N;
-			sb[t] = (T) xb[idx];
-		}
-		__syncthreads();
-		if(sizeof(T) == 4)
-			for(int t = 0; t < len; t++)
-				acc += sa[t] * sb[t + threadIdx.x];
-		else
-			for(int t = part; t < len; t += P)
-				acc += sa[t] * sbl[t];
-		__syncthreads();
-	}
-	if(sizeof(T) == 4) {
-		const int uu = uu0 + threadIdx.x;
-		if(uu < B.lags)
-			c[B.off_c + uu] = acc * (T) N;
-		return;
-	}
-	/* sum the parts of each lag (sa is free after the last barrier) */
-	sa[threadIdx.x] = acc;
-	__syncthreads();
-	if(part == 0 && li < nl) {
-		T sum = 0;
-		for(int q = 0; q < P; q++)
-			sum += sa[q * LR + li];
-		c[B.off_c + uu0 + li] = sum * (T) N;
-	}
-}
-
 /*
- * float32 lag correlation with four consecutive lags per thread: thread
- * tid of the workgroup sums lags uu0 + 4 tid + r, r < 4, over a register
- * window of the shifted series.  Per four samples it makes one 16-byte
- * LDS read of x_a (the same address in every lane) and two of x_b, and
- * sixteen multiply-adds: a tenth of the LDS instructions of one lag per
- * thread (two 4-byte reads per multiply-add), which bounded that form.
+ * c[u] = N sum_s x_a[s] x_b[(s+u) mod N] for the baseline's lags, four
+ * consecutive lags per thread: thread tid of the workgroup sums lags
+ * uu0 + 4 tid + r, r < 4, over a register window of the shifted series,
+ * through LDS tiles of sc samples (sc a multiple of four).  Per four
+ * samples it reads four values of x_a (the same address in every lane)
+ * and eight of x_b, as 16-byte LDS reads, for sixteen multiply-adds: a
+ * tenth of the LDS instructions of one lag per thread (two reads per
+ * multiply-add), which bounded that form (float32 4.24 -> 1.84 ms).
  * The x_b chunk starts at sample s0 + u0 (mod N): one modulo per chunk,
  * the wrap of each sample by subtraction.
  */
+template <typename T>
+struct lag4 {
+	T x, y, z, w;
+};
+
+__device__ __forceinline__ lag4<float> lag_ld4(const float *p)
+{
+	const float4 v = *reinterpret_cast<const float4 *>(p);
+	return { v.x, v.y, v.z, v.w };
+}
+
+__device__ __forceinline__ lag4<double> lag_ld4(const double *p)
+{
+	const double2 u = *reinterpret_cast<const double2 *>(p), v = *reinterpret_cast<const double2 *>(p + 2);
+	return { u.x, u.y, v.x, v.y };
+}
+
 #define LC_R 4
-__global__ void k_lagcorr4(const struct jb_job *__restrict__ jobs, const struct jb_bl *__restrict__ bl, const float *__restrict__ in, int sc, float *__restrict__ c)
+template <typename T>
+__global__ void k_lagcorr4(const struct jb_job *__restrict__ jobs, const struct jb_bl *__restrict__ bl, const float *__restrict__ in, int sc, T *__restrict__ c)
 {
 	extern __shared__ __align__(16) unsigned char lag4_sm_raw[];
-	float *sa = reinterpret_cast<float *>(lag4_sm_raw);	/* [sc] */
-	float *sb = sa + sc;					/* [sc + LC_R nt + LC_R] */
+	T *sa = reinterpret_cast<T *>(lag4_sm_raw);	/* [sc] */
+	T *sb = sa + sc;				/* [sc + LC_R nt + LC_R] */
 	const struct jb_bl &B = bl[blockIdx.y];
 	const struct jb_job &J = jobs[B.job];
 	const int nt = blockDim.x;
@@ -625,12 +578,12 @@ __global__ void k_lagcorr4(const struct jb_job *__restrict__ jobs, const struct 
 	const long u0 = uu0 - (B.np - 1);
 	const float *xa = in + J.off_in + (size_t) B.ia * N, *xb = in + J.off_in + (size_t) B.ib * N;
 	const int wlen = sc + LC_R * nt + LC_R;
-	const float *bw = sb + LC_R * threadIdx.x;
-	float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+	const T *bw = sb + LC_R * threadIdx.x;
+	T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 	for(int s0 = 0; s0 < N; s0 += sc) {
 		const int len = N - s0 < sc ? N - s0 : sc;
 		for(int t = threadIdx.x; t < sc; t += nt)
-			sa[t] = t < len ? xa[s0 + t] : 0.f;
+			sa[t] = t < len ? (T) xa[s0 + t] : (T) 0;
 		int base = (int) (((long) s0 + u0) % N);
 		if(base < 0)
 			base += N;
@@ -638,13 +591,12 @@ __global__ void k_lagcorr4(const struct jb_job *__restrict__ jobs, const struct 
 			int idx = base + t;
 			while(idx >= N)
 				idx -= N;
-			sb[t] = xb[idx];
+			sb[t] = (T) xb[idx];
 		}
 		__syncthreads();
 		/* sa is zero past len, so the last block of four needs no test */
 		for(int t = 0; t < len; t += 4) {
-			const float4 a = *reinterpret_cast<const float4 *>(sa + t);
-			const float4 w = *reinterpret_cast<const float4 *>(bw + t), v = *reinterpret_cast<const float4 *>(bw + t + 4);
+			const lag4<T> a = lag_ld4(sa + t), w = lag_ld4(bw + t), v = lag_ld4(bw + t + 4);
 			a0 += a.x * w.x + a.y * w.y + a.z * w.z + a.w * w.w;
 			a1 += a.x * w.y + a.y * w.z + a.z * w.w + a.w * v.x;
 			a2 += a.x * w.z + a.y * w.w + a.z * v.x + a.w * v.y;
@@ -653,11 +605,11 @@ __global__ void k_lagcorr4(const struct jb_job *__restrict__ jobs, const struct 
 		__syncthreads();
 	}
 	const int uu = uu0 + LC_R * threadIdx.x;
-	const float acc[LC_R] = { a0, a1, a2, a3 };
+	const T acc[LC_R] = { a0, a1, a2, a3 };
 #pragma unroll
 	for(int r = 0; r < LC_R; r++)
 		if(uu + r < B.lags)
-			c[B.off_c + uu + r] = acc[r] * (float) N;
+			c[B.off_c + uu + r] = acc[r] * (T) N;
 }
 
 /* per job: power_1d_b = c_b M_b, a_lm = sum_b R_b power_1d_b[l] */
@@ -1164,10 +1116,7 @@ static int launch_run(struct sphr_jobs *J, const float *in, void *alm, void *map
 	int lags_max = 0;
 	for(int b = 0; b < J->n_bl; b++)
 		lags_max = J->bl[b].lags > lags_max ? J->bl[b].lags : lags_max;
-	if(sizeof(T) == sizeof(float))
-		hipLaunchKernelGGL(k_lagcorr4, dim3((lags_max + LC_R * nt - 1) / (LC_R * nt), J->n_bl), dim3(nt), (size_t) (2 * sc + LC_R * nt + LC_R) * sizeof(float), st, J->d_jobs, J->d_bl, in, sc, (float *) J->c);
-	else
-		hipLaunchKernelGGL((k_lagcorr<T>), dim3((lags_max + nt - 1) / nt, J->n_bl), dim3(nt), (size_t) (2 * sc + nt) * sizeof(T), st, J->d_jobs, J->d_bl, in, sc, (T *) J->c);
+	hipLaunchKernelGGL((k_lagcorr4<T>), dim3((lags_max + LC_R * nt - 1) / (LC_R * nt), J->n_bl), dim3(nt), (size_t) (2 * sc + LC_R * nt + LC_R) * sizeof(T), st, J->d_jobs, J->d_bl, in, sc, (T *) J->c);
 	size_t pw = 0;
 	for(int j = 0; j < J->n_jobs; j++) {
 		size_t t = 0;
